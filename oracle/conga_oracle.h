@@ -1,0 +1,164 @@
+/*
+ * conga_oracle.h -- C interface of the CPU oracle.
+ *
+ * TEST INFRASTRUCTURE ONLY.  The oracle is a plain C++17/OpenMP FP64
+ * restatement of SuperConga's self-consistency path (the reference's CPU
+ * backend semantics), written to check the HIP product against.  Only the
+ * tests, the smoke check and the CPU-baseline leg of the benchmark may load
+ * it.  The product never links, loads or calls anything in this directory.
+ *
+ * Every function cites the reference file:line it restates in
+ * conga_oracle.cpp.
+ */
+#ifndef CONGA_ORACLE_H_
+#define CONGA_ORACLE_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OC_MAX_COMPONENTS 4
+#define OC_MAX_GEOMETRY 16
+
+/* Geometry component kinds. */
+enum { OC_DISC = 0, OC_REGULAR_POLYGON = 1, OC_POLYGON = 2 };
+/* Order-parameter symmetries. */
+enum { OC_SWAVE = 0, OC_DX2Y2 = 1, OC_DXY = 2, OC_GWAVE = 3 };
+/* Gauges of the external vector potential. */
+enum { OC_LANDAU = 0, OC_SYMMETRIC = 1, OC_SOLENOID = 2 };
+/* Residual norms. */
+enum { OC_L1 = 0, OC_L2 = 1, OC_LINF = 2 };
+/* Accelerators. */
+enum { OC_PICARD = 0, OC_ANDERSON = 1, OC_CONGACC = 2, OC_BB = 3, OC_POLYAK = 4 };
+
+typedef struct {
+  /* physics */
+  double temperature;
+  double external_flux_quanta;
+  double penetration_depth;
+  double crystal_axes_rotation; /* radians */
+  int gauge;
+  int charge_sign;
+  int num_components;
+  int symmetry[OC_MAX_COMPONENTS];
+  double critical_temperature[OC_MAX_COMPONENTS];
+
+  /* geometry.  geometry_internal = 0: user units as in simulation_config.json
+   * (bounding box sets the grain width).  geometry_internal = 1: grain_width
+   * is given and the components use the internal [-0.5, 0.5] frame:
+   * disc (cx, cy, radius), regular polygon (cx, cy, num_edges, rotation
+   * [radians], side_length; side_length < 0 means circumradius 0.5). */
+  int geometry_internal;
+  double grain_width;
+  int num_geometry;
+  int geometry_type[OC_MAX_GEOMETRY];
+  int geometry_add[OC_MAX_GEOMETRY];
+  double geometry_param[OC_MAX_GEOMETRY][5];
+  int polygon_num_vertices[OC_MAX_GEOMETRY];
+  int polygon_vertex_offset[OC_MAX_GEOMETRY];
+  const double *polygon_vertices_x;
+  const double *polygon_vertices_y;
+
+  /* numerics */
+  double convergence_criterion;
+  double energy_cutoff;
+  int norm;
+  int num_fermi_momenta;
+  double points_per_coherence_length;
+  double vector_potential_error; /* only 0 (the exact Green function) */
+
+  /* compute objects */
+  int compute_current;
+  int compute_free_energy;
+
+  /* accelerator; non-positive numbers select the reference defaults */
+  int accelerator;
+  double step_size;
+  int rank;
+  double drag;
+  double step_size_factor;
+  double cos_similarity_min;
+  double step_size_max;
+  double step_size_min;
+
+  /* 1: unwritten coherence-function sites are zero instead of keeping the
+   * values of a previous angle (diagnostic; 0 restates the reference). */
+  int zero_stale;
+  int num_threads; /* <= 0: OpenMP default */
+} oc_config;
+
+void *oc_create(const oc_config *config);
+void oc_destroy(void *handle);
+const char *oc_last_error(void);
+
+/* Scalar queries: "N0", "N1", "num_energies", "num_angles",
+ * "num_grain_sites", "num_storage_in_gamma", "num_storage_out_gamma",
+ * "num_storage_in_gammatilde", "num_storage_out_gammatilde",
+ * "num_iterations", "traj_steps" */
+long long oc_get_int(void *handle, const char *key);
+/* "residual", "residual_boundary", "residual_order_parameter",
+ * "residual_current", "residual_free_energy", "residual_vector_potential",
+ * "free_energy", "free_energy_per_area", "magnetic_free_energy",
+ * "magnetic_free_energy_per_area", "grain_fraction", "h",
+ * "coupling_0".."coupling_3", "delta_bulk_0".."delta_bulk_3",
+ * "energy_<i>", "residue_<i>", "system_angle_<k>" */
+double oc_get_double(void *handle, const char *key);
+
+/* Arrays: "delta" (context order parameter, Nc*2*N0^2, planar re/im per
+ * component), "delta_new" (gap-equation result), "current" (2*N0^2),
+ * "vector_potential" (context A scaled by lambda^2, 2*N0^2),
+ * "vector_potential_new", "free_energy_grid" (2*N0^2), "domain" (N0^2),
+ * "boundary_in_gamma", "boundary_out_gamma", "boundary_in_gammatilde",
+ * "boundary_out_gammatilde" (2*count each, planar). Returns the number of
+ * doubles written or -1. */
+long long oc_get_array(void *handle, const char *name, double *out,
+                       long long capacity);
+/* Setters (planar re/im). The order parameter is masked outside the grain
+ * like OrderParameter::initialize; the vector potential is scaled by
+ * lambda^2 like the load path. */
+int oc_set_order_parameter(void *handle, int component, const double *re,
+                           const double *im);
+int oc_set_vector_potential(void *handle, const double *ax, const double *ay);
+int oc_set_array(void *handle, const char *name, const double *in,
+                 long long count);
+int oc_initial_bulk(void *handle, double phase);
+
+/* Geometry of one rotational frame: labels, distances and normals on the
+ * N1 x N1 grid for angle index k (frame -systemAngle(k)). */
+int oc_get_geometry(void *handle, int angle, signed char *labels,
+                    double *distance, double *normal_x, double *normal_y);
+
+int oc_compute(void *handle); /* Context::compute */
+int oc_burnin(void *handle);  /* Context::burnIn */
+/* runSimulation(burnin, min, max): returns 1 if converged. */
+int oc_run(void *handle, int num_burnin, int min_iter, int max_iter);
+
+/* Timing helper for the CPU baseline: runs only the Riccati sweeps (both
+ * directions) for angles [angle_begin, angle_end) with the current state and
+ * returns the number of trajectory steps taken. */
+long long oc_sweep_angles(void *handle, int angle_begin, int angle_end);
+
+/* ----- unit-level restatements (closed-form checks) ----- */
+void oc_riccati_step(const double *delta, const double *energy,
+                     const double *initial, double step, double *out);
+void oc_homogeneous(const double *delta, const double *energy, double *gamma,
+                    double *gammatilde);
+void oc_greens(const double *gamma, const double *gammatilde, double *g,
+               double *f_sum);
+int oc_num_ozaki(double temperature, double cutoff);
+int oc_ozaki(int num_poles, double *poles, double *residues);
+void oc_lattice(double width, double ppxi, int *n0, int *n1,
+                double *grain_fraction);
+double oc_bulk_gap(double temperature);
+double oc_bulk_free_energy(double temperature, double delta);
+double oc_yoshida(double temperature, double delta, double cutoff);
+/* ComputeVectorPotential with the exact Green function. */
+int oc_vector_potential(int n, double h, int num_grain, int enforce_mean,
+                        const double *current, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
