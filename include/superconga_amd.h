@@ -1,0 +1,144 @@
+/*
+ * superconga_amd.h -- C ABI of the MI355X self-consistency path.
+ *
+ * A drop-in for SuperConga's self-consistency hot path: the Riccati solver,
+ * the specular boundary condition, the angle/energy reductions into order
+ * parameter, current and free energy, the vector-potential solve and the
+ * accelerators, all on one GPU.  The entry points replace the reference's
+ * C++ module interfaces (file:line citations below are into the reference
+ * tree, superconga 1.1.0).  Host buffers use the reference's GridData layout:
+ * row-major y*N + x per field, separate real and imaginary planes.
+ *
+ * Every function returns SC_OK or an error code; sc_last_error() returns a
+ * thread-local message for the last failure.  Handles are independent; one
+ * handle must not be used from two threads at once.
+ */
+#ifndef SUPERCONGA_AMD_H_
+#define SUPERCONGA_AMD_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SC_OK 0
+#define SC_ERR_ARGUMENT 1
+#define SC_ERR_CONFIG 2
+#define SC_ERR_DEVICE 3
+#define SC_ERR_STATE 4
+
+/* Arithmetic of the Riccati stepping. */
+#define SC_PRECISION_FP64 0  /* everything in FP64 */
+#define SC_PRECISION_MIXED 1 /* FP32 stepping, FP64 energy sums and reductions */
+
+typedef struct sc_context sc_context;
+
+typedef struct {
+  int n0;               /* reference lattice side (Parameters::getGridResolutionBase) */
+  int n1;               /* rotated-frame lattice side (getGridResolutionCoherence) */
+  int num_energies;     /* Ozaki poles */
+  int num_angles;       /* Fermi-surface angles */
+  int num_components;   /* order-parameter components */
+  int num_grain_sites;  /* GeometryGroup::getNumGrainLatticeSites */
+  long long num_in_gamma, num_out_gamma, num_in_gammatilde, num_out_gammatilde;
+  long long traj_steps; /* trajectory steps per iteration (2 Ne sum of inside sites) */
+  double h;             /* lattice spacing in xi0 */
+  double grain_fraction;
+} sc_dims;
+
+typedef struct {
+  double residual; /* Context::getResidual (Context.h:462-483) */
+  double residual_boundary, residual_order_parameter, residual_current,
+      residual_free_energy, residual_vector_potential;
+  double free_energy, free_energy_per_area, magnetic_free_energy,
+      magnetic_free_energy_per_area; /* ComputeFreeEnergy getters */
+  int iterations;
+  int converged; /* Context::isPartiallyConverged (Context.h:485-497) */
+  double last_sweep_ms;     /* Riccati sweep kernel time of the last iteration */
+  double last_iteration_ms; /* whole iteration */
+} sc_results;
+
+const char *sc_last_error(void);
+int sc_version(void);
+
+/* Builds a context from the text of a simulation_config.json (the file the
+ * reference's simulation binary reads, src/simulation.cu:64-463): parameters,
+ * geometry, Ozaki energies, order parameter with its initial guess,
+ * accelerator and compute objects.  Replaces Context<T>::initialize
+ * (Context.h:285-375). */
+int sc_create(const char *config_json, sc_context **out);
+int sc_destroy(sc_context *ctx);
+int sc_get_dims(sc_context *ctx, sc_dims *out);
+/* Ozaki energies (imaginary parts) and residues, IntegrationIteratorOzaki.h:165-200. */
+int sc_get_energies(sc_context *ctx, double *energies, double *residues);
+
+/* Order parameter component `comp` (OrderParameter::getComponentGrid,
+ * OrderParameter.h:214-230).  which = 0: the iterate x; which = 1: the
+ * gap-equation result g(x) of the last compute (ComputeOrderParameter::
+ * getResult).  Setting masks the sites outside the grain like
+ * OrderParameter::initialize (OrderParameter.h:180-199). */
+int sc_set_order_parameter(sc_context *ctx, int comp, const double *re,
+                           const double *im);
+int sc_get_order_parameter(sc_context *ctx, int which, int comp, double *re,
+                           double *im);
+/* Induced vector potential in units of A0 (unscaled).  which = 0: iterate,
+ * 1: last Poisson solution.  The reference stores it scaled by lambda^2
+ * (src/simulation.cu:369-376). */
+int sc_set_vector_potential(sc_context *ctx, const double *ax, const double *ay);
+int sc_get_vector_potential(sc_context *ctx, int which, double *ax, double *ay);
+/* Current density of the last compute (ComputeCurrent::getResult). */
+int sc_get_current(sc_context *ctx, double *jx, double *jy);
+/* Free-energy density T Re I(r) summed over angles (ComputeFreeEnergy grid). */
+int sc_get_free_energy_density(sc_context *ctx, double *out);
+/* Sparse boundary storage (BoundaryStorage.h:158-169, layout
+ * (crossing * Ne + energy)).  in_out: 0 in, 1 out; dir: 0 gamma, 1 gamma~. */
+int sc_get_boundary(sc_context *ctx, int in_out, int dir, double *re, double *im);
+int sc_set_boundary(sc_context *ctx, int dir, const double *re, const double *im);
+
+/* One self-consistency iteration, Context<T>::compute (Context.h:391-447):
+ * accelerator step (from the second call on), Riccati sweeps over all angles
+ * and energies fused with the Green-function sums, angle integration, gap
+ * equation, current, free energy, specular boundary condition and, with a
+ * finite penetration depth, the vector-potential solve. */
+int sc_compute(sc_context *ctx, int precision);
+/* One burn-in iteration, Context<T>::burnIn (Context.h:377-389). */
+int sc_burnin(sc_context *ctx, int precision);
+/* runSimulation (run_simulation.h:124-187) with the limits of the config
+ * when the arguments are negative; *converged receives the final status. */
+int sc_run(sc_context *ctx, int num_burnin, int num_min, int num_max,
+           int precision, int *converged);
+int sc_get_results(sc_context *ctx, sc_results *out);
+
+/* Individual stages. */
+/* RiccatiSolver::computeCoherenceFunctions for every angle (RiccatiSolverConfined.h:513-671)
+ * fused with ComputeProperty::computeGreensAngle; burnin = 1 only updates the
+ * boundary out-values (computeCoherenceFunctionsBurnIn). */
+int sc_riccati_sweep(sc_context *ctx, int precision, int burnin);
+/* BoundaryConditionSpecular::computeBoundaryCondition (BoundaryConditionSpecular.h:361-486). */
+int sc_boundary_condition(sc_context *ctx, double *residual);
+/* Lane layout of the sweep: Ozaki energies per lane (4, 8 or 16) and
+ * persistent wavefronts per compute unit (the autotuned shape). */
+int sc_set_sweep_shape(sc_context *ctx, int energies_per_lane, int slots_per_cu);
+
+/* ComputeVectorPotential with vector_potential_error = 0
+ * (ComputeVectorPotential.h:143-201,647-745): A = -h^2 G * (j - <j>_grain).
+ * current and out are 2*n*n (x plane then y plane). */
+int sc_poisson_solve(int n, double h, int num_grain, int enforce_mean,
+                     const double *current, double *out);
+
+/* Unit-level device arithmetic for closed-form checks: n midpoint steps
+ * (RiccatiSolutions.h:65-94).  in = n records of (delta re, im, energy re,
+ * im, initial re, im, step); out = n records of (re, im). */
+int sc_riccati_steps(int n, const double *in, double *out, int precision);
+/* Host-side tables exposed for checks. */
+int sc_num_ozaki(double temperature, double cutoff);
+int sc_ozaki(int num_poles, double *poles, double *residues);
+double sc_bulk_gap(double temperature);
+/* Geometry labels of frame `angle` (N1*N1, GeometryGroup::create(-theta)). */
+int sc_get_labels(sc_context *ctx, int angle, signed char *labels);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

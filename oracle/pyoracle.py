@@ -264,11 +264,13 @@ class Oracle:
     def domain(self):
         return self.array("domain", self.N0 ** 2).reshape(self.N0, self.N0)
 
-    def boundary(self, name):
-        cnt = self.get_int("num_storage_" + name.split("_", 2)[2]) * self.Ne
-        inout = name.split("_")[1]
-        base = name.split("_", 2)[2]
-        a = self.array("boundary_%s_%s" % (inout, base), 2 * cnt)
+    def boundary(self, in_out, direction):
+        """Sparse boundary storage; in_out 0/1 = in/out, direction 0/1 =
+        gamma/gamma-tilde (same convention as the product's C ABI)."""
+        io = ("in", "out")[in_out]
+        base = ("gamma", "gammatilde")[direction]
+        cnt = self.get_int("num_storage_%s_%s" % (io, base)) * self.Ne
+        a = self.array("boundary_%s_%s" % (io, base), 2 * cnt)
         return a[:cnt] + 1j * a[cnt:]
 
     def set_order_parameter(self, comp, values):

@@ -1,0 +1,276 @@
+"""superconga_amd -- MI355X (gfx950) self-consistency path of SuperConga.
+
+Python mirror of the reference's simulation interface over the C ABI of
+libsuperconga_amd.so (include/superconga_amd.h).  The heavy lifting is done
+by hand-written HIP kernels; this module only moves configuration and host
+arrays across the boundary.  It fails loudly when the library is missing:
+there is no CPU fallback.
+"""
+import ctypes
+import json
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libsuperconga_amd.so")
+
+FP64 = 0
+MIXED = 1
+_PRECISION = {"fp64": FP64, "mixed": MIXED, FP64: FP64, MIXED: MIXED}
+
+
+class SupercongaError(RuntimeError):
+    """Error reported by the C ABI (status code and message)."""
+
+    def __init__(self, code, message):
+        super().__init__("superconga_amd error %d: %s" % (code, message))
+        self.code = code
+
+
+class Dims(ctypes.Structure):
+    _fields_ = [("n0", ctypes.c_int), ("n1", ctypes.c_int),
+                ("num_energies", ctypes.c_int), ("num_angles", ctypes.c_int),
+                ("num_components", ctypes.c_int),
+                ("num_grain_sites", ctypes.c_int),
+                ("num_in_gamma", ctypes.c_longlong),
+                ("num_out_gamma", ctypes.c_longlong),
+                ("num_in_gammatilde", ctypes.c_longlong),
+                ("num_out_gammatilde", ctypes.c_longlong),
+                ("traj_steps", ctypes.c_longlong),
+                ("h", ctypes.c_double), ("grain_fraction", ctypes.c_double)]
+
+
+class Results(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "residual", "residual_boundary", "residual_order_parameter",
+        "residual_current", "residual_free_energy",
+        "residual_vector_potential", "free_energy", "free_energy_per_area",
+        "magnetic_free_energy", "magnetic_free_energy_per_area")] + [
+        ("iterations", ctypes.c_int), ("converged", ctypes.c_int),
+        ("last_sweep_ms", ctypes.c_double),
+        ("last_iteration_ms", ctypes.c_double)]
+
+
+_lib = None
+_DP = ctypes.POINTER(ctypes.c_double)
+
+
+def lib():
+    """Loads libsuperconga_amd.so (built by superconga_amd.build)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                "libsuperconga_amd.so is not built (%s); run "
+                "python -m superconga_amd.build" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.sc_last_error.restype = ctypes.c_char_p
+        L.sc_create.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
+        L.sc_destroy.argtypes = [vp]
+        L.sc_get_dims.argtypes = [vp, ctypes.POINTER(Dims)]
+        L.sc_get_energies.argtypes = [vp, _DP, _DP]
+        L.sc_set_order_parameter.argtypes = [vp, ctypes.c_int, _DP, _DP]
+        L.sc_get_order_parameter.argtypes = [vp, ctypes.c_int, ctypes.c_int, _DP, _DP]
+        L.sc_set_vector_potential.argtypes = [vp, _DP, _DP]
+        L.sc_get_vector_potential.argtypes = [vp, ctypes.c_int, _DP, _DP]
+        L.sc_get_current.argtypes = [vp, _DP, _DP]
+        L.sc_get_free_energy_density.argtypes = [vp, _DP]
+        L.sc_get_boundary.argtypes = [vp, ctypes.c_int, ctypes.c_int, _DP, _DP]
+        L.sc_set_boundary.argtypes = [vp, ctypes.c_int, _DP, _DP]
+        L.sc_compute.argtypes = [vp, ctypes.c_int]
+        L.sc_burnin.argtypes = [vp, ctypes.c_int]
+        L.sc_run.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                             ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.sc_get_results.argtypes = [vp, ctypes.POINTER(Results)]
+        L.sc_riccati_sweep.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.sc_boundary_condition.argtypes = [vp, _DP]
+        L.sc_set_sweep_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.sc_poisson_solve.argtypes = [ctypes.c_int, ctypes.c_double,
+                                       ctypes.c_int, ctypes.c_int, _DP, _DP]
+        L.sc_riccati_steps.argtypes = [ctypes.c_int, _DP, _DP, ctypes.c_int]
+        L.sc_num_ozaki.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.sc_ozaki.argtypes = [ctypes.c_int, _DP, _DP]
+        L.sc_bulk_gap.restype = ctypes.c_double
+        L.sc_bulk_gap.argtypes = [ctypes.c_double]
+        L.sc_get_labels.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_byte)]
+        _lib = L
+    return _lib
+
+
+def _check(code):
+    if code != 0:
+        raise SupercongaError(code, lib().sc_last_error().decode())
+
+
+def _dp(a):
+    return a.ctypes.data_as(_DP)
+
+
+def _precision(p):
+    try:
+        return _PRECISION[p]
+    except KeyError:
+        raise ValueError("precision must be 'fp64' or 'mixed'") from None
+
+
+class Simulation:
+    """One grain: Context<T> with its modules (Context.h:43-183)."""
+
+    def __init__(self, config):
+        text = config if isinstance(config, str) else json.dumps(config)
+        h = ctypes.c_void_p()
+        _check(lib().sc_create(text.encode(), ctypes.byref(h)))
+        self._h = h
+        d = Dims()
+        _check(lib().sc_get_dims(self._h, ctypes.byref(d)))
+        self.dims = d
+        self.N0, self.N1 = d.n0, d.n1
+        self.Ne, self.Na, self.Nc = d.num_energies, d.num_angles, d.num_components
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ----- state I/O -----
+    def energies(self):
+        e = np.empty(self.Ne)
+        r = np.empty(self.Ne)
+        _check(lib().sc_get_energies(self._h, _dp(e), _dp(r)))
+        return e, r
+
+    def set_order_parameter(self, comp, values):
+        v = np.asarray(values, dtype=np.complex128).reshape(self.N0, self.N0)
+        re = np.ascontiguousarray(v.real)
+        im = np.ascontiguousarray(v.imag)
+        _check(lib().sc_set_order_parameter(self._h, comp, _dp(re), _dp(im)))
+
+    def order_parameter(self, which=0):
+        out = np.empty((self.Nc, self.N0, self.N0), dtype=np.complex128)
+        re = np.empty((self.N0, self.N0))
+        im = np.empty((self.N0, self.N0))
+        for c in range(self.Nc):
+            _check(lib().sc_get_order_parameter(self._h, which, c, _dp(re), _dp(im)))
+            out[c] = re + 1j * im
+        return out
+
+    def set_vector_potential(self, ax, ay):
+        ax = np.ascontiguousarray(ax, dtype=np.float64)
+        ay = np.ascontiguousarray(ay, dtype=np.float64)
+        _check(lib().sc_set_vector_potential(self._h, _dp(ax), _dp(ay)))
+
+    def vector_potential(self, which=0):
+        ax = np.empty((self.N0, self.N0))
+        ay = np.empty((self.N0, self.N0))
+        _check(lib().sc_get_vector_potential(self._h, which, _dp(ax), _dp(ay)))
+        return np.stack([ax, ay])
+
+    def current(self):
+        jx = np.empty((self.N0, self.N0))
+        jy = np.empty((self.N0, self.N0))
+        _check(lib().sc_get_current(self._h, _dp(jx), _dp(jy)))
+        return np.stack([jx, jy])
+
+    def free_energy_density(self):
+        out = np.empty((self.N0, self.N0))
+        _check(lib().sc_get_free_energy_density(self._h, _dp(out)))
+        return out
+
+    def boundary(self, in_out, direction):
+        d = self.dims
+        n = {(0, 0): d.num_in_gamma, (1, 0): d.num_out_gamma,
+             (0, 1): d.num_in_gammatilde, (1, 1): d.num_out_gammatilde}[
+            (in_out, direction)] * self.Ne
+        re = np.empty(n)
+        im = np.empty(n)
+        _check(lib().sc_get_boundary(self._h, in_out, direction, _dp(re), _dp(im)))
+        return re + 1j * im
+
+    def set_boundary(self, direction, values):
+        v = np.asarray(values, dtype=np.complex128)
+        re = np.ascontiguousarray(v.real)
+        im = np.ascontiguousarray(v.imag)
+        _check(lib().sc_set_boundary(self._h, direction, _dp(re), _dp(im)))
+
+    def labels(self, angle):
+        out = np.empty(self.N1 * self.N1, dtype=np.int8)
+        _check(lib().sc_get_labels(self._h, angle,
+                                   out.ctypes.data_as(ctypes.POINTER(ctypes.c_byte))))
+        return out.reshape(self.N1, self.N1)
+
+    # ----- iteration -----
+    def compute(self, precision="fp64"):
+        _check(lib().sc_compute(self._h, _precision(precision)))
+
+    def burnin(self, precision="fp64"):
+        _check(lib().sc_burnin(self._h, _precision(precision)))
+
+    def run(self, burnin=-1, nmin=-1, nmax=-1, precision="fp64"):
+        c = ctypes.c_int()
+        _check(lib().sc_run(self._h, burnin, nmin, nmax, _precision(precision),
+                            ctypes.byref(c)))
+        return bool(c.value)
+
+    def results(self):
+        r = Results()
+        _check(lib().sc_get_results(self._h, ctypes.byref(r)))
+        return {n: getattr(r, n) for n, _ in Results._fields_}
+
+    def riccati_sweep(self, precision="fp64", burnin=False):
+        _check(lib().sc_riccati_sweep(self._h, _precision(precision), int(burnin)))
+
+    def boundary_condition(self):
+        r = ctypes.c_double()
+        _check(lib().sc_boundary_condition(self._h, ctypes.byref(r)))
+        return r.value
+
+    def set_sweep_shape(self, energies_per_lane, slots_per_cu):
+        _check(lib().sc_set_sweep_shape(self._h, energies_per_lane, slots_per_cu))
+
+
+def poisson_solve(current, h, num_grain, enforce_mean=True):
+    """ComputeVectorPotential::solve with the exact Green function."""
+    j = np.ascontiguousarray(current, dtype=np.float64)
+    n = j.shape[-1]
+    out = np.empty((2, n, n))
+    _check(lib().sc_poisson_solve(n, h, num_grain, int(enforce_mean), _dp(j), _dp(out)))
+    return out
+
+
+def riccati_steps(delta, energy, initial, step, precision="fp64"):
+    """Midpoint Riccati steps on the device for arrays of inputs."""
+    d = np.atleast_1d(np.asarray(delta, dtype=np.complex128))
+    e = np.broadcast_to(np.asarray(energy, dtype=np.complex128), d.shape)
+    y = np.broadcast_to(np.asarray(initial, dtype=np.complex128), d.shape)
+    s = np.broadcast_to(np.asarray(step, dtype=np.float64), d.shape)
+    rec = np.ascontiguousarray(np.stack([d.real, d.imag, e.real, e.imag, y.real,
+                                         y.imag, s], axis=-1).reshape(-1, 7))
+    out = np.empty((rec.shape[0], 2))
+    _check(lib().sc_riccati_steps(rec.shape[0], _dp(rec), _dp(out), _precision(precision)))
+    return (out[:, 0] + 1j * out[:, 1]).reshape(d.shape)
+
+
+def num_ozaki(temperature, cutoff):
+    return lib().sc_num_ozaki(temperature, cutoff)
+
+
+def ozaki(n):
+    p = np.empty(n)
+    r = np.empty(n)
+    _check(lib().sc_ozaki(n, _dp(p), _dp(r)))
+    return p, r
+
+
+def bulk_gap(temperature):
+    return lib().sc_bulk_gap(temperature)

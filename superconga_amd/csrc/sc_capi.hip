@@ -1,0 +1,342 @@
+// sc_capi.hip -- the extern "C" boundary of libsuperconga_amd.so.
+
+#include "../../include/superconga_amd.h"
+#include "sc_engine.hpp"
+#include "sc_device.hpp"
+
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+struct sc_context {
+  std::unique_ptr<scamd::Grain> grain;
+};
+
+namespace {
+thread_local std::string g_lastError;
+
+int fail(int code, const std::string &msg) {
+  g_lastError = msg;
+  return code;
+}
+
+template <class F> int guarded(F &&f) {
+  try {
+    return f();
+  } catch (const std::invalid_argument &e) {
+    return fail(SC_ERR_ARGUMENT, e.what());
+  } catch (const std::runtime_error &e) {
+    const std::string m = e.what();
+    return fail(m.rfind("HIP error", 0) == 0 ? SC_ERR_DEVICE : SC_ERR_CONFIG, m);
+  } catch (const std::exception &e) {
+    return fail(SC_ERR_STATE, e.what());
+  }
+}
+
+#define SC_CHECK_CTX(ctx)                                                      \
+  if (!(ctx) || !(ctx)->grain) return fail(SC_ERR_ARGUMENT, "null context")
+#define SC_CHECK_PTR(p)                                                        \
+  if (!(p)) return fail(SC_ERR_ARGUMENT, "null pointer argument: " #p)
+
+void checkPrecision(int p) {
+  if (p != SC_PRECISION_FP64 && p != SC_PRECISION_MIXED)
+    throw std::invalid_argument("precision must be SC_PRECISION_FP64 or SC_PRECISION_MIXED");
+}
+
+__global__ void riccatiStepsKernel(int n, const double *in, double *out, int prec) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double *r = in + 7 * size_t(i);
+  if (prec == SC_PRECISION_FP64) {
+    const scamd::Cx<double> y = scamd::riccatiStep(scamd::Cx<double>{r[0], r[1]},
+                                                   scamd::Cx<double>{r[2], r[3]},
+                                                   scamd::Cx<double>{r[4], r[5]}, r[6]);
+    out[2 * i] = y.re;
+    out[2 * i + 1] = y.im;
+  } else {
+    const scamd::Cx<float> y = scamd::riccatiStep(
+        scamd::Cx<float>{float(r[0]), float(r[1])}, scamd::Cx<float>{float(r[2]), float(r[3])},
+        scamd::Cx<float>{float(r[4]), float(r[5])}, float(r[6]));
+    out[2 * i] = y.re;
+    out[2 * i + 1] = y.im;
+  }
+}
+
+void hipCheck(hipError_t e, const char *what) {
+  if (e != hipSuccess)
+    throw std::runtime_error(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+}
+
+} // namespace
+
+extern "C" {
+
+const char *sc_last_error(void) { return g_lastError.c_str(); }
+int sc_version(void) { return 1; }
+
+int sc_create(const char *config_json, sc_context **out) {
+  SC_CHECK_PTR(config_json);
+  SC_CHECK_PTR(out);
+  *out = nullptr;
+  return guarded([&] {
+    scamd::Config cfg = scamd::parseConfig(config_json);
+    auto setup = scamd::buildSetup(cfg);
+    auto ctx = std::make_unique<sc_context>();
+    ctx->grain = std::make_unique<scamd::Grain>(std::move(setup));
+    ctx->grain->initialGuess();
+    *out = ctx.release();
+    return SC_OK;
+  });
+}
+
+int sc_destroy(sc_context *ctx) {
+  if (!ctx) return SC_OK;
+  return guarded([&] {
+    delete ctx;
+    return SC_OK;
+  });
+}
+
+int sc_get_dims(sc_context *ctx, sc_dims *out) {
+  SC_CHECK_CTX(ctx);
+  SC_CHECK_PTR(out);
+  const scamd::Setup &s = ctx->grain->setup();
+  out->n0 = s.N0;
+  out->n1 = s.N1;
+  out->num_energies = s.Ne;
+  out->num_angles = s.Na;
+  out->num_components = s.Nc;
+  out->num_grain_sites = s.numGrain;
+  out->num_in_gamma = s.nInG;
+  out->num_out_gamma = s.nOutG;
+  out->num_in_gammatilde = s.nInGt;
+  out->num_out_gammatilde = s.nOutGt;
+  out->traj_steps = s.trajSteps;
+  out->h = s.h;
+  out->grain_fraction = s.grainFraction;
+  return SC_OK;
+}
+
+int sc_get_energies(sc_context *ctx, double *e, double *r) {
+  SC_CHECK_CTX(ctx);
+  SC_CHECK_PTR(e);
+  SC_CHECK_PTR(r);
+  const scamd::Setup &s = ctx->grain->setup();
+  std::memcpy(e, s.energy.data(), s.energy.size() * sizeof(double));
+  std::memcpy(r, s.residue.data(), s.residue.size() * sizeof(double));
+  return SC_OK;
+}
+
+int sc_set_order_parameter(sc_context *ctx, int comp, const double *re, const double *im) {
+  SC_CHECK_CTX(ctx);
+  SC_CHECK_PTR(re);
+  SC_CHECK_PTR(im);
+  return guarded([&] {
+    ctx->grain->setOrderParameter(comp, re, im);
+    return SC_OK;
+  });
+}
+int sc_get_order_parameter(sc_context *ctx, int which, int comp, double *re, double *im) {
+  SC_CHECK_CTX(ctx);
+  SC_CHECK_PTR(re);
+  SC_CHECK_PTR(im);
+  if (which != 0 && which != 1) return fail(SC_ERR_ARGUMENT, "which must be 0 or 1");
+  return guarded([&] {
+    ctx->grain->getOrderParameter(which, comp, re, im);
+    return SC_OK;
+  });
+}
+int sc_set_vector_potential(sc_context *ctx, const double *ax, const double *ay) {
+  SC_CHECK_CTX(ctx);
+  SC_CHECK_PTR(ax);
+  SC_CHECK_PTR(ay);
+  return guarded([&] {
+    ctx->grain->setVectorPotential(ax, ay, false);
+    return SC_OK;
+  });
+}
+int sc_get_vector_potential(sc_context *ctx, int which, double *ax, double *ay) {
+  SC_CHECK_CTX(ctx);
+  SC_CHECK_PTR(ax);
+  SC_CHECK_PTR(ay);
+  if (which != 0 && which != 1) return fail(SC_ERR_ARGUMENT, "which must be 0 or 1");
+  return guarded([&] {
+    ctx->grain->getVectorPotential(which, ax, ay, false);
+    return SC_OK;
+  });
+}
+int sc_get_current(sc_context *ctx, double *jx, double *jy) {
+  SC_CHECK_CTX(ctx);
+  SC_CHECK_PTR(jx);
+  SC_CHECK_PTR(jy);
+  return guarded([&] {
+    ctx->grain->getCurrent(jx, jy);
+    return SC_OK;
+  });
+}
+int sc_get_free_energy_density(sc_context *ctx, double *out) {
+  SC_CHECK_CTX(ctx);
+  SC_CHECK_PTR(out);
+  return guarded([&] {
+    ctx->grain->getEnergyDensity(out);
+    return SC_OK;
+  });
+}
+int sc_get_boundary(sc_context *ctx, int in_out, int dir, double *re, double *im) {
+  SC_CHECK_CTX(ctx);
+  SC_CHECK_PTR(re);
+  SC_CHECK_PTR(im);
+  if ((in_out != 0 && in_out != 1) || (dir != 0 && dir != 1))
+    return fail(SC_ERR_ARGUMENT, "in_out and dir must be 0 or 1");
+  return guarded([&] {
+    ctx->grain->getBoundary(in_out, dir, re, im);
+    return SC_OK;
+  });
+}
+int sc_set_boundary(sc_context *ctx, int dir, const double *re, const double *im) {
+  SC_CHECK_CTX(ctx);
+  SC_CHECK_PTR(re);
+  SC_CHECK_PTR(im);
+  if (dir != 0 && dir != 1) return fail(SC_ERR_ARGUMENT, "dir must be 0 or 1");
+  return guarded([&] {
+    ctx->grain->setBoundary(dir, re, im);
+    return SC_OK;
+  });
+}
+
+int sc_compute(sc_context *ctx, int precision) {
+  SC_CHECK_CTX(ctx);
+  return guarded([&] {
+    checkPrecision(precision);
+    ctx->grain->compute(precision);
+    return SC_OK;
+  });
+}
+int sc_burnin(sc_context *ctx, int precision) {
+  SC_CHECK_CTX(ctx);
+  return guarded([&] {
+    checkPrecision(precision);
+    ctx->grain->burnin(precision);
+    return SC_OK;
+  });
+}
+int sc_run(sc_context *ctx, int nb, int nmin, int nmax, int precision, int *converged) {
+  SC_CHECK_CTX(ctx);
+  return guarded([&] {
+    checkPrecision(precision);
+    const scamd::Config &c = ctx->grain->setup().cfg;
+    const bool ok = ctx->grain->run(nb < 0 ? c.burnin : nb, nmin < 0 ? c.itMin : nmin,
+                                    nmax < 0 ? c.itMax : nmax, precision);
+    if (converged) *converged = ok ? 1 : 0;
+    return SC_OK;
+  });
+}
+int sc_get_results(sc_context *ctx, sc_results *out) {
+  SC_CHECK_CTX(ctx);
+  SC_CHECK_PTR(out);
+  const scamd::Results r = ctx->grain->results();
+  const scamd::Timing t = ctx->grain->timing();
+  out->residual = r.residual;
+  out->residual_boundary = r.resBoundary;
+  out->residual_order_parameter = r.resDelta;
+  out->residual_current = r.resCurrent;
+  out->residual_free_energy = r.resFreeEnergy;
+  out->residual_vector_potential = r.resVectorPotential;
+  out->free_energy = r.freeEnergy;
+  out->free_energy_per_area = r.freeEnergyPerArea;
+  out->magnetic_free_energy = r.magneticFreeEnergy;
+  out->magnetic_free_energy_per_area = r.magneticFreeEnergyPerArea;
+  out->iterations = r.iterations;
+  out->converged = r.converged;
+  out->last_sweep_ms = t.sweepMs;
+  out->last_iteration_ms = t.iterMs;
+  return SC_OK;
+}
+
+int sc_riccati_sweep(sc_context *ctx, int precision, int burnin) {
+  SC_CHECK_CTX(ctx);
+  return guarded([&] {
+    checkPrecision(precision);
+    ctx->grain->sweep(precision, burnin != 0);
+    hipCheck(hipStreamSynchronize(ctx->grain->stream()), "sweep");
+    return SC_OK;
+  });
+}
+int sc_boundary_condition(sc_context *ctx, double *residual) {
+  SC_CHECK_CTX(ctx);
+  return guarded([&] {
+    ctx->grain->boundaryCondition();
+    if (residual) *residual = ctx->grain->results().resBoundary;
+    return SC_OK;
+  });
+}
+int sc_set_sweep_shape(sc_context *ctx, int el, int slots) {
+  SC_CHECK_CTX(ctx);
+  return guarded([&] {
+    ctx->grain->setSweepShape(el, slots);
+    return SC_OK;
+  });
+}
+
+int sc_poisson_solve(int n, double h, int numGrain, int enforce, const double *current,
+                     double *out) {
+  SC_CHECK_PTR(current);
+  SC_CHECK_PTR(out);
+  if (n < 2 || !(h > 0.0) || numGrain < 1)
+    return fail(SC_ERR_ARGUMENT, "need n >= 2, h > 0 and num_grain >= 1");
+  return guarded([&] {
+    scamd::poissonSolve(n, h, numGrain, enforce != 0, current, out);
+    return SC_OK;
+  });
+}
+
+int sc_riccati_steps(int n, const double *in, double *out, int precision) {
+  SC_CHECK_PTR(in);
+  SC_CHECK_PTR(out);
+  if (n < 0) return fail(SC_ERR_ARGUMENT, "n must be non-negative");
+  if (n == 0) return SC_OK;
+  return guarded([&] {
+    checkPrecision(precision);
+    double *din = nullptr, *dout = nullptr;
+    hipCheck(hipMalloc(reinterpret_cast<void **>(&din), size_t(n) * 7 * sizeof(double)), "alloc");
+    hipCheck(hipMalloc(reinterpret_cast<void **>(&dout), size_t(n) * 2 * sizeof(double)), "alloc");
+    hipCheck(hipMemcpy(din, in, size_t(n) * 7 * sizeof(double), hipMemcpyHostToDevice), "copy");
+    riccatiStepsKernel<<<(n + 255) / 256, 256>>>(n, din, dout, precision);
+    hipCheck(hipGetLastError(), "riccati steps");
+    hipCheck(hipMemcpy(out, dout, size_t(n) * 2 * sizeof(double), hipMemcpyDeviceToHost), "copy");
+    (void)hipFree(din);
+    (void)hipFree(dout);
+    return SC_OK;
+  });
+}
+
+int sc_num_ozaki(double t, double cutoff) {
+  if (!(t > 0.0) || !(cutoff > 0.0)) return fail(SC_ERR_ARGUMENT, "need t > 0, cutoff > 0") * -1;
+  return scamd::numOzakiEnergies(t, cutoff);
+}
+int sc_ozaki(int n, double *poles, double *residues) {
+  SC_CHECK_PTR(poles);
+  SC_CHECK_PTR(residues);
+  if (n < 1) return fail(SC_ERR_ARGUMENT, "need at least one pole");
+  std::vector<double> p, r;
+  scamd::ozakiPoles(n, p, r);
+  std::memcpy(poles, p.data(), n * sizeof(double));
+  std::memcpy(residues, r.data(), n * sizeof(double));
+  return SC_OK;
+}
+double sc_bulk_gap(double t) { return scamd::bulkGapSWave(t); }
+
+int sc_get_labels(sc_context *ctx, int angle, signed char *labels) {
+  SC_CHECK_CTX(ctx);
+  SC_CHECK_PTR(labels);
+  const scamd::Setup &s = ctx->grain->setup();
+  if (angle < 0 || angle >= s.Na) return fail(SC_ERR_ARGUMENT, "angle out of range");
+  const size_t n1 = size_t(s.N1) * s.N1;
+  std::memcpy(labels, s.labels.data() + size_t(angle) * n1, n1);
+  return SC_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,939 @@
+// sc_engine.hip -- device state of one grain and the self-consistency
+// iteration (Context::compute / Context::burnIn, Context.h:377-447) on one
+// MI355X.  All buffers stay resident in HBM between iterations; the host only
+// reads back scalars (residuals, the accelerator's Gram matrix).
+
+#include "sc_engine.hpp"
+#include "sc_kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <stdexcept>
+
+namespace scamd {
+
+#define SC_HIP(expr)                                                            \
+  do {                                                                          \
+    const hipError_t err_ = (expr);                                             \
+    if (err_ != hipSuccess)                                                     \
+      throw std::runtime_error(std::string("HIP error in ") + #expr + ": " +    \
+                               hipGetErrorString(err_));                        \
+  } while (0)
+
+namespace {
+
+template <class T> T *devAlloc(size_t n) {
+  T *p = nullptr;
+  if (n == 0) n = 1;
+  SC_HIP(hipMalloc(reinterpret_cast<void **>(&p), n * sizeof(T)));
+  SC_HIP(hipMemset(p, 0, n * sizeof(T)));
+  return p;
+}
+template <class T> T *devUpload(const std::vector<T> &v) {
+  T *p = devAlloc<T>(v.size());
+  if (!v.empty())
+    SC_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return p;
+}
+template <class T> void devFree(T *&p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+inline unsigned blocksFor(long long n, int bs = 256) {
+  return static_cast<unsigned>((n + bs - 1) / bs);
+}
+
+} // namespace
+
+Grain::Grain(std::unique_ptr<Setup> setup) : s_(std::move(setup)) {
+  SC_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  SC_HIP(hipEventCreate(&ev0_));
+  SC_HIP(hipEventCreate(&ev1_));
+  SC_HIP(hipEventCreate(&ev2_));
+  upload();
+  tim_.trajSteps = s_->trajSteps;
+}
+
+Grain::~Grain() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  for (double *p : accX_) devFree(p);
+  for (double *p : accD_) devFree(p);
+  devFree(dAccMean_);
+  devFree(dAccPrev_);
+  devFree(dAccVel_);
+  devFree(dEnergy_);
+  devFree(dResidue_);
+  devFree(dSaCos_);
+  devFree(dSaSin_);
+  devFree(dVFx_);
+  devFree(dVFy_);
+  devFree(dWeight_);
+  devFree(dBasis_);
+  devFree(dCoupling_);
+  devFree(dTrig_);
+  devFree(dLabels_);
+  devFree(dDomain_);
+  devFree(dDomainSites_);
+  devFree(dIdx_);
+  devFree(dDist_);
+  devFree(dBcOffG_);
+  devFree(dBcSrcG_);
+  devFree(dBcOffGt_);
+  devFree(dBcSrcGt_);
+  devFree(dBcWG_);
+  devFree(dBcWsG_);
+  devFree(dBcWGt_);
+  devFree(dBcWsGt_);
+  devFree(dExtRefT_);
+  devFree(dExtRefO_);
+  devFree(dExtRefS_);
+  devFree(dExtRotT_);
+  devFree(dExtRotO_);
+  devFree(dExtRotS_);
+  devFree(dDelta_); // also owns dA_
+  devFree(dDeltaNew_); // also owns dANew_
+  devFree(dExtDelta_);
+  devFree(dCurrent_);
+  devFree(dCurrentOld_);
+  devFree(dEnergyDens_);
+  devFree(dBInG_);
+  devFree(dBOutG_);
+  devFree(dBInGt_);
+  devFree(dBOutGt_);
+  devFree(dG_);
+  devFree(dPartials_);
+  devFree(dWork_);
+  devFree(dWork2_);
+  devFree(dKernel_);
+  if (dScratch_) (void)hipFree(dScratch_);
+  if (dItems_) (void)hipFree(dItems_);
+  devFree(dCounter_);
+  if (ev0_) (void)hipEventDestroy(ev0_);
+  if (ev1_) (void)hipEventDestroy(ev1_);
+  if (ev2_) (void)hipEventDestroy(ev2_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Grain::upload() {
+  const Setup &s = *s_;
+  const size_t n0 = size_t(s.N0) * s.N0, n1 = size_t(s.N1) * s.N1;
+  dEnergy_ = devUpload(s.energy);
+  dResidue_ = devUpload(s.residue);
+  dSaCos_ = devUpload(s.saCos);
+  dSaSin_ = devUpload(s.saSin);
+  dVFx_ = devUpload(s.vFx);
+  dVFy_ = devUpload(s.vFy);
+  dWeight_ = devUpload(s.weight);
+  dBasis_ = devUpload(s.basis);
+  dCoupling_ = devUpload(s.coupling);
+  std::vector<float> trig;
+  for (const auto *v : {&s.opCos, &s.opSin, &s.aCos, &s.aSin, &s.bkCos, &s.bkSin})
+    trig.insert(trig.end(), v->begin(), v->end());
+  dTrig_ = devUpload(trig);
+  dLabels_ = devUpload(s.labels);
+  dDomain_ = devUpload(s.domain);
+  dDomainSites_ = devUpload(s.domainSites);
+  std::vector<int> idx;
+  for (const auto *v : {&s.idxInG, &s.idxOutG, &s.idxInGt, &s.idxOutGt})
+    idx.insert(idx.end(), v->begin(), v->end());
+  dIdx_ = devUpload(idx);
+  std::vector<double> dist;
+  for (const auto *v : {&s.distInG, &s.distOutG, &s.distInGt, &s.distOutGt})
+    dist.insert(dist.end(), v->begin(), v->end());
+  dDist_ = devUpload(dist);
+  dBcOffG_ = devUpload(s.bcG.offset);
+  dBcSrcG_ = devUpload(s.bcG.src);
+  dBcWG_ = devUpload(s.bcG.w);
+  dBcWsG_ = devUpload(s.bcG.wsum);
+  dBcOffGt_ = devUpload(s.bcGt.offset);
+  dBcSrcGt_ = devUpload(s.bcGt.src);
+  dBcWGt_ = devUpload(s.bcGt.w);
+  dBcWsGt_ = devUpload(s.bcGt.wsum);
+  dExtRefT_ = devUpload(s.extRef.target);
+  dExtRefO_ = devUpload(s.extRef.offset);
+  dExtRefS_ = devUpload(s.extRef.source);
+  dExtRotT_ = devUpload(s.extRot.target);
+  dExtRotO_ = devUpload(s.extRot.offset);
+  dExtRotS_ = devUpload(s.extRot.source);
+  // state: order parameter and vector potential are contiguous so that the
+  // accelerators see the concatenated field [Re D_0, Im D_0, ..., Ax, Ay]
+  // (accelerators/utils.h:58-186) without copies.
+  const size_t nf = size_t(2 * s.Nc + 2) * n0;
+  dDelta_ = devAlloc<double>(nf);
+  dA_ = dDelta_ + size_t(2 * s.Nc) * n0;
+  dDeltaNew_ = devAlloc<double>(nf);
+  dANew_ = dDeltaNew_ + size_t(2 * s.Nc) * n0;
+  dExtDelta_ = devAlloc<double>(size_t(2 * s.Nc) * n0);
+  dCurrent_ = devAlloc<double>(2 * n0);
+  dCurrentOld_ = devAlloc<double>(2 * n0);
+  dEnergyDens_ = devAlloc<double>(n0);
+  dBInG_ = devAlloc<double>(size_t(2 * s.nInG * s.Ne));
+  dBOutG_ = devAlloc<double>(size_t(2 * s.nOutG * s.Ne));
+  dBInGt_ = devAlloc<double>(size_t(2 * s.nInGt * s.Ne));
+  dBOutGt_ = devAlloc<double>(size_t(2 * s.nOutGt * s.Ne));
+  dG_ = devAlloc<double>(size_t(s.Na) * 4 * n1);
+  partialsCap_ = 4096;
+  dPartials_ = devAlloc<double>(size_t(partialsCap_) * 6);
+  dWork_ = devAlloc<double>(nf);
+  dWork2_ = devAlloc<double>(nf);
+  if (s.lambda > 0.0) {
+    std::vector<double> g;
+    poissonKernel(s.N0, s.h, g);
+    dKernel_ = devUpload(g);
+  }
+  dCounter_ = devAlloc<int>(1);
+}
+
+void Grain::setSweepShape(int el, int slotsPerCU) {
+  if (el != 4 && el != 8 && el != 16)
+    throw std::runtime_error("energies per lane must be 4, 8 or 16");
+  if (slotsPerCU < 1 || slotsPerCU > 32)
+    throw std::runtime_error("slots per CU must be in [1, 32]");
+  if (el != el_ || slotsPerCU != slotsPerCU_) sweepReady_ = false;
+  el_ = el;
+  slotsPerCU_ = slotsPerCU;
+}
+
+void Grain::prepareSweep() {
+  if (sweepReady_) return;
+  const Setup &s = *s_;
+  groups_ = (s.Ne + el_ - 1) / el_;
+  int gp = 1;
+  while (gp < groups_) gp <<= 1;
+  if (gp > 64)
+    throw std::runtime_error("too many Ozaki energies for the lane layout; "
+                             "raise the energies per lane");
+  cols_ = 64 / gp;
+  std::vector<SweepItem> items = sweepItems(s, cols_, &maxLen_);
+  numItems_ = static_cast<int>(items.size());
+  std::vector<SweepItemDev> di(items.size());
+  for (size_t i = 0; i < items.size(); ++i)
+    di[i] = {items[i].angle, items[i].x0, items[i].ncols, items[i].ylo, items[i].yhi};
+  if (dItems_) (void)hipFree(dItems_);
+  SC_HIP(hipMalloc(&dItems_, std::max<size_t>(1, di.size()) * sizeof(SweepItemDev)));
+  if (!di.empty())
+    SC_HIP(hipMemcpy(dItems_, di.data(), di.size() * sizeof(SweepItemDev),
+                     hipMemcpyHostToDevice));
+  int dev = 0, cus = 0;
+  SC_HIP(hipGetDevice(&dev));
+  SC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  numSlots_ = std::max(1, std::min(numItems_, slotsPerCU_ * std::max(cus, 1)));
+  const size_t stride = size_t(maxLen_) * el_ * 64; // complex elements
+  const size_t bytes = size_t(numSlots_) * stride * sizeof(Cx<double>);
+  if (bytes > scratchBytes_) {
+    if (dScratch_) (void)hipFree(dScratch_);
+    SC_HIP(hipMalloc(&dScratch_, bytes));
+    scratchBytes_ = bytes;
+  }
+  sweepReady_ = true;
+}
+
+static GrainView makeView(const Setup &s, const float *trig, const int *idx,
+                          const double *dist, double *bInG, double *bOutG,
+                          double *bInGt, double *bOutGt, const double *extDelta,
+                          const double *A, double *G, const double *energy,
+                          const double *residue, const double *saCos,
+                          const double *saSin, const double *vFx,
+                          const double *vFy, const double *weight,
+                          const double *basis, const int8_t *labels) {
+  GrainView v;
+  v.N0 = s.N0;
+  v.N1 = s.N1;
+  v.Ne = s.Ne;
+  v.Na = s.Na;
+  v.Nc = s.Nc;
+  v.h = s.h;
+  v.useInduced = s.lambda > 0.0 ? 1 : 0;
+  v.lambdaInv2 = s.lambda > 0.0 ? 1.0 / (s.lambda * s.lambda) : 0.0;
+  v.area = s.h * s.h * static_cast<double>(s.numGrain);
+  v.fluxDensity = M_PI * s.cfg.fluxQuanta / v.area;
+  v.useField = (v.useInduced || std::fabs(v.fluxDensity) > 0.0) ? 1 : 0;
+  v.gauge = s.cfg.gauge;
+  v.charge = s.cfg.chargeSign;
+  v.energy = energy;
+  v.residue = residue;
+  const int Na = s.Na;
+  v.opCos = trig;
+  v.opSin = trig + Na;
+  v.aCos = trig + 2 * Na;
+  v.aSin = trig + 3 * Na;
+  v.bkCos = trig + 4 * Na;
+  v.bkSin = trig + 5 * Na;
+  v.saCos = saCos;
+  v.saSin = saSin;
+  v.vFx = vFx;
+  v.vFy = vFy;
+  v.weight = weight;
+  v.basis = basis;
+  v.labels = labels;
+  const size_t nc = size_t(Na) * s.N1;
+  v.idxInG = idx;
+  v.idxOutG = idx + nc;
+  v.idxInGt = idx + 2 * nc;
+  v.idxOutGt = idx + 3 * nc;
+  v.distInG = dist;
+  v.distOutG = dist + s.nInG;
+  v.distInGt = dist + s.nInG + s.nOutG;
+  v.distOutGt = dist + s.nInG + s.nOutG + s.nInGt;
+  v.bInG = bInG;
+  v.bOutG = bOutG;
+  v.bInGt = bInGt;
+  v.bOutGt = bOutGt;
+  v.nInG = s.nInG;
+  v.nOutG = s.nOutG;
+  v.nInGt = s.nInGt;
+  v.nOutGt = s.nOutGt;
+  v.extDelta = extDelta;
+  v.A = A;
+  v.G = G;
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// host I/O
+// ---------------------------------------------------------------------------
+void Grain::setOrderParameter(int c, const double *re, const double *im) {
+  const Setup &s = *s_;
+  if (c < 0 || c >= s.Nc) throw std::runtime_error("component index out of range");
+  const size_t n0 = size_t(s.N0) * s.N0;
+  std::vector<double> buf(2 * n0);
+  for (size_t i = 0; i < n0; ++i) {
+    // OrderParameter::initialize masks the sites outside the grain.
+    buf[i] = s.domain[i] ? re[i] : 0.0;
+    buf[n0 + i] = s.domain[i] ? im[i] : 0.0;
+  }
+  SC_HIP(hipMemcpy(dDelta_ + size_t(c) * 2 * n0, buf.data(), 2 * n0 * sizeof(double),
+                   hipMemcpyHostToDevice));
+}
+void Grain::getOrderParameter(int which, int c, double *re, double *im) {
+  const Setup &s = *s_;
+  if (c < 0 || c >= s.Nc) throw std::runtime_error("component index out of range");
+  const size_t n0 = size_t(s.N0) * s.N0;
+  const double *src = (which == 0 ? dDelta_ : dDeltaNew_) + size_t(c) * 2 * n0;
+  SC_HIP(hipStreamSynchronize(stream_));
+  SC_HIP(hipMemcpy(re, src, n0 * sizeof(double), hipMemcpyDeviceToHost));
+  SC_HIP(hipMemcpy(im, src + n0, n0 * sizeof(double), hipMemcpyDeviceToHost));
+}
+void Grain::setVectorPotential(const double *ax, const double *ay, bool scaled) {
+  const Setup &s = *s_;
+  const size_t n0 = size_t(s.N0) * s.N0;
+  // The induced potential is stored scaled by lambda^2 (src/simulation.cu:369-376).
+  const double f = scaled ? 1.0 : (s.lambda > 0.0 ? s.lambda * s.lambda : 0.0);
+  std::vector<double> buf(2 * n0);
+  for (size_t i = 0; i < n0; ++i) {
+    buf[i] = ax[i] * f;
+    buf[n0 + i] = ay[i] * f;
+  }
+  SC_HIP(hipMemcpy(dA_, buf.data(), 2 * n0 * sizeof(double), hipMemcpyHostToDevice));
+}
+void Grain::getVectorPotential(int which, double *ax, double *ay, bool scaled) {
+  const Setup &s = *s_;
+  const size_t n0 = size_t(s.N0) * s.N0;
+  SC_HIP(hipStreamSynchronize(stream_));
+  const double *src = which == 0 ? dA_ : dANew_;
+  SC_HIP(hipMemcpy(ax, src, n0 * sizeof(double), hipMemcpyDeviceToHost));
+  SC_HIP(hipMemcpy(ay, src + n0, n0 * sizeof(double), hipMemcpyDeviceToHost));
+  if (!scaled && s.lambda > 0.0) {
+    const double f = 1.0 / (s.lambda * s.lambda);
+    for (size_t i = 0; i < n0; ++i) {
+      ax[i] *= f;
+      ay[i] *= f;
+    }
+  }
+}
+void Grain::getCurrent(double *jx, double *jy) {
+  const size_t n0 = size_t(s_->N0) * s_->N0;
+  SC_HIP(hipStreamSynchronize(stream_));
+  SC_HIP(hipMemcpy(jx, dCurrent_, n0 * sizeof(double), hipMemcpyDeviceToHost));
+  SC_HIP(hipMemcpy(jy, dCurrent_ + n0, n0 * sizeof(double), hipMemcpyDeviceToHost));
+}
+void Grain::getEnergyDensity(double *out) {
+  const size_t n0 = size_t(s_->N0) * s_->N0;
+  SC_HIP(hipStreamSynchronize(stream_));
+  SC_HIP(hipMemcpy(out, dEnergyDens_, n0 * sizeof(double), hipMemcpyDeviceToHost));
+}
+void Grain::getBoundary(int inOut, int dir, double *re, double *im) {
+  const Setup &s = *s_;
+  const long long n = (dir == 0 ? (inOut == 0 ? s.nInG : s.nOutG)
+                                : (inOut == 0 ? s.nInGt : s.nOutGt)) *
+                      s.Ne;
+  const double *p = dir == 0 ? (inOut == 0 ? dBInG_ : dBOutG_)
+                             : (inOut == 0 ? dBInGt_ : dBOutGt_);
+  SC_HIP(hipStreamSynchronize(stream_));
+  SC_HIP(hipMemcpy(re, p, n * sizeof(double), hipMemcpyDeviceToHost));
+  SC_HIP(hipMemcpy(im, p + n, n * sizeof(double), hipMemcpyDeviceToHost));
+}
+void Grain::setBoundary(int dir, const double *re, const double *im) {
+  const Setup &s = *s_;
+  const long long n = (dir == 0 ? s.nInG : s.nInGt) * s.Ne;
+  double *p = dir == 0 ? dBInG_ : dBInGt_;
+  SC_HIP(hipMemcpy(p, re, n * sizeof(double), hipMemcpyHostToDevice));
+  SC_HIP(hipMemcpy(p + n, im, n * sizeof(double), hipMemcpyHostToDevice));
+  boundaryInit_ = true;
+}
+
+// Initial<T>::bulk / ::random / ::perturbedPhaseWindings
+// (OrderParameterInitial.h:186-238,286-427).  The noise is Gaussian from a
+// seeded generator (the reference seeds with clock()).
+void Grain::initialGuess() {
+  const Setup &s = *s_;
+  const int N0 = s.N0;
+  const size_t n0 = size_t(N0) * N0;
+  std::mt19937_64 rng(s.cfg.seed);
+  std::normal_distribution<double> nd(0.0, 1.0);
+  for (int c = 0; c < s.Nc; ++c) {
+    std::vector<double> re(n0), im(n0);
+    const double half = 0.5 * s.cfg.grainWidth;
+    for (int j = 0; j < N0; ++j)
+      for (int i = 0; i < N0; ++i) {
+        double phase = s.cfg.phaseShift[c];
+        for (const Vortex &vx : s.cfg.vortices[c]) {
+          const double cx = (vx.cx - s.cfg.geomOffX) / half;
+          const double cy = (vx.cy - s.cfg.geomOffY) / half;
+          const double x = double(2 * i) / double(N0 - 1) - 1.0 - cx;
+          const double y = double(2 * j) / double(N0 - 1) - 1.0 - cy;
+          const double r = std::hypot(x, y);
+          phase += (r > 0.0 ? std::atan2(y, x) : 0.0) * vx.winding;
+        }
+        const size_t q = size_t(j) * N0 + i;
+        const double nr = s.cfg.noise[c] * nd(rng), ni = s.cfg.noise[c] * nd(rng);
+        re[q] = s.deltaBulk[c] * std::cos(phase) + nr;
+        im[q] = s.deltaBulk[c] * std::sin(phase) + ni;
+      }
+    setOrderParameter(c, re.data(), im.data());
+  }
+}
+
+// ---------------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------------
+namespace {
+double residualFromStats(const double *st, long long numElements, int norm) {
+  // ComputeResidual.h:69-106
+  double numer, denom, eps = 1e-4;
+  if (norm == NORM_L1) {
+    numer = st[0];
+    denom = st[1];
+    eps *= static_cast<double>(numElements);
+  } else if (norm == NORM_L2) {
+    numer = std::sqrt(st[2]);
+    denom = std::sqrt(st[3]);
+    eps *= static_cast<double>(numElements);
+  } else {
+    numer = st[4];
+    denom = st[5];
+  }
+  return numer / std::max(denom, eps);
+}
+} // namespace
+
+double Grain::residualOf(const double *oldv, const double *newv, long long n) {
+  const long long blocks = std::min<long long>(blocksFor(n), 1024);
+  statsKernel<<<blocks, 256, 0, stream_>>>(oldv, newv, n, dPartials_);
+  SC_HIP(hipGetLastError());
+  std::vector<double> h(size_t(blocks) * 6);
+  SC_HIP(hipMemcpyAsync(h.data(), dPartials_, h.size() * sizeof(double),
+                        hipMemcpyDeviceToHost, stream_));
+  SC_HIP(hipStreamSynchronize(stream_));
+  double st[6] = {0, 0, 0, 0, 0, 0};
+  for (long long b = 0; b < blocks; ++b)
+    for (int q = 0; q < 6; ++q)
+      st[q] = q < 4 ? st[q] + h[size_t(b) * 6 + q] : std::max(st[q], h[size_t(b) * 6 + q]);
+  return residualFromStats(st, n, s_->cfg.norm);
+}
+
+double Grain::deviceSum(const double *a, const double *b, const uint8_t *mask,
+                        long long n, int op) {
+  const long long blocks = std::min<long long>(blocksFor(n), 1024);
+  sumKernel<<<blocks, 256, 0, stream_>>>(a, b, mask, n, op, dPartials_);
+  SC_HIP(hipGetLastError());
+  std::vector<double> h(blocks);
+  SC_HIP(hipMemcpyAsync(h.data(), dPartials_, h.size() * sizeof(double),
+                        hipMemcpyDeviceToHost, stream_));
+  SC_HIP(hipStreamSynchronize(stream_));
+  double s = 0.0;
+  for (double v : h) s += v;
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// the iteration
+// ---------------------------------------------------------------------------
+void Grain::extendDelta() {
+  const Setup &s = *s_;
+  const size_t n0 = size_t(s.N0) * s.N0;
+  SC_HIP(hipMemcpyAsync(dExtDelta_, dDelta_, size_t(2 * s.Nc) * n0 * sizeof(double),
+                        hipMemcpyDeviceToDevice, stream_));
+  const int nt = static_cast<int>(s.extRef.target.size());
+  if (nt > 0) {
+    extendKernel<<<blocksFor(nt), 256, 0, stream_>>>(
+        dExtDelta_, dExtRefT_, dExtRefO_, dExtRefS_, nt, 2 * s.Nc,
+        static_cast<long long>(n0), static_cast<long long>(n0), 0);
+    SC_HIP(hipGetLastError());
+  }
+}
+
+void Grain::ensureBoundaryInit() {
+  if (boundaryInit_) return;
+  const Setup &s = *s_;
+  extendDelta();
+  GrainView v = makeView(s, dTrig_, dIdx_, dDist_, dBInG_, dBOutG_, dBInGt_, dBOutGt_,
+                         dExtDelta_, dA_, dG_, dEnergy_, dResidue_, dSaCos_, dSaSin_,
+                         dVFx_, dVFy_, dWeight_, dBasis_, dLabels_);
+  const long long n = static_cast<long long>(s.Na) * s.N1 * s.Ne;
+  initialBoundaryKernel<<<blocksFor(n), 256, 0, stream_>>>(v, nullptr);
+  SC_HIP(hipGetLastError());
+  boundaryCondition();
+  boundaryInit_ = true;
+}
+
+template <typename T, int EL> static void launchSweep(const SweepArgs &a, int slots,
+                                                      hipStream_t st) {
+  sweepKernel<T, EL><<<slots, 64, 0, st>>>(a);
+}
+
+void Grain::sweep(int precision, bool burnin) {
+  prepareSweep();
+  const Setup &s = *s_;
+  SweepArgs a;
+  a.v = makeView(s, dTrig_, dIdx_, dDist_, dBInG_, dBOutG_, dBInGt_, dBOutGt_,
+                 dExtDelta_, dA_, dG_, dEnergy_, dResidue_, dSaCos_, dSaSin_, dVFx_,
+                 dVFy_, dWeight_, dBasis_, dLabels_);
+  a.items = reinterpret_cast<const SweepItemDev *>(dItems_);
+  a.numItems = numItems_;
+  a.cols = cols_;
+  a.groups = groups_;
+  a.counter = dCounter_;
+  a.scratch = dScratch_;
+  a.slotStride = static_cast<long long>(maxLen_) * el_ * 64;
+  a.burnin = burnin ? 1 : 0;
+  resetCounter<<<1, 1, 0, stream_>>>(dCounter_);
+  SC_HIP(hipEventRecord(ev0_, stream_));
+  const bool dp = precision == PREC_FP64;
+  switch (el_) {
+  case 4: dp ? launchSweep<double, 4>(a, numSlots_, stream_) : launchSweep<float, 4>(a, numSlots_, stream_); break;
+  case 8: dp ? launchSweep<double, 8>(a, numSlots_, stream_) : launchSweep<float, 8>(a, numSlots_, stream_); break;
+  default: dp ? launchSweep<double, 16>(a, numSlots_, stream_) : launchSweep<float, 16>(a, numSlots_, stream_); break;
+  }
+  SC_HIP(hipGetLastError());
+  SC_HIP(hipEventRecord(ev1_, stream_));
+}
+
+void Grain::boundaryCondition() {
+  // BoundaryConditionSpecular::computeBoundaryCondition (:361-486)
+  const Setup &s = *s_;
+  double r = 0.0;
+  for (int dir = 0; dir < 2; ++dir) {
+    const long long nIn = dir == 0 ? s.nInG : s.nInGt;
+    const long long nOut = dir == 0 ? s.nOutG : s.nOutGt;
+    const long long total = nIn * s.Ne;
+    if (total == 0) continue;
+    const long long blocks = blocksFor(total);
+    if (blocks > partialsCap_) {
+      devFree(dPartials_);
+      partialsCap_ = blocks;
+      dPartials_ = devAlloc<double>(size_t(partialsCap_) * 6);
+    }
+    boundaryKernel<<<blocks, 256, 0, stream_>>>(
+        dir == 0 ? dBInG_ : dBInGt_, dir == 0 ? dBOutG_ : dBOutGt_, nIn, nOut, s.Ne,
+        dir == 0 ? dBcOffG_ : dBcOffGt_, dir == 0 ? dBcSrcG_ : dBcSrcGt_,
+        dir == 0 ? dBcWG_ : dBcWGt_, dir == 0 ? dBcWsG_ : dBcWsGt_, dPartials_);
+    SC_HIP(hipGetLastError());
+    std::vector<double> h(size_t(blocks) * 6);
+    SC_HIP(hipMemcpyAsync(h.data(), dPartials_, h.size() * sizeof(double),
+                          hipMemcpyDeviceToHost, stream_));
+    SC_HIP(hipStreamSynchronize(stream_));
+    double st[6] = {0, 0, 0, 0, 0, 0};
+    for (long long b = 0; b < blocks; ++b)
+      for (int q = 0; q < 6; ++q)
+        st[q] = q < 4 ? st[q] + h[size_t(b) * 6 + q] : std::max(st[q], h[size_t(b) * 6 + q]);
+    r = std::max(r, residualFromStats(st, 2 * total, s.cfg.norm));
+  }
+  res_.resBoundary = r;
+}
+
+void Grain::integrate(int precision, bool burnin) {
+  // IntegrationIteratorOzaki::compute (IntegrationIteratorOzaki.h:303-404)
+  const Setup &s = *s_;
+  const size_t n0 = size_t(s.N0) * s.N0, n1 = size_t(s.N1) * s.N1;
+  ensureBoundaryInit();
+  extendDelta();
+  sweep(precision, burnin);
+  if (!burnin) {
+    const int nt = static_cast<int>(s.extRot.target.size());
+    if (nt > 0) {
+      extendKernel<<<blocksFor(nt), 256, 0, stream_>>>(
+          dG_, dExtRotT_, dExtRotO_, dExtRotS_, nt, 4, static_cast<long long>(n1),
+          static_cast<long long>(n1), static_cast<long long>(4 * n1));
+      SC_HIP(hipGetLastError());
+    }
+    SC_HIP(hipMemcpyAsync(dCurrentOld_, dCurrent_, 2 * n0 * sizeof(double),
+                          hipMemcpyDeviceToDevice, stream_));
+    GrainView v = makeView(s, dTrig_, dIdx_, dDist_, dBInG_, dBOutG_, dBInGt_, dBOutGt_,
+                           dExtDelta_, dA_, dG_, dEnergy_, dResidue_, dSaCos_, dSaSin_,
+                           dVFx_, dVFy_, dWeight_, dBasis_, dLabels_);
+    const int ns = static_cast<int>(s.domainSites.size());
+    const double jf = 2.0 * static_cast<double>(s.cfg.chargeSign) * s.temperature;
+    gatherKernel<<<blocksFor(ns), 256, 0, stream_>>>(v, dDomainSites_, ns, dCoupling_, jf,
+                                                     dDeltaNew_, dCurrent_, dEnergyDens_);
+    SC_HIP(hipGetLastError());
+    SC_HIP(hipEventRecord(ev2_, stream_));
+    // ComputeOrderParameter::computePropertyFromGreens (:212-262)
+    double rd = 0.0;
+    for (int c = 0; c < s.Nc; ++c)
+      rd = std::max(rd, residualOf(dDelta_ + size_t(c) * 2 * n0,
+                                   dDeltaNew_ + size_t(c) * 2 * n0,
+                                   static_cast<long long>(2 * n0)));
+    res_.resDelta = rd;
+    convDelta_ = rd < s.cfg.convergence;
+    if (s.cfg.computeCurrent) {
+      // ComputeCurrent::computePropertyFromGreens (:175-222)
+      const double actual = residualOf(dCurrentOld_, dCurrent_, static_cast<long long>(2 * n0));
+      res_.resCurrent = firstCurrent_ ? s.cfg.convergence * 0.1 : actual;
+      convCurrent_ = res_.resCurrent < s.cfg.convergence;
+      firstCurrent_ = false;
+    }
+    if (s.cfg.computeFreeEnergy) {
+      // ComputeFreeEnergy::computePropertyFromGreens (:281-414)
+      double nonMag = s.temperature * deviceSum(dEnergyDens_, nullptr, nullptr,
+                                                static_cast<long long>(n0), 0);
+      for (int c = 0; c < s.Nc; ++c) {
+        const double f = std::log(s.temperature / s.cfg.tc[c]) + s.sumInverseEnergy;
+        const double sq =
+            deviceSum(dDelta_ + size_t(c) * 2 * n0, nullptr, dDomain_,
+                      static_cast<long long>(n0), 2) +
+            deviceSum(dDelta_ + size_t(c) * 2 * n0 + n0, nullptr, dDomain_,
+                      static_cast<long long>(n0), 2);
+        nonMag += sq * f;
+      }
+      double mag = 0.0;
+      if (s.lambda > 0.0) {
+        curlSquareKernel<<<blocksFor(n0), 256, 0, stream_>>>(dA_, s.N0, 1.0 / s.h, dWork_);
+        SC_HIP(hipGetLastError());
+        mag = 0.5 / (s.lambda * s.lambda) *
+              deviceSum(dWork_, nullptr, nullptr, static_cast<long long>(n0), 0);
+      }
+      const double area = s.h * s.h;
+      const double perArea = (nonMag + mag) / static_cast<double>(s.numGrain);
+      const double actual =
+          std::fabs(fePerAreaOld_ - perArea) / std::max(std::fabs(fePerAreaOld_), 1e-4);
+      res_.resFreeEnergy = firstFE_ ? s.cfg.convergence * 0.1 : actual;
+      convFE_ = res_.resFreeEnergy < s.cfg.convergence;
+      res_.freeEnergy = (nonMag + mag) * area;
+      res_.freeEnergyPerArea = perArea;
+      res_.magneticFreeEnergy = mag * area;
+      res_.magneticFreeEnergyPerArea = mag / static_cast<double>(s.numGrain);
+      fePerAreaOld_ = perArea;
+      firstFE_ = false;
+    }
+  }
+  boundaryCondition();
+  float ms = 0.f;
+  SC_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+  tim_.sweepMs = ms;
+}
+
+void Grain::solveVectorPotential(const double *dj, double *dout) {
+  // ComputeVectorPotential::solve (:647-745) with the exact Green function:
+  // subtract the grain mean of j, correlate with G, scale by -h^2.
+  const Setup &s = *s_;
+  const size_t n0 = size_t(s.N0) * s.N0;
+  const double mx = deviceSum(dj, nullptr, nullptr, static_cast<long long>(n0), 0) /
+                    static_cast<double>(s.numGrain);
+  const double my = deviceSum(dj + n0, nullptr, nullptr, static_cast<long long>(n0), 0) /
+                    static_cast<double>(s.numGrain);
+  subtractMeanKernel<<<blocksFor(2 * n0), 256, 0, stream_>>>(dj, mx, my,
+                                                             static_cast<long long>(n0), dWork2_);
+  SC_HIP(hipGetLastError());
+  poissonDirectKernel<<<blocksFor(2 * n0), 256, 0, stream_>>>(dWork2_, dKernel_, s.N0,
+                                                              -(s.h * s.h), dout);
+  SC_HIP(hipGetLastError());
+}
+
+void Grain::acceleratorStep() {
+  const Setup &s = *s_;
+  const size_t n0 = size_t(s.N0) * s.N0;
+  const bool doA = s.cfg.computeCurrent && s.lambda > 0.0;
+  const int F = 2 * s.Nc + (doA ? 2 : 0);
+  const long long n = static_cast<long long>(F) * static_cast<long long>(n0);
+  double *X = dDelta_;
+  const double *G = dDeltaNew_;
+  switch (s.cfg.accel) {
+  case ACC_PICARD: {
+    const double st = s.cfg.stepSize > 0 ? s.cfg.stepSize : 1.0;
+    picardKernel<<<blocksFor(n), 256, 0, stream_>>>(X, G, n, st);
+    break;
+  }
+  case ACC_ANDERSON: {
+    // Anderson.h:267-355
+    const size_t rank = s.cfg.rank > 0 ? size_t(s.cfg.rank) : 4;
+    const double st = s.cfg.stepSize > 0 ? s.cfg.stepSize : 1.0;
+    size_t slot;
+    if (accD_.size() == rank) {
+      slot = size_t(std::min_element(accIdx_.begin(), accIdx_.end()) - accIdx_.begin());
+    } else {
+      accX_.push_back(devAlloc<double>(size_t(n)));
+      accD_.push_back(devAlloc<double>(size_t(n)));
+      accSq_.push_back(0.0);
+      accIdx_.push_back(0);
+      slot = accD_.size() - 1;
+    }
+    SC_HIP(hipMemcpyAsync(accX_[slot], X, size_t(n) * sizeof(double),
+                          hipMemcpyDeviceToDevice, stream_));
+    diffKernel<<<blocksFor(n), 256, 0, stream_>>>(accD_[slot], G, X, n);
+    accSq_[slot] = deviceSum(accD_[slot], nullptr, nullptr, n, 2);
+    accIdx_[slot] = accIter_;
+    const size_t m = accD_.size(), M = m + 1;
+    std::vector<double> A(M * M, 1.0), b(M, 0.0);
+    double sr = 0.0;
+    for (double v : accSq_) sr += v;
+    const double scale = 1.0 / (sr / static_cast<double>(m));
+    for (size_t i = 0; i < m; ++i) {
+      A[i * M + i] = accSq_[i] * scale;
+      for (size_t j = i + 1; j < m; ++j) {
+        double dot = 0.0;
+        for (int f = 0; f < F; ++f)
+          dot += deviceSum(accD_[i] + size_t(f) * n0, accD_[j] + size_t(f) * n0, nullptr,
+                           static_cast<long long>(n0), 1);
+        A[i * M + j] = dot * scale;
+        A[j * M + i] = dot * scale;
+      }
+    }
+    A[(M - 1) * M + (M - 1)] = 0.0;
+    b[M - 1] = 1.0;
+    // Gaussian elimination with partial pivoting on the bordered system.
+    for (size_t c = 0; c < M; ++c) {
+      size_t p = c;
+      for (size_t r = c + 1; r < M; ++r)
+        if (std::fabs(A[r * M + c]) > std::fabs(A[p * M + c])) p = r;
+      if (p != c) {
+        for (size_t q = 0; q < M; ++q) std::swap(A[c * M + q], A[p * M + q]);
+        std::swap(b[c], b[p]);
+      }
+      if (A[c * M + c] == 0.0) continue;
+      for (size_t r = c + 1; r < M; ++r) {
+        const double f = A[r * M + c] / A[c * M + c];
+        for (size_t q = c; q < M; ++q) A[r * M + q] -= f * A[c * M + q];
+        b[r] -= f * b[c];
+      }
+    }
+    std::vector<double> w(M, 0.0);
+    for (size_t r = M; r-- > 0;) {
+      double acc = b[r];
+      for (size_t q = r + 1; q < M; ++q) acc -= A[r * M + q] * w[q];
+      w[r] = A[r * M + r] != 0.0 ? acc / A[r * M + r] : 0.0;
+    }
+    double **ptrs = nullptr;
+    const size_t pb = 2 * m * sizeof(double *) + m * sizeof(double);
+    SC_HIP(hipMalloc(reinterpret_cast<void **>(&ptrs), pb));
+    std::vector<unsigned char> hostBuf(pb);
+    std::memcpy(hostBuf.data(), accX_.data(), m * sizeof(double *));
+    std::memcpy(hostBuf.data() + m * sizeof(double *), accD_.data(), m * sizeof(double *));
+    std::memcpy(hostBuf.data() + 2 * m * sizeof(double *), w.data(), m * sizeof(double));
+    SC_HIP(hipMemcpyAsync(ptrs, hostBuf.data(), pb, hipMemcpyHostToDevice, stream_));
+    andersonCombine<<<blocksFor(n), 256, 0, stream_>>>(
+        X, ptrs, ptrs + m, reinterpret_cast<double *>(ptrs + 2 * m), static_cast<int>(m), n, st);
+    SC_HIP(hipStreamSynchronize(stream_));
+    (void)hipFree(ptrs);
+    ++accIter_;
+    break;
+  }
+  case ACC_CONGACC: {
+    // CongAcc.h:271-419
+    const double st0 = s.cfg.stepSize > 0 ? s.cfg.stepSize : 0.5;
+    const double fac = s.cfg.stepFactor > 0 ? s.cfg.stepFactor : 1.234;
+    const double minCos = s.cfg.cosMin > 0 ? s.cfg.cosMin : 0.7;
+    const double smax = s.cfg.stepMax > 0 ? s.cfg.stepMax : 100.0;
+    const double smin = s.cfg.stepMin > 0 ? s.cfg.stepMin : 0.001;
+    if (!accInit_) {
+      dAccMean_ = devAlloc<double>(size_t(n));
+      accStep_.assign(F, st0);
+      accInertia_.assign(F, 0.0);
+      accInit_ = true;
+    }
+    diffKernel<<<blocksFor(n), 256, 0, stream_>>>(dWork_, G, X, n);
+    for (int f = 0; f < F; ++f) {
+      const double *Df = dWork_ + size_t(f) * n0, *Mf = dAccMean_ + size_t(f) * n0;
+      const double d = std::sqrt(deviceSum(Df, nullptr, nullptr, static_cast<long long>(n0), 2));
+      const double m = std::sqrt(deviceSum(Mf, nullptr, nullptr, static_cast<long long>(n0), 2));
+      const double den = d * m;
+      if (den > 0.0) {
+        const double dot = deviceSum(Df, Mf, nullptr, static_cast<long long>(n0), 1);
+        const double cs = dot / den;
+        accStep_[f] *= std::pow(fac, cs);
+        if (cs < minCos) {
+          // bisection for the inertia (CongAcc.h:299-358)
+          double eA = dot / (d * m) - minCos, xA = 1.0, eB = 1.0 - minCos, xB = 0.0;
+          double xx = 0.5 * (xA + xB);
+          bool ok = false;
+          while (!ok) {
+            const double y = 1.0 - xx;
+            const double num = xx * dot + y * d * d;
+            const double dd = d * std::sqrt(xx * xx * m * m + 2.0 * xx * y * dot + y * y * d * d);
+            const double err = num / dd - minCos;
+            ok = std::fabs(err) < 0.01;
+            const int se = (0.0 < err) - (err < 0.0), sa = (0.0 < eA) - (eA < 0.0);
+            if (se == sa) {
+              eA = err;
+              xA = xx;
+            } else {
+              eB = err;
+              xB = xx;
+            }
+            const int sb = (0.0 < eB) - (eB < 0.0), sa2 = (0.0 < eA) - (eA < 0.0);
+            if (sa2 == sb) break;
+            xx = 0.5 * (xA + xB);
+          }
+          accInertia_[f] = xx;
+        }
+      }
+      accStep_[f] = std::clamp(accStep_[f], smin, smax);
+    }
+    double *stepDev = dWork2_, *inDev = dWork2_ + F;
+    SC_HIP(hipMemcpyAsync(stepDev, accStep_.data(), F * sizeof(double),
+                          hipMemcpyHostToDevice, stream_));
+    SC_HIP(hipMemcpyAsync(inDev, accInertia_.data(), F * sizeof(double),
+                          hipMemcpyHostToDevice, stream_));
+    congaccKernel<<<blocksFor(n), 256, 0, stream_>>>(X, dAccMean_, dWork_, stepDev, inDev,
+                                                     static_cast<long long>(n0), n);
+    SC_HIP(hipStreamSynchronize(stream_));
+    break;
+  }
+  case ACC_BB: {
+    // BarzilaiBorwein.h:209-243
+    const double smax = s.cfg.stepMax > 0 ? s.cfg.stepMax : 100.0;
+    const double smin = s.cfg.stepMin > 0 ? s.cfg.stepMin : 0.001;
+    if (!accInit_) {
+      bbStep_ = s.cfg.stepSize > 0 ? s.cfg.stepSize : 1.0;
+      dAccPrev_ = devAlloc<double>(size_t(n));
+      accInit_ = true;
+    }
+    diffKernel<<<blocksFor(n), 256, 0, stream_>>>(dWork_, G, X, n);
+    if (accIter_ > 0) {
+      diffKernel<<<blocksFor(n), 256, 0, stream_>>>(dWork2_, dWork_, dAccPrev_, n);
+      double dot = 0.0;
+      for (int f = 0; f < F; ++f)
+        dot += deviceSum(dAccPrev_ + size_t(f) * n0, dWork2_ + size_t(f) * n0, nullptr,
+                         static_cast<long long>(n0), 1);
+      dot = std::fabs(dot);
+      const double sc = (accIter_ % 2 == 0)
+                            ? dot / deviceSum(dWork2_, nullptr, nullptr, n, 2)
+                            : deviceSum(dAccPrev_, nullptr, nullptr, n, 2) / dot;
+      bbStep_ *= sc;
+    }
+    bbStep_ = std::clamp(bbStep_, smin, smax);
+    SC_HIP(hipMemcpyAsync(dAccPrev_, dWork_, size_t(n) * sizeof(double),
+                          hipMemcpyDeviceToDevice, stream_));
+    axpyKernel<<<blocksFor(n), 256, 0, stream_>>>(X, dWork_, n, bbStep_);
+    ++accIter_;
+    break;
+  }
+  case ACC_POLYAK: {
+    // Polyak.h:216-265
+    const double st = s.cfg.stepSize > 0 ? s.cfg.stepSize : 0.5;
+    const double drag = s.cfg.drag > 0 ? s.cfg.drag : 0.5;
+    if (!accInit_) {
+      dAccVel_ = devAlloc<double>(size_t(n));
+      accInit_ = true;
+    }
+    polyakKernel<<<blocksFor(n), 256, 0, stream_>>>(X, dAccVel_, G, n, st, drag);
+    break;
+  }
+  }
+  SC_HIP(hipGetLastError());
+}
+
+void Grain::compute(int precision) {
+  // Context::compute (Context.h:391-447)
+  const Setup &s = *s_;
+  const size_t n0 = size_t(s.N0) * s.N0;
+  const bool doA = s.cfg.computeCurrent && s.lambda > 0.0;
+  hipEvent_t start;
+  SC_HIP(hipEventCreate(&start));
+  SC_HIP(hipEventRecord(start, stream_));
+  if (!firstCompute_) acceleratorStep();
+  integrate(precision, false);
+  if (doA) {
+    solveVectorPotential(dCurrent_, dANew_);
+    res_.resVectorPotential = residualOf(dA_, dANew_, static_cast<long long>(2 * n0));
+  }
+  res_.iterations++;
+  firstCompute_ = false;
+  // Context::getResidual / isPartiallyConverged (Context.h:462-497)
+  double r = doA ? res_.resVectorPotential : 0.0;
+  r = std::max(r, res_.resBoundary);
+  r = std::max(r, res_.resDelta);
+  if (s.cfg.computeCurrent) r = std::max(r, res_.resCurrent);
+  if (s.cfg.computeFreeEnergy) r = std::max(r, res_.resFreeEnergy);
+  res_.residual = r;
+  bool ok = r < s.cfg.convergence && convDelta_;
+  if (s.cfg.computeCurrent) ok = ok && convCurrent_;
+  if (s.cfg.computeFreeEnergy) ok = ok && convFE_;
+  res_.converged = ok ? 1 : 0;
+  hipEvent_t stop;
+  SC_HIP(hipEventCreate(&stop));
+  SC_HIP(hipEventRecord(stop, stream_));
+  SC_HIP(hipEventSynchronize(stop));
+  float ms = 0.f;
+  SC_HIP(hipEventElapsedTime(&ms, start, stop));
+  tim_.iterMs = ms;
+  (void)hipEventDestroy(start);
+  (void)hipEventDestroy(stop);
+}
+
+void Grain::burnin(int precision) {
+  // Context::burnIn (Context.h:377-389)
+  integrate(precision, true);
+  SC_HIP(hipStreamSynchronize(stream_));
+  res_.iterations++;
+  res_.residual = res_.resBoundary;
+}
+
+bool Grain::run(int nBurnin, int itMin, int itMax, int precision) {
+  // runSimulation (run_simulation.h:124-187), save_frequency < 0
+  for (int i = 1; nBurnin > 0; ++i) {
+    burnin(precision);
+    if (i >= nBurnin) break;
+  }
+  if (itMin == 0 && itMax == 0) return false;
+  bool conv = false;
+  for (int i = 1;; ++i) {
+    compute(precision);
+    conv = res_.converged != 0;
+    const bool minReached = i >= itMin;
+    const bool maxReached = itMax < 0 ? false : i >= itMax;
+    if ((conv && minReached) || maxReached) break;
+  }
+  return conv;
+}
+
+void poissonSolve(int n, double h, int numGrain, bool enforce, const double *current,
+                  double *out) {
+  const size_t n0 = size_t(n) * n;
+  std::vector<double> g;
+  poissonKernel(n, h, g);
+  double mx = 0.0, my = 0.0;
+  if (enforce) {
+    for (size_t i = 0; i < n0; ++i) {
+      mx += current[i];
+      my += current[n0 + i];
+    }
+    mx /= numGrain;
+    my /= numGrain;
+  }
+  double *dj = devUpload(std::vector<double>(current, current + 2 * n0));
+  double *dk = devUpload(g);
+  double *dsrc = devAlloc<double>(2 * n0), *dout = devAlloc<double>(2 * n0);
+  subtractMeanKernel<<<blocksFor(2 * n0), 256>>>(dj, mx, my, static_cast<long long>(n0), dsrc);
+  poissonDirectKernel<<<blocksFor(2 * n0), 256>>>(dsrc, dk, n, -(h * h), dout);
+  SC_HIP(hipGetLastError());
+  SC_HIP(hipMemcpy(out, dout, 2 * n0 * sizeof(double), hipMemcpyDeviceToHost));
+  devFree(dj);
+  devFree(dk);
+  devFree(dsrc);
+  devFree(dout);
+}
+
+} // namespace scamd

@@ -1,0 +1,140 @@
+// sc_engine.hpp -- one grain on one GPU: device state and the iteration.
+#pragma once
+
+#include "sc_setup.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace scamd {
+
+enum Precision { PREC_FP64 = 0, PREC_MIXED = 1 };
+
+struct Results {
+  double residual = 0, resBoundary = 0, resDelta = 0, resCurrent = 0,
+         resFreeEnergy = 0, resVectorPotential = 0;
+  double freeEnergy = 0, freeEnergyPerArea = 0, magneticFreeEnergy = 0,
+         magneticFreeEnergyPerArea = 0;
+  int iterations = 0;
+  int converged = 0;
+};
+
+struct Timing {
+  double sweepMs = 0;  // last sweep kernel time (HIP events)
+  double iterMs = 0;   // last whole iteration
+  long long trajSteps = 0;
+};
+
+class Grain {
+public:
+  explicit Grain(std::unique_ptr<Setup> setup);
+  ~Grain();
+  Grain(const Grain &) = delete;
+  Grain &operator=(const Grain &) = delete;
+
+  const Setup &setup() const { return *s_; }
+
+  // host I/O (planar re/im, reference GridData layout)
+  void setOrderParameter(int comp, const double *re, const double *im);
+  void getOrderParameter(int which, int comp, double *re, double *im);
+  void setVectorPotential(const double *ax, const double *ay, bool scaled);
+  void getVectorPotential(int which, double *ax, double *ay, bool scaled);
+  void getCurrent(double *jx, double *jy);
+  void getBoundary(int inOut, int dir, double *re, double *im);
+  void setBoundary(int dir, const double *re, const double *im);
+  void getEnergyDensity(double *out);
+  void initialGuess(); // Initial<T>::bulk / perturbedPhaseWindings
+
+  // the self-consistency iteration (Context.h:377-447)
+  void compute(int precision);
+  void burnin(int precision);
+  bool run(int burnin, int itMin, int itMax, int precision);
+  // individual stages
+  void sweep(int precision, bool burnin);
+  void boundaryCondition();
+  void solveVectorPotential(const double *dCurrent, double *dOut);
+
+  Results results() const { return res_; }
+  Timing timing() const { return tim_; }
+  void setSweepShape(int energiesPerLane, int slotsPerCU);
+  int energiesPerLane() const { return el_; }
+  hipStream_t stream() const { return stream_; }
+
+  // device pointers for zero-copy callers
+  double *devDelta() { return dDelta_; }
+  double *devVectorPotential() { return dA_; }
+
+private:
+  void upload();
+  void extendDelta();
+  void ensureBoundaryInit();
+  void integrate(int precision, bool burnin);
+  void acceleratorStep();
+  double residualOf(const double *oldv, const double *newv, long long n);
+  double deviceSum(const double *a, const double *b, const uint8_t *mask,
+                   long long n, int op);
+  void prepareSweep();
+
+  std::unique_ptr<Setup> s_;
+  hipStream_t stream_ = nullptr;
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr, ev2_ = nullptr;
+  // constant tables
+  double *dEnergy_ = nullptr, *dResidue_ = nullptr, *dSaCos_ = nullptr,
+         *dSaSin_ = nullptr, *dVFx_ = nullptr, *dVFy_ = nullptr,
+         *dWeight_ = nullptr, *dBasis_ = nullptr, *dCoupling_ = nullptr;
+  float *dTrig_ = nullptr; // opCos, opSin, aCos, aSin, bkCos, bkSin
+  int8_t *dLabels_ = nullptr;
+  uint8_t *dDomain_ = nullptr;
+  int *dDomainSites_ = nullptr;
+  int *dIdx_ = nullptr; // idxInG, idxOutG, idxInGt, idxOutGt
+  double *dDist_ = nullptr;
+  int *dBcOffG_ = nullptr, *dBcSrcG_ = nullptr, *dBcOffGt_ = nullptr,
+      *dBcSrcGt_ = nullptr;
+  double *dBcWG_ = nullptr, *dBcWsG_ = nullptr, *dBcWGt_ = nullptr,
+         *dBcWsGt_ = nullptr;
+  int *dExtRefT_ = nullptr, *dExtRefO_ = nullptr, *dExtRefS_ = nullptr;
+  int *dExtRotT_ = nullptr, *dExtRotO_ = nullptr, *dExtRotS_ = nullptr;
+  // state
+  double *dDelta_ = nullptr, *dDeltaNew_ = nullptr, *dExtDelta_ = nullptr;
+  double *dA_ = nullptr, *dANew_ = nullptr, *dCurrent_ = nullptr,
+         *dCurrentOld_ = nullptr, *dEnergyDens_ = nullptr;
+  double *dBInG_ = nullptr, *dBOutG_ = nullptr, *dBInGt_ = nullptr,
+         *dBOutGt_ = nullptr;
+  double *dG_ = nullptr;
+  double *dPartials_ = nullptr, *dWork_ = nullptr, *dWork2_ = nullptr;
+  double *dKernel_ = nullptr;
+  long long partialsCap_ = 0;
+  // sweep
+  int el_ = 8, slotsPerCU_ = 4, cols_ = 0, groups_ = 0, maxLen_ = 0,
+      numItems_ = 0, numSlots_ = 0;
+  void *dScratch_ = nullptr;
+  size_t scratchBytes_ = 0;
+  void *dItems_ = nullptr;
+  int *dCounter_ = nullptr;
+  bool sweepReady_ = false;
+  // accelerator state (history lives on the device)
+  int accIter_ = 0;
+  std::vector<double *> accX_, accD_;
+  std::vector<double> accSq_;
+  std::vector<long long> accIdx_;
+  double *dAccMean_ = nullptr, *dAccPrev_ = nullptr, *dAccVel_ = nullptr;
+  std::vector<double> accStep_, accInertia_;
+  double bbStep_ = 0;
+  bool accInit_ = false;
+  // bookkeeping
+  bool boundaryInit_ = false, firstCompute_ = true, firstCurrent_ = true,
+       firstFE_ = true;
+  double fePerAreaOld_ = 0;
+  Results res_;
+  Timing tim_;
+  bool convDelta_ = false, convCurrent_ = false, convFE_ = false;
+};
+
+// Stand-alone ComputeVectorPotential solve with host buffers (2*n*n each).
+void poissonSolve(int n, double h, int numGrain, bool enforceMean,
+                  const double *current, double *out);
+
+} // namespace scamd

@@ -1,0 +1,635 @@
+// sc_kernels.hpp -- HIP kernels of the self-consistency iteration (gfx950).
+#pragma once
+
+#include "sc_device.hpp"
+
+namespace scamd {
+
+// Per-grain constants and device pointers consumed by the kernels.
+struct GrainView {
+  int N0, N1, Ne, Na, Nc;
+  double h;
+  double lambdaInv2;  // 1 / lambda^2, 0 when the induced potential is off
+  double fluxDensity; // pi * Phi / area
+  double area;
+  int gauge;          // 0 landau, 1 symmetric, 2 solenoid
+  int charge;
+  int useInduced;     // lambda > 0
+  int useField;       // any vector potential (induced or external)
+  const double *energy, *residue;
+  const float *opCos, *opSin, *aCos, *aSin, *bkCos, *bkSin;
+  const double *saCos, *saSin, *vFx, *vFy, *weight, *basis;
+  const int8_t *labels;
+  const int *idxInG, *idxOutG, *idxInGt, *idxOutGt;
+  const double *distInG, *distOutG, *distInGt, *distOutGt;
+  double *bInG, *bOutG, *bInGt, *bOutGt; // planar [re: n*Ne][im: n*Ne]
+  long long nInG, nOutG, nInGt, nOutGt;
+  const double *extDelta; // [Nc][2][N0^2]
+  const double *A;        // [2][N0^2] (scaled by lambda^2)
+  double *G;              // [Na][4][N1^2]: Re/Im of f + f~*, Re g, Re i(D* g - D g~)
+};
+
+struct SweepItemDev {
+  int angle, x0, ncols, ylo, yhi;
+};
+
+struct SweepArgs {
+  GrainView v;
+  const SweepItemDev *items;
+  int numItems;
+  int cols;   // columns per item (lanes per energy group)
+  int groups; // energy groups actually used (ceil(Ne / EL))
+  int *counter;
+  void *scratch;       // per slot: [maxLen][EL][64] complex T
+  long long slotStride; // elements of complex T per slot
+  int burnin;
+};
+
+// Order parameter and Doppler shift at site (x, y) of frame k, from the
+// extended order parameter and the vector potential on the N0 grid
+// (OrderParameter.h:232-269, RiccatiSolver.h:127-176,
+// VectorPotential_Kernel.h:21-99, Gauges.h:25-62).
+__device__ __forceinline__ void siteFields(const GrainView &v, int k, int x,
+                                           int y, double &dre, double &dim,
+                                           double &es) {
+  const int N0 = v.N0, N1 = v.N1;
+  const size_t n0 = size_t(N0) * N0;
+  dre = 0.0;
+  dim = 0.0;
+  const RotSample r = rotatedCell(x, y, N1, N0, double(v.opCos[k]), double(v.opSin[k]));
+  if (r.valid) {
+    for (int c = 0; c < v.Nc; ++c) {
+      const double b = v.basis[c * v.Na + k];
+      const double *p = v.extDelta + size_t(c) * 2 * n0;
+      dre += b * bilinear(p, N0, r);
+      dim += b * bilinear(p + n0, N0, r);
+    }
+  }
+  es = 0.0;
+  if (v.useField) {
+    double ax = 0.0, ay = 0.0;
+    if (v.useInduced) {
+      const RotSample q = rotatedCell(x, y, N1, N0, double(v.aCos[k]), double(v.aSin[k]));
+      if (q.valid) {
+        ax = bilinear(v.A, N0, q) * v.lambdaInv2;
+        ay = bilinear(v.A + n0, N0, q) * v.lambdaInv2;
+      }
+    }
+    const double off = -0.5 * double(N1 - 1);
+    const double px0 = (double(x) + off) * v.h, py0 = (double(y) + off) * v.h;
+    const double c = v.saCos[k], s = v.saSin[k];
+    const double px = px0 * c - py0 * s, py = px0 * s + py0 * c;
+    double ex, ey;
+    if (v.gauge == 0) {
+      ex = 0.0;
+      ey = px * v.fluxDensity;
+    } else if (v.gauge == 1) {
+      const double f = 0.5 * v.fluxDensity;
+      ex = -py * f;
+      ey = px * f;
+    } else {
+      const double r2 = fmax(px * px + py * py, 1e-7);
+      const double f = v.fluxDensity * v.area / (2.0 * M_PI * r2);
+      ex = -py * f;
+      ey = px * f;
+    }
+    es = double(v.charge) * (v.vFx[k] * (ax + ex) + v.vFy[k] * (ay + ey));
+  }
+}
+
+__global__ void resetCounter(int *c) { *c = 0; }
+
+// The Riccati sweep.  One wavefront handles an item: `cols` adjacent columns
+// of one frame, each column split over `groups` lanes of EL Ozaki energies.
+// Pass 1 integrates gamma~ from the top of the item to the bottom and keeps
+// it in the slot's scratch; pass 2 integrates gamma upwards and forms the
+// energy-summed Green functions at every inside site.  Boundary in-values are
+// read and out-values written as in RiccatiSolverConfined.h:129-312.
+template <typename T, int EL>
+__global__ __launch_bounds__(64) void sweepKernel(SweepArgs a) {
+  const GrainView &v = a.v;
+  const int lane = threadIdx.x;
+  const int g = lane / a.cols;
+  const int c = lane - g * a.cols;
+  const int N1 = v.N1, Ne = v.Ne;
+  const size_t n1 = size_t(N1) * N1;
+  Cx<T> *scr = reinterpret_cast<Cx<T> *>(a.scratch) + size_t(blockIdx.x) * a.slotStride;
+  const T h = T(v.h);
+
+  for (;;) {
+    int item = 0;
+    if (lane == 0) item = atomicAdd(a.counter, 1);
+    item = __shfl(item, 0);
+    if (item >= a.numItems) break;
+    const SweepItemDev it = a.items[item];
+    const int k = it.angle;
+    const int x = it.x0 + (c < it.ncols ? c : it.ncols - 1);
+    const bool active = (c < it.ncols) && (g < a.groups);
+    const int8_t *lab = v.labels + size_t(k) * n1 + x;
+    T eIm[EL], res[EL];
+    bool eOk[EL];
+#pragma unroll
+    for (int j = 0; j < EL; ++j) {
+      const int e = g * EL + j;
+      eOk[j] = active && e < Ne;
+      eIm[j] = eOk[j] ? T(v.energy[e]) : T(0);
+      res[j] = eOk[j] ? T(v.residue[e]) : T(0);
+    }
+    const int len = it.yhi - it.ylo + 1;
+    const size_t col = size_t(k) * N1 + x;
+
+    // ---------------- pass 1: gamma~ along -y ----------------
+    {
+      const int baseIn = v.idxInGt[col], baseOut = v.idxOutGt[col];
+      int cntIn = 0, cntOut = 0;
+      Cx<T> coh[EL];
+#pragma unroll
+      for (int j = 0; j < EL; ++j) coh[j] = {T(0), T(0)};
+      Cx<T> dPrev = {T(0), T(0)};
+      T esPrev = T(0);
+      bool havePrev = false;
+      int8_t lcur = lab[size_t(it.yhi) * N1];
+      for (int t = 0; t < len; ++t) {
+        const int y = it.yhi - t;
+        const int8_t lnext = (t + 1 < len) ? lab[size_t(y - 1) * N1] : int8_t(0);
+        Cx<T> D = {T(0), T(0)};
+        T es = T(0);
+        if (active && (lcur != 0 || lnext != 0)) {
+          double dre, dim, e;
+          siteFields(v, k, x, y, dre, dim, e);
+          D = {T(-dre), T(dim)}; // -conj(Delta) for gamma~
+          es = T(e);
+        }
+        if (active) {
+          const T hp = havePrev ? T(1) : T(0);
+          const Cx<T> Dm = {(dPrev.re + D.re) * T(0.5), (dPrev.im + D.im) * T(0.5)};
+          const T esm = (esPrev + es) * T(0.5);
+          if (lcur == 1 || lcur == 2) {
+            // inside (1) or exit of gamma~ (2): midpoint step
+#pragma unroll
+            for (int j = 0; j < EL; ++j) {
+              const Cx<T> em = {esm, havePrev ? eIm[j] : eIm[j] * T(0.5)};
+              coh[j] = riccatiStep(Dm, em, coh[j], h);
+            }
+            (void)hp;
+            if (lcur == 2) {
+              const int ci = baseOut + cntOut;
+              const T d = T(v.distOutGt[ci]) * h;
+#pragma unroll
+              for (int j = 0; j < EL; ++j) {
+                if (!eOk[j]) continue;
+                const Cx<T> bv = riccatiStep(D, Cx<T>{es, eIm[j]}, coh[j], d);
+                const size_t si = size_t(ci) * Ne + g * EL + j;
+                v.bOutGt[si] = double(bv.re);
+                v.bOutGt[size_t(v.nOutGt) * Ne + si] = double(bv.im);
+              }
+              ++cntOut;
+            }
+          } else if (lcur == 3) {
+            // entry of gamma~: start from the boundary value
+            const int ci = baseIn + cntIn;
+            const T d = T(v.distInGt[ci]) * h;
+#pragma unroll
+            for (int j = 0; j < EL; ++j) {
+              const size_t si = size_t(ci) * Ne + g * EL + j;
+              const Cx<T> b0 = eOk[j] ? Cx<T>{T(v.bInGt[si]), T(v.bInGt[size_t(v.nInGt) * Ne + si])}
+                                      : Cx<T>{T(0), T(0)};
+              coh[j] = riccatiStep(D, Cx<T>{es, eIm[j]}, b0, d);
+            }
+            ++cntIn;
+          } else {
+#pragma unroll
+            for (int j = 0; j < EL; ++j) coh[j] = {T(0), T(0)};
+          }
+          if (!a.burnin && lcur > 0) {
+#pragma unroll
+            for (int j = 0; j < EL; ++j) scr[(size_t(t) * EL + j) * 64 + lane] = coh[j];
+          }
+        }
+        dPrev = D;
+        esPrev = es;
+        havePrev = true;
+        lcur = lnext;
+      }
+    }
+
+    // ---------------- pass 2: gamma along +y, Green functions ----------------
+    {
+      const int baseIn = v.idxInG[col], baseOut = v.idxOutG[col];
+      int cntIn = 0, cntOut = 0;
+      Cx<T> coh[EL];
+#pragma unroll
+      for (int j = 0; j < EL; ++j) coh[j] = {T(0), T(0)};
+      Cx<T> dPrev = {T(0), T(0)};
+      T esPrev = T(0);
+      bool havePrev = false;
+      int8_t lcur = lab[size_t(it.ylo) * N1];
+      for (int t = 0; t < len; ++t) {
+        const int y = it.ylo + t;
+        const int8_t lnext = (t + 1 < len) ? lab[size_t(y + 1) * N1] : int8_t(0);
+        Cx<T> D = {T(0), T(0)};
+        T es = T(0);
+        double dre = 0.0, dim = 0.0;
+        if (active && (lcur != 0 || lnext != 0)) {
+          double e;
+          siteFields(v, k, x, y, dre, dim, e);
+          D = {T(dre), T(dim)};
+          es = T(e);
+        }
+        if (active) {
+          const Cx<T> Dm = {(dPrev.re + D.re) * T(0.5), (dPrev.im + D.im) * T(0.5)};
+          const T esm = (esPrev + es) * T(0.5);
+          if (lcur == 1 || lcur == 3) {
+#pragma unroll
+            for (int j = 0; j < EL; ++j) {
+              const Cx<T> em = {esm, havePrev ? eIm[j] : eIm[j] * T(0.5)};
+              coh[j] = riccatiStep(Dm, em, coh[j], h);
+            }
+            if (lcur == 3) {
+              const int ci = baseOut + cntOut;
+              const T d = T(v.distOutG[ci]) * h;
+#pragma unroll
+              for (int j = 0; j < EL; ++j) {
+                if (!eOk[j]) continue;
+                const Cx<T> bv = riccatiStep(D, Cx<T>{es, eIm[j]}, coh[j], d);
+                const size_t si = size_t(ci) * Ne + g * EL + j;
+                v.bOutG[si] = double(bv.re);
+                v.bOutG[size_t(v.nOutG) * Ne + si] = double(bv.im);
+              }
+              ++cntOut;
+            }
+          } else if (lcur == 2) {
+            const int ci = baseIn + cntIn;
+            const T d = T(v.distInG[ci]) * h;
+#pragma unroll
+            for (int j = 0; j < EL; ++j) {
+              const size_t si = size_t(ci) * Ne + g * EL + j;
+              const Cx<T> b0 = eOk[j] ? Cx<T>{T(v.bInG[si]), T(v.bInG[size_t(v.nInG) * Ne + si])}
+                                      : Cx<T>{T(0), T(0)};
+              coh[j] = riccatiStep(D, Cx<T>{es, eIm[j]}, b0, d);
+            }
+            ++cntIn;
+          } else {
+#pragma unroll
+            for (int j = 0; j < EL; ++j) coh[j] = {T(0), T(0)};
+          }
+        }
+        // Green functions at inside sites (GreensFunctionStatic.h:125-157,
+        // LocalGreens.h:32-70, ComputeFreeEnergy.h:95-160).
+        if (!a.burnin && lcur > 0) {
+          double gdr = 0.0, gdi = 0.0, gj = 0.0, gf = 0.0;
+          if (active) {
+            const int tt = len - 1 - t;
+#pragma unroll
+            for (int j = 0; j < EL; ++j) {
+              if (!eOk[j]) continue;
+              const Cx<T> ga = coh[j];
+              const Cx<T> gt = scr[(size_t(tt) * EL + j) * 64 + lane];
+              const Cx<T> p = ga * gt;
+              const Cx<T> inv = cdiv(Cx<T>{T(1), T(0)}, Cx<T>{T(1) + p.re, p.im});
+              // f + f~* with factor = 2 i / (1 + p)
+              const Cx<T> fac = {T(-2) * inv.im, T(2) * inv.re};
+              const Cx<T> f = fac * ga;
+              const Cx<T> ft = fac * gt;
+              const double r = double(res[j]);
+              gdr += r * double(-f.re + ft.re);
+              gdi += r * double(-f.im - ft.im);
+              // Re g = Im((1 - p) / (1 + p))
+              const Cx<T> q = Cx<T>{T(1) - p.re, T(0) - p.im} * inv;
+              gj += r * double(q.im);
+              // Re[i (D* gamma - D gamma~)] = -Im(D* gamma - D gamma~)
+              const double im1 = dre * double(ga.im) - dim * double(ga.re);
+              const double im2 = dre * double(gt.im) + dim * double(gt.re);
+              gf -= r * (im1 - im2);
+            }
+          }
+          // sum the energy groups of this column (lanes c, c + cols, ...)
+          for (int m = a.cols; m < 64; m <<= 1) {
+            gdr += __shfl_xor(gdr, m);
+            gdi += __shfl_xor(gdi, m);
+            gj += __shfl_xor(gj, m);
+            gf += __shfl_xor(gf, m);
+          }
+          if (g == 0 && c < it.ncols) {
+            double *Gk = v.G + size_t(k) * 4 * n1 + size_t(y) * N1 + x;
+            Gk[0] = gdr;
+            Gk[n1] = gdi;
+            Gk[2 * n1] = gj;
+            Gk[3 * n1] = gf;
+          }
+        }
+        dPrev = D;
+        esPrev = es;
+        havePrev = true;
+        lcur = lnext;
+      }
+    }
+  }
+}
+
+// Homogeneous coherence functions at every exit crossing
+// (RiccatiSolverConfined.h:400-468): one thread per (frame, column, energy).
+__global__ void initialBoundaryKernel(GrainView v, const int *xEnd) {
+  const long long tid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const int N1 = v.N1, Ne = v.Ne;
+  if (tid >= (long long)v.Na * N1 * Ne) return;
+  const int e = int(tid % Ne);
+  const int x = int((tid / Ne) % N1);
+  const int k = int(tid / ((long long)Ne * N1));
+  (void)xEnd;
+  const size_t n1 = size_t(N1) * N1;
+  const int8_t *lab = v.labels + size_t(k) * n1 + x;
+  const size_t col = size_t(k) * N1 + x;
+  int cg = 0, cgt = 0;
+  const int baseG = v.idxOutG[col], baseGt = v.idxOutGt[col];
+  // exits of gamma (label 3) in ascending y and of gamma~ (label 2) in
+  // descending y; the sparse index only depends on the count so a single
+  // ascending pass with a reversed count for gamma~ suffices.
+  int nGt = 0;
+  for (int y = 0; y < N1; ++y) nGt += lab[size_t(y) * N1] == 2;
+  for (int y = 0; y < N1; ++y) {
+    const int8_t l = lab[size_t(y) * N1];
+    if (l != 2 && l != 3) continue;
+    double dre, dim, es;
+    siteFields(v, k, x, y, dre, dim, es);
+    Cx<double> ga, gt;
+    homogeneous(Cx<double>{dre, dim}, Cx<double>{es, v.energy[e]}, ga, gt);
+    if (l == 3) {
+      const size_t si = size_t(baseG + cg) * Ne + e;
+      v.bOutG[si] = ga.re;
+      v.bOutG[size_t(v.nOutG) * Ne + si] = ga.im;
+      ++cg;
+    } else {
+      const size_t si = size_t(baseGt + (nGt - 1 - cgt)) * Ne + e;
+      v.bOutGt[si] = gt.re;
+      v.bOutGt[size_t(v.nOutGt) * Ne + si] = gt.im;
+      ++cgt;
+    }
+  }
+}
+
+// Extension of values outwards from the boundary
+// (geometry/domain_ops.h:11-57): target <- mean of its listed sources, for
+// `planes` planes spaced `planeStride` apart starting at
+// base + (target / siteStride) * blockStride.
+__global__ void extendKernel(double *base, const int *target, const int *offset,
+                             const int *source, int numTargets, int planes,
+                             long long planeStride, long long siteStride,
+                             long long blockStride) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= numTargets) return;
+  const long long tg = target[t];
+  const long long blk = tg / siteStride;
+  const long long site = tg - blk * siteStride;
+  double *p0 = base + blk * blockStride;
+  const int o0 = offset[t], o1 = offset[t + 1];
+  const double inv = double(o1 - o0);
+  for (int q = 0; q < planes; ++q) {
+    double *p = p0 + q * planeStride;
+    double val = 0.0;
+    for (int o = o0; o < o1; ++o) {
+      const long long s = source[o] - blk * siteStride;
+      val = (o == o0) ? p[s] : val + p[s];
+    }
+    p[site] = val / inv;
+  }
+}
+
+// Back-rotation of the Green functions of every frame to the reference grid
+// and the angle integration (ComputeOrderParameter.h:174-262,
+// ComputeCurrent.h:142-222, ComputeFreeEnergy.h:260-279), one thread per
+// inside site of the reference grain.
+__global__ void gatherKernel(GrainView v, const int *sites, int numSites,
+                             const double *coupling, double currentFactor,
+                             double *deltaOut, double *currentOut,
+                             double *energyDensity) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= numSites) return;
+  const int N0 = v.N0, N1 = v.N1, Na = v.Na, Nc = v.Nc;
+  const int site = sites[t];
+  const int x = site % N0, y = site / N0;
+  const size_t n0 = size_t(N0) * N0, n1 = size_t(N1) * N1;
+  double dre[4] = {0, 0, 0, 0}, dim[4] = {0, 0, 0, 0};
+  double jx = 0.0, jy = 0.0, fe = 0.0;
+  for (int k = 0; k < Na; ++k) {
+    const RotSample r = rotatedCell(x, y, N0, N1, double(v.bkCos[k]), double(v.bkSin[k]));
+    if (!r.valid) continue;
+    const double *Gk = v.G + size_t(k) * 4 * n1;
+    const double w = v.weight[k];
+    const double gre = bilinear(Gk, N1, r);
+    const double gim = bilinear(Gk + n1, N1, r);
+    const double gj = bilinear(Gk + 2 * n1, N1, r);
+    const double gf = bilinear(Gk + 3 * n1, N1, r);
+    for (int c = 0; c < Nc && c < 4; ++c) {
+      const double b = v.basis[c * Na + k];
+      dre[c] += (gre * b) * w;
+      dim[c] += (gim * b) * w;
+    }
+    jx += (v.vFx[k] * w) * gj;
+    jy += (v.vFy[k] * w) * gj;
+    fe += w * gf;
+  }
+  for (int c = 0; c < Nc && c < 4; ++c) {
+    deltaOut[size_t(c) * 2 * n0 + site] = dre[c] * coupling[c];
+    deltaOut[size_t(c) * 2 * n0 + n0 + site] = dim[c] * coupling[c];
+  }
+  currentOut[site] = jx * currentFactor;
+  currentOut[n0 + site] = jy * currentFactor;
+  energyDensity[site] = fe;
+}
+
+// Specular boundary condition as a precomputed stencil per entry crossing
+// (BoundaryConditionSpecular.h:108-358), one thread per (crossing, energy).
+// The previous in-values feed the residual statistics
+// (ComputeResidual.h:54-106): per block sums of |d|, |old|, d^2, old^2 and
+// maxima of |d|, |old|.
+__global__ void boundaryKernel(double *bin, const double *bout, long long nIn,
+                               long long nOut, int Ne, const int *offset,
+                               const int *src, const double *w,
+                               const double *wsum, double *partials) {
+  const long long tid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  double s[6] = {0, 0, 0, 0, 0, 0};
+  const long long total = nIn * Ne;
+  if (tid < total) {
+    const long long i = tid / Ne;
+    const int e = int(tid - i * Ne);
+    double re = 0.0, im = 0.0;
+    for (int o = offset[i]; o < offset[i + 1]; ++o) {
+      const size_t si = size_t(src[o]) * Ne + e;
+      re += bout[si] * w[o];
+      im += bout[size_t(nOut) * Ne + si] * w[o];
+    }
+    const double ws = wsum[i];
+    const double nre = ws > 0.0 ? re / ws : 0.0;
+    const double nim = ws > 0.0 ? im / ws : 0.0;
+    const double ore = bin[tid], oim = bin[total + tid];
+    bin[tid] = nre;
+    bin[total + tid] = nim;
+    const double d1 = ore - nre, d2 = oim - nim;
+    s[0] = fabs(d1) + fabs(d2);
+    s[1] = fabs(ore) + fabs(oim);
+    s[2] = d1 * d1 + d2 * d2;
+    s[3] = ore * ore + oim * oim;
+    s[4] = fmax(fabs(d1), fabs(d2));
+    s[5] = fmax(fabs(ore), fabs(oim));
+  }
+  __shared__ double sh[6][256];
+  for (int q = 0; q < 6; ++q) sh[q][threadIdx.x] = s[q];
+  __syncthreads();
+  for (int m = blockDim.x / 2; m > 0; m >>= 1) {
+    if (threadIdx.x < m)
+      for (int q = 0; q < 6; ++q)
+        sh[q][threadIdx.x] = q < 4 ? sh[q][threadIdx.x] + sh[q][threadIdx.x + m]
+                                   : fmax(sh[q][threadIdx.x], sh[q][threadIdx.x + m]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 6; ++q) partials[size_t(blockIdx.x) * 6 + q] = sh[q][0];
+}
+
+// Residual statistics of old/new arrays (ComputeResidual.h:54-106), same six
+// per-block quantities as boundaryKernel.
+__global__ void statsKernel(const double *oldv, const double *newv, long long n,
+                            double *partials) {
+  double s[6] = {0, 0, 0, 0, 0, 0};
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double o = oldv[i];
+    const double d = o - (newv ? newv[i] : 0.0);
+    s[0] += fabs(d);
+    s[1] += fabs(o);
+    s[2] += d * d;
+    s[3] += o * o;
+    s[4] = fmax(s[4], fabs(d));
+    s[5] = fmax(s[5], fabs(o));
+  }
+  __shared__ double sh[6][256];
+  for (int q = 0; q < 6; ++q) sh[q][threadIdx.x] = s[q];
+  __syncthreads();
+  for (int m = blockDim.x / 2; m > 0; m >>= 1) {
+    if (threadIdx.x < m)
+      for (int q = 0; q < 6; ++q)
+        sh[q][threadIdx.x] = q < 4 ? sh[q][threadIdx.x] + sh[q][threadIdx.x + m]
+                                   : fmax(sh[q][threadIdx.x], sh[q][threadIdx.x + m]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 6; ++q) partials[size_t(blockIdx.x) * 6 + q] = sh[q][0];
+}
+
+// Generic block sums: partial[b] = sum_i f(i) with f selected by `op`:
+// 0: a[i]; 1: a[i]*b[i]; 2: a[i]^2 masked by mask[i] (mask may be null).
+__global__ void sumKernel(const double *a, const double *b, const uint8_t *mask,
+                          long long n, int op, double *partials) {
+  double s = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    if (mask && !mask[i]) continue;
+    const double x = a[i];
+    s += op == 0 ? x : (op == 1 ? x * b[i] : x * x);
+  }
+  __shared__ double sh[256];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int m = blockDim.x / 2; m > 0; m >>= 1) {
+    if (threadIdx.x < m) sh[threadIdx.x] += sh[threadIdx.x + m];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
+}
+
+// B = dAy/dx - dAx/dy with one-sided differences at the edges
+// (GridFunctions_GPU.h:326-353); writes B^2.
+__global__ void curlSquareKernel(const double *A, int N, double invH, double *out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N * N) return;
+  const int x = t % N, y = t / N;
+  const size_t n = size_t(N) * N;
+  const int xn = min(x + 1, N - 1), xp = max(x - 1, 0);
+  const int yn = min(y + 1, N - 1), yp = max(y - 1, 0);
+  const double dAy = (A[n + size_t(y) * N + xn] - A[n + size_t(y) * N + xp]) / double(xn - xp);
+  const double dAx = (A[size_t(yn) * N + x] - A[size_t(yp) * N + x]) / double(yn - yp);
+  const double b = (dAy - dAx) * invH;
+  out[t] = b * b;
+}
+
+// ---------------- elementwise kernels of the accelerators ----------------
+// x += (g - x) * s  (Picard.h:133-138)
+__global__ void picardKernel(double *x, const double *g, long long n, double s) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) x[i] += (g[i] - x[i]) * s;
+}
+// d = g - x
+__global__ void diffKernel(double *d, const double *g, const double *x, long long n) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) d[i] = g[i] - x[i];
+}
+// x = sum_i (X_i + D_i s) w_i over the history (Anderson.h:336-355)
+__global__ void andersonCombine(double *x, const double *const *X,
+                                const double *const *D, const double *w, int m,
+                                long long n, double s) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double acc = 0.0;
+  for (int q = 0; q < m; ++q) acc += (X[q][i] + D[q][i] * s) * w[q];
+  x[i] = acc;
+}
+// running mean and step per field (CongAcc.h:271-297)
+__global__ void congaccKernel(double *x, double *mean, const double *d,
+                              const double *step, const double *inertia,
+                              long long fieldSize, long long n) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int f = int(i / fieldSize);
+  const double m = mean[i] * inertia[f] + d[i] * (1.0 - inertia[f]);
+  mean[i] = m;
+  x[i] += step[f] * m;
+}
+// x += d * s (BarzilaiBorwein.h:237-240)
+__global__ void axpyKernel(double *x, const double *d, long long n, double s) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) x[i] += d[i] * s;
+}
+// heavy ball (Polyak.h:216-240)
+__global__ void polyakKernel(double *x, double *vel, const double *g,
+                             long long n, double s, double drag) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double d = g[i] - x[i];
+  const double u = vel[i] * (1.0 - drag) + d * s;
+  vel[i] = u;
+  x[i] += u;
+}
+// zero the sites outside the reference grain
+__global__ void maskKernel(double *p, const uint8_t *dom, long long n0, int planes) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n0 * planes) return;
+  if (!dom[i % n0]) p[i] = 0.0;
+}
+
+// Direct cross-correlation with the full Green function
+// (ComputeVectorPotential.h:519-563), used to check the FFT path.
+__global__ void poissonDirectKernel(const double *src, const double *g, int N,
+                                    double scale, double *out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 2 * N * N) return;
+  const int d = t / (N * N), r = t % (N * N), y = r / N, x = r % N;
+  const double *s = src + size_t(d) * N * N;
+  double sum = 0.0;
+  for (int j = 0; j < N; ++j) {
+    double ps = 0.0;
+    const double *grow = g + size_t(abs(j - y)) * N;
+    for (int i = 0; i < N; ++i) ps += s[size_t(j) * N + i] * grow[abs(i - x)];
+    sum += ps;
+  }
+  out[t] = sum * scale;
+}
+// j - mean over the whole grid (ComputeVectorPotential.h:575-616)
+__global__ void subtractMeanKernel(const double *j, double mx, double my,
+                                   long long n0, double *out) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= 2 * n0) return;
+  out[i] = j[i] - (i < n0 ? mx : my);
+}
+
+} // namespace scamd

@@ -1,0 +1,164 @@
+// sc_setup.hpp -- iteration-invariant host data for one grain.
+//
+// Everything the reference recomputes on every angle of every iteration but
+// which never changes (geometry labels, boundary storage bookkeeping,
+// specular-reflection interpolation stencils, extension stencils, Ozaki poles,
+// rotation trigonometry) is computed once here in FP64 on the host and
+// uploaded.  The references to the reference's files state which routine each
+// table replaces.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace scamd {
+
+enum Symmetry { SYM_S = 0, SYM_DX2Y2 = 1, SYM_DXY = 2, SYM_G = 3 };
+enum Gauge { GAUGE_LANDAU = 0, GAUGE_SYMMETRIC = 1, GAUGE_SOLENOID = 2 };
+enum Norm { NORM_L1 = 0, NORM_L2 = 1, NORM_LINF = 2 };
+enum Accel {
+  ACC_PICARD = 0,
+  ACC_ANDERSON = 1,
+  ACC_CONGACC = 2,
+  ACC_BB = 3,
+  ACC_POLYAK = 4
+};
+
+constexpr int kMaxComponents = 4;
+
+// A geometry component in the internal frame, already scaled by the grain
+// fraction (DiscGeometry.h:249-261, PolygonGeometry.h:265-342).
+struct Shape {
+  bool disc = true;
+  bool add = true;
+  double radius = 0.5, cx = 0.0, cy = 0.0; // disc
+  std::vector<double> vx, vy;              // polygon vertices
+};
+
+struct Vortex {
+  double cx, cy, winding;
+};
+
+struct Config {
+  // physics (src/simulation.cu:101-155)
+  double temperature = 0.5;
+  double fluxQuanta = 0.0;
+  double lambda = -1.0;
+  double crystalRotation = 0.0; // radians
+  int gauge = GAUGE_SYMMETRIC;
+  int chargeSign = -1;
+  int numComponents = 0;
+  int symmetry[kMaxComponents] = {0, 0, 0, 0};
+  double tc[kMaxComponents] = {1, 1, 1, 1};
+  double phaseShift[kMaxComponents] = {0, 0, 0, 0};
+  double noise[kMaxComponents] = {0, 0, 0, 0};
+  std::vector<Vortex> vortices[kMaxComponents];
+  // geometry
+  double grainWidth = 20.0;
+  double geomOffX = 0.0, geomOffY = 0.0; // bounding-box centre (user units)
+  std::vector<Shape> shapes;
+  // numerics
+  double convergence = 1e-6;
+  double cutoff = 10.0;
+  int norm = NORM_L2;
+  int numAngles = 64;
+  double ppxi = 5.0;
+  double vpError = 0.0;
+  int burnin = 0, itMin = 0, itMax = 10000;
+  bool computeCurrent = true, computeFreeEnergy = true;
+  // accelerator (accelerators/*.h defaults)
+  int accel = ACC_CONGACC;
+  double stepSize = -1, drag = -1, stepFactor = -1, cosMin = -1, stepMax = -1,
+         stepMin = -1;
+  int rank = -1;
+  // misc
+  std::string loadPath;
+  unsigned long long seed = 0;
+  bool verbose = false;
+};
+
+// Parses simulation_config.json text.  Throws std::runtime_error.
+Config parseConfig(const std::string &text);
+
+// Compressed lists "target <- mean of sources" for extendValuesAtBoundary
+// (geometry/domain_ops.h:11-57).  Sources are listed in the reference's scan
+// order so that the sums are formed in the same order.
+struct ExtensionList {
+  std::vector<int> target; // flat site index
+  std::vector<int> offset; // size target.size() + 1
+  std::vector<int> source; // flat site index
+};
+
+// Specular-reflection stencil per boundary entry crossing
+// (BoundaryConditionSpecular.h:108-358): in = sum_j w_j out[src_j] / wsum.
+struct BcTable {
+  std::vector<int> offset; // size numIn + 1
+  std::vector<int> src;    // crossing index into the out storage
+  std::vector<double> w;
+  std::vector<double> wsum;
+};
+
+// One unit of work of the Riccati sweep: a block of adjacent columns of one
+// rotational frame and the rows the block's trajectories touch.
+struct SweepItem {
+  int angle;
+  int x0;
+  int ncols;
+  int ylo, yhi;
+};
+
+struct Setup {
+  Config cfg;
+  int N0 = 0, N1 = 0, Ne = 0, Na = 0, Nc = 0, numGrain = 0;
+  double h = 0, grainFraction = 0, temperature = 0, lambda = -1;
+  // energies (IntegrationIteratorOzaki.h:108-200)
+  std::vector<double> energy, residue;
+  // Fermi surface (FermiSurfaceCircle.h:83-163)
+  std::vector<double> systemAngle, polar, weight, vFx, vFy, saCos, saSin;
+  // single-precision rotation trigonometry of insertRotatedGrid(Additive)
+  // (GridFunctions_GPU.h:436-544): cos(-phi), sin(-phi) with phi a float.
+  std::vector<float> opCos, opSin; // order parameter into frame k
+  std::vector<float> aCos, aSin;   // vector potential into frame k
+  std::vector<float> bkCos, bkSin; // Green functions back from frame k
+  // order parameter (OrderParameterBasis.h, OrderParameterComponent.h)
+  std::vector<double> basis; // [c * Na + k]
+  std::vector<double> basisNorm, coupling, deltaBulk;
+  double sumInverseEnergy = 0; // T * sum_n R_n / Im eps_n
+  // geometry
+  std::vector<int8_t> labels;  // [Na][N1][N1]
+  std::vector<uint8_t> domain; // [N0][N0] reference-frame labels (0/1)
+  std::vector<int> domainSites;
+  // boundary storage (BoundaryStorage.h:298-527)
+  std::vector<int> idxInG, idxOutG, idxInGt, idxOutGt, numIsG, numIsGt;
+  long long nInG = 0, nOutG = 0, nInGt = 0, nOutGt = 0;
+  std::vector<int> coordG, coordGt;
+  std::vector<double> distInG, distOutG, distInGt, distOutGt;
+  std::vector<int> xmin, xmax, ymin, ymax;
+  std::vector<int> colFirst, colLast; // [Na][N1], -1 where empty
+  BcTable bcG, bcGt;
+  ExtensionList extRef, extRot;
+  long long trajSteps = 0; // 2 * Ne * #{sites with label > 0 over all frames}
+};
+
+std::unique_ptr<Setup> buildSetup(const Config &cfg);
+
+// Work list of the sweep for a given number of columns per item, sorted by
+// decreasing length.
+std::vector<SweepItem> sweepItems(const Setup &s, int colsPerItem,
+                                  int *maxLength);
+
+// Stand-alone pieces exposed through the C ABI for unit checks.
+int numOzakiEnergies(double temperature, double cutoff);
+void ozakiPoles(int numPoles, std::vector<double> &poles,
+                std::vector<double> &residues);
+double bulkGapSWave(double temperature);
+void latticeSizes(double width, double ppxi, int &n0, int &n1,
+                  double &grainFraction);
+// Real, even Green-function kernel of ComputeVectorPotential with
+// vector_potential_error = 0 (ComputeVectorPotential.h:212-294,396-415):
+// returns G(|dx|, |dy|) for 0 <= dx, dy < n as g[dy * n + dx].
+void poissonKernel(int n, double h, std::vector<double> &g);
+
+} // namespace scamd
