@@ -137,6 +137,10 @@ int sc_ozaki(int num_poles, double *poles, double *residues);
 double sc_bulk_gap(double temperature);
 /* Geometry labels of frame `angle` (N1*N1, GeometryGroup::create(-theta)). */
 int sc_get_labels(sc_context *ctx, int angle, signed char *labels);
+/* Host-only planning (no device needed): the sizes and, if labels is not
+ * null, the labels of frame `angle` that sc_create would build for a config. */
+int sc_describe(const char *config_json, int angle, sc_dims *out,
+                signed char *labels);
 
 #ifdef __cplusplus
 }

@@ -2023,6 +2023,13 @@ static Oracle *build(const oc_config &cfg) {
         c.radius = p[2] * gf;
         c.cx = p[0] * gf;
         c.cy = p[1] * gf;
+      } else if (cfg.geometry_type[i] == OC_POLYGON) {
+        // PolygonGeometry.h:61-77: vertices scaled by the grain fraction
+        c.type = OC_POLYGON;
+        for (int v = 0; v < cfg.polygon_num_vertices[i]; ++v) {
+          c.vx.push_back(cfg.polygon_vertices_x[cfg.polygon_vertex_offset[i] + v] * gf);
+          c.vy.push_back(cfg.polygon_vertices_y[cfg.polygon_vertex_offset[i] + v] * gf);
+        }
       } else {
         // PolygonGeometry.h:265-291
         c.type = OC_POLYGON;

@@ -148,7 +148,11 @@ def make_config(cfg):
     c.gauge = GAUGES[ph.get("gauge", "symmetric")]
     c.charge_sign = ph.get("charge_sign", -1)
     comps = []
-    for name in ("dx2-y2", "dxy", "g", "s"):  # simulation.cu:201 order
+    # simulation.cu:201 adds the components in this order; the extension key
+    # "component_order" overrides it (the reference's C++ tests add them in
+    # any order, tests/free_energy_test.cu:107-117).
+    order = cfg.get("component_order", ("dx2-y2", "dxy", "g", "s"))
+    for name in order:
         if name in ph["order_parameter"]:
             comps.append((name, ph["order_parameter"][name]))
     c.num_components = len(comps)

@@ -95,6 +95,8 @@ def lib():
         L.sc_bulk_gap.restype = ctypes.c_double
         L.sc_bulk_gap.argtypes = [ctypes.c_double]
         L.sc_get_labels.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_byte)]
+        L.sc_describe.argtypes = [ctypes.c_char_p, ctypes.c_int,
+                                  ctypes.POINTER(Dims), ctypes.POINTER(ctypes.c_byte)]
         _lib = L
     return _lib
 
@@ -237,6 +239,23 @@ class Simulation:
 
     def set_sweep_shape(self, energies_per_lane, slots_per_cu):
         _check(lib().sc_set_sweep_shape(self._h, energies_per_lane, slots_per_cu))
+
+
+def describe(config, angle=None):
+    """Host-only plan of a config: its sizes and, for one angle, its labels.
+
+    Needs no device; returns (dims, labels or None)."""
+    text = config if isinstance(config, str) else json.dumps(config)
+    d = Dims()
+    if angle is None:
+        _check(lib().sc_describe(text.encode(), 0, ctypes.byref(d), None))
+        return d, None
+    # sizes first, then the labels buffer of the right size
+    _check(lib().sc_describe(text.encode(), 0, ctypes.byref(d), None))
+    out = np.empty(d.n1 * d.n1, dtype=np.int8)
+    _check(lib().sc_describe(text.encode(), angle, ctypes.byref(d),
+                             out.ctypes.data_as(ctypes.POINTER(ctypes.c_byte))))
+    return d, out.reshape(d.n1, d.n1)
 
 
 def poisson_solve(current, h, num_grain, enforce_mean=True):

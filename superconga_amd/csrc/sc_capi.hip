@@ -329,6 +329,33 @@ int sc_ozaki(int n, double *poles, double *residues) {
 }
 double sc_bulk_gap(double t) { return scamd::bulkGapSWave(t); }
 
+int sc_describe(const char *config_json, int angle, sc_dims *out, signed char *labels) {
+  SC_CHECK_PTR(config_json);
+  SC_CHECK_PTR(out);
+  return guarded([&] {
+    const auto s = scamd::buildSetup(scamd::parseConfig(config_json));
+    out->n0 = s->N0;
+    out->n1 = s->N1;
+    out->num_energies = s->Ne;
+    out->num_angles = s->Na;
+    out->num_components = s->Nc;
+    out->num_grain_sites = s->numGrain;
+    out->num_in_gamma = s->nInG;
+    out->num_out_gamma = s->nOutG;
+    out->num_in_gammatilde = s->nInGt;
+    out->num_out_gammatilde = s->nOutGt;
+    out->traj_steps = s->trajSteps;
+    out->h = s->h;
+    out->grain_fraction = s->grainFraction;
+    if (labels) {
+      if (angle < 0 || angle >= s->Na) throw std::invalid_argument("angle out of range");
+      const size_t n1 = size_t(s->N1) * s->N1;
+      std::memcpy(labels, s->labels.data() + size_t(angle) * n1, n1);
+    }
+    return SC_OK;
+  });
+}
+
 int sc_get_labels(sc_context *ctx, int angle, signed char *labels) {
   SC_CHECK_CTX(ctx);
   SC_CHECK_PTR(labels);

@@ -1,0 +1,117 @@
+"""The C ABI without a device: the library loads, exports every entry point
+include/superconga_amd.h declares, reports errors the documented way, and
+its host-side planning (lattice sizes, Ozaki tables, geometry labels,
+boundary-storage sizes) agrees exactly with the oracle."""
+import ctypes
+import json
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import cases
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "superconga_amd.h")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sc_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_exports_every_declared_symbol(product):
+    names = declared_symbols()
+    assert len(names) >= 25
+    L = ctypes.CDLL(product.LIB_PATH)
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing
+
+
+def test_version_and_error_reporting(product):
+    L = product.lib()
+    assert L.sc_version() >= 1
+    h = ctypes.c_void_p()
+    code = L.sc_create(b"{not json", ctypes.byref(h))
+    assert code != 0
+    assert L.sc_last_error()
+    code = L.sc_create(json.dumps({"physics": {}}).encode(), ctypes.byref(h))
+    assert code != 0
+    assert L.sc_get_dims(None, None) != 0
+    assert L.sc_ozaki(0, None, None) != 0
+
+
+def test_describe_rejects_bad_angle(product):
+    with pytest.raises(product.SupercongaError):
+        product.describe(cases.base(angles=8), angle=8)
+
+
+def test_host_tables_match_oracle(product, oracle):
+    table = json.load(open(os.path.join(GOLDEN, "ozaki_max_poles.json")))["max_pole"]
+    for n in (1, 2, 3, 9, 14, 28, 61, 200):
+        p, r = product.ozaki(n)
+        po, ro = oracle.ozaki(n)
+        assert abs(p.max() - table[n - 1]) < 1e-9 * table[n - 1]
+        # Different eigen-solvers (QL here, Jacobi in the oracle): agree to
+        # rounding of the eigenproblem.
+        # (3.9e-12 relative on the residues at 200 poles, 1.2e-13 at 28).
+        assert np.allclose(np.sort(p), np.sort(po), rtol=1e-11, atol=0)
+        assert np.allclose(r[np.argsort(p)], ro[np.argsort(po)], rtol=2e-11, atol=0)
+        # Ozaki's expansion of the Fermi function reproduces tanh(x/2).
+        if n >= 9:
+            for x in (0.1, 1.0, 5.0, 15.0):
+                assert abs(np.sum(4 * x * r / (x * x + p * p)) - math.tanh(x / 2)) < 1e-12
+    for t, cut in ((0.5, 10 / (2 * math.pi)), (0.5, 16.0), (0.2, 16.0), (0.1, 32.0),
+                   (0.3, 16.0), (0.9, 16.0), (0.01, 64.0)):
+        assert product.num_ozaki(t, cut) == oracle.num_ozaki(t, cut)
+    for t in (0.001, 0.1, 0.5, 0.9, 0.999, 1.0):
+        assert abs(product.bulk_gap(t) - oracle.bulk_gap(t)) < 1e-13
+
+
+PLAN_CASES = {
+    "disc": cases.base(angles=16),
+    "square_rotated_axes": cases.base(
+        order_parameter={"dx2-y2": {"critical_temperature": 1.0}}, rotation=0.125,
+        components=[cases.regular_polygon_unit(4)], angles=20, ppxi=4.0),
+    "octagon_hole": cases.mixed_small(),
+    "polygon": cases.base(components=[{"polygon": {
+        "vertices_x": [-0.5, 0.5, 0.3, -0.4], "vertices_y": [-0.45, -0.5, 0.5, 0.2]}}],
+        angles=12, ppxi=4.0, width=12.0),
+    "two_discs": cases.base(components=[
+        {"disc": {"radius": 0.25, "center_x": -0.22, "center_y": 0.0}},
+        {"disc": {"radius": 0.25, "center_x": 0.22, "center_y": 0.1}}],
+        angles=16, ppxi=3.0, width=14.0),
+    "physical_units_square": cases.base(geometry=[
+        {"regular_polygon": {"num_edges": 4, "side_length": 8.0, "rotation": 0.1}}],
+        angles=12, ppxi=3.0),
+}
+
+
+@pytest.mark.parametrize("name", sorted(PLAN_CASES))
+def test_plan_matches_oracle(product, oracle, name):
+    cfg = PLAN_CASES[name]
+    d, _ = product.describe(cfg)
+    o = oracle.Oracle(cfg)
+    assert (d.n0, d.n1, d.num_energies, d.num_angles, d.num_components) == (
+        o.N0, o.N1, o.Ne, o.Na, o.Nc)
+    assert d.num_grain_sites == o.get_int("num_grain_sites")
+    assert d.num_in_gamma == o.get_int("num_storage_in_gamma")
+    assert d.num_out_gamma == o.get_int("num_storage_out_gamma")
+    assert d.num_in_gammatilde == o.get_int("num_storage_in_gammatilde")
+    assert d.num_out_gammatilde == o.get_int("num_storage_out_gammatilde")
+    assert d.traj_steps == o.get_int("traj_steps")
+    for k in range(o.Na):
+        _, lab = product.describe(cfg, angle=k)
+        assert np.array_equal(lab, o.geometry(k)[0]), k
+
+
+def test_plan_octagon_benchmark_sizes(product):
+    d, _ = product.describe(cases.octagon_dwave())
+    assert (d.n0, d.n1, d.num_energies, d.num_angles) == (930, 1316, 28, 128)
+    # About 2 Na Ne N_grain steps: the in-grain share of the frames.
+    approx = 2 * d.num_angles * d.num_energies * d.num_grain_sites
+    assert 0.9 < d.traj_steps / approx < 1.2
