@@ -89,7 +89,8 @@ int sc_set_vector_potential(sc_context *ctx, const double *ax, const double *ay)
 int sc_get_vector_potential(sc_context *ctx, int which, double *ax, double *ay);
 /* Current density of the last compute (ComputeCurrent::getResult). */
 int sc_get_current(sc_context *ctx, double *jx, double *jy);
-/* Free-energy density T Re I(r) summed over angles (ComputeFreeEnergy grid). */
+/* Free-energy density Re I(r) summed over angles: the ComputeFreeEnergy grid
+ * (ComputeFreeEnergy.h:281-300) before its factor T. */
 int sc_get_free_energy_density(sc_context *ctx, double *out);
 /* Sparse boundary storage (BoundaryStorage.h:158-169, layout
  * (crossing * Ne + energy)).  in_out: 0 in, 1 out; dir: 0 gamma, 1 gamma~. */

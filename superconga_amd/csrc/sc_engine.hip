@@ -53,6 +53,9 @@ Grain::Grain(std::unique_ptr<Setup> setup) : s_(std::move(setup)) {
   SC_HIP(hipEventCreate(&ev1_));
   SC_HIP(hipEventCreate(&ev2_));
   upload();
+  // Allocation clears and uploads ran on the null stream, which a
+  // non-blocking stream does not wait for.
+  SC_HIP(hipDeviceSynchronize());
   tim_.trajSteps = s_->trajSteps;
 }
 
@@ -106,7 +109,7 @@ Grain::~Grain() {
   devFree(dPartials_);
   devFree(dWork_);
   devFree(dWork2_);
-  devFree(dKernel_);
+  fft_.reset();
   if (dScratch_) (void)hipFree(dScratch_);
   if (dItems_) (void)hipFree(dItems_);
   devFree(dCounter_);
@@ -178,11 +181,7 @@ void Grain::upload() {
   dPartials_ = devAlloc<double>(size_t(partialsCap_) * 6);
   dWork_ = devAlloc<double>(nf);
   dWork2_ = devAlloc<double>(nf);
-  if (s.lambda > 0.0) {
-    std::vector<double> g;
-    poissonKernel(s.N0, s.h, g);
-    dKernel_ = devUpload(g);
-  }
+  if (s.lambda > 0.0) fft_ = std::make_unique<PoissonFFT>(s.N0, s.h, stream_);
   dCounter_ = devAlloc<int>(1);
 }
 
@@ -644,12 +643,7 @@ void Grain::solveVectorPotential(const double *dj, double *dout) {
                     static_cast<double>(s.numGrain);
   const double my = deviceSum(dj + n0, nullptr, nullptr, static_cast<long long>(n0), 0) /
                     static_cast<double>(s.numGrain);
-  subtractMeanKernel<<<blocksFor(2 * n0), 256, 0, stream_>>>(dj, mx, my,
-                                                             static_cast<long long>(n0), dWork2_);
-  SC_HIP(hipGetLastError());
-  poissonDirectKernel<<<blocksFor(2 * n0), 256, 0, stream_>>>(dWork2_, dKernel_, s.N0,
-                                                              -(s.h * s.h), dout);
-  SC_HIP(hipGetLastError());
+  fft_->solve(dj, mx, my, dout);
 }
 
 void Grain::acceleratorStep() {
@@ -912,8 +906,6 @@ bool Grain::run(int nBurnin, int itMin, int itMax, int precision) {
 void poissonSolve(int n, double h, int numGrain, bool enforce, const double *current,
                   double *out) {
   const size_t n0 = size_t(n) * n;
-  std::vector<double> g;
-  poissonKernel(n, h, g);
   double mx = 0.0, my = 0.0;
   if (enforce) {
     for (size_t i = 0; i < n0; ++i) {
@@ -923,17 +915,26 @@ void poissonSolve(int n, double h, int numGrain, bool enforce, const double *cur
     mx /= numGrain;
     my /= numGrain;
   }
-  double *dj = devUpload(std::vector<double>(current, current + 2 * n0));
-  double *dk = devUpload(g);
-  double *dsrc = devAlloc<double>(2 * n0), *dout = devAlloc<double>(2 * n0);
-  subtractMeanKernel<<<blocksFor(2 * n0), 256>>>(dj, mx, my, static_cast<long long>(n0), dsrc);
-  poissonDirectKernel<<<blocksFor(2 * n0), 256>>>(dsrc, dk, n, -(h * h), dout);
-  SC_HIP(hipGetLastError());
-  SC_HIP(hipMemcpy(out, dout, 2 * n0 * sizeof(double), hipMemcpyDeviceToHost));
+  hipStream_t stream = nullptr;
+  SC_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  double *dj = nullptr, *dout = nullptr;
+  try {
+    PoissonFFT fft(n, h, stream);
+    dj = devAlloc<double>(2 * n0);
+    dout = devAlloc<double>(2 * n0);
+    SC_HIP(hipMemcpyAsync(dj, current, 2 * n0 * sizeof(double), hipMemcpyHostToDevice, stream));
+    fft.solve(dj, mx, my, dout);
+    SC_HIP(hipMemcpyAsync(out, dout, 2 * n0 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    SC_HIP(hipStreamSynchronize(stream));
+  } catch (...) {
+    devFree(dj);
+    devFree(dout);
+    (void)hipStreamDestroy(stream);
+    throw;
+  }
   devFree(dj);
-  devFree(dk);
-  devFree(dsrc);
   devFree(dout);
+  SC_HIP(hipStreamDestroy(stream));
 }
 
 } // namespace scamd

@@ -1,6 +1,7 @@
 // sc_engine.hpp -- one grain on one GPU: device state and the iteration.
 #pragma once
 
+#include "sc_fft.hpp"
 #include "sc_setup.hpp"
 
 #include <hip/hip_runtime.h>
@@ -105,7 +106,7 @@ private:
          *dBOutGt_ = nullptr;
   double *dG_ = nullptr;
   double *dPartials_ = nullptr, *dWork_ = nullptr, *dWork2_ = nullptr;
-  double *dKernel_ = nullptr;
+  std::unique_ptr<PoissonFFT> fft_;
   long long partialsCap_ = 0;
   // sweep
   int el_ = 8, slotsPerCU_ = 4, cols_ = 0, groups_ = 0, maxLen_ = 0,

@@ -1,0 +1,55 @@
+// sc_fft.hpp -- FFT solver of the vector-potential Poisson convolution.
+//
+// ComputeVectorPotential::solve (ComputeVectorPotential.h:647-745) with the
+// exact lattice Green function (vector_potential_error = 0) computes
+//     A(r) = -h^2 sum_r' G(|x - x'|, |y - y'|) (j(r') - <j>_grain)
+// on the N0 x N0 grid, a direct O(N0^4) sum in the reference.  Here the
+// correlation is a product in Fourier space on an M x M periodic grid,
+// M = the smallest power of two >= 2 N0 - 1, so that no wrapped term reaches
+// the N0 x N0 output window.  G is real and even in both directions, so its
+// spectrum is real; jx and jy travel together as one complex field
+// jx + i jy and come back as Ax + i Ay.
+//
+// Kernels (all FP64, radix-2 in LDS, one sequence per row or per group of
+// adjacent columns):
+//   rows forward  : N0 rows of the packed current, zero-padded to M
+//   columns       : forward FFT of each column (first N0 rows non-zero),
+//                   multiply by the kernel spectrum, inverse FFT, keep N0 rows
+//   rows inverse  : N0 rows, scaled and written out as two N0 x N0 planes
+// Work buffer: N0 x M complex doubles.  Kernel spectrum: M x M doubles.
+#ifndef SC_FFT_HPP_
+#define SC_FFT_HPP_
+
+#include <hip/hip_runtime.h>
+
+namespace scamd {
+
+class PoissonFFT {
+public:
+  // Largest transform length: one column of M complex doubles in 64 KiB LDS.
+  static constexpr int kMaxLength = 4096;
+
+  PoissonFFT(int n, double h, hipStream_t stream);
+  ~PoissonFFT();
+  PoissonFFT(const PoissonFFT &) = delete;
+  PoissonFFT &operator=(const PoissonFFT &) = delete;
+
+  // out (2 planes n x n) = -h^2 G * (j - (mx, my)); j is 2 planes n x n.
+  // Device pointers; runs on the stream given at construction.
+  void solve(const double *dj, double mx, double my, double *dout);
+
+  int length() const { return m_; }
+  int n() const { return n_; }
+
+private:
+  int n_, m_, logm_;
+  double h_;
+  hipStream_t stream_;
+  double *khat_ = nullptr;  // M x M real spectrum of the kernel
+  double2 *work_ = nullptr; // max(N0, M) x M complex
+  double2 *tw_ = nullptr;   // exp(-2 pi i k / M), k < M/2
+};
+
+} // namespace scamd
+
+#endif

@@ -607,29 +607,4 @@ __global__ void maskKernel(double *p, const uint8_t *dom, long long n0, int plan
   if (!dom[i % n0]) p[i] = 0.0;
 }
 
-// Direct cross-correlation with the full Green function
-// (ComputeVectorPotential.h:519-563), used to check the FFT path.
-__global__ void poissonDirectKernel(const double *src, const double *g, int N,
-                                    double scale, double *out) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 2 * N * N) return;
-  const int d = t / (N * N), r = t % (N * N), y = r / N, x = r % N;
-  const double *s = src + size_t(d) * N * N;
-  double sum = 0.0;
-  for (int j = 0; j < N; ++j) {
-    double ps = 0.0;
-    const double *grow = g + size_t(abs(j - y)) * N;
-    for (int i = 0; i < N; ++i) ps += s[size_t(j) * N + i] * grow[abs(i - x)];
-    sum += ps;
-  }
-  out[t] = sum * scale;
-}
-// j - mean over the whole grid (ComputeVectorPotential.h:575-616)
-__global__ void subtractMeanKernel(const double *j, double mx, double my,
-                                   long long n0, double *out) {
-  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (i >= 2 * n0) return;
-  out[i] = j[i] - (i < n0 ? mx : my);
-}
-
 } // namespace scamd

@@ -57,8 +57,9 @@ def compare_state(s, o, cfg, tol=TOL, what=""):
         "delta_result": rel(s.order_parameter(1), o.delta(True)),
         "delta_iterate": rel(s.order_parameter(0), o.delta(False)),
         "current": rel(s.current(), o.current()),
-        "free_energy_density": rel(s.free_energy_density(),
-                                   o.array("free_energy_grid", o.N0 ** 2).reshape(o.N0, o.N0)),
+        "free_energy_density": rel(
+            s.free_energy_density(),
+            o.array("free_energy_grid", 2 * o.N0 ** 2)[:o.N0 ** 2].reshape(o.N0, o.N0)),
     }
     if cfg["physics"].get("penetration_depth", -1.0) > 0:
         errs["vector_potential"] = rel(s.vector_potential(1),
