@@ -252,7 +252,7 @@ def test_order_parameter_outside_grain_is_masked(product, oracle):
 # ---------------- full size: the d-wave octagon ----------------
 
 def test_octagon_full_size_invariants(product):
-    cfg = cases.octagon_dwave(lam=-1.0)
+    cfg = cases.octagon_dwave()
     s = product.Simulation(cfg)
     assert (s.N0, s.N1, s.Ne, s.Na) == (930, 1316, 28, 128)
     s.compute("fp64")
