@@ -1493,9 +1493,73 @@ static void zeroOutside(const Oracle &o, double *plane) {
 }
 
 // IntegrationIteratorOzaki.h:303-404 + compute objects.
+// One angle of IntegrationIteratorOzaki::compute (IntegrationIteratorOzaki.h:
+// 300-420): rotate the order parameter, energy shift, Riccati sweep, Green
+// functions, and their integration into the angle sums.  Returns the number
+// of trajectory steps taken.
+static long long integrateAngle(Oracle &o, int k, std::vector<double> &drot,
+                                std::vector<double> &es, std::vector<double> &gD,
+                                std::vector<double> &gJ, std::vector<double> &gF,
+                                std::vector<double> &tmp) {
+  const size_t n0 = o.n0sq(), n1 = o.n1sq();
+  rotatedOrderParameter(o, k, drot);
+  energyShift(o, k, es);
+  const long long steps = riccatiAngle(o, k, drot, es, true);
+  greensAngle(o, drot, gD, gJ, gF);
+  const signed char *lab = o.frames[k].label.data();
+  const double sa = o.systemAngle[k];
+  const float phi = static_cast<float>(sa);
+  const double wk = angleWeight(k, o.Na) / static_cast<double>(o.Na);
+  const double pphi = polarAngle(k, o.Na);
+  const double vx = std::cos(pphi), vy = std::sin(pphi);
+  // ComputeOrderParameter::integrateGreensAngle (ComputeOrderParameter.h:174-210)
+  extendValues(lab, o.N1, gD.data());
+  extendValues(lab, o.N1, gD.data() + n1);
+  for (int c = 0; c < o.Nc; ++c) {
+    std::fill(tmp.begin(), tmp.end(), 0.0);
+    insertRotated(tmp.data(), o.N0, o.N0, gD.data(), o.N1, o.N1, phi, false);
+    insertRotated(tmp.data() + n0, o.N0, o.N0, gD.data() + n1, o.N1, o.N1,
+                  phi, false);
+    const double b = basisMomentum(o, c, k);
+    const double bi = -0.0; // conj of a real basis value
+    double *acc = o.deltaNew[c].data();
+    for (size_t i = 0; i < n0; ++i) {
+      const double re = tmp[i], im = tmp[n0 + i];
+      const double nre = re * b - im * bi, nim = re * bi + im * b;
+      acc[i] += nre * wk;
+      acc[n0 + i] += nim * wk;
+    }
+  }
+  if (o.cfg.compute_current) {
+    // ComputeCurrent::integrateGreensAngle (ComputeCurrent.h:142-173)
+    extendValues(lab, o.N1, gJ.data());
+    extendValues(lab, o.N1, gJ.data() + n1);
+    for (int f = 0; f < 2; ++f) {
+      const double fac = (f == 0 ? vx : vy) * wk;
+      std::vector<double> t2(gJ);
+      for (double &v : t2) v *= fac;
+      insertRotated(o.currentAcc.data() + f * 2 * n0, o.N0, o.N0, t2.data(),
+                    o.N1, o.N1, phi, true);
+      insertRotated(o.currentAcc.data() + f * 2 * n0 + n0, o.N0, o.N0,
+                    t2.data() + n1, o.N1, o.N1, phi, true);
+    }
+  }
+  if (o.cfg.compute_free_energy) {
+    // ComputeFreeEnergy::integrateGreensAngle (ComputeFreeEnergy.h:260-279)
+    extendValues(lab, o.N1, gF.data());
+    extendValues(lab, o.N1, gF.data() + n1);
+    for (double &v : gF) v *= wk;
+    insertRotated(o.feGrid.data(), o.N0, o.N0, gF.data(), o.N1, o.N1, phi,
+                  true);
+    insertRotated(o.feGrid.data() + n0, o.N0, o.N0, gF.data() + n1, o.N1,
+                  o.N1, phi, true);
+  }
+  return steps;
+}
+
 static void integrate(Oracle &o, bool burnin) {
   if (!o.boundaryInit) setBoundaryInitialCondition(o);
-  const size_t n0 = o.n0sq(), n1 = o.n1sq();
+  const size_t n0 = o.n0sq();
   std::vector<double> drot, es, gD, gJ, gF, tmp(2 * n0);
   if (burnin) {
     for (int k = 0; k < o.Na; ++k) {
@@ -1508,60 +1572,7 @@ static void integrate(Oracle &o, bool burnin) {
       std::fill(o.deltaNew[c].begin(), o.deltaNew[c].end(), 0.0);
     std::fill(o.currentAcc.begin(), o.currentAcc.end(), 0.0);
     std::fill(o.feGrid.begin(), o.feGrid.end(), 0.0);
-    for (int k = 0; k < o.Na; ++k) {
-      rotatedOrderParameter(o, k, drot);
-      energyShift(o, k, es);
-      riccatiAngle(o, k, drot, es, true);
-      greensAngle(o, drot, gD, gJ, gF);
-      const signed char *lab = o.frames[k].label.data();
-      const double sa = o.systemAngle[k];
-      const float phi = static_cast<float>(sa);
-      const double wk = angleWeight(k, o.Na) / static_cast<double>(o.Na);
-      const double pphi = polarAngle(k, o.Na);
-      const double vx = std::cos(pphi), vy = std::sin(pphi);
-      // ComputeOrderParameter::integrateGreensAngle (ComputeOrderParameter.h:174-210)
-      extendValues(lab, o.N1, gD.data());
-      extendValues(lab, o.N1, gD.data() + n1);
-      for (int c = 0; c < o.Nc; ++c) {
-        std::fill(tmp.begin(), tmp.end(), 0.0);
-        insertRotated(tmp.data(), o.N0, o.N0, gD.data(), o.N1, o.N1, phi, false);
-        insertRotated(tmp.data() + n0, o.N0, o.N0, gD.data() + n1, o.N1, o.N1,
-                      phi, false);
-        const double b = basisMomentum(o, c, k);
-        const double bi = -0.0; // conj of a real basis value
-        double *acc = o.deltaNew[c].data();
-        for (size_t i = 0; i < n0; ++i) {
-          const double re = tmp[i], im = tmp[n0 + i];
-          const double nre = re * b - im * bi, nim = re * bi + im * b;
-          acc[i] += nre * wk;
-          acc[n0 + i] += nim * wk;
-        }
-      }
-      if (o.cfg.compute_current) {
-        // ComputeCurrent::integrateGreensAngle (ComputeCurrent.h:142-173)
-        extendValues(lab, o.N1, gJ.data());
-        extendValues(lab, o.N1, gJ.data() + n1);
-        for (int f = 0; f < 2; ++f) {
-          const double fac = (f == 0 ? vx : vy) * wk;
-          std::vector<double> t2(gJ);
-          for (double &v : t2) v *= fac;
-          insertRotated(o.currentAcc.data() + f * 2 * n0, o.N0, o.N0, t2.data(),
-                        o.N1, o.N1, phi, true);
-          insertRotated(o.currentAcc.data() + f * 2 * n0 + n0, o.N0, o.N0,
-                        t2.data() + n1, o.N1, o.N1, phi, true);
-        }
-      }
-      if (o.cfg.compute_free_energy) {
-        // ComputeFreeEnergy::integrateGreensAngle (ComputeFreeEnergy.h:260-279)
-        extendValues(lab, o.N1, gF.data());
-        extendValues(lab, o.N1, gF.data() + n1);
-        for (double &v : gF) v *= wk;
-        insertRotated(o.feGrid.data(), o.N0, o.N0, gF.data(), o.N1, o.N1, phi,
-                      true);
-        insertRotated(o.feGrid.data() + n0, o.N0, o.N0, gF.data() + n1, o.N1,
-                      o.N1, phi, true);
-      }
-    }
+    for (int k = 0; k < o.Na; ++k) integrateAngle(o, k, drot, es, gD, gJ, gF, tmp);
     // ComputeOrderParameter::computePropertyFromGreens (:212-262)
     double res = 0.0;
     for (int c = 0; c < o.Nc; ++c) {
@@ -2338,13 +2349,9 @@ int oc_run(void *hh, int nb, int mn, int mx) {
 long long oc_sweep_angles(void *hh, int a0, int a1) {
   Oracle &o = *static_cast<Oracle *>(hh);
   if (!o.boundaryInit) setBoundaryInitialCondition(o);
-  std::vector<double> drot, es;
+  std::vector<double> drot, es, gD, gJ, gF, tmp(2 * o.n0sq());
   long long steps = 0;
-  for (int k = a0; k < a1; ++k) {
-    rotatedOrderParameter(o, k, drot);
-    energyShift(o, k, es);
-    steps += riccatiAngle(o, k, drot, es, true);
-  }
+  for (int k = a0; k < a1; ++k) steps += integrateAngle(o, k, drot, es, gD, gJ, gF, tmp);
   return steps;
 }
 

@@ -135,9 +135,11 @@ int oc_burnin(void *handle);  /* Context::burnIn */
 /* runSimulation(burnin, min, max): returns 1 if converged. */
 int oc_run(void *handle, int num_burnin, int min_iter, int max_iter);
 
-/* Timing helper for the CPU baseline: runs only the Riccati sweeps (both
- * directions) for angles [angle_begin, angle_end) with the current state and
- * returns the number of trajectory steps taken. */
+/* Timing helper for the CPU baseline: the per-angle work of one iteration
+ * (rotation, energy shift, Riccati sweeps in both directions, Green
+ * functions and their integration into the angle sums) for angles
+ * [angle_begin, angle_end) with the current state.  Leaves the angle sums
+ * partial; returns the number of trajectory steps taken. */
 long long oc_sweep_angles(void *handle, int angle_begin, int angle_end);
 
 /* ----- unit-level restatements (closed-form checks) ----- */
