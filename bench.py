@@ -1,0 +1,210 @@
+#!/usr/bin/env python3
+"""Throughput of the self-consistency iteration on one MI355X.
+
+A step is one self-consistency iteration (sc_compute: accelerator, Riccati
+sweeps over every angle and Ozaki energy fused with the Green-function sums,
+angle integration, gap equation, current, free energy, specular boundary
+condition, FFT vector-potential solve) on the d-wave octagon, with every
+input already resident in HBM.  The metric is Riccati trajectory steps per
+second: S = 2 Ne #{sites inside the grain in each frame} summed over frames,
+per iteration, times iterations, over the wall time of the timed iterations.
+
+Prints one JSON line.  Besides the FP64 line it reports the mixed path
+(FP32 stepping, FP64 sums) with its error against the FP64 path, the
+roofline of the dominant kernel (the Riccati sweep, timed per launch with HIP
+events on the stream it runs on) and the CPU oracle timed on a bounded
+sample of the same workload on this host.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload octagon_dwave]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# Algorithmic FP64 flops per trajectory step of the sweep kernel, counted
+# from the arithmetic of the step and of the Green-function sums (square
+# root and reciprocal count one flop each):
+#   midpoint step (RiccatiSolutions.h:65-94): 67 add/mul + 2 sqrt + 2 rcp = 71
+#   Green sums per site and energy (LocalGreens.h:32-70,
+#   ComputeFreeEnergy.h:95-160): 58 add/mul + 1 rcp = 59, shared by the
+#   gamma and gamma~ steps of that site: 29.5 per trajectory step.
+STEP_FLOPS = 71.0
+GREEN_FLOPS_PER_SITE_ENERGY = 59.0
+FLOPS_PER_TRAJ_STEP = STEP_FLOPS + 0.5 * GREEN_FLOPS_PER_SITE_ENERGY
+PEAK_FP64_TFLOPS = 78.6   # MI355X vector FP64
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 (packed)
+PEAK_HBM_GBS = 8000.0
+METRIC = "Riccati traj-steps/s; wall time to 1e-8 residual, mixed/NK vs plain FP64"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def timed_iterations(sim, precision, warmup, steps):
+    for _ in range(warmup):
+        sim.compute(precision)
+    # sc_compute returns after its stream has drained (it reads the residual
+    # back), so each call is a synchronisation point on both sides.
+    sim.results()
+    sweep = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.compute(precision)
+        sweep.append(sim.results()["last_sweep_ms"])
+    t1 = time.perf_counter()
+    return (t1 - t0), np.array(sweep)
+
+
+def roofline(traj_steps, sweep_ms, peak):
+    t = float(np.mean(sweep_ms)) * 1e-3
+    achieved = traj_steps * FLOPS_PER_TRAJ_STEP / t / 1e12
+    return {"bound": "valu", "achieved": round(achieved, 3), "peak": peak,
+            "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
+            "traffic": None, "kernel": "sweepKernel",
+            "flops_per_traj_step": FLOPS_PER_TRAJ_STEP,
+            "traj_steps_per_launch": int(traj_steps),
+            "avg_launch_ms": round(float(np.mean(sweep_ms)), 4)}
+
+
+def load_traffic(workload, precision):
+    """HBM bytes per sweep launch from the committed rocprofv3 counter runs."""
+    path = os.path.join(ROOT, "profiles", "sweep_%s_%s_traffic.json" % (workload, precision))
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def cpu_baseline(cfg, seconds, threads):
+    """The oracle (FP64 OpenMP restatement of the reference's CPU path) on
+    a bounded sample: the per-angle iteration work for consecutive angles
+    until `seconds` of CPU time have been spent."""
+    from oracle import pyoracle
+    c = dict(cfg, num_threads=threads)
+    t = time.perf_counter()
+    o = pyoracle.Oracle(c)
+    o.initial_bulk()
+    o.sweep_angles(0, 1)  # boundary initial condition and first angle, untimed
+    log("cpu oracle ready in %.1fs" % (time.perf_counter() - t))
+    steps, spent, k = 0, 0.0, 1
+    while k < o.Na and (spent < seconds or k < 3):
+        t0 = time.perf_counter()
+        steps += o.sweep_angles(k, k + 1)
+        spent += time.perf_counter() - t0
+        k += 1
+    cpu = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": steps / spent, "unit": "traj-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": "angles 1-%d of %d of the same grain (rotation, energy shift, "
+                      "both Riccati sweeps, Green functions, angle integration; "
+                      "%d traj-steps in %.1f s), FP64, %d OpenMP threads on %s"
+                      % (k - 1, o.Na, steps, spent, threads, cpu or "host CPU")}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="octagon_dwave")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-threads", type=int,
+                    default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+                    or min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    if int(os.environ.get("RANK", "0")) != 0:
+        sys.exit(0)
+
+    import superconga_amd as sa
+    from superconga_amd import workloads
+
+    cfg = workloads.WORKLOADS[args.workload]()
+    t = time.perf_counter()
+    sim = sa.Simulation(cfg)
+    d = sim.dims
+    log("setup %.1fs: N0=%d N1=%d Ne=%d Na=%d grain sites=%d traj-steps/iter=%d"
+        % (time.perf_counter() - t, d.n0, d.n1, d.num_energies, d.num_angles,
+           d.num_grain_sites, d.traj_steps))
+    # First FP64 iteration from the initial state: the reference for the
+    # mixed path's error.
+    sim.compute("fp64")
+    ref = (sim.order_parameter(1), sim.current(), sim.vector_potential(1),
+           sim.results()["free_energy"])
+    wall, sweep = timed_iterations(sim, "fp64", max(args.warmup - 1, 0), args.steps)
+    value = d.traj_steps * args.steps / wall
+    rf = roofline(d.traj_steps, sweep, PEAK_FP64_TFLOPS)
+    rf["traffic"] = load_traffic(args.workload, "fp64")
+    log("fp64: %.2f ms/iter, sweep %.2f ms, %.3e traj-steps/s, %.2f TFLOP/s (%.1f%%)"
+        % (1e3 * wall / args.steps, np.mean(sweep), value, rf["achieved"],
+           100 * rf["frac"]))
+    sim.close()
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "traj-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "n0": d.n0, "n1": d.n1,
+                   "num_energies": d.num_energies, "num_angles": d.num_angles,
+                   "grain_sites": d.num_grain_sites,
+                   "traj_steps_per_iteration": d.traj_steps,
+                   "step": "one self-consistency iteration"},
+        "roofline": rf,
+    }
+
+    if not args.no_mixed:
+        sim = sa.Simulation(cfg)
+        sim.compute("mixed")
+        got = (sim.order_parameter(1), sim.current(), sim.vector_potential(1),
+               sim.results()["free_energy"])
+        err = {"delta": rel(got[0], ref[0]),
+               "current": rel(got[1], ref[1]) if np.any(ref[1]) else float(np.max(np.abs(got[1]))),
+               "vector_potential": rel(got[2], ref[2]) if np.any(ref[2]) else float(np.max(np.abs(got[2]))),
+               "free_energy": abs(got[3] - ref[3]) / abs(ref[3])}
+        mwall, msweep = timed_iterations(sim, "mixed", max(args.warmup - 1, 0), args.steps)
+        mval = d.traj_steps * args.steps / mwall
+        mrf = roofline(d.traj_steps, msweep, PEAK_FP32_TFLOPS)
+        mrf["traffic"] = load_traffic(args.workload, "mixed")
+        log("mixed: %.2f ms/iter, sweep %.2f ms, %.3e traj-steps/s; error vs fp64 %s"
+            % (1e3 * mwall / args.steps, np.mean(msweep), mval, err))
+        out["mixed"] = {"value": mval, "unit": "traj-steps/s",
+                        "ms_per_step": 1e3 * mwall / args.steps,
+                        "dtype": "f32 stepping, f64 sums",
+                        "error_vs_fp64_first_iteration": err, "roofline": mrf}
+        sim.close()
+
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds, args.cpu_threads)
+        log("cpu: %.3e traj-steps/s" % out["cpu_baseline"]["value"])
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1143,7 +1143,11 @@ static long long riccatiAngle(Oracle &o, int k, const std::vector<double> &drot,
       dprev = dl;
     }
   }
-  return 2LL * W * Ne * H;
+  // trajectory steps at the sites inside the grain (label > 0)
+  long long inside = 0;
+  for (int y = y0; y <= y1; ++y)
+    for (int x = x0; x <= x1; ++x) inside += g.label[size_t(y) * N1 + x] > 0 ? 1 : 0;
+  return 2LL * Ne * inside;
 }
 
 // GridFunctions_GPU.h:1121-1189 -- threads per block along x for a 2D

@@ -1,0 +1,81 @@
+"""Named benchmark grains in the simulation_config.json schema.
+
+Each function returns a config dict that both the HIP path (Simulation) and
+the reference's simulation binary (src/simulation.cu:64-463) accept; sizes
+follow Parameters::setLatticeResolution (Parameters.h:422-458).
+"""
+import math
+
+
+def _numerics(angles, ppxi, cutoff, criterion, norm="l2"):
+    return {"convergence_criterion": criterion, "energy_cutoff": cutoff,
+            "norm": norm, "num_fermi_momenta": angles,
+            "points_per_coherence_length": ppxi, "vector_potential_error": 0.0,
+            "num_iterations_burnin": 0, "num_iterations_min": 0,
+            "num_iterations_max": 10000}
+
+
+def octagon_dwave(penetration_depth=5.0, seed=1234, noise=0.01):
+    """d-wave octagon: regular 8-gon of circumradius 25 xi0 (side
+    2 R sin(pi/8) = 19.134 xi0), 20 points per xi0 (N0 = 930, N1 = 1316),
+    d_{x2-y2} with the crystal axes at 45 degrees so that the straight edges
+    are pair-breaking, 128 Fermi-surface angles, T = 0.1 Tc, cutoff 32
+    (28 Ozaki poles), zero external flux with a self-consistent vector
+    potential (penetration depth 5 xi0), initial Delta = bulk d-wave
+    amplitude with 1% seeded noise, Anderson mixing with history 8."""
+    side = 2.0 * 25.0 * math.sin(math.pi / 8.0)
+    return {
+        "physics": {"temperature": 0.1, "external_flux_quanta": 0.0,
+                    "penetration_depth": penetration_depth, "gauge": "symmetric",
+                    "charge_sign": -1, "crystal_axes_rotation": 0.125,
+                    "order_parameter": {"dx2-y2": {
+                        "critical_temperature": 1.0,
+                        "initial_noise_stddev": noise}}},
+        "geometry": [{"regular_polygon": {"num_edges": 8, "side_length": side,
+                                          "rotation": 0.0, "center_x": 0.0,
+                                          "center_y": 0.0, "add": True}}],
+        "numerics": _numerics(128, 20.0, 32.0, 1e-8),
+        "accelerator": {"name": "anderson", "rank": 8, "step_size": 1.0},
+        "misc": {"seed": seed},
+    }
+
+
+def vortex_disc(seed=1234, noise=0.01):
+    """s-wave disc of radius 12.5 xi0 at 20 points per xi0 (N0 = 506),
+    64 angles, T = 0.2 Tc, cutoff 16, 2 flux quanta, penetration depth
+    3 xi0, one vortex at the centre with 1% seeded noise."""
+    return {
+        "physics": {"temperature": 0.2, "external_flux_quanta": 2.0,
+                    "penetration_depth": 3.0, "gauge": "symmetric",
+                    "charge_sign": -1,
+                    "order_parameter": {"s": {
+                        "critical_temperature": 1.0, "initial_noise_stddev": noise,
+                        "vortices": [{"center_x": 0.0, "center_y": 0.0,
+                                      "winding_number": 1}]}}},
+        "geometry": [{"disc": {"radius": 12.5, "center_x": 0.0, "center_y": 0.0,
+                               "add": True}}],
+        "numerics": _numerics(64, 20.0, 16.0, 1e-8),
+        "accelerator": {"name": "congacc"},
+        "misc": {"seed": seed},
+    }
+
+
+def swave_disc(seed=1234, noise=0.01):
+    """s-wave disc of radius 10 xi0 at 10 points per xi0 (N0 = 206), 32
+    angles, T = 0.5 Tc, cutoff 16, no field, Picard, L2 criterion 1e-6."""
+    return {
+        "physics": {"temperature": 0.5, "external_flux_quanta": 0.0,
+                    "penetration_depth": -1.0, "gauge": "symmetric",
+                    "charge_sign": -1,
+                    "order_parameter": {"s": {
+                        "critical_temperature": 1.0, "initial_noise_stddev": noise}}},
+        "geometry": [{"disc": {"radius": 10.0, "center_x": 0.0, "center_y": 0.0,
+                               "add": True}}],
+        "numerics": _numerics(32, 10.0, 16.0, 1e-6),
+        "accelerator": {"name": "picard", "step_size": 1.0},
+        "misc": {"seed": seed},
+    }
+
+
+WORKLOADS = {"octagon_dwave": octagon_dwave, "vortex_disc": vortex_disc,
+             "swave_disc": swave_disc}
