@@ -262,6 +262,7 @@ int sc_riccati_sweep(sc_context *ctx, int precision, int burnin) {
     checkPrecision(precision);
     ctx->grain->sweep(precision, burnin != 0);
     hipCheck(hipStreamSynchronize(ctx->grain->stream()), "sweep");
+    ctx->grain->updateSweepTime();
     return SC_OK;
   });
 }

@@ -629,6 +629,11 @@ void Grain::integrate(int precision, bool burnin) {
     }
   }
   boundaryCondition();
+  updateSweepTime();
+}
+
+void Grain::updateSweepTime() {
+  SC_HIP(hipEventSynchronize(ev1_));
   float ms = 0.f;
   SC_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
   tim_.sweepMs = ms;

@@ -54,6 +54,8 @@ public:
   void burnin(int precision);
   bool run(int burnin, int itMin, int itMax, int precision);
   // individual stages
+  // Reads the HIP-event time of the last sweep launch into the timing.
+  void updateSweepTime();
   void sweep(int precision, bool burnin);
   void boundaryCondition();
   void solveVectorPotential(const double *dCurrent, double *dOut);
