@@ -118,9 +118,19 @@ int sc_get_results(sc_context *ctx, sc_results *out);
 int sc_riccati_sweep(sc_context *ctx, int precision, int burnin);
 /* BoundaryConditionSpecular::computeBoundaryCondition (BoundaryConditionSpecular.h:361-486). */
 int sc_boundary_condition(sc_context *ctx, double *residual);
-/* Lane layout of the sweep: Ozaki energies per lane (4, 8 or 16) and
- * persistent wavefronts per compute unit (the autotuned shape). */
-int sc_set_sweep_shape(sc_context *ctx, int energies_per_lane, int slots_per_cu);
+/* Lane layout of the sweep for one precision (no reference counterpart: the
+ * reference's shape rule is getKernelDimensions, GridFunctions_GPU.h:1121-1189):
+ * Ozaki energies per lane, the register budget the kernel variant is compiled
+ * for (wavefronts per SIMD) and persistent wavefronts per compute unit.
+ * Compiled variants (energies, waves): (4,2) (4,4) (7,1) (7,2) (7,4) (8,1)
+ * (8,2) (8,4) (14,1) (14,2) (16,1) (16,2). */
+int sc_set_sweep_shape(sc_context *ctx, int precision, int energies_per_lane,
+                       int waves_per_simd, int slots_per_cu);
+int sc_get_sweep_shape(sc_context *ctx, int precision, int *energies_per_lane,
+                       int *waves_per_simd, int *slots_per_cu);
+/* Autotuner: times every variant on the current state and keeps the fastest
+ * for `precision`.  The sweeps it runs do not change the iteration state. */
+int sc_autotune_sweep(sc_context *ctx, int precision);
 
 /* ComputeVectorPotential with vector_potential_error = 0
  * (ComputeVectorPotential.h:143-201,647-745): A = -h^2 G * (j - <j>_grain).

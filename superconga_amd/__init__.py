@@ -86,7 +86,11 @@ def lib():
         L.sc_get_results.argtypes = [vp, ctypes.POINTER(Results)]
         L.sc_riccati_sweep.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.sc_boundary_condition.argtypes = [vp, _DP]
-        L.sc_set_sweep_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.sc_set_sweep_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int]
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.sc_get_sweep_shape.argtypes = [vp, ctypes.c_int, ip, ip, ip]
+        L.sc_autotune_sweep.argtypes = [vp, ctypes.c_int]
         L.sc_poisson_solve.argtypes = [ctypes.c_int, ctypes.c_double,
                                        ctypes.c_int, ctypes.c_int, _DP, _DP]
         L.sc_riccati_steps.argtypes = [ctypes.c_int, _DP, _DP, ctypes.c_int]
@@ -237,8 +241,21 @@ class Simulation:
         _check(lib().sc_boundary_condition(self._h, ctypes.byref(r)))
         return r.value
 
-    def set_sweep_shape(self, energies_per_lane, slots_per_cu):
-        _check(lib().sc_set_sweep_shape(self._h, energies_per_lane, slots_per_cu))
+    def set_sweep_shape(self, energies_per_lane, waves_per_simd, slots_per_cu,
+                        precision="fp64"):
+        _check(lib().sc_set_sweep_shape(self._h, _precision(precision), energies_per_lane,
+                                        waves_per_simd, slots_per_cu))
+
+    def sweep_shape(self, precision="fp64"):
+        e, w, s = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().sc_get_sweep_shape(self._h, _precision(precision), ctypes.byref(e),
+                                        ctypes.byref(w), ctypes.byref(s)))
+        return e.value, w.value, s.value
+
+    def autotune(self, precision="fp64"):
+        """Picks the fastest sweep variant for this grain and precision."""
+        _check(lib().sc_autotune_sweep(self._h, _precision(precision)))
+        return self.sweep_shape(precision)
 
 
 def describe(config, angle=None):

@@ -274,10 +274,31 @@ int sc_boundary_condition(sc_context *ctx, double *residual) {
     return SC_OK;
   });
 }
-int sc_set_sweep_shape(sc_context *ctx, int el, int slots) {
+int sc_set_sweep_shape(sc_context *ctx, int precision, int el, int waves, int slots) {
   SC_CHECK_CTX(ctx);
   return guarded([&] {
-    ctx->grain->setSweepShape(el, slots);
+    checkPrecision(precision);
+    ctx->grain->setSweepShape(precision, el, waves, slots);
+    return SC_OK;
+  });
+}
+int sc_get_sweep_shape(sc_context *ctx, int precision, int *el, int *waves, int *slots) {
+  SC_CHECK_CTX(ctx);
+  SC_CHECK_PTR(el);
+  SC_CHECK_PTR(waves);
+  SC_CHECK_PTR(slots);
+  return guarded([&] {
+    checkPrecision(precision);
+    ctx->grain->sweepShape(precision, *el, *waves, *slots);
+    return SC_OK;
+  });
+}
+int sc_autotune_sweep(sc_context *ctx, int precision) {
+  SC_CHECK_CTX(ctx);
+  return guarded([&] {
+    checkPrecision(precision);
+    ctx->grain->autotune(precision);
+    hipCheck(hipStreamSynchronize(ctx->grain->stream()), "autotune");
     return SC_OK;
   });
 }

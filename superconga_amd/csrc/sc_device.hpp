@@ -6,7 +6,9 @@
 
 namespace scamd {
 
-template <typename T> struct Cx {
+// Complex value aligned to its full size so that it moves as one vector
+// access.
+template <typename T> struct alignas(2 * sizeof(T)) Cx {
   T re, im;
 };
 
@@ -26,29 +28,54 @@ template <typename T> __device__ __forceinline__ Cx<T> scale(Cx<T> a, T s) {
   return {a.re * s, a.im * s};
 }
 
-// Square root and reciprocal: FP64 uses the IEEE operations; FP32 uses the
-// hardware v_sqrt_f32 / v_rcp_f32 (1 ulp).
-__device__ __forceinline__ double dsqrt(double x) { return __builtin_sqrt(x); }
-__device__ __forceinline__ float dsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
-__device__ __forceinline__ double drcp(double x) { return 1.0 / x; }
-__device__ __forceinline__ float drcp(float x) { return __builtin_amdgcn_rcpf(x); }
+// Reciprocal and reciprocal square root.  FP32: the hardware v_rcp_f32 /
+// v_rsq_f32 (1 ulp).  FP64: the hardware v_rcp_f64 / v_rsq_f64 estimate
+// refined by kNewtonSteps Newton-Raphson iterations (each doubles the number
+// of correct bits of the roughly single-precision estimate; two reach the
+// last bits of the double), instead of the longer IEEE division and square
+// root sequences.
+constexpr int kNewtonSteps = 2;
+__device__ __forceinline__ float rcpT(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float rsqT(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double rcpT(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+#pragma unroll
+  for (int i = 0; i < kNewtonSteps; ++i) r = fma(r, fma(-x, r, 1.0), r);
+  return r;
+}
+__device__ __forceinline__ double rsqT(double x) {
+  double r = __builtin_amdgcn_rsq(x);
+#pragma unroll
+  for (int i = 0; i < kNewtonSteps; ++i) {
+    const double e = fma(-(x * r), r, 1.0); // 1 - x r^2
+    r = fma(0.5 * r, e, r);
+  }
+  return r;
+}
+// Kept for the host-visible unit checks and the bulk solution.
+__device__ __forceinline__ double drcp(double x) { return rcpT(x); }
+__device__ __forceinline__ float drcp(float x) { return rcpT(x); }
 
 // Principal complex square root (Kahan's formulation): with
 // t = sqrt((|x| + |z|) / 2), sqrt(z) = (t, y / 2t) for x >= 0 and
-// (|y| / 2t, copysign(t, y)) otherwise.
+// (|y| / 2t, copysign(t, y)) otherwise.  Both square roots come from
+// reciprocal square roots: |z| = n rsq(n) with n = |z|^2, and with
+// u = (|x| + |z|) / 2, t = u rsq(u) and y / 2t = y rsq(u) / 2.
 template <typename T> __device__ __forceinline__ Cx<T> csqrt(Cx<T> z) {
-  const T ax = fabs(z.re);
-  const T mod = dsqrt(z.re * z.re + z.im * z.im);
-  const T t = dsqrt(T(0.5) * (ax + mod));
-  if (t == T(0)) return {T(0), z.im};
-  const T q = z.im * drcp(T(2) * t);
+  const T n = z.re * z.re + z.im * z.im;
+  if (n == T(0)) return {T(0), z.im};
+  const T m = n * rsqT(n);
+  const T u = T(0.5) * (fabs(z.re) + m);
+  const T r = rsqT(u);
+  const T t = u * r;
+  const T q = (T(0.5) * z.im) * r;
   if (z.re >= T(0)) return {t, q};
   return {fabs(q), copysign(t, z.im)};
 }
 
 // a / b through the reciprocal of |b|^2 (|b| is of order one here).
 template <typename T> __device__ __forceinline__ Cx<T> cdiv(Cx<T> a, Cx<T> b) {
-  const T r = drcp(b.re * b.re + b.im * b.im);
+  const T r = rcpT(b.re * b.re + b.im * b.im);
   return {(a.re * b.re + a.im * b.im) * r, (a.im * b.re - a.re * b.im) * r};
 }
 
@@ -56,19 +83,38 @@ template <typename T> __device__ __forceinline__ Cx<T> cdiv(Cx<T> a, Cx<T> b) {
 // a y^2 + b y + c = 0 (RiccatiSolutions.h:65-94):
 //   s = i h, a = s conj(D) / 4, b = 2 a y0 + s e - 1,
 //   c = (a y0 + s e + 1) y0 + s D,  y = 2 c / (-b + sqrt(b^2 - 4 a c)).
+// The quantities that only depend on the site (D, the energy shift Re e and
+// h) are formed once per site and shared by every Ozaki energy.
+template <typename T> struct StepSite {
+  Cx<T> a;  // s conj(D) / 4
+  Cx<T> a4; // 4 a
+  Cx<T> sD; // s D = (-4 a.re, 4 a.im)
+  T sei;    // Im(s e) = h Re e
+};
+template <typename T>
+__device__ __forceinline__ StepSite<T> stepSite(Cx<T> D, T shift, T h) {
+  StepSite<T> p;
+  p.a4 = {h * D.im, h * D.re};
+  p.a = {T(0.25) * p.a4.re, T(0.25) * p.a4.im};
+  p.sD = {-p.a4.re, p.a4.im};
+  p.sei = h * shift;
+  return p;
+}
+// ser = Re(s e) = -h Im e (the Matsubara part).
+template <typename T>
+__device__ __forceinline__ Cx<T> stepEnergy(const StepSite<T> &p, T ser, Cx<T> y0) {
+  const Cx<T> ay = p.a * y0;
+  const Cx<T> b = {T(2) * ay.re + (ser - T(1)), T(2) * ay.im + p.sei};
+  const Cx<T> t = {ay.re + (ser + T(1)), ay.im + p.sei};
+  const Cx<T> c = t * y0 + p.sD;
+  const Cx<T> ac4 = p.a4 * c;
+  const Cx<T> disc = {b.re * b.re - b.im * b.im - ac4.re, T(2) * b.re * b.im - ac4.im};
+  const Cx<T> sq = csqrt(disc);
+  return cdiv(c + c, sq - b);
+}
 template <typename T>
 __device__ __forceinline__ Cx<T> riccatiStep(Cx<T> D, Cx<T> e, Cx<T> y0, T h) {
-  const T qh = T(0.25) * h;
-  const Cx<T> a = {qh * D.im, qh * D.re};
-  const Cx<T> se = {-h * e.im, h * e.re};
-  const Cx<T> ay = a * y0;
-  const Cx<T> b = {T(2) * ay.re + se.re - T(1), T(2) * ay.im + se.im};
-  const Cx<T> t = {ay.re + se.re + T(1), ay.im + se.im};
-  const Cx<T> c = t * y0 + Cx<T>{-h * D.im, h * D.re};
-  const Cx<T> bb = b * b;
-  const Cx<T> ac = a * c;
-  const Cx<T> disc = csqrt(Cx<T>{bb.re - T(4) * ac.re, bb.im - T(4) * ac.im});
-  return cdiv(Cx<T>{T(2) * c.re, T(2) * c.im}, Cx<T>{disc.re - b.re, disc.im - b.im});
+  return stepEnergy(stepSite(D, e.re, h), -h * e.im, y0);
 }
 
 // Homogeneous (bulk) solutions (RiccatiSolutions.h:36-54).

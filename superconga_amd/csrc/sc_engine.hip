@@ -111,7 +111,8 @@ Grain::~Grain() {
   devFree(dWork2_);
   fft_.reset();
   if (dScratch_) (void)hipFree(dScratch_);
-  if (dItems_) (void)hipFree(dItems_);
+  for (SweepPlan &pl : plan_)
+    if (pl.dItems) (void)hipFree(pl.dItems);
   devFree(dCounter_);
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
@@ -185,48 +186,79 @@ void Grain::upload() {
   dCounter_ = devAlloc<int>(1);
 }
 
-void Grain::setSweepShape(int el, int slotsPerCU) {
-  if (el != 4 && el != 8 && el != 16)
-    throw std::runtime_error("energies per lane must be 4, 8 or 16");
-  if (slotsPerCU < 1 || slotsPerCU > 32)
-    throw std::runtime_error("slots per CU must be in [1, 32]");
-  if (el != el_ || slotsPerCU != slotsPerCU_) sweepReady_ = false;
-  el_ = el;
-  slotsPerCU_ = slotsPerCU;
+// Compiled sweep variants: (energies per lane, wavefronts per SIMD).
+#define SC_SWEEP_VARIANTS(X)                                                         \
+  X(4, 2) X(4, 4) X(7, 1) X(7, 2) X(7, 4) X(8, 1) X(8, 2) X(8, 4) X(14, 1) X(14, 2)   \
+  X(16, 1) X(16, 2)
+
+static bool sweepVariantExists(int el, int waves) {
+#define SC_HAS(E, W) if (el == E && waves == W) return true;
+  SC_SWEEP_VARIANTS(SC_HAS)
+#undef SC_HAS
+  return false;
 }
 
-void Grain::prepareSweep() {
-  if (sweepReady_) return;
+void Grain::setSweepShape(int precision, int el, int waves, int slotsPerCU) {
+  if (precision != PREC_FP64 && precision != PREC_MIXED)
+    throw std::runtime_error("unknown precision");
+  if (!sweepVariantExists(el, waves))
+    throw std::runtime_error("no sweep variant with " + std::to_string(el) +
+                             " energies per lane and " + std::to_string(waves) +
+                             " wavefronts per SIMD");
+  if (slotsPerCU < 1 || slotsPerCU > 32)
+    throw std::runtime_error("slots per CU must be in [1, 32]");
+  SweepPlan &pl = plan_[precision];
+  if (el != pl.el || waves != pl.waves || slotsPerCU != pl.slotsPerCU) pl.ready = false;
+  pl.el = el;
+  pl.waves = waves;
+  pl.slotsPerCU = slotsPerCU;
+}
+
+void Grain::sweepShape(int precision, int &el, int &waves, int &slotsPerCU) const {
+  const SweepPlan &pl = plan_[precision == PREC_FP64 ? 0 : 1];
+  el = pl.el;
+  waves = pl.waves;
+  slotsPerCU = pl.slotsPerCU;
+}
+
+void Grain::prepareSweep(int precision) {
+  SweepPlan &pl = plan_[precision];
+  if (pl.ready) return;
   const Setup &s = *s_;
-  groups_ = (s.Ne + el_ - 1) / el_;
+  pl.groups = (s.Ne + pl.el - 1) / pl.el;
   int gp = 1;
-  while (gp < groups_) gp <<= 1;
+  while (gp < pl.groups) gp <<= 1;
   if (gp > 64)
     throw std::runtime_error("too many Ozaki energies for the lane layout; "
                              "raise the energies per lane");
-  cols_ = 64 / gp;
-  std::vector<SweepItem> items = sweepItems(s, cols_, &maxLen_);
-  numItems_ = static_cast<int>(items.size());
+  pl.cols = 64 / gp;
+  std::vector<SweepItem> items = sweepItems(s, pl.cols, &pl.maxLen);
+  pl.numItems = static_cast<int>(items.size());
   std::vector<SweepItemDev> di(items.size());
   for (size_t i = 0; i < items.size(); ++i)
     di[i] = {items[i].angle, items[i].x0, items[i].ncols, items[i].ylo, items[i].yhi};
-  if (dItems_) (void)hipFree(dItems_);
-  SC_HIP(hipMalloc(&dItems_, std::max<size_t>(1, di.size()) * sizeof(SweepItemDev)));
+  if (pl.dItems) (void)hipFree(pl.dItems);
+  pl.dItems = nullptr;
+  SC_HIP(hipMalloc(&pl.dItems, std::max<size_t>(1, di.size()) * sizeof(SweepItemDev)));
   if (!di.empty())
-    SC_HIP(hipMemcpy(dItems_, di.data(), di.size() * sizeof(SweepItemDev),
+    SC_HIP(hipMemcpy(pl.dItems, di.data(), di.size() * sizeof(SweepItemDev),
                      hipMemcpyHostToDevice));
   int dev = 0, cus = 0;
   SC_HIP(hipGetDevice(&dev));
   SC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  numSlots_ = std::max(1, std::min(numItems_, slotsPerCU_ * std::max(cus, 1)));
-  const size_t stride = size_t(maxLen_) * el_ * 64; // complex elements
-  const size_t bytes = size_t(numSlots_) * stride * sizeof(Cx<double>);
+  pl.numSlots = std::max(1, std::min(pl.numItems, pl.slotsPerCU * std::max(cus, 1)));
+  const size_t elemBytes = precision == PREC_FP64 ? sizeof(Cx<double>) : sizeof(Cx<float>);
+  const size_t stride = size_t(pl.maxLen) * pl.el * 64; // complex elements per slot
+  const size_t bytes = size_t(pl.numSlots) * stride * elemBytes;
   if (bytes > scratchBytes_) {
+    SC_HIP(hipStreamSynchronize(stream_));
     if (dScratch_) (void)hipFree(dScratch_);
+    dScratch_ = nullptr;
+    scratchBytes_ = 0;
     SC_HIP(hipMalloc(&dScratch_, bytes));
     scratchBytes_ = bytes;
   }
-  sweepReady_ = true;
+  pl.ready = true;
 }
 
 static GrainView makeView(const Setup &s, const float *trig, const int *idx,
@@ -488,36 +520,79 @@ void Grain::ensureBoundaryInit() {
   boundaryInit_ = true;
 }
 
-template <typename T, int EL> static void launchSweep(const SweepArgs &a, int slots,
-                                                      hipStream_t st) {
-  sweepKernel<T, EL><<<slots, 64, 0, st>>>(a);
+template <typename T, int EL, int W>
+static void launchSweep(const SweepArgs &a, int slots, hipStream_t st) {
+  sweepKernel<T, EL, W><<<slots, 64, 0, st>>>(a);
 }
 
 void Grain::sweep(int precision, bool burnin) {
-  prepareSweep();
+  prepareSweep(precision);
+  const SweepPlan &pl = plan_[precision];
   const Setup &s = *s_;
   SweepArgs a;
   a.v = makeView(s, dTrig_, dIdx_, dDist_, dBInG_, dBOutG_, dBInGt_, dBOutGt_,
                  dExtDelta_, dA_, dG_, dEnergy_, dResidue_, dSaCos_, dSaSin_, dVFx_,
                  dVFy_, dWeight_, dBasis_, dLabels_);
-  a.items = reinterpret_cast<const SweepItemDev *>(dItems_);
-  a.numItems = numItems_;
-  a.cols = cols_;
-  a.groups = groups_;
+  a.items = reinterpret_cast<const SweepItemDev *>(pl.dItems);
+  a.numItems = pl.numItems;
+  a.cols = pl.cols;
+  a.groups = pl.groups;
   a.counter = dCounter_;
   a.scratch = dScratch_;
-  a.slotStride = static_cast<long long>(maxLen_) * el_ * 64;
+  a.slotStride = static_cast<long long>(pl.maxLen) * pl.el * 64;
   a.burnin = burnin ? 1 : 0;
   resetCounter<<<1, 1, 0, stream_>>>(dCounter_);
   SC_HIP(hipEventRecord(ev0_, stream_));
   const bool dp = precision == PREC_FP64;
-  switch (el_) {
-  case 4: dp ? launchSweep<double, 4>(a, numSlots_, stream_) : launchSweep<float, 4>(a, numSlots_, stream_); break;
-  case 8: dp ? launchSweep<double, 8>(a, numSlots_, stream_) : launchSweep<float, 8>(a, numSlots_, stream_); break;
-  default: dp ? launchSweep<double, 16>(a, numSlots_, stream_) : launchSweep<float, 16>(a, numSlots_, stream_); break;
-  }
+#define SC_LAUNCH(E, W)                                                              \
+  if (pl.el == E && pl.waves == W) {                                                 \
+    if (dp) launchSweep<double, E, W>(a, pl.numSlots, stream_);                      \
+    else launchSweep<float, E, W>(a, pl.numSlots, stream_);                          \
+  } else
+  SC_SWEEP_VARIANTS(SC_LAUNCH)
+#undef SC_LAUNCH
+  throw std::runtime_error("sweep variant not compiled");
   SC_HIP(hipGetLastError());
   SC_HIP(hipEventRecord(ev1_, stream_));
+}
+
+void Grain::autotune(int precision) {
+  if (precision != PREC_FP64 && precision != PREC_MIXED)
+    throw std::runtime_error("unknown precision");
+  ensureBoundaryInit();
+  extendDelta();
+  SweepPlan &pl = plan_[precision];
+  const int keepEl = pl.el, keepW = pl.waves, keepSlots = pl.slotsPerCU;
+  float best = -1.f;
+  int bestEl = keepEl, bestW = keepW, bestSlots = keepSlots;
+  const int slotChoices[] = {4, 8, 16};
+  auto tryShape = [&](int el, int w, int slots) {
+    const int groups = (s_->Ne + el - 1) / el;
+    int gp = 1;
+    while (gp < groups) gp <<= 1;
+    if (gp > 64) return;
+    // skip layouts that leave more than a quarter of the energy slots idle
+    if (4 * (gp * el - s_->Ne) > gp * el) return;
+    setSweepShape(precision, el, w, slots);
+    sweep(precision, false);
+    updateSweepTime();
+    const float ms = static_cast<float>(tim_.sweepMs);
+    if (best < 0.f || ms < best) {
+      best = ms;
+      bestEl = el;
+      bestW = w;
+      bestSlots = slots;
+    }
+  };
+#define SC_TRY(E, W)                                                                 \
+  for (int slots : slotChoices)                                                       \
+    if (slots >= 4 * W || slots == 16) tryShape(E, W, slots);
+  SC_SWEEP_VARIANTS(SC_TRY)
+#undef SC_TRY
+  setSweepShape(precision, bestEl, bestW, bestSlots);
+  (void)keepEl;
+  (void)keepW;
+  (void)keepSlots;
 }
 
 void Grain::boundaryCondition() {

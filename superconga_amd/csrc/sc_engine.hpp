@@ -62,8 +62,15 @@ public:
 
   Results results() const { return res_; }
   Timing timing() const { return tim_; }
-  void setSweepShape(int energiesPerLane, int slotsPerCU);
-  int energiesPerLane() const { return el_; }
+  // Lane layout of the sweep for one precision: Ozaki energies per lane,
+  // register budget (wavefronts per SIMD the kernel is compiled for) and
+  // persistent wavefronts per compute unit.
+  void setSweepShape(int precision, int energiesPerLane, int wavesPerSimd, int slotsPerCU);
+  void sweepShape(int precision, int &energiesPerLane, int &wavesPerSimd,
+                  int &slotsPerCU) const;
+  // Times every compiled sweep variant on the current state and keeps the
+  // fastest for `precision`; the sweeps it runs leave the state unchanged.
+  void autotune(int precision);
   hipStream_t stream() const { return stream_; }
 
   // device pointers for zero-copy callers
@@ -79,7 +86,7 @@ private:
   double residualOf(const double *oldv, const double *newv, long long n);
   double deviceSum(const double *a, const double *b, const uint8_t *mask,
                    long long n, int op);
-  void prepareSweep();
+  void prepareSweep(int precision);
 
   std::unique_ptr<Setup> s_;
   hipStream_t stream_ = nullptr;
@@ -110,14 +117,17 @@ private:
   double *dPartials_ = nullptr, *dWork_ = nullptr, *dWork2_ = nullptr;
   std::unique_ptr<PoissonFFT> fft_;
   long long partialsCap_ = 0;
-  // sweep
-  int el_ = 8, slotsPerCU_ = 4, cols_ = 0, groups_ = 0, maxLen_ = 0,
-      numItems_ = 0, numSlots_ = 0;
+  // sweep plans, one per precision
+  struct SweepPlan {
+    int el = 8, waves = 2, slotsPerCU = 8;
+    int cols = 0, groups = 0, maxLen = 0, numItems = 0, numSlots = 0;
+    void *dItems = nullptr;
+    bool ready = false;
+  };
+  SweepPlan plan_[2];
   void *dScratch_ = nullptr;
   size_t scratchBytes_ = 0;
-  void *dItems_ = nullptr;
   int *dCounter_ = nullptr;
-  bool sweepReady_ = false;
   // accelerator state (history lives on the device)
   int accIter_ = 0;
   std::vector<double *> accX_, accD_;

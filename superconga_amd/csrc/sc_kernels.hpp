@@ -100,85 +100,92 @@ __device__ __forceinline__ void siteFields(const GrainView &v, int k, int x,
 __global__ void resetCounter(int *c) { *c = 0; }
 
 // The Riccati sweep.  One wavefront handles an item: `cols` adjacent columns
-// of one frame, each column split over `groups` lanes of EL Ozaki energies.
-// Pass 1 integrates gamma~ from the top of the item to the bottom and keeps
-// it in the slot's scratch; pass 2 integrates gamma upwards and forms the
-// energy-summed Green functions at every inside site.  Boundary in-values are
-// read and out-values written as in RiccatiSolverConfined.h:129-312.
+// of one frame; each column is carried by 64 / cols lanes, of which `groups`
+// hold EL Ozaki energies each.  Pass 1 integrates gamma~ from the top of the
+// item to the bottom and keeps it in the slot's scratch; pass 2 integrates
+// gamma upwards and forms the energy-summed Green functions at every inside
+// site.  Boundary in-values are read and out-values written as in
+// RiccatiSolverConfined.h:129-312.
+//
+// The order parameter and Doppler shift of a row are computed once per
+// column: the lanes of a column take consecutive rows of a chunk and pass
+// the values round with __shfl.
 template <typename T, int EL>
-__global__ __launch_bounds__(64) void sweepKernel(SweepArgs a) {
+__device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev &it,
+                                          Cx<T> *scr) {
   const GrainView &v = a.v;
   const int lane = threadIdx.x;
-  const int g = lane / a.cols;
-  const int c = lane - g * a.cols;
+  const int cols = a.cols;
+  const int gp = 64 / cols; // lanes per column
+  const int g = lane / cols;
+  const int c = lane - g * cols;
   const int N1 = v.N1, Ne = v.Ne;
   const size_t n1 = size_t(N1) * N1;
-  Cx<T> *scr = reinterpret_cast<Cx<T> *>(a.scratch) + size_t(blockIdx.x) * a.slotStride;
   const T h = T(v.h);
-
-  for (;;) {
-    int item = 0;
-    if (lane == 0) item = atomicAdd(a.counter, 1);
-    item = __shfl(item, 0);
-    if (item >= a.numItems) break;
-    const SweepItemDev it = a.items[item];
-    const int k = it.angle;
-    const int x = it.x0 + (c < it.ncols ? c : it.ncols - 1);
-    const bool active = (c < it.ncols) && (g < a.groups);
-    const int8_t *lab = v.labels + size_t(k) * n1 + x;
-    T eIm[EL], res[EL];
-    bool eOk[EL];
+  const int k = it.angle;
+  const bool colActive = c < it.ncols;
+  const int x = it.x0 + (colActive ? c : it.ncols - 1);
+  const bool active = colActive && (g < a.groups);
+  const int8_t *lab = v.labels + size_t(k) * n1 + x;
+  T ser[EL], res[EL];
+  bool eOk[EL];
 #pragma unroll
-    for (int j = 0; j < EL; ++j) {
-      const int e = g * EL + j;
-      eOk[j] = active && e < Ne;
-      eIm[j] = eOk[j] ? T(v.energy[e]) : T(0);
-      res[j] = eOk[j] ? T(v.residue[e]) : T(0);
-    }
-    const int len = it.yhi - it.ylo + 1;
-    const size_t col = size_t(k) * N1 + x;
+  for (int j = 0; j < EL; ++j) {
+    const int e = g * EL + j;
+    eOk[j] = active && e < Ne;
+    ser[j] = eOk[j] ? -h * T(v.energy[e]) : T(0);
+    res[j] = eOk[j] ? T(v.residue[e]) : T(0);
+  }
+  const int len = it.yhi - it.ylo + 1;
+  const size_t col = size_t(k) * N1 + x;
 
-    // ---------------- pass 1: gamma~ along -y ----------------
-    {
-      const int baseIn = v.idxInGt[col], baseOut = v.idxOutGt[col];
-      int cntIn = 0, cntOut = 0;
-      Cx<T> coh[EL];
+  // ---------------- pass 1: gamma~ along -y ----------------
+  {
+    const int baseIn = v.idxInGt[col], baseOut = v.idxOutGt[col];
+    int cntIn = 0, cntOut = 0;
+    Cx<T> coh[EL];
 #pragma unroll
-      for (int j = 0; j < EL; ++j) coh[j] = {T(0), T(0)};
-      Cx<T> dPrev = {T(0), T(0)};
-      T esPrev = T(0);
-      bool havePrev = false;
-      int8_t lcur = lab[size_t(it.yhi) * N1];
-      for (int t = 0; t < len; ++t) {
+    for (int j = 0; j < EL; ++j) coh[j] = {T(0), T(0)};
+    Cx<T> dPrev = {T(0), T(0)};
+    T esPrev = T(0);
+    bool havePrev = false;
+    int8_t lcur = lab[size_t(it.yhi) * N1];
+    for (int t0 = 0; t0 < len; t0 += gp) {
+      double fre = 0.0, fim = 0.0, fes = 0.0;
+      {
+        const int tt = t0 + g;
+        if (colActive && tt < len) {
+          const int y = it.yhi - tt;
+          const int8_t l0 = lab[size_t(y) * N1];
+          const int8_t l1 = (tt + 1 < len) ? lab[size_t(y - 1) * N1] : int8_t(0);
+          if (l0 != 0 || l1 != 0) siteFields(v, k, x, y, fre, fim, fes);
+        }
+      }
+      const int tEnd = min(gp, len - t0);
+      for (int i = 0; i < tEnd; ++i) {
+        const int src = i * cols + c;
+        const double dre = __shfl(fre, src), dim = __shfl(fim, src), e = __shfl(fes, src);
+        const int t = t0 + i;
         const int y = it.yhi - t;
         const int8_t lnext = (t + 1 < len) ? lab[size_t(y - 1) * N1] : int8_t(0);
-        Cx<T> D = {T(0), T(0)};
-        T es = T(0);
-        if (active && (lcur != 0 || lnext != 0)) {
-          double dre, dim, e;
-          siteFields(v, k, x, y, dre, dim, e);
-          D = {T(-dre), T(dim)}; // -conj(Delta) for gamma~
-          es = T(e);
-        }
+        const Cx<T> D = {T(-dre), T(dim)}; // -conj(Delta) for gamma~
+        const T es = T(e);
         if (active) {
-          const T hp = havePrev ? T(1) : T(0);
-          const Cx<T> Dm = {(dPrev.re + D.re) * T(0.5), (dPrev.im + D.im) * T(0.5)};
-          const T esm = (esPrev + es) * T(0.5);
           if (lcur == 1 || lcur == 2) {
             // inside (1) or exit of gamma~ (2): midpoint step
+            const Cx<T> Dm = {(dPrev.re + D.re) * T(0.5), (dPrev.im + D.im) * T(0.5)};
+            const StepSite<T> ps = stepSite(Dm, (esPrev + es) * T(0.5), h);
+            const T half = havePrev ? T(1) : T(0.5);
 #pragma unroll
-            for (int j = 0; j < EL; ++j) {
-              const Cx<T> em = {esm, havePrev ? eIm[j] : eIm[j] * T(0.5)};
-              coh[j] = riccatiStep(Dm, em, coh[j], h);
-            }
-            (void)hp;
+            for (int j = 0; j < EL; ++j) coh[j] = stepEnergy(ps, ser[j] * half, coh[j]);
             if (lcur == 2) {
               const int ci = baseOut + cntOut;
-              const T d = T(v.distOutGt[ci]) * h;
+              const T dr = T(v.distOutGt[ci]), d = dr * h;
+              const StepSite<T> pe = stepSite(D, es, d);
 #pragma unroll
               for (int j = 0; j < EL; ++j) {
                 if (!eOk[j]) continue;
-                const Cx<T> bv = riccatiStep(D, Cx<T>{es, eIm[j]}, coh[j], d);
+                const Cx<T> bv = stepEnergy(pe, ser[j] * dr, coh[j]);
                 const size_t si = size_t(ci) * Ne + g * EL + j;
                 v.bOutGt[si] = double(bv.re);
                 v.bOutGt[size_t(v.nOutGt) * Ne + si] = double(bv.im);
@@ -188,13 +195,14 @@ __global__ __launch_bounds__(64) void sweepKernel(SweepArgs a) {
           } else if (lcur == 3) {
             // entry of gamma~: start from the boundary value
             const int ci = baseIn + cntIn;
-            const T d = T(v.distInGt[ci]) * h;
+            const T dr = T(v.distInGt[ci]), d = dr * h;
+            const StepSite<T> pe = stepSite(D, es, d);
 #pragma unroll
             for (int j = 0; j < EL; ++j) {
               const size_t si = size_t(ci) * Ne + g * EL + j;
               const Cx<T> b0 = eOk[j] ? Cx<T>{T(v.bInGt[si]), T(v.bInGt[size_t(v.nInGt) * Ne + si])}
                                       : Cx<T>{T(0), T(0)};
-              coh[j] = riccatiStep(D, Cx<T>{es, eIm[j]}, b0, d);
+              coh[j] = stepEnergy(pe, ser[j] * dr, b0);
             }
             ++cntIn;
           } else {
@@ -212,46 +220,54 @@ __global__ __launch_bounds__(64) void sweepKernel(SweepArgs a) {
         lcur = lnext;
       }
     }
+  }
 
-    // ---------------- pass 2: gamma along +y, Green functions ----------------
-    {
-      const int baseIn = v.idxInG[col], baseOut = v.idxOutG[col];
-      int cntIn = 0, cntOut = 0;
-      Cx<T> coh[EL];
+  // ---------------- pass 2: gamma along +y, Green functions ----------------
+  {
+    const int baseIn = v.idxInG[col], baseOut = v.idxOutG[col];
+    int cntIn = 0, cntOut = 0;
+    Cx<T> coh[EL];
 #pragma unroll
-      for (int j = 0; j < EL; ++j) coh[j] = {T(0), T(0)};
-      Cx<T> dPrev = {T(0), T(0)};
-      T esPrev = T(0);
-      bool havePrev = false;
-      int8_t lcur = lab[size_t(it.ylo) * N1];
-      for (int t = 0; t < len; ++t) {
+    for (int j = 0; j < EL; ++j) coh[j] = {T(0), T(0)};
+    Cx<T> dPrev = {T(0), T(0)};
+    T esPrev = T(0);
+    bool havePrev = false;
+    int8_t lcur = lab[size_t(it.ylo) * N1];
+    for (int t0 = 0; t0 < len; t0 += gp) {
+      double fre = 0.0, fim = 0.0, fes = 0.0;
+      {
+        const int tt = t0 + g;
+        if (colActive && tt < len) {
+          const int y = it.ylo + tt;
+          const int8_t l0 = lab[size_t(y) * N1];
+          const int8_t l1 = (tt + 1 < len) ? lab[size_t(y + 1) * N1] : int8_t(0);
+          if (l0 != 0 || l1 != 0) siteFields(v, k, x, y, fre, fim, fes);
+        }
+      }
+      const int tEnd = min(gp, len - t0);
+      for (int i = 0; i < tEnd; ++i) {
+        const int src = i * cols + c;
+        const double dre = __shfl(fre, src), dim = __shfl(fim, src), e = __shfl(fes, src);
+        const int t = t0 + i;
         const int y = it.ylo + t;
         const int8_t lnext = (t + 1 < len) ? lab[size_t(y + 1) * N1] : int8_t(0);
-        Cx<T> D = {T(0), T(0)};
-        T es = T(0);
-        double dre = 0.0, dim = 0.0;
-        if (active && (lcur != 0 || lnext != 0)) {
-          double e;
-          siteFields(v, k, x, y, dre, dim, e);
-          D = {T(dre), T(dim)};
-          es = T(e);
-        }
+        const Cx<T> D = {T(dre), T(dim)};
+        const T es = T(e);
         if (active) {
-          const Cx<T> Dm = {(dPrev.re + D.re) * T(0.5), (dPrev.im + D.im) * T(0.5)};
-          const T esm = (esPrev + es) * T(0.5);
           if (lcur == 1 || lcur == 3) {
+            const Cx<T> Dm = {(dPrev.re + D.re) * T(0.5), (dPrev.im + D.im) * T(0.5)};
+            const StepSite<T> ps = stepSite(Dm, (esPrev + es) * T(0.5), h);
+            const T half = havePrev ? T(1) : T(0.5);
 #pragma unroll
-            for (int j = 0; j < EL; ++j) {
-              const Cx<T> em = {esm, havePrev ? eIm[j] : eIm[j] * T(0.5)};
-              coh[j] = riccatiStep(Dm, em, coh[j], h);
-            }
+            for (int j = 0; j < EL; ++j) coh[j] = stepEnergy(ps, ser[j] * half, coh[j]);
             if (lcur == 3) {
               const int ci = baseOut + cntOut;
-              const T d = T(v.distOutG[ci]) * h;
+              const T dr = T(v.distOutG[ci]), d = dr * h;
+              const StepSite<T> pe = stepSite(D, es, d);
 #pragma unroll
               for (int j = 0; j < EL; ++j) {
                 if (!eOk[j]) continue;
-                const Cx<T> bv = riccatiStep(D, Cx<T>{es, eIm[j]}, coh[j], d);
+                const Cx<T> bv = stepEnergy(pe, ser[j] * dr, coh[j]);
                 const size_t si = size_t(ci) * Ne + g * EL + j;
                 v.bOutG[si] = double(bv.re);
                 v.bOutG[size_t(v.nOutG) * Ne + si] = double(bv.im);
@@ -260,13 +276,14 @@ __global__ __launch_bounds__(64) void sweepKernel(SweepArgs a) {
             }
           } else if (lcur == 2) {
             const int ci = baseIn + cntIn;
-            const T d = T(v.distInG[ci]) * h;
+            const T dr = T(v.distInG[ci]), d = dr * h;
+            const StepSite<T> pe = stepSite(D, es, d);
 #pragma unroll
             for (int j = 0; j < EL; ++j) {
               const size_t si = size_t(ci) * Ne + g * EL + j;
               const Cx<T> b0 = eOk[j] ? Cx<T>{T(v.bInG[si]), T(v.bInG[size_t(v.nInG) * Ne + si])}
                                       : Cx<T>{T(0), T(0)};
-              coh[j] = riccatiStep(D, Cx<T>{es, eIm[j]}, b0, d);
+              coh[j] = stepEnergy(pe, ser[j] * dr, b0);
             }
             ++cntIn;
           } else {
@@ -275,9 +292,14 @@ __global__ __launch_bounds__(64) void sweepKernel(SweepArgs a) {
           }
         }
         // Green functions at inside sites (GreensFunctionStatic.h:125-157,
-        // LocalGreens.h:32-70, ComputeFreeEnergy.h:95-160).
+        // LocalGreens.h:32-70, ComputeFreeEnergy.h:95-160).  Per energy, with
+        // p = gamma gamma~ and F = 2 i / (1 + p):
+        //   f + f~* = F (gamma~ - gamma) [real part], -F (gamma + gamma~) [imag]
+        //   Re g = Im((1 - p) / (1 + p)) = -2 Im p / |1 + p|^2
+        //   Re i(D* gamma - D gamma~) summed as D.re Su + D.im Sw with
+        //   Su = sum r Im(gamma~ - gamma), Sw = sum r Re(gamma + gamma~).
         if (!a.burnin && lcur > 0) {
-          double gdr = 0.0, gdi = 0.0, gj = 0.0, gf = 0.0;
+          double gdr = 0.0, gdi = 0.0, gj = 0.0, su = 0.0, sw = 0.0;
           if (active) {
             const int tt = len - 1 - t;
 #pragma unroll
@@ -286,31 +308,28 @@ __global__ __launch_bounds__(64) void sweepKernel(SweepArgs a) {
               const Cx<T> ga = coh[j];
               const Cx<T> gt = scr[(size_t(tt) * EL + j) * 64 + lane];
               const Cx<T> p = ga * gt;
-              const Cx<T> inv = cdiv(Cx<T>{T(1), T(0)}, Cx<T>{T(1) + p.re, p.im});
-              // f + f~* with factor = 2 i / (1 + p)
-              const Cx<T> fac = {T(-2) * inv.im, T(2) * inv.re};
-              const Cx<T> f = fac * ga;
-              const Cx<T> ft = fac * gt;
+              const T pr = T(1) + p.re;
+              const T r2 = T(2) * rcpT(pr * pr + p.im * p.im);
+              const Cx<T> F = {p.im * r2, pr * r2};
+              const Cx<T> u = gt - ga;
+              const Cx<T> w = ga + gt;
               const double r = double(res[j]);
-              gdr += r * double(-f.re + ft.re);
-              gdi += r * double(-f.im - ft.im);
-              // Re g = Im((1 - p) / (1 + p))
-              const Cx<T> q = Cx<T>{T(1) - p.re, T(0) - p.im} * inv;
-              gj += r * double(q.im);
-              // Re[i (D* gamma - D gamma~)] = -Im(D* gamma - D gamma~)
-              const double im1 = dre * double(ga.im) - dim * double(ga.re);
-              const double im2 = dre * double(gt.im) + dim * double(gt.re);
-              gf -= r * (im1 - im2);
+              gdr += r * double(F.re * u.re - F.im * u.im);
+              gdi -= r * double(F.re * w.im + F.im * w.re);
+              gj -= r * double(p.im * r2);
+              su += r * double(u.im);
+              sw += r * double(w.re);
             }
           }
+          double gf = dre * su + dim * sw;
           // sum the energy groups of this column (lanes c, c + cols, ...)
-          for (int m = a.cols; m < 64; m <<= 1) {
+          for (int m = cols; m < 64; m <<= 1) {
             gdr += __shfl_xor(gdr, m);
             gdi += __shfl_xor(gdi, m);
             gj += __shfl_xor(gj, m);
             gf += __shfl_xor(gf, m);
           }
-          if (g == 0 && c < it.ncols) {
+          if (g == 0 && colActive) {
             double *Gk = v.G + size_t(k) * 4 * n1 + size_t(y) * N1 + x;
             Gk[0] = gdr;
             Gk[n1] = gdi;
@@ -324,6 +343,21 @@ __global__ __launch_bounds__(64) void sweepKernel(SweepArgs a) {
         lcur = lnext;
       }
     }
+  }
+}
+
+// Persistent wavefronts pulling items from the work counter.  The second
+// launch bound asks the compiler for a register budget that leaves room for
+// WAVES wavefronts per SIMD.
+template <typename T, int EL, int WAVES>
+__global__ __launch_bounds__(64, WAVES) void sweepKernel(SweepArgs a) {
+  Cx<T> *scr = reinterpret_cast<Cx<T> *>(a.scratch) + size_t(blockIdx.x) * a.slotStride;
+  for (;;) {
+    int item = 0;
+    if (threadIdx.x == 0) item = atomicAdd(a.counter, 1);
+    item = __shfl(item, 0);
+    if (item >= a.numItems) break;
+    sweepItem<T, EL>(a, a.items[item], scr);
   }
 }
 

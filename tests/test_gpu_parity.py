@@ -191,15 +191,43 @@ def test_burnin_then_iterations(product, oracle):
 def test_sweep_shapes_agree(product, oracle):
     cfg = cases.mixed_small()
     out = []
-    for el, slots in ((4, 2), (8, 4), (16, 1)):
+    for el, waves, slots in ((4, 4, 16), (7, 2, 8), (8, 1, 4), (14, 2, 8), (16, 1, 2)):
         s, o = pair(product, oracle, cfg)
-        s.set_sweep_shape(el, slots)
+        s.set_sweep_shape(el, waves, slots)
         s.compute("fp64")
         out.append((s.order_parameter(1), s.current()))
     o.compute()
     for d, j in out:
         assert rel(d, o.delta(True)) <= TOL
         assert rel(j, o.current()) <= TOL
+
+
+def test_autotune_leaves_state_and_parity(product, oracle):
+    cfg = cases.mixed_small()
+    s, o = pair(product, oracle, cfg)
+    s.compute("fp64")
+    o.compute()
+    for prec in ("fp64", "mixed"):
+        shape = s.autotune(prec)
+        print("autotuned", prec, shape)
+        assert len(shape) == 3
+    for it in range(2):
+        s.compute("fp64")
+        o.compute()
+        compare_state(s, o, cfg, what="after autotune %d" % it)
+
+
+def test_mixed_shapes_agree(product):
+    cfg = cases.mixed_small()
+    out = []
+    for el, waves, slots in ((4, 2, 8), (7, 1, 4), (8, 2, 8), (16, 2, 8)):
+        s = product.Simulation(cfg)
+        s.set_sweep_shape(el, waves, slots, precision="mixed")
+        s.compute("mixed")
+        out.append(s.order_parameter(1))
+    for d in out[1:]:
+        # FP32 stepping: shapes differ only in the order of the energy sums
+        assert rel(d, out[0]) < 1e-12
 
 
 def test_run_to_convergence_known_answer(product, oracle):
