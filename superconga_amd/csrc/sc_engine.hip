@@ -24,11 +24,15 @@ namespace scamd {
 
 namespace {
 
+// Allocates and clears.  The clear runs on the null stream, which the
+// engine's non-blocking stream does not wait for, so it is completed here
+// before any stream work can touch the buffer.
 template <class T> T *devAlloc(size_t n) {
   T *p = nullptr;
   if (n == 0) n = 1;
   SC_HIP(hipMalloc(reinterpret_cast<void **>(&p), n * sizeof(T)));
   SC_HIP(hipMemset(p, 0, n * sizeof(T)));
+  SC_HIP(hipDeviceSynchronize());
   return p;
 }
 template <class T> T *devUpload(const std::vector<T> &v) {
@@ -224,6 +228,8 @@ void Grain::sweepShape(int precision, int &el, int &waves, int &slotsPerCU) cons
 void Grain::prepareSweep(int precision) {
   SweepPlan &pl = plan_[precision];
   if (pl.ready) return;
+  // the buffers below may still be read by a sweep in flight
+  SC_HIP(hipStreamSynchronize(stream_));
   const Setup &s = *s_;
   pl.groups = (s.Ne + pl.el - 1) / pl.el;
   int gp = 1;
@@ -240,9 +246,7 @@ void Grain::prepareSweep(int precision) {
   if (pl.dItems) (void)hipFree(pl.dItems);
   pl.dItems = nullptr;
   SC_HIP(hipMalloc(&pl.dItems, std::max<size_t>(1, di.size()) * sizeof(SweepItemDev)));
-  if (!di.empty())
-    SC_HIP(hipMemcpy(pl.dItems, di.data(), di.size() * sizeof(SweepItemDev),
-                     hipMemcpyHostToDevice));
+  if (!di.empty()) toDevice(pl.dItems, di.data(), di.size() * sizeof(SweepItemDev));
   int dev = 0, cus = 0;
   SC_HIP(hipGetDevice(&dev));
   SC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -251,7 +255,6 @@ void Grain::prepareSweep(int precision) {
   const size_t stride = size_t(pl.maxLen) * pl.el * 64; // complex elements per slot
   const size_t bytes = size_t(pl.numSlots) * stride * elemBytes;
   if (bytes > scratchBytes_) {
-    SC_HIP(hipStreamSynchronize(stream_));
     if (dScratch_) (void)hipFree(dScratch_);
     dScratch_ = nullptr;
     scratchBytes_ = 0;
@@ -325,6 +328,16 @@ static GrainView makeView(const Setup &s, const float *trig, const int *idx,
 // ---------------------------------------------------------------------------
 // host I/O
 // ---------------------------------------------------------------------------
+// Host transfers are ordered on the grain's stream and complete on return.
+void Grain::toDevice(void *dst, const void *src, size_t bytes) {
+  SC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_));
+  SC_HIP(hipStreamSynchronize(stream_));
+}
+void Grain::fromDevice(void *dst, const void *src, size_t bytes) {
+  SC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream_));
+  SC_HIP(hipStreamSynchronize(stream_));
+}
+
 void Grain::setOrderParameter(int c, const double *re, const double *im) {
   const Setup &s = *s_;
   if (c < 0 || c >= s.Nc) throw std::runtime_error("component index out of range");
@@ -335,17 +348,15 @@ void Grain::setOrderParameter(int c, const double *re, const double *im) {
     buf[i] = s.domain[i] ? re[i] : 0.0;
     buf[n0 + i] = s.domain[i] ? im[i] : 0.0;
   }
-  SC_HIP(hipMemcpy(dDelta_ + size_t(c) * 2 * n0, buf.data(), 2 * n0 * sizeof(double),
-                   hipMemcpyHostToDevice));
+  toDevice(dDelta_ + size_t(c) * 2 * n0, buf.data(), 2 * n0 * sizeof(double));
 }
 void Grain::getOrderParameter(int which, int c, double *re, double *im) {
   const Setup &s = *s_;
   if (c < 0 || c >= s.Nc) throw std::runtime_error("component index out of range");
   const size_t n0 = size_t(s.N0) * s.N0;
   const double *src = (which == 0 ? dDelta_ : dDeltaNew_) + size_t(c) * 2 * n0;
-  SC_HIP(hipStreamSynchronize(stream_));
-  SC_HIP(hipMemcpy(re, src, n0 * sizeof(double), hipMemcpyDeviceToHost));
-  SC_HIP(hipMemcpy(im, src + n0, n0 * sizeof(double), hipMemcpyDeviceToHost));
+  fromDevice(re, src, n0 * sizeof(double));
+  fromDevice(im, src + n0, n0 * sizeof(double));
 }
 void Grain::setVectorPotential(const double *ax, const double *ay, bool scaled) {
   const Setup &s = *s_;
@@ -357,15 +368,14 @@ void Grain::setVectorPotential(const double *ax, const double *ay, bool scaled) 
     buf[i] = ax[i] * f;
     buf[n0 + i] = ay[i] * f;
   }
-  SC_HIP(hipMemcpy(dA_, buf.data(), 2 * n0 * sizeof(double), hipMemcpyHostToDevice));
+  toDevice(dA_, buf.data(), 2 * n0 * sizeof(double));
 }
 void Grain::getVectorPotential(int which, double *ax, double *ay, bool scaled) {
   const Setup &s = *s_;
   const size_t n0 = size_t(s.N0) * s.N0;
-  SC_HIP(hipStreamSynchronize(stream_));
   const double *src = which == 0 ? dA_ : dANew_;
-  SC_HIP(hipMemcpy(ax, src, n0 * sizeof(double), hipMemcpyDeviceToHost));
-  SC_HIP(hipMemcpy(ay, src + n0, n0 * sizeof(double), hipMemcpyDeviceToHost));
+  fromDevice(ax, src, n0 * sizeof(double));
+  fromDevice(ay, src + n0, n0 * sizeof(double));
   if (!scaled && s.lambda > 0.0) {
     const double f = 1.0 / (s.lambda * s.lambda);
     for (size_t i = 0; i < n0; ++i) {
@@ -376,14 +386,12 @@ void Grain::getVectorPotential(int which, double *ax, double *ay, bool scaled) {
 }
 void Grain::getCurrent(double *jx, double *jy) {
   const size_t n0 = size_t(s_->N0) * s_->N0;
-  SC_HIP(hipStreamSynchronize(stream_));
-  SC_HIP(hipMemcpy(jx, dCurrent_, n0 * sizeof(double), hipMemcpyDeviceToHost));
-  SC_HIP(hipMemcpy(jy, dCurrent_ + n0, n0 * sizeof(double), hipMemcpyDeviceToHost));
+  fromDevice(jx, dCurrent_, n0 * sizeof(double));
+  fromDevice(jy, dCurrent_ + n0, n0 * sizeof(double));
 }
 void Grain::getEnergyDensity(double *out) {
   const size_t n0 = size_t(s_->N0) * s_->N0;
-  SC_HIP(hipStreamSynchronize(stream_));
-  SC_HIP(hipMemcpy(out, dEnergyDens_, n0 * sizeof(double), hipMemcpyDeviceToHost));
+  fromDevice(out, dEnergyDens_, n0 * sizeof(double));
 }
 void Grain::getBoundary(int inOut, int dir, double *re, double *im) {
   const Setup &s = *s_;
@@ -392,16 +400,15 @@ void Grain::getBoundary(int inOut, int dir, double *re, double *im) {
                       s.Ne;
   const double *p = dir == 0 ? (inOut == 0 ? dBInG_ : dBOutG_)
                              : (inOut == 0 ? dBInGt_ : dBOutGt_);
-  SC_HIP(hipStreamSynchronize(stream_));
-  SC_HIP(hipMemcpy(re, p, n * sizeof(double), hipMemcpyDeviceToHost));
-  SC_HIP(hipMemcpy(im, p + n, n * sizeof(double), hipMemcpyDeviceToHost));
+  fromDevice(re, p, n * sizeof(double));
+  fromDevice(im, p + n, n * sizeof(double));
 }
 void Grain::setBoundary(int dir, const double *re, const double *im) {
   const Setup &s = *s_;
   const long long n = (dir == 0 ? s.nInG : s.nInGt) * s.Ne;
   double *p = dir == 0 ? dBInG_ : dBInGt_;
-  SC_HIP(hipMemcpy(p, re, n * sizeof(double), hipMemcpyHostToDevice));
-  SC_HIP(hipMemcpy(p + n, im, n * sizeof(double), hipMemcpyHostToDevice));
+  toDevice(p, re, n * sizeof(double));
+  toDevice(p + n, im, n * sizeof(double));
   boundaryInit_ = true;
 }
 

@@ -79,6 +79,8 @@ public:
 
 private:
   void upload();
+  void toDevice(void *dst, const void *src, size_t bytes);
+  void fromDevice(void *dst, const void *src, size_t bytes);
   void extendDelta();
   void ensureBoundaryInit();
   void integrate(int precision, bool burnin);
