@@ -83,6 +83,26 @@ def load_traffic(workload, precision):
         return json.load(f).get("bytes_per_launch")
 
 
+def copy_probe(mib):
+    """One device-to-device copy of `mib` MiB through the HIP runtime: the
+    reference kernel of known traffic for scaling the HBM counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = int(mib) << 20
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, ctypes.c_size_t(n))
+    hip.hipDeviceSynchronize()
+    if hip.hipMemcpy(dst, src, ctypes.c_size_t(n), 3) != 0:  # device to device
+        raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    log("copy probe: %d MiB device-to-device" % mib)
+
+
 def rel(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
 
@@ -130,6 +150,9 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="run one device-to-device copy of this size first "
+                         "(calibrates HBM counters of a profiler run)")
     ap.add_argument("--cpu-threads", type=int,
                     default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
                     or min(16, os.cpu_count() or 1))
@@ -143,6 +166,9 @@ def main():
 
     import superconga_amd as sa
     from superconga_amd import workloads
+
+    if args.copy_probe_mib > 0:
+        copy_probe(args.copy_probe_mib)
 
     cfg = workloads.WORKLOADS[args.workload]()
     t = time.perf_counter()

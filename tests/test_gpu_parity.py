@@ -226,8 +226,10 @@ def test_mixed_shapes_agree(product):
         s.compute("mixed")
         out.append(s.order_parameter(1))
     for d in out[1:]:
-        # FP32 stepping: shapes differ only in the order of the energy sums
-        assert rel(d, out[0]) < 1e-12
+        # FP32 stepping: the compiled variants round differently (their
+        # multiply-adds are contracted differently), so they agree to FP32
+        # rounding, not bit for bit.
+        assert rel(d, out[0]) < 1e-6
 
 
 def test_run_to_convergence_known_answer(product, oracle):
