@@ -182,6 +182,11 @@ def main():
     sim.compute("fp64")
     ref = (sim.order_parameter(1), sim.current(), sim.vector_potential(1),
            sim.results()["free_energy"])
+    t = time.perf_counter()
+    shape = sim.autotune("fp64")
+    tune_s = time.perf_counter() - t
+    log("autotune fp64: energies/lane %d, waves/SIMD %d, slots/CU %d (%.1fs)"
+        % (shape + (tune_s,)))
     wall, sweep = timed_iterations(sim, "fp64", max(args.warmup - 1, 0), args.steps)
     value = d.traj_steps * args.steps / wall
     rf = roofline(d.traj_steps, sweep, PEAK_FP64_TFLOPS)
@@ -203,6 +208,8 @@ def main():
                    "traj_steps_per_iteration": d.traj_steps,
                    "step": "one self-consistency iteration"},
         "roofline": rf,
+        "sweep_shape": {"energies_per_lane": shape[0], "waves_per_simd": shape[1],
+                        "slots_per_cu": shape[2], "autotune_s": round(tune_s, 2)},
     }
 
     if not args.no_mixed:
@@ -210,6 +217,8 @@ def main():
         sim.compute("mixed")
         got = (sim.order_parameter(1), sim.current(), sim.vector_potential(1),
                sim.results()["free_energy"])
+        mshape = sim.autotune("mixed")
+        log("autotune mixed: energies/lane %d, waves/SIMD %d, slots/CU %d" % mshape)
         err = {"delta": rel(got[0], ref[0]),
                "current": rel(got[1], ref[1]) if np.any(ref[1]) else float(np.max(np.abs(got[1]))),
                "vector_potential": rel(got[2], ref[2]) if np.any(ref[2]) else float(np.max(np.abs(got[2]))),
@@ -223,7 +232,10 @@ def main():
         out["mixed"] = {"value": mval, "unit": "traj-steps/s",
                         "ms_per_step": 1e3 * mwall / args.steps,
                         "dtype": "f32 stepping, f64 sums",
-                        "error_vs_fp64_first_iteration": err, "roofline": mrf}
+                        "error_vs_fp64_first_iteration": err, "roofline": mrf,
+                        "sweep_shape": {"energies_per_lane": mshape[0],
+                                        "waves_per_simd": mshape[1],
+                                        "slots_per_cu": mshape[2]}}
         sim.close()
 
     if not args.no_cpu:
