@@ -31,6 +31,7 @@ extern "C" {
 /* Arithmetic of the Riccati stepping. */
 #define SC_PRECISION_FP64 0  /* everything in FP64 */
 #define SC_PRECISION_MIXED 1 /* FP32 stepping, FP64 energy sums and reductions */
+#define SC_PRECISION_MIXED_REFINE 2 /* sc_run only: MIXED to the FP32 floor, then FP64 */
 
 typedef struct sc_context sc_context;
 
@@ -57,6 +58,7 @@ typedef struct {
   int converged; /* Context::isPartiallyConverged (Context.h:485-497) */
   double last_sweep_ms;     /* Riccati sweep kernel time of the last iteration */
   double last_iteration_ms; /* whole iteration */
+  int mixed_iterations;     /* FP32-stepping iterations of the last sc_run */
 } sc_results;
 
 const char *sc_last_error(void);
@@ -106,7 +108,10 @@ int sc_compute(sc_context *ctx, int precision);
 /* One burn-in iteration, Context<T>::burnIn (Context.h:377-389). */
 int sc_burnin(sc_context *ctx, int precision);
 /* runSimulation (run_simulation.h:124-187) with the limits of the config
- * when the arguments are negative; *converged receives the final status. */
+ * when the arguments are negative; *converged receives the final status.
+ * With SC_PRECISION_MIXED_REFINE the iterations step in FP32 until the
+ * residual reaches the FP32 floor and then in FP64; convergence is only
+ * declared on an FP64 iteration. */
 int sc_run(sc_context *ctx, int num_burnin, int num_min, int num_max,
            int precision, int *converged);
 int sc_get_results(sc_context *ctx, sc_results *out);

@@ -17,7 +17,9 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libsuperconga_amd.so")
 
 FP64 = 0
 MIXED = 1
-_PRECISION = {"fp64": FP64, "mixed": MIXED, FP64: FP64, MIXED: MIXED}
+MIXED_REFINE = 2
+_PRECISION = {"fp64": FP64, "mixed": MIXED, "mixed_refine": MIXED_REFINE,
+              FP64: FP64, MIXED: MIXED, MIXED_REFINE: MIXED_REFINE}
 
 
 class SupercongaError(RuntimeError):
@@ -49,7 +51,8 @@ class Results(ctypes.Structure):
         "magnetic_free_energy", "magnetic_free_energy_per_area")] + [
         ("iterations", ctypes.c_int), ("converged", ctypes.c_int),
         ("last_sweep_ms", ctypes.c_double),
-        ("last_iteration_ms", ctypes.c_double)]
+        ("last_iteration_ms", ctypes.c_double),
+        ("mixed_iterations", ctypes.c_int)]
 
 
 _lib = None
@@ -118,7 +121,7 @@ def _precision(p):
     try:
         return _PRECISION[p]
     except KeyError:
-        raise ValueError("precision must be 'fp64' or 'mixed'") from None
+        raise ValueError("precision must be 'fp64', 'mixed' or 'mixed_refine'") from None
 
 
 class Simulation:

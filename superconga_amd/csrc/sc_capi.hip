@@ -226,7 +226,7 @@ int sc_burnin(sc_context *ctx, int precision) {
 int sc_run(sc_context *ctx, int nb, int nmin, int nmax, int precision, int *converged) {
   SC_CHECK_CTX(ctx);
   return guarded([&] {
-    checkPrecision(precision);
+    if (precision != SC_PRECISION_MIXED_REFINE) checkPrecision(precision);
     const scamd::Config &c = ctx->grain->setup().cfg;
     const bool ok = ctx->grain->run(nb < 0 ? c.burnin : nb, nmin < 0 ? c.itMin : nmin,
                                     nmax < 0 ? c.itMax : nmax, precision);
@@ -240,6 +240,7 @@ int sc_get_results(sc_context *ctx, sc_results *out) {
   const scamd::Results r = ctx->grain->results();
   const scamd::Timing t = ctx->grain->timing();
   out->residual = r.residual;
+  out->mixed_iterations = ctx->grain->mixedIterations();
   out->residual_boundary = r.resBoundary;
   out->residual_order_parameter = r.resDelta;
   out->residual_current = r.resCurrent;

@@ -12,7 +12,7 @@
 
 namespace scamd {
 
-enum Precision { PREC_FP64 = 0, PREC_MIXED = 1 };
+enum Precision { PREC_FP64 = 0, PREC_MIXED = 1, PREC_MIXED_REFINE = 2 };
 
 struct Results {
   double residual = 0, resBoundary = 0, resDelta = 0, resCurrent = 0,
@@ -62,6 +62,8 @@ public:
 
   Results results() const { return res_; }
   Timing timing() const { return tim_; }
+  // FP32-stepping iterations of the last run()
+  int mixedIterations() const { return mixedIterations_; }
   // Lane layout of the sweep for one precision: Ozaki energies per lane,
   // register budget (wavefronts per SIMD the kernel is compiled for) and
   // persistent wavefronts per compute unit.
@@ -143,6 +145,7 @@ private:
   bool boundaryInit_ = false, firstCompute_ = true, firstCurrent_ = true,
        firstFE_ = true;
   double fePerAreaOld_ = 0;
+  int mixedIterations_ = 0;
   Results res_;
   Timing tim_;
   bool convDelta_ = false, convCurrent_ = false, convFE_ = false;
