@@ -150,6 +150,9 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--shape", default="",
+                    help="fixed sweep shape 'energies,waves,slots' for both "
+                         "precisions instead of the autotuner")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one device-to-device copy of this size first "
                          "(calibrates HBM counters of a profiler run)")
@@ -182,8 +185,13 @@ def main():
     sim.compute("fp64")
     ref = (sim.order_parameter(1), sim.current(), sim.vector_potential(1),
            sim.results()["free_energy"])
+    fixed = tuple(int(v) for v in args.shape.split(",")) if args.shape else None
     t = time.perf_counter()
-    shape = sim.autotune("fp64")
+    if fixed:
+        sim.set_sweep_shape(*fixed, precision="fp64")
+        shape = sim.sweep_shape("fp64")
+    else:
+        shape = sim.autotune("fp64")
     tune_s = time.perf_counter() - t
     log("autotune fp64: energies/lane %d, waves/SIMD %d, slots/CU %d (%.1fs)"
         % (shape + (tune_s,)))
@@ -217,7 +225,11 @@ def main():
         sim.compute("mixed")
         got = (sim.order_parameter(1), sim.current(), sim.vector_potential(1),
                sim.results()["free_energy"])
-        mshape = sim.autotune("mixed")
+        if fixed:
+            sim.set_sweep_shape(*fixed, precision="mixed")
+            mshape = sim.sweep_shape("mixed")
+        else:
+            mshape = sim.autotune("mixed")
         log("autotune mixed: energies/lane %d, waves/SIMD %d, slots/CU %d" % mshape)
         err = {"delta": rel(got[0], ref[0]),
                "current": rel(got[1], ref[1]) if np.any(ref[1]) else float(np.max(np.abs(got[1]))),
