@@ -496,6 +496,29 @@ double Grain::deviceSum(const double *a, const double *b, const uint8_t *mask,
   return s;
 }
 
+std::vector<double> Grain::multiDot(const double *a, const std::vector<double *> &bs,
+                                    long long n) {
+  std::vector<double> out(bs.size(), 0.0);
+  const long long blocks = std::min<long long>(blocksFor(n), 512);
+  for (size_t first = 0; first < bs.size(); first += kMaxDots) {
+    const int m = static_cast<int>(std::min<size_t>(kMaxDots, bs.size() - first));
+    if (blocks * m > partialsCap_ * 6) {
+      devFree(dPartials_);
+      partialsCap_ = (blocks * m + 5) / 6;
+      dPartials_ = devAlloc<double>(size_t(partialsCap_) * 6);
+    }
+    DotOperands ops{};
+    for (int i = 0; i < m; ++i) ops.b[i] = bs[first + i];
+    multiDotKernel<<<blocks, 256, 0, stream_>>>(a, ops, m, n, dPartials_);
+    SC_HIP(hipGetLastError());
+    std::vector<double> h(size_t(blocks) * m);
+    fromDevice(h.data(), dPartials_, h.size() * sizeof(double));
+    for (long long blk = 0; blk < blocks; ++blk)
+      for (int i = 0; i < m; ++i) out[first + i] += h[size_t(blk) * m + i];
+  }
+  return out;
+}
+
 // ---------------------------------------------------------------------------
 // the iteration
 // ---------------------------------------------------------------------------
@@ -508,7 +531,7 @@ void Grain::extendDelta() {
   if (nt > 0) {
     extendKernel<<<blocksFor(nt), 256, 0, stream_>>>(
         dExtDelta_, dExtRefT_, dExtRefO_, dExtRefS_, nt, 2 * s.Nc,
-        static_cast<long long>(n0), static_cast<long long>(n0), 0);
+        static_cast<long long>(n0), static_cast<long long>(n0), 0, 1);
     SC_HIP(hipGetLastError());
   }
 }
@@ -646,8 +669,8 @@ void Grain::integrate(int precision, bool burnin) {
     const int nt = static_cast<int>(s.extRot.target.size());
     if (nt > 0) {
       extendKernel<<<blocksFor(nt), 256, 0, stream_>>>(
-          dG_, dExtRotT_, dExtRotO_, dExtRotS_, nt, 4, static_cast<long long>(n1),
-          static_cast<long long>(n1), static_cast<long long>(4 * n1));
+          dG_, dExtRotT_, dExtRotO_, dExtRotS_, nt, 4, 1, static_cast<long long>(n1),
+          static_cast<long long>(4 * n1), 4);
       SC_HIP(hipGetLastError());
     }
     SC_HIP(hipMemcpyAsync(dCurrentOld_, dCurrent_, 2 * n0 * sizeof(double),
@@ -764,9 +787,17 @@ void Grain::acceleratorStep() {
     SC_HIP(hipMemcpyAsync(accX_[slot], X, size_t(n) * sizeof(double),
                           hipMemcpyDeviceToDevice, stream_));
     diffKernel<<<blocksFor(n), 256, 0, stream_>>>(accD_[slot], G, X, n);
-    accSq_[slot] = deviceSum(accD_[slot], nullptr, nullptr, n, 2);
     accIdx_[slot] = accIter_;
     const size_t m = accD_.size(), M = m + 1;
+    // Only the replaced difference changes the Gram matrix: one pass of dot
+    // products of it with every stored difference (itself included).
+    if (accGram_.size() != rank * rank) accGram_.assign(rank * rank, 0.0);
+    const std::vector<double> dots = multiDot(accD_[slot], accD_, n);
+    for (size_t i = 0; i < m; ++i) {
+      accGram_[slot * rank + i] = dots[i];
+      accGram_[i * rank + slot] = dots[i];
+    }
+    accSq_[slot] = dots[slot];
     std::vector<double> A(M * M, 1.0), b(M, 0.0);
     double sr = 0.0;
     for (double v : accSq_) sr += v;
@@ -774,12 +805,8 @@ void Grain::acceleratorStep() {
     for (size_t i = 0; i < m; ++i) {
       A[i * M + i] = accSq_[i] * scale;
       for (size_t j = i + 1; j < m; ++j) {
-        double dot = 0.0;
-        for (int f = 0; f < F; ++f)
-          dot += deviceSum(accD_[i] + size_t(f) * n0, accD_[j] + size_t(f) * n0, nullptr,
-                           static_cast<long long>(n0), 1);
-        A[i * M + j] = dot * scale;
-        A[j * M + i] = dot * scale;
+        A[i * M + j] = accGram_[i * rank + j] * scale;
+        A[j * M + i] = accGram_[i * rank + j] * scale;
       }
     }
     A[(M - 1) * M + (M - 1)] = 0.0;

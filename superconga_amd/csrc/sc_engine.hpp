@@ -91,6 +91,8 @@ private:
   double deviceSum(const double *a, const double *b, const uint8_t *mask,
                    long long n, int op);
   void prepareSweep(int precision);
+  std::vector<double> multiDot(const double *a, const std::vector<double *> &bs,
+                               long long n);
 
   std::unique_ptr<Setup> s_;
   hipStream_t stream_ = nullptr;
@@ -136,6 +138,7 @@ private:
   int accIter_ = 0;
   std::vector<double *> accX_, accD_;
   std::vector<double> accSq_;
+  std::vector<double> accGram_; // rank x rank dot products of the differences
   std::vector<long long> accIdx_;
   double *dAccMean_ = nullptr, *dAccPrev_ = nullptr, *dAccVel_ = nullptr;
   std::vector<double> accStep_, accInertia_;

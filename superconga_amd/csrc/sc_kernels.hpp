@@ -26,7 +26,7 @@ struct GrainView {
   long long nInG, nOutG, nInGt, nOutGt;
   const double *extDelta; // [Nc][2][N0^2]
   const double *A;        // [2][N0^2] (scaled by lambda^2)
-  double *G;              // [Na][4][N1^2]: Re/Im of f + f~*, Re g, Re i(D* g - D g~)
+  double *G;              // [Na][N1^2][4]: Re/Im of f + f~*, Re g, Re i(D* g - D g~)
 };
 
 struct SweepItemDev {
@@ -330,11 +330,10 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev
             gf += __shfl_xor(gf, m);
           }
           if (g == 0 && colActive) {
-            double *Gk = v.G + size_t(k) * 4 * n1 + size_t(y) * N1 + x;
-            Gk[0] = gdr;
-            Gk[n1] = gdi;
-            Gk[2 * n1] = gj;
-            Gk[3 * n1] = gf;
+            double2 *Gk = reinterpret_cast<double2 *>(
+                v.G + (size_t(k) * n1 + size_t(y) * N1 + x) * 4);
+            Gk[0] = make_double2(gdr, gdi);
+            Gk[1] = make_double2(gj, gf);
           }
         }
         dPrev = D;
@@ -404,12 +403,13 @@ __global__ void initialBoundaryKernel(GrainView v, const int *xEnd) {
 
 // Extension of values outwards from the boundary
 // (geometry/domain_ops.h:11-57): target <- mean of its listed sources, for
-// `planes` planes spaced `planeStride` apart starting at
-// base + (target / siteStride) * blockStride.
+// `planes` values per site.  A target index t addresses block t / siteStride
+// and site t % siteStride; value q of a site lives at
+// base + block * blockStride + site * siteScale + q * planeStride.
 __global__ void extendKernel(double *base, const int *target, const int *offset,
                              const int *source, int numTargets, int planes,
                              long long planeStride, long long siteStride,
-                             long long blockStride) {
+                             long long blockStride, long long siteScale) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= numTargets) return;
   const long long tg = target[t];
@@ -423,9 +423,9 @@ __global__ void extendKernel(double *base, const int *target, const int *offset,
     double val = 0.0;
     for (int o = o0; o < o1; ++o) {
       const long long s = source[o] - blk * siteStride;
-      val = (o == o0) ? p[s] : val + p[s];
+      val = (o == o0) ? p[s * siteScale] : val + p[s * siteScale];
     }
-    p[site] = val / inv;
+    p[site * siteScale] = val / inv;
   }
 }
 
@@ -448,12 +448,24 @@ __global__ void gatherKernel(GrainView v, const int *sites, int numSites,
   for (int k = 0; k < Na; ++k) {
     const RotSample r = rotatedCell(x, y, N0, N1, double(v.bkCos[k]), double(v.bkSin[k]));
     if (!r.valid) continue;
-    const double *Gk = v.G + size_t(k) * 4 * n1;
+    // four values per site: taps (x, y), (x+1, y), (x, y+1), (x+1, y+1)
+    const double2 *Gk = reinterpret_cast<const double2 *>(v.G + size_t(k) * n1 * 4);
+    const size_t i00 = size_t(r.idx) * 2, i10 = i00 + 2, i01 = i00 + size_t(N1) * 2,
+                 i11 = i01 + 2;
+    const double2 a00 = Gk[i00], b00 = Gk[i00 + 1];
+    const double2 a10 = Gk[i10], b10 = Gk[i10 + 1];
+    const double2 a01 = Gk[i01], b01 = Gk[i01 + 1];
+    const double2 a11 = Gk[i11], b11 = Gk[i11 + 1];
+    double gre, gim, gj, gf;
+    {
+#pragma clang fp contract(off)
+      const double wx0 = 1.0 - r.fx, wx1 = r.fx, wy0 = 1.0 - r.fy, wy1 = r.fy;
+      gre = wy0 * (wx0 * a00.x + wx1 * a10.x) + wy1 * (wx0 * a01.x + wx1 * a11.x);
+      gim = wy0 * (wx0 * a00.y + wx1 * a10.y) + wy1 * (wx0 * a01.y + wx1 * a11.y);
+      gj = wy0 * (wx0 * b00.x + wx1 * b10.x) + wy1 * (wx0 * b01.x + wx1 * b11.x);
+      gf = wy0 * (wx0 * b00.y + wx1 * b10.y) + wy1 * (wx0 * b01.y + wx1 * b11.y);
+    }
     const double w = v.weight[k];
-    const double gre = bilinear(Gk, N1, r);
-    const double gim = bilinear(Gk + n1, N1, r);
-    const double gj = bilinear(Gk + 2 * n1, N1, r);
-    const double gf = bilinear(Gk + 3 * n1, N1, r);
     for (int c = 0; c < Nc && c < 4; ++c) {
       const double b = v.basis[c * Na + k];
       dre[c] += (gre * b) * w;
@@ -623,6 +635,37 @@ __global__ void congaccKernel(double *x, double *mean, const double *d,
 __global__ void axpyKernel(double *x, const double *d, long long n, double s) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i < n) x[i] += d[i] * s;
+}
+// Dot products of one vector with up to kMaxDots others in one pass over it:
+// partials[block * m + i] = block-partial of sum a * b_i.
+constexpr int kMaxDots = 16;
+struct DotOperands {
+  const double *b[kMaxDots];
+};
+__global__ __launch_bounds__(256) void multiDotKernel(const double *a, DotOperands b,
+                                                      int m, long long n,
+                                                      double *partials) {
+  double s[kMaxDots];
+#pragma unroll
+  for (int i = 0; i < kMaxDots; ++i) s[i] = 0.0;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+       e += (long long)gridDim.x * blockDim.x) {
+    const double x = a[e];
+#pragma unroll
+    for (int i = 0; i < kMaxDots; ++i)
+      if (i < m) s[i] += x * b.b[i][e];
+  }
+  __shared__ double sh[256];
+  for (int i = 0; i < m; ++i) {
+    sh[threadIdx.x] = s[i];
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+      if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[size_t(blockIdx.x) * m + i] = sh[0];
+    __syncthreads();
+  }
 }
 // heavy ball (Polyak.h:216-240)
 __global__ void polyakKernel(double *x, double *vel, const double *g,
