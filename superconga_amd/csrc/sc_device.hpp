@@ -28,6 +28,25 @@ template <typename T> __device__ __forceinline__ Cx<T> scale(Cx<T> a, T s) {
   return {a.re * s, a.im * s};
 }
 
+// Streaming access to the sweep scratch: every element is written once and
+// read once, so the accesses carry the non-temporal hint and leave the cache
+// to the grids the sweep samples.
+template <typename T> struct VecOf;
+template <> struct VecOf<double> { typedef double type __attribute__((ext_vector_type(2))); };
+template <> struct VecOf<float> { typedef float type __attribute__((ext_vector_type(2))); };
+template <typename T> __device__ __forceinline__ void storeStream(Cx<T> *p, Cx<T> v) {
+  typedef typename VecOf<T>::type V;
+  V w;
+  w.x = v.re;
+  w.y = v.im;
+  __builtin_nontemporal_store(w, reinterpret_cast<V *>(p));
+}
+template <typename T> __device__ __forceinline__ Cx<T> loadStream(const Cx<T> *p) {
+  typedef typename VecOf<T>::type V;
+  const V w = __builtin_nontemporal_load(reinterpret_cast<const V *>(p));
+  return Cx<T>{w.x, w.y};
+}
+
 // Reciprocal and reciprocal square root.  FP32: the hardware v_rcp_f32 /
 // v_rsq_f32 (1 ulp).  FP64: the hardware v_rcp_f64 / v_rsq_f64 estimate
 // refined by kNewtonSteps Newton-Raphson iterations (each doubles the number

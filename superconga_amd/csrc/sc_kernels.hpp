@@ -211,7 +211,7 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev
           }
           if (!a.burnin && lcur > 0) {
 #pragma unroll
-            for (int j = 0; j < EL; ++j) scr[(size_t(t) * EL + j) * 64 + lane] = coh[j];
+            for (int j = 0; j < EL; ++j) storeStream(scr + (size_t(t) * EL + j) * 64 + lane, coh[j]);
           }
         }
         dPrev = D;
@@ -233,6 +233,16 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev
     T esPrev = T(0);
     bool havePrev = false;
     int8_t lcur = lab[size_t(it.ylo) * N1];
+    // gamma~ of the next site, loaded one site ahead of its use
+    Cx<T> gtNext[EL];
+    const bool green = active && !a.burnin;
+#pragma unroll
+    for (int j = 0; j < EL; ++j) gtNext[j] = {T(0), T(0)};
+    if (green && lcur > 0) {
+#pragma unroll
+      for (int j = 0; j < EL; ++j)
+        gtNext[j] = loadStream(scr + (size_t(len - 1) * EL + j) * 64 + lane);
+    }
     for (int t0 = 0; t0 < len; t0 += gp) {
       double fre = 0.0, fim = 0.0, fes = 0.0;
       {
@@ -253,6 +263,14 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev
         const int8_t lnext = (t + 1 < len) ? lab[size_t(y + 1) * N1] : int8_t(0);
         const Cx<T> D = {T(dre), T(dim)};
         const T es = T(e);
+        Cx<T> gtCur[EL];
+#pragma unroll
+        for (int j = 0; j < EL; ++j) gtCur[j] = gtNext[j];
+        if (green && lnext > 0) {
+#pragma unroll
+          for (int j = 0; j < EL; ++j)
+            gtNext[j] = loadStream(scr + (size_t(len - 2 - t) * EL + j) * 64 + lane);
+        }
         if (active) {
           if (lcur == 1 || lcur == 3) {
             const Cx<T> Dm = {(dPrev.re + D.re) * T(0.5), (dPrev.im + D.im) * T(0.5)};
@@ -301,12 +319,11 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev
         if (!a.burnin && lcur > 0) {
           double gdr = 0.0, gdi = 0.0, gj = 0.0, su = 0.0, sw = 0.0;
           if (active) {
-            const int tt = len - 1 - t;
 #pragma unroll
             for (int j = 0; j < EL; ++j) {
               if (!eOk[j]) continue;
               const Cx<T> ga = coh[j];
-              const Cx<T> gt = scr[(size_t(tt) * EL + j) * 64 + lane];
+              const Cx<T> gt = gtCur[j];
               const Cx<T> p = ga * gt;
               const T pr = T(1) + p.re;
               const T r2 = T(2) * rcpT(pr * pr + p.im * p.im);
