@@ -142,7 +142,20 @@ void Grain::upload() {
   dTrig_ = devUpload(trig);
   dLabels_ = devUpload(s.labels);
   dDomain_ = devUpload(s.domain);
-  dDomainSites_ = devUpload(s.domainSites);
+  {
+    // The gather visits the grain sites in 16 x 16 tiles so that the
+    // threads of a block sample a compact rotated patch of every frame and
+    // share its bilinear taps in the caches.
+    std::vector<int> tiled(s.domainSites);
+    const int N0 = s.N0;
+    std::stable_sort(tiled.begin(), tiled.end(), [N0](int a, int b) {
+      const int ax = a % N0, ay = a / N0, bx = b % N0, by = b / N0;
+      const long long ka = (long long)(ay / 16) * N0 + ax / 16;
+      const long long kb = (long long)(by / 16) * N0 + bx / 16;
+      return ka < kb;
+    });
+    dDomainSites_ = devUpload(tiled);
+  }
   std::vector<int> idx;
   for (const auto *v : {&s.idxInG, &s.idxOutG, &s.idxInGt, &s.idxOutGt})
     idx.insert(idx.end(), v->begin(), v->end());
@@ -773,6 +786,8 @@ void Grain::acceleratorStep() {
   case ACC_ANDERSON: {
     // Anderson.h:267-355
     const size_t rank = s.cfg.rank > 0 ? size_t(s.cfg.rank) : 4;
+    if (rank > size_t(kMaxHistory))
+      throw std::runtime_error("Anderson history rank above " + std::to_string(kMaxHistory));
     const double st = s.cfg.stepSize > 0 ? s.cfg.stepSize : 1.0;
     size_t slot;
     if (accD_.size() == rank) {
@@ -833,18 +848,14 @@ void Grain::acceleratorStep() {
       for (size_t q = r + 1; q < M; ++q) acc -= A[r * M + q] * w[q];
       w[r] = A[r * M + r] != 0.0 ? acc / A[r * M + r] : 0.0;
     }
-    double **ptrs = nullptr;
-    const size_t pb = 2 * m * sizeof(double *) + m * sizeof(double);
-    SC_HIP(hipMalloc(reinterpret_cast<void **>(&ptrs), pb));
-    std::vector<unsigned char> hostBuf(pb);
-    std::memcpy(hostBuf.data(), accX_.data(), m * sizeof(double *));
-    std::memcpy(hostBuf.data() + m * sizeof(double *), accD_.data(), m * sizeof(double *));
-    std::memcpy(hostBuf.data() + 2 * m * sizeof(double *), w.data(), m * sizeof(double));
-    SC_HIP(hipMemcpyAsync(ptrs, hostBuf.data(), pb, hipMemcpyHostToDevice, stream_));
-    andersonCombine<<<blocksFor(n), 256, 0, stream_>>>(
-        X, ptrs, ptrs + m, reinterpret_cast<double *>(ptrs + 2 * m), static_cast<int>(m), n, st);
-    SC_HIP(hipStreamSynchronize(stream_));
-    (void)hipFree(ptrs);
+    AndersonTerms terms{};
+    for (size_t q = 0; q < m; ++q) {
+      terms.X[q] = accX_[q];
+      terms.D[q] = accD_[q];
+      terms.w[q] = w[q];
+    }
+    andersonCombine<<<blocksFor(n), 256, 0, stream_>>>(X, terms, static_cast<int>(m), n, st);
+    SC_HIP(hipGetLastError());
     ++accIter_;
     break;
   }

@@ -628,13 +628,19 @@ __global__ void diffKernel(double *d, const double *g, const double *x, long lon
   if (i < n) d[i] = g[i] - x[i];
 }
 // x = sum_i (X_i + D_i s) w_i over the history (Anderson.h:336-355)
-__global__ void andersonCombine(double *x, const double *const *X,
-                                const double *const *D, const double *w, int m,
-                                long long n, double s) {
+// x = sum_q w_q (X_q + s D_q) (Anderson.h:329-352); the history pointers and
+// weights travel in the kernel arguments.
+constexpr int kMaxHistory = 16;
+struct AndersonTerms {
+  const double *X[kMaxHistory];
+  const double *D[kMaxHistory];
+  double w[kMaxHistory];
+};
+__global__ void andersonCombine(double *x, AndersonTerms t, int m, long long n, double s) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= n) return;
   double acc = 0.0;
-  for (int q = 0; q < m; ++q) acc += (X[q][i] + D[q][i] * s) * w[q];
+  for (int q = 0; q < m; ++q) acc += (t.X[q][i] + t.D[q][i] * s) * t.w[q];
   x[i] = acc;
 }
 // running mean and step per field (CongAcc.h:271-297)
