@@ -65,7 +65,7 @@ def build(verbose=False):
     objs = [j[-1] for j in jobs]
     tmp = LIB + ".tmp"
     _run([hipcc, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
-         + ["-pthread"])
+         + ["-pthread", "-Wl,--no-undefined"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -111,6 +111,8 @@ Grain::~Grain() {
   devFree(dBOutGt_);
   devFree(dG_);
   devFree(dPartials_);
+  devFree(dRed_);
+  devFree(dMean_);
   devFree(dWork_);
   devFree(dWork2_);
   fft_.reset();
@@ -195,6 +197,9 @@ void Grain::upload() {
   dBInGt_ = devAlloc<double>(size_t(2 * s.nInGt * s.Ne));
   dBOutGt_ = devAlloc<double>(size_t(2 * s.nOutGt * s.Ne));
   dG_ = devAlloc<double>(size_t(s.Na) * 4 * n1);
+  redCap_ = size_t(64) * 1024 * 6;
+  dRed_ = devAlloc<double>(redCap_);
+  dMean_ = devAlloc<double>(2);
   partialsCap_ = 4096;
   dPartials_ = devAlloc<double>(size_t(partialsCap_) * 6);
   dWork_ = devAlloc<double>(nf);
@@ -480,33 +485,98 @@ double residualFromStats(const double *st, long long numElements, int norm) {
 }
 } // namespace
 
-double Grain::residualOf(const double *oldv, const double *newv, long long n) {
+// Deferred reductions.  Each queued reduction writes its block partials to
+// its own range of dRed_; flushReductions() brings every pending range back
+// with one transfer and combines the partials in block order.
+double *Grain::reserveReduction(int kind, long long blocks, long long n, int &handle) {
+  const size_t need = size_t(blocks) * (kind == 0 ? 1 : 6);
+  if (redUsed_ + need > redCap_) {
+    flushReductions(); // keeps the results of everything queued so far
+    if (need > redCap_) {
+      devFree(dRed_);
+      redCap_ = std::max(need, 2 * redCap_);
+      dRed_ = devAlloc<double>(redCap_);
+    }
+  }
+  Pending p;
+  p.kind = kind;
+  p.blocks = blocks;
+  p.n = n;
+  p.offset = redUsed_;
+  p.handle = static_cast<int>(redOut_.size());
+  redOut_.push_back({0, 0, 0, 0, 0, 0});
+  redN_.push_back(n);
+  pend_.push_back(p);
+  redUsed_ += need;
+  handle = p.handle;
+  return dRed_ + p.offset;
+}
+
+int Grain::queueSum(const double *a, const double *b, const uint8_t *mask, long long n,
+                    int op) {
   const long long blocks = std::min<long long>(blocksFor(n), 1024);
-  statsKernel<<<blocks, 256, 0, stream_>>>(oldv, newv, n, dPartials_);
+  int h = -1;
+  double *dst = reserveReduction(0, blocks, n, h);
+  sumKernel<<<blocks, 256, 0, stream_>>>(a, b, mask, n, op, dst);
   SC_HIP(hipGetLastError());
-  std::vector<double> h(size_t(blocks) * 6);
-  SC_HIP(hipMemcpyAsync(h.data(), dPartials_, h.size() * sizeof(double),
-                        hipMemcpyDeviceToHost, stream_));
-  SC_HIP(hipStreamSynchronize(stream_));
-  double st[6] = {0, 0, 0, 0, 0, 0};
-  for (long long b = 0; b < blocks; ++b)
-    for (int q = 0; q < 6; ++q)
-      st[q] = q < 4 ? st[q] + h[size_t(b) * 6 + q] : std::max(st[q], h[size_t(b) * 6 + q]);
-  return residualFromStats(st, n, s_->cfg.norm);
+  return h;
+}
+
+int Grain::queueStats(const double *oldv, const double *newv, long long n) {
+  const long long blocks = std::min<long long>(blocksFor(n), 1024);
+  int h = -1;
+  double *dst = reserveReduction(1, blocks, n, h);
+  statsKernel<<<blocks, 256, 0, stream_>>>(oldv, newv, n, dst);
+  SC_HIP(hipGetLastError());
+  return h;
+}
+
+void Grain::flushReductions() {
+  if (pend_.empty()) return;
+  redHost_.resize(redUsed_);
+  fromDevice(redHost_.data(), dRed_, redUsed_ * sizeof(double));
+  for (const Pending &p : pend_) {
+    double *out = redOut_[size_t(p.handle)].data();
+    const double *h = redHost_.data() + p.offset;
+    if (p.kind == 0) {
+      double sum = 0.0;
+      for (long long b = 0; b < p.blocks; ++b) sum += h[b];
+      out[0] = sum;
+    } else {
+      double st[6] = {0, 0, 0, 0, 0, 0};
+      for (long long b = 0; b < p.blocks; ++b)
+        for (int q = 0; q < 6; ++q)
+          st[q] = q < 4 ? st[q] + h[size_t(b) * 6 + q] : std::max(st[q], h[size_t(b) * 6 + q]);
+      for (int q = 0; q < 6; ++q) out[q] = st[q];
+    }
+  }
+  pend_.clear();
+  redUsed_ = 0;
+}
+
+void Grain::clearReductions() {
+  flushReductions();
+  redOut_.clear();
+  redN_.clear();
+}
+
+double Grain::reductionSum(int h) const { return redOut_[size_t(h)][0]; }
+
+double Grain::reductionResidual(int h) const {
+  return residualFromStats(redOut_[size_t(h)].data(), redN_[size_t(h)], s_->cfg.norm);
+}
+
+double Grain::residualOf(const double *oldv, const double *newv, long long n) {
+  const int h = queueStats(oldv, newv, n);
+  flushReductions();
+  return reductionResidual(h);
 }
 
 double Grain::deviceSum(const double *a, const double *b, const uint8_t *mask,
                         long long n, int op) {
-  const long long blocks = std::min<long long>(blocksFor(n), 1024);
-  sumKernel<<<blocks, 256, 0, stream_>>>(a, b, mask, n, op, dPartials_);
-  SC_HIP(hipGetLastError());
-  std::vector<double> h(blocks);
-  SC_HIP(hipMemcpyAsync(h.data(), dPartials_, h.size() * sizeof(double),
-                        hipMemcpyDeviceToHost, stream_));
-  SC_HIP(hipStreamSynchronize(stream_));
-  double s = 0.0;
-  for (double v : h) s += v;
-  return s;
+  const int h = queueSum(a, b, mask, n, op);
+  flushReductions();
+  return reductionSum(h);
 }
 
 std::vector<double> Grain::multiDot(const double *a, const std::vector<double *> &bs,
@@ -638,46 +708,51 @@ void Grain::autotune(int precision) {
   (void)keepSlots;
 }
 
-void Grain::boundaryCondition() {
-  // BoundaryConditionSpecular::computeBoundaryCondition (:361-486)
+void Grain::queueBoundaryCondition(int handles[2]) {
+  // BoundaryConditionSpecular::computeBoundaryCondition (:361-486): new
+  // in-values from the out-values of the reflected angles, with the change
+  // statistics for the residual.
   const Setup &s = *s_;
-  double r = 0.0;
   for (int dir = 0; dir < 2; ++dir) {
+    handles[dir] = -1;
     const long long nIn = dir == 0 ? s.nInG : s.nInGt;
     const long long nOut = dir == 0 ? s.nOutG : s.nOutGt;
     const long long total = nIn * s.Ne;
     if (total == 0) continue;
     const long long blocks = blocksFor(total);
-    if (blocks > partialsCap_) {
-      devFree(dPartials_);
-      partialsCap_ = blocks;
-      dPartials_ = devAlloc<double>(size_t(partialsCap_) * 6);
-    }
+    double *dst = reserveReduction(1, blocks, 2 * total, handles[dir]);
     boundaryKernel<<<blocks, 256, 0, stream_>>>(
         dir == 0 ? dBInG_ : dBInGt_, dir == 0 ? dBOutG_ : dBOutGt_, nIn, nOut, s.Ne,
         dir == 0 ? dBcOffG_ : dBcOffGt_, dir == 0 ? dBcSrcG_ : dBcSrcGt_,
-        dir == 0 ? dBcWG_ : dBcWGt_, dir == 0 ? dBcWsG_ : dBcWsGt_, dPartials_);
+        dir == 0 ? dBcWG_ : dBcWGt_, dir == 0 ? dBcWsG_ : dBcWsGt_, dst);
     SC_HIP(hipGetLastError());
-    std::vector<double> h(size_t(blocks) * 6);
-    SC_HIP(hipMemcpyAsync(h.data(), dPartials_, h.size() * sizeof(double),
-                          hipMemcpyDeviceToHost, stream_));
-    SC_HIP(hipStreamSynchronize(stream_));
-    double st[6] = {0, 0, 0, 0, 0, 0};
-    for (long long b = 0; b < blocks; ++b)
-      for (int q = 0; q < 6; ++q)
-        st[q] = q < 4 ? st[q] + h[size_t(b) * 6 + q] : std::max(st[q], h[size_t(b) * 6 + q]);
-    r = std::max(r, residualFromStats(st, 2 * total, s.cfg.norm));
   }
-  res_.resBoundary = r;
+}
+
+double Grain::boundaryResidual(const int handles[2]) const {
+  double r = 0.0;
+  for (int dir = 0; dir < 2; ++dir)
+    if (handles[dir] >= 0) r = std::max(r, reductionResidual(handles[dir]));
+  return r;
+}
+
+void Grain::boundaryCondition() {
+  int hb[2];
+  queueBoundaryCondition(hb);
+  flushReductions();
+  res_.resBoundary = boundaryResidual(hb);
 }
 
 void Grain::integrate(int precision, bool burnin) {
-  // IntegrationIteratorOzaki::compute (IntegrationIteratorOzaki.h:303-404)
+  // IntegrationIteratorOzaki::compute (IntegrationIteratorOzaki.h:303-404):
+  // sweeps, back-rotation and angle integration, then the reductions of the
+  // compute objects and the boundary condition, queued for one read-back.
   const Setup &s = *s_;
   const size_t n0 = size_t(s.N0) * s.N0, n1 = size_t(s.N1) * s.N1;
   ensureBoundaryInit();
   extendDelta();
   sweep(precision, burnin);
+  pendingIntegrate_ = !burnin;
   if (!burnin) {
     const int nt = static_cast<int>(s.extRot.target.size());
     if (nt > 0) {
@@ -698,56 +773,75 @@ void Grain::integrate(int precision, bool burnin) {
     SC_HIP(hipGetLastError());
     SC_HIP(hipEventRecord(ev2_, stream_));
     // ComputeOrderParameter::computePropertyFromGreens (:212-262)
-    double rd = 0.0;
+    hDelta_.assign(size_t(s.Nc), -1);
     for (int c = 0; c < s.Nc; ++c)
-      rd = std::max(rd, residualOf(dDelta_ + size_t(c) * 2 * n0,
-                                   dDeltaNew_ + size_t(c) * 2 * n0,
-                                   static_cast<long long>(2 * n0)));
-    res_.resDelta = rd;
-    convDelta_ = rd < s.cfg.convergence;
-    if (s.cfg.computeCurrent) {
-      // ComputeCurrent::computePropertyFromGreens (:175-222)
-      const double actual = residualOf(dCurrentOld_, dCurrent_, static_cast<long long>(2 * n0));
-      res_.resCurrent = firstCurrent_ ? s.cfg.convergence * 0.1 : actual;
-      convCurrent_ = res_.resCurrent < s.cfg.convergence;
-      firstCurrent_ = false;
-    }
+      hDelta_[size_t(c)] = queueStats(dDelta_ + size_t(c) * 2 * n0,
+                                      dDeltaNew_ + size_t(c) * 2 * n0,
+                                      static_cast<long long>(2 * n0));
+    // ComputeCurrent::computePropertyFromGreens (:175-222)
+    hCurrent_ = s.cfg.computeCurrent
+                    ? queueStats(dCurrentOld_, dCurrent_, static_cast<long long>(2 * n0))
+                    : -1;
+    // ComputeFreeEnergy::computePropertyFromGreens (:281-414)
+    hEnergy_ = -1;
+    hCurl_ = -1;
+    hDeltaSq_.assign(size_t(2 * s.Nc), -1);
     if (s.cfg.computeFreeEnergy) {
-      // ComputeFreeEnergy::computePropertyFromGreens (:281-414)
-      double nonMag = s.temperature * deviceSum(dEnergyDens_, nullptr, nullptr,
-                                                static_cast<long long>(n0), 0);
-      for (int c = 0; c < s.Nc; ++c) {
-        const double f = std::log(s.temperature / s.cfg.tc[c]) + s.sumInverseEnergy;
-        const double sq =
-            deviceSum(dDelta_ + size_t(c) * 2 * n0, nullptr, dDomain_,
-                      static_cast<long long>(n0), 2) +
-            deviceSum(dDelta_ + size_t(c) * 2 * n0 + n0, nullptr, dDomain_,
-                      static_cast<long long>(n0), 2);
-        nonMag += sq * f;
-      }
-      double mag = 0.0;
+      hEnergy_ = queueSum(dEnergyDens_, nullptr, nullptr, static_cast<long long>(n0), 0);
+      for (int c = 0; c < s.Nc; ++c)
+        for (int q = 0; q < 2; ++q)
+          hDeltaSq_[size_t(2 * c + q)] =
+              queueSum(dDelta_ + size_t(c) * 2 * n0 + size_t(q) * n0, nullptr, dDomain_,
+                       static_cast<long long>(n0), 2);
       if (s.lambda > 0.0) {
         curlSquareKernel<<<blocksFor(n0), 256, 0, stream_>>>(dA_, s.N0, 1.0 / s.h, dWork_);
         SC_HIP(hipGetLastError());
-        mag = 0.5 / (s.lambda * s.lambda) *
-              deviceSum(dWork_, nullptr, nullptr, static_cast<long long>(n0), 0);
+        hCurl_ = queueSum(dWork_, nullptr, nullptr, static_cast<long long>(n0), 0);
       }
-      const double area = s.h * s.h;
-      const double perArea = (nonMag + mag) / static_cast<double>(s.numGrain);
-      const double actual =
-          std::fabs(fePerAreaOld_ - perArea) / std::max(std::fabs(fePerAreaOld_), 1e-4);
-      res_.resFreeEnergy = firstFE_ ? s.cfg.convergence * 0.1 : actual;
-      convFE_ = res_.resFreeEnergy < s.cfg.convergence;
-      res_.freeEnergy = (nonMag + mag) * area;
-      res_.freeEnergyPerArea = perArea;
-      res_.magneticFreeEnergy = mag * area;
-      res_.magneticFreeEnergyPerArea = mag / static_cast<double>(s.numGrain);
-      fePerAreaOld_ = perArea;
-      firstFE_ = false;
     }
   }
-  boundaryCondition();
-  updateSweepTime();
+  queueBoundaryCondition(hBoundary_);
+}
+
+void Grain::finishIntegrate() {
+  // Host side of the compute objects once the queued reductions are back.
+  const Setup &s = *s_;
+  res_.resBoundary = boundaryResidual(hBoundary_);
+  if (!pendingIntegrate_) return;
+  pendingIntegrate_ = false;
+  double rd = 0.0;
+  for (int c = 0; c < s.Nc; ++c) rd = std::max(rd, reductionResidual(hDelta_[size_t(c)]));
+  res_.resDelta = rd;
+  convDelta_ = rd < s.cfg.convergence;
+  if (s.cfg.computeCurrent) {
+    const double actual = reductionResidual(hCurrent_);
+    res_.resCurrent = firstCurrent_ ? s.cfg.convergence * 0.1 : actual;
+    convCurrent_ = res_.resCurrent < s.cfg.convergence;
+    firstCurrent_ = false;
+  }
+  if (s.cfg.computeFreeEnergy) {
+    double nonMag = s.temperature * reductionSum(hEnergy_);
+    for (int c = 0; c < s.Nc; ++c) {
+      const double f = std::log(s.temperature / s.cfg.tc[c]) + s.sumInverseEnergy;
+      const double sq = reductionSum(hDeltaSq_[size_t(2 * c)]) +
+                        reductionSum(hDeltaSq_[size_t(2 * c + 1)]);
+      nonMag += sq * f;
+    }
+    double mag = 0.0;
+    if (hCurl_ >= 0) mag = 0.5 / (s.lambda * s.lambda) * reductionSum(hCurl_);
+    const double area = s.h * s.h;
+    const double perArea = (nonMag + mag) / static_cast<double>(s.numGrain);
+    const double actual =
+        std::fabs(fePerAreaOld_ - perArea) / std::max(std::fabs(fePerAreaOld_), 1e-4);
+    res_.resFreeEnergy = firstFE_ ? s.cfg.convergence * 0.1 : actual;
+    convFE_ = res_.resFreeEnergy < s.cfg.convergence;
+    res_.freeEnergy = (nonMag + mag) * area;
+    res_.freeEnergyPerArea = perArea;
+    res_.magneticFreeEnergy = mag * area;
+    res_.magneticFreeEnergyPerArea = mag / static_cast<double>(s.numGrain);
+    fePerAreaOld_ = perArea;
+    firstFE_ = false;
+  }
 }
 
 void Grain::updateSweepTime() {
@@ -759,14 +853,22 @@ void Grain::updateSweepTime() {
 
 void Grain::solveVectorPotential(const double *dj, double *dout) {
   // ComputeVectorPotential::solve (:647-745) with the exact Green function:
-  // subtract the grain mean of j, correlate with G, scale by -h^2.
+  // subtract the grain mean of j, correlate with G, scale by -h^2.  The
+  // means are reduced on the device so that the solve needs no read-back.
   const Setup &s = *s_;
   const size_t n0 = size_t(s.N0) * s.N0;
-  const double mx = deviceSum(dj, nullptr, nullptr, static_cast<long long>(n0), 0) /
-                    static_cast<double>(s.numGrain);
-  const double my = deviceSum(dj + n0, nullptr, nullptr, static_cast<long long>(n0), 0) /
-                    static_cast<double>(s.numGrain);
-  fft_->solve(dj, mx, my, dout);
+  for (int d = 0; d < 2; ++d) {
+    const long long blocks = std::min<long long>(blocksFor(static_cast<long long>(n0)), 1024);
+    int h = -1;
+    double *part = reserveReduction(0, blocks, static_cast<long long>(n0), h);
+    sumKernel<<<blocks, 256, 0, stream_>>>(dj + size_t(d) * n0, nullptr, nullptr,
+                                           static_cast<long long>(n0), 0, part);
+    SC_HIP(hipGetLastError());
+    finishMeanKernel<<<1, 1, 0, stream_>>>(part, blocks, static_cast<double>(s.numGrain),
+                                           dMean_ + d);
+    SC_HIP(hipGetLastError());
+  }
+  fft_->solve(dj, dMean_, dout);
 }
 
 void Grain::acceleratorStep() {
@@ -972,12 +1074,18 @@ void Grain::compute(int precision) {
   hipEvent_t start;
   SC_HIP(hipEventCreate(&start));
   SC_HIP(hipEventRecord(start, stream_));
+  clearReductions();
   if (!firstCompute_) acceleratorStep();
   integrate(precision, false);
+  int hA = -1;
   if (doA) {
     solveVectorPotential(dCurrent_, dANew_);
-    res_.resVectorPotential = residualOf(dA_, dANew_, static_cast<long long>(2 * n0));
+    hA = queueStats(dA_, dANew_, static_cast<long long>(2 * n0));
   }
+  flushReductions();
+  finishIntegrate();
+  updateSweepTime();
+  if (doA) res_.resVectorPotential = reductionResidual(hA);
   res_.iterations++;
   firstCompute_ = false;
   // Context::getResidual / isPartiallyConverged (Context.h:462-497)
@@ -1004,8 +1112,11 @@ void Grain::compute(int precision) {
 
 void Grain::burnin(int precision) {
   // Context::burnIn (Context.h:377-389)
+  clearReductions();
   integrate(precision, true);
-  SC_HIP(hipStreamSynchronize(stream_));
+  flushReductions();
+  finishIntegrate();
+  updateSweepTime();
   res_.iterations++;
   res_.residual = res_.resBoundary;
 }
@@ -1062,10 +1173,13 @@ void poissonSolve(int n, double h, int numGrain, bool enforce, const double *cur
   double *dj = nullptr, *dout = nullptr;
   try {
     PoissonFFT fft(n, h, stream);
-    dj = devAlloc<double>(2 * n0);
+    dj = devAlloc<double>(2 * n0 + 2);
     dout = devAlloc<double>(2 * n0);
+    const double means[2] = {mx, my};
     SC_HIP(hipMemcpyAsync(dj, current, 2 * n0 * sizeof(double), hipMemcpyHostToDevice, stream));
-    fft.solve(dj, mx, my, dout);
+    SC_HIP(hipMemcpyAsync(dj + 2 * n0, means, 2 * sizeof(double), hipMemcpyHostToDevice,
+                          stream));
+    fft.solve(dj, dj + 2 * n0, dout);
     SC_HIP(hipMemcpyAsync(out, dout, 2 * n0 * sizeof(double), hipMemcpyDeviceToHost, stream));
     SC_HIP(hipStreamSynchronize(stream));
   } catch (...) {

@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <memory>
 #include <string>
 #include <vector>
@@ -86,10 +87,21 @@ private:
   void extendDelta();
   void ensureBoundaryInit();
   void integrate(int precision, bool burnin);
+  void finishIntegrate();
   void acceleratorStep();
   double residualOf(const double *oldv, const double *newv, long long n);
   double deviceSum(const double *a, const double *b, const uint8_t *mask,
                    long long n, int op);
+  // deferred reductions: queue now, read back together at flushReductions()
+  double *reserveReduction(int kind, long long blocks, long long n, int &handle);
+  int queueSum(const double *a, const double *b, const uint8_t *mask, long long n, int op);
+  int queueStats(const double *oldv, const double *newv, long long n);
+  void flushReductions();
+  void clearReductions();
+  double reductionSum(int handle) const;
+  double reductionResidual(int handle) const;
+  void queueBoundaryCondition(int handles[2]);
+  double boundaryResidual(const int handles[2]) const;
   void prepareSweep(int precision);
   std::vector<double> multiDot(const double *a, const std::vector<double *> &bs,
                                long long n);
@@ -122,6 +134,23 @@ private:
   double *dG_ = nullptr;
   double *dPartials_ = nullptr, *dWork_ = nullptr, *dWork2_ = nullptr;
   std::unique_ptr<PoissonFFT> fft_;
+  struct Pending {
+    int kind = 0; // 0: sum (one partial per block), 1: change statistics (six)
+    long long blocks = 0, n = 0;
+    size_t offset = 0;
+    int handle = -1;
+  };
+  std::vector<Pending> pend_;
+  std::vector<std::array<double, 6>> redOut_;
+  std::vector<long long> redN_;
+  std::vector<double> redHost_;
+  double *dRed_ = nullptr;
+  size_t redCap_ = 0, redUsed_ = 0;
+  double *dMean_ = nullptr; // grain means of the current (Poisson solve)
+  // handles of the reductions queued by integrate()
+  std::vector<int> hDelta_, hDeltaSq_;
+  int hCurrent_ = -1, hEnergy_ = -1, hCurl_ = -1, hBoundary_[2] = {-1, -1};
+  bool pendingIntegrate_ = false;
   long long partialsCap_ = 0;
   // sweep plans, one per precision
   struct SweepPlan {

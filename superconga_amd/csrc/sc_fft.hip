@@ -78,16 +78,17 @@ __device__ void ditInverse(double2 *x, int M, int logM, int count, const double2
   }
 }
 
-// Rows forward: row y < nRows of (re - subRe) + i (im - subIm), columns
+// Rows forward: row y < nRows of (re - sub[0]) + i (im - sub[1]), columns
 // x < nCols (zero beyond), transformed and stored to work[y][.].
 __global__ __launch_bounds__(kThreads) void rowsForwardKernel(
-    const double *re, const double *im, int ld, int nRows, int nCols, double subRe,
-    double subIm, int M, int logM, const double2 *tw, double2 *work) {
+    const double *re, const double *im, int ld, int nRows, int nCols, const double *sub,
+    int M, int logM, const double2 *tw, double2 *work) {
   extern __shared__ double2 lds[];
   const int y = blockIdx.x;
   if (y >= nRows) return;
   const double *rr = re + size_t(y) * ld;
   const double *ri = im ? im + size_t(y) * ld : nullptr;
+  const double subRe = sub ? sub[0] : 0.0, subIm = sub ? sub[1] : 0.0;
   for (int x = threadIdx.x; x < M; x += blockDim.x) {
     double2 v = make_double2(0.0, 0.0);
     if (x < nCols) {
@@ -215,7 +216,7 @@ PoissonFFT::PoissonFFT(int n, double h, hipStream_t stream)
                               hipMemcpyHostToDevice, stream_));
     const int cb = columnsPerBlock(M);
     rowsForwardKernel<<<M, kThreads, sizeof(double2) * M, stream_>>>(
-        dk, nullptr, M, M, M, 0.0, 0.0, M, logm_, tw_, work_);
+        dk, nullptr, M, M, M, nullptr, M, logm_, tw_, work_);
     SC_FFT_HIP(hipGetLastError());
     columnsKernel<<<M / cb, kThreads, sizeof(double2) * M * cb, stream_>>>(
         work_, M, 0, M, logm_, cb, tw_, nullptr, khat_);
@@ -237,12 +238,12 @@ PoissonFFT::~PoissonFFT() {
   (void)hipFree(work_);
 }
 
-void PoissonFFT::solve(const double *dj, double mx, double my, double *dout) {
+void PoissonFFT::solve(const double *dj, const double *dMean, double *dout) {
   const int M = m_, n = n_;
   const int cb = columnsPerBlock(M);
   const size_t plane = size_t(n) * n;
   rowsForwardKernel<<<n, kThreads, sizeof(double2) * M, stream_>>>(
-      dj, dj + plane, n, n, n, mx, my, M, logm_, tw_, work_);
+      dj, dj + plane, n, n, n, dMean, M, logm_, tw_, work_);
   SC_FFT_HIP(hipGetLastError());
   columnsKernel<<<M / cb, kThreads, sizeof(double2) * M * cb, stream_>>>(
       work_, n, n, M, logm_, cb, tw_, khat_, nullptr);

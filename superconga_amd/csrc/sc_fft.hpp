@@ -34,9 +34,10 @@ public:
   PoissonFFT(const PoissonFFT &) = delete;
   PoissonFFT &operator=(const PoissonFFT &) = delete;
 
-  // out (2 planes n x n) = -h^2 G * (j - (mx, my)); j is 2 planes n x n.
-  // Device pointers; runs on the stream given at construction.
-  void solve(const double *dj, double mx, double my, double *dout);
+  // out (2 planes n x n) = -h^2 G * (j - mean); j is 2 planes n x n and
+  // mean points to the two means (x, y).  Device pointers; runs on the
+  // stream given at construction.
+  void solve(const double *dj, const double *dMean, double *dout);
 
   int length() const { return m_; }
   int n() const { return n_; }

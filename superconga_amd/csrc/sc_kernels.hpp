@@ -659,6 +659,14 @@ __global__ void axpyKernel(double *x, const double *d, long long n, double s) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i < n) x[i] += d[i] * s;
 }
+// mean = (sum of the block partials, in block order) / count
+__global__ void finishMeanKernel(const double *partials, long long blocks, double count,
+                                 double *mean) {
+  double s = 0.0;
+  for (long long b = 0; b < blocks; ++b) s += partials[b];
+  *mean = s / count;
+}
+
 // Dot products of one vector with up to kMaxDots others in one pass over it:
 // partials[block * m + i] = block-partial of sum a * b_i.
 constexpr int kMaxDots = 16;
