@@ -151,6 +151,9 @@ int sc_riccati_steps(int n, const double *in, double *out, int precision);
 int sc_num_ozaki(double temperature, double cutoff);
 int sc_ozaki(int num_poles, double *poles, double *residues);
 double sc_bulk_gap(double temperature);
+/* Reference-frame domain, 1 inside the grain and 0 outside (N0*N0;
+ * GeometryGroup::getDomainGrid, GeometryGroup.h:906-925). */
+int sc_get_domain(sc_context *ctx, signed char *domain);
 /* Geometry labels of frame `angle` (N1*N1, GeometryGroup::create(-theta)). */
 int sc_get_labels(sc_context *ctx, int angle, signed char *labels);
 /* Host-only planning (no device needed): the sizes and, if labels is not

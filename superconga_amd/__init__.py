@@ -102,6 +102,7 @@ def lib():
         L.sc_bulk_gap.restype = ctypes.c_double
         L.sc_bulk_gap.argtypes = [ctypes.c_double]
         L.sc_get_labels.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_byte)]
+        L.sc_get_domain.argtypes = [vp, ctypes.POINTER(ctypes.c_byte)]
         L.sc_describe.argtypes = [ctypes.c_char_p, ctypes.c_int,
                                   ctypes.POINTER(Dims), ctypes.POINTER(ctypes.c_byte)]
         _lib = L
@@ -211,6 +212,12 @@ class Simulation:
         re = np.ascontiguousarray(v.real)
         im = np.ascontiguousarray(v.imag)
         _check(lib().sc_set_boundary(self._h, direction, _dp(re), _dp(im)))
+
+    def domain(self):
+        """Reference-frame domain: 1 inside the grain, 0 outside."""
+        out = np.empty(self.N0 * self.N0, dtype=np.int8)
+        _check(lib().sc_get_domain(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_byte))))
+        return out.reshape(self.N0, self.N0)
 
     def labels(self, angle):
         out = np.empty(self.N1 * self.N1, dtype=np.int8)

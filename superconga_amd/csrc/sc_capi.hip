@@ -379,6 +379,14 @@ int sc_describe(const char *config_json, int angle, sc_dims *out, signed char *l
   });
 }
 
+int sc_get_domain(sc_context *ctx, signed char *domain) {
+  SC_CHECK_CTX(ctx);
+  SC_CHECK_PTR(domain);
+  const scamd::Setup &s = ctx->grain->setup();
+  for (size_t i = 0; i < s.domain.size(); ++i) domain[i] = static_cast<signed char>(s.domain[i]);
+  return SC_OK;
+}
+
 int sc_get_labels(sc_context *ctx, int angle, signed char *labels) {
   SC_CHECK_CTX(ctx);
   SC_CHECK_PTR(labels);

@@ -12,6 +12,9 @@ Fixtures written (data only, no reference code):
                            solver, so it pins any pole solver.
   known_answers.json    -- scalar known answers held by the reference's tests,
                            each with the file:line it is taken from.
+  examples/<name>.json  -- the reference's example simulation configs
+                           (examples/<name>/simulation_config.json), inputs
+                           for the drop-in config tests.
 """
 import json
 import os
@@ -64,6 +67,16 @@ def main():
     }
     with open(os.path.join(HERE, "known_answers.json"), "w") as f:
         json.dump(answers, f, indent=1)
+    ex_dir = os.path.join(HERE, "examples")
+    os.makedirs(ex_dir, exist_ok=True)
+    ex_root = os.path.join(root, "examples")
+    for name in sorted(os.listdir(ex_root)):
+        cfg_path = os.path.join(ex_root, name, "simulation_config.json")
+        if os.path.isfile(cfg_path):
+            with open(cfg_path) as f:
+                cfg = json.load(f)
+            with open(os.path.join(ex_dir, name + ".json"), "w") as f:
+                json.dump(cfg, f, indent=1)
     print("wrote %d poles from %s; omega %.10f from %s" % (len(poles), src, omega, osrc))
 
 

@@ -115,3 +115,34 @@ def test_plan_octagon_benchmark_sizes(product):
     # About 2 Na Ne N_grain steps: the in-grain share of the frames.
     approx = 2 * d.num_angles * d.num_energies * d.num_grain_sites
     assert 0.9 < d.traj_steps / approx < 1.2
+
+
+EXAMPLES = os.path.join(GOLDEN, "examples")
+
+
+def example_config(name):
+    with open(os.path.join(EXAMPLES, name + ".json")) as f:
+        cfg = json.load(f)
+    # The oracle implements the exact Poisson kernel only (error 0); the
+    # product uses the exact kernel for every setting.
+    cfg["numerics"]["vector_potential_error"] = 0.0
+    return cfg
+
+
+@pytest.mark.parametrize("name", sorted(n[:-5] for n in os.listdir(EXAMPLES)))
+def test_reference_example_configs_plan_like_oracle(product, oracle, name):
+    """The reference's own example configs (examples/*/simulation_config.json)
+    are accepted unchanged and planned exactly like the oracle plans them."""
+    cfg = example_config(name)
+    d, _ = product.describe(cfg)
+    o = oracle.Oracle(cfg)
+    assert (d.n0, d.n1, d.num_energies, d.num_angles, d.num_components) == (
+        o.N0, o.N1, o.Ne, o.Na, o.Nc)
+    assert d.num_grain_sites == o.get_int("num_grain_sites")
+    assert d.traj_steps == o.get_int("traj_steps")
+    assert (d.num_in_gamma, d.num_out_gamma, d.num_in_gammatilde, d.num_out_gammatilde) == (
+        o.get_int("num_storage_in_gamma"), o.get_int("num_storage_out_gamma"),
+        o.get_int("num_storage_in_gammatilde"), o.get_int("num_storage_out_gammatilde"))
+    for k in sorted({0, o.Na // 8, o.Na // 3, o.Na - 1}):
+        _, lab = product.describe(cfg, angle=k)
+        assert np.array_equal(lab, o.geometry(k)[0]), k

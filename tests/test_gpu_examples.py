@@ -1,0 +1,55 @@
+"""The reference's example configs and the on-disk formats on the device.
+
+Parity bar as in test_gpu_parity.py: every grid point within 1e-10 of the
+oracle relative to the field maximum, FP64."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from test_capi import example_config
+from test_gpu_parity import TOL, compare_state, pair, rel
+
+pytestmark = pytest.mark.gpu
+
+SMALL = ["dwave_chiral", "dwave_octagon", "dwave_phase_crystal", "dwave_plus_swave",
+         "swave_disc_meissner"]
+
+
+@pytest.mark.parametrize("name", SMALL)
+def test_reference_example_iterations(product, oracle, name):
+    cfg = example_config(name)
+    s, o = pair(product, oracle, cfg)
+    for it in range(2):
+        s.compute("fp64")
+        o.compute()
+        compare_state(s, o, cfg, what="%s it %d" % (name, it))
+
+
+def test_simulate_writes_reference_files_and_warm_starts(product, oracle, tmp_path):
+    from superconga_amd import io as scio
+    cfg = example_config("swave_disc_meissner")
+    cfg["numerics"]["num_iterations_max"] = 3
+    cfg["misc"] = {"save_path": str(tmp_path / "run"), "save_frequency": -1,
+                   "load_path": "", "data_format": "csv", "verbose": False}
+    converged, results = scio.simulate(cfg)
+    run = tmp_path / "run"
+    for f in ("domain.csv", "order_parameter.csv", "current_density.csv",
+              "vector_potential.csv", "magnetic_flux_density.csv",
+              "simulation_results.json", "internal_parameters.json",
+              "simulation_convergence.csv", "simulation_config.json"):
+        assert (run / f).exists(), f
+    conv = (run / "simulation_convergence.csv").read_text().splitlines()
+    assert len(conv) == 1 + 3
+    assert results["free_energy_per_unit_area"] < 0
+    # A warm start reads the written gap-equation result back as the iterate.
+    saved = scio.read_csv(str(run / "order_parameter.csv"))
+    s = product.Simulation(cfg)
+    scio.load_state(s, cfg, str(run))
+    got = s.order_parameter(0)[0]
+    want = (saved["s_re"] + 1j * saved["s_im"]).reshape(s.N0, s.N0)
+    dom = s.domain().astype(bool)
+    assert rel(np.where(dom, got, 0), np.where(dom, want, 0)) < 1e-8
+    s.compute("fp64")
+    assert np.isfinite(s.results()["free_energy_per_area"])
