@@ -117,8 +117,10 @@ Grain::~Grain() {
   devFree(dWork2_);
   fft_.reset();
   if (dScratch_) (void)hipFree(dScratch_);
-  for (SweepPlan &pl : plan_)
+  for (SweepPlan &pl : plan_) {
     if (pl.dItems) (void)hipFree(pl.dItems);
+    if (pl.dView) (void)hipFree(pl.dView);
+  }
   devFree(dCounter_);
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
@@ -249,22 +251,20 @@ void Grain::prepareSweep(int precision) {
   // the buffers below may still be read by a sweep in flight
   SC_HIP(hipStreamSynchronize(stream_));
   const Setup &s = *s_;
-  pl.groups = (s.Ne + pl.el - 1) / pl.el;
-  int gp = 1;
-  while (gp < pl.groups) gp <<= 1;
-  if (gp > 64)
-    throw std::runtime_error("too many Ozaki energies for the lane layout; "
-                             "raise the energies per lane");
-  pl.cols = 64 / gp;
+  const GrainView gv = sweepView(pl.el);
+  pl.groups = gv.groups;
+  pl.cols = gv.cols;
   std::vector<SweepItem> items = sweepItems(s, pl.cols, &pl.maxLen);
   pl.numItems = static_cast<int>(items.size());
   std::vector<SweepItemDev> di(items.size());
   for (size_t i = 0; i < items.size(); ++i)
-    di[i] = {items[i].angle, items[i].x0, items[i].ncols, items[i].ylo, items[i].yhi};
+    di[i] = {0, items[i].angle, items[i].x0, items[i].ncols, items[i].ylo, items[i].yhi};
   if (pl.dItems) (void)hipFree(pl.dItems);
   pl.dItems = nullptr;
   SC_HIP(hipMalloc(&pl.dItems, std::max<size_t>(1, di.size()) * sizeof(SweepItemDev)));
   if (!di.empty()) toDevice(pl.dItems, di.data(), di.size() * sizeof(SweepItemDev));
+  if (!pl.dView) SC_HIP(hipMalloc(&pl.dView, sizeof(GrainView)));
+  toDevice(pl.dView, &gv, sizeof(GrainView));
   int dev = 0, cus = 0;
   SC_HIP(hipGetDevice(&dev));
   SC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -290,7 +290,7 @@ static GrainView makeView(const Setup &s, const float *trig, const int *idx,
                           const double *saSin, const double *vFx,
                           const double *vFy, const double *weight,
                           const double *basis, const int8_t *labels) {
-  GrainView v;
+  GrainView v{};
   v.N0 = s.N0;
   v.N1 = s.N1;
   v.Ne = s.Ne;
@@ -619,13 +619,29 @@ void Grain::extendDelta() {
   }
 }
 
+GrainView Grain::view() const {
+  return makeView(*s_, dTrig_, dIdx_, dDist_, dBInG_, dBOutG_, dBInGt_, dBOutGt_,
+                  dExtDelta_, dA_, dG_, dEnergy_, dResidue_, dSaCos_, dSaSin_, dVFx_, dVFy_,
+                  dWeight_, dBasis_, dLabels_);
+}
+
+GrainView Grain::sweepView(int el) const {
+  GrainView v = view();
+  v.groups = (s_->Ne + el - 1) / el;
+  int gp = 1;
+  while (gp < v.groups) gp <<= 1;
+  if (gp > 64)
+    throw std::runtime_error("too many Ozaki energies for the lane layout; "
+                             "raise the energies per lane");
+  v.cols = 64 / gp;
+  return v;
+}
+
 void Grain::ensureBoundaryInit() {
   if (boundaryInit_) return;
   const Setup &s = *s_;
   extendDelta();
-  GrainView v = makeView(s, dTrig_, dIdx_, dDist_, dBInG_, dBOutG_, dBInGt_, dBOutGt_,
-                         dExtDelta_, dA_, dG_, dEnergy_, dResidue_, dSaCos_, dSaSin_,
-                         dVFx_, dVFy_, dWeight_, dBasis_, dLabels_);
+  GrainView v = view();
   const long long n = static_cast<long long>(s.Na) * s.N1 * s.Ne;
   initialBoundaryKernel<<<blocksFor(n), 256, 0, stream_>>>(v, nullptr);
   SC_HIP(hipGetLastError());
@@ -638,34 +654,34 @@ static void launchSweep(const SweepArgs &a, int slots, hipStream_t st) {
   sweepKernel<T, EL, W><<<slots, 64, 0, st>>>(a);
 }
 
+void launchSweepVariant(int precision, int el, int waves, const SweepArgs &a, int slots,
+                        hipStream_t stream) {
+  const bool dp = precision == PREC_FP64;
+#define SC_LAUNCH(E, W)                                                              \
+  if (el == E && waves == W) {                                                       \
+    if (dp) launchSweep<double, E, W>(a, slots, stream);                             \
+    else launchSweep<float, E, W>(a, slots, stream);                                 \
+  } else
+  SC_SWEEP_VARIANTS(SC_LAUNCH)
+#undef SC_LAUNCH
+  throw std::runtime_error("sweep variant not compiled");
+  SC_HIP(hipGetLastError());
+}
+
 void Grain::sweep(int precision, bool burnin) {
   prepareSweep(precision);
   const SweepPlan &pl = plan_[precision];
-  const Setup &s = *s_;
   SweepArgs a;
-  a.v = makeView(s, dTrig_, dIdx_, dDist_, dBInG_, dBOutG_, dBInGt_, dBOutGt_,
-                 dExtDelta_, dA_, dG_, dEnergy_, dResidue_, dSaCos_, dSaSin_, dVFx_,
-                 dVFy_, dWeight_, dBasis_, dLabels_);
+  a.views = reinterpret_cast<const GrainView *>(pl.dView);
   a.items = reinterpret_cast<const SweepItemDev *>(pl.dItems);
   a.numItems = pl.numItems;
-  a.cols = pl.cols;
-  a.groups = pl.groups;
   a.counter = dCounter_;
   a.scratch = dScratch_;
   a.slotStride = static_cast<long long>(pl.maxLen) * pl.el * 64;
   a.burnin = burnin ? 1 : 0;
   resetCounter<<<1, 1, 0, stream_>>>(dCounter_);
   SC_HIP(hipEventRecord(ev0_, stream_));
-  const bool dp = precision == PREC_FP64;
-#define SC_LAUNCH(E, W)                                                              \
-  if (pl.el == E && pl.waves == W) {                                                 \
-    if (dp) launchSweep<double, E, W>(a, pl.numSlots, stream_);                      \
-    else launchSweep<float, E, W>(a, pl.numSlots, stream_);                          \
-  } else
-  SC_SWEEP_VARIANTS(SC_LAUNCH)
-#undef SC_LAUNCH
-  throw std::runtime_error("sweep variant not compiled");
-  SC_HIP(hipGetLastError());
+  launchSweepVariant(precision, pl.el, pl.waves, a, pl.numSlots, stream_);
   SC_HIP(hipEventRecord(ev1_, stream_));
 }
 
@@ -763,9 +779,7 @@ void Grain::integrate(int precision, bool burnin) {
     }
     SC_HIP(hipMemcpyAsync(dCurrentOld_, dCurrent_, 2 * n0 * sizeof(double),
                           hipMemcpyDeviceToDevice, stream_));
-    GrainView v = makeView(s, dTrig_, dIdx_, dDist_, dBInG_, dBOutG_, dBInGt_, dBOutGt_,
-                           dExtDelta_, dA_, dG_, dEnergy_, dResidue_, dSaCos_, dSaSin_,
-                           dVFx_, dVFy_, dWeight_, dBasis_, dLabels_);
+    GrainView v = view();
     const int ns = static_cast<int>(s.domainSites.size());
     const double jf = 2.0 * static_cast<double>(s.cfg.chargeSign) * s.temperature;
     gatherKernel<<<blocksFor(ns), 256, 0, stream_>>>(v, dDomainSites_, ns, dCoupling_, jf,

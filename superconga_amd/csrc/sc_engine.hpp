@@ -2,6 +2,7 @@
 #pragma once
 
 #include "sc_fft.hpp"
+#include "sc_view.hpp"
 #include "sc_setup.hpp"
 
 #include <hip/hip_runtime.h>
@@ -75,6 +76,11 @@ public:
   // fastest for `precision`; the sweeps it runs leave the state unchanged.
   void autotune(int precision);
   hipStream_t stream() const { return stream_; }
+
+  // Device view of the grain, and with the lane layout of `el` energies
+  // per lane (for batched sweeps).
+  GrainView view() const;
+  GrainView sweepView(int el) const;
 
   // device pointers for zero-copy callers
   double *devDelta() { return dDelta_; }
@@ -157,6 +163,7 @@ private:
     int el = 8, waves = 2, slotsPerCU = 8;
     int cols = 0, groups = 0, maxLen = 0, numItems = 0, numSlots = 0;
     void *dItems = nullptr;
+    void *dView = nullptr; // device copy of the grain view with this lane layout
     bool ready = false;
   };
   SweepPlan plan_[2];

@@ -2,48 +2,9 @@
 #pragma once
 
 #include "sc_device.hpp"
+#include "sc_view.hpp"
 
 namespace scamd {
-
-// Per-grain constants and device pointers consumed by the kernels.
-struct GrainView {
-  int N0, N1, Ne, Na, Nc;
-  double h;
-  double lambdaInv2;  // 1 / lambda^2, 0 when the induced potential is off
-  double fluxDensity; // pi * Phi / area
-  double area;
-  int gauge;          // 0 landau, 1 symmetric, 2 solenoid
-  int charge;
-  int useInduced;     // lambda > 0
-  int useField;       // any vector potential (induced or external)
-  const double *energy, *residue;
-  const float *opCos, *opSin, *aCos, *aSin, *bkCos, *bkSin;
-  const double *saCos, *saSin, *vFx, *vFy, *weight, *basis;
-  const int8_t *labels;
-  const int *idxInG, *idxOutG, *idxInGt, *idxOutGt;
-  const double *distInG, *distOutG, *distInGt, *distOutGt;
-  double *bInG, *bOutG, *bInGt, *bOutGt; // planar [re: n*Ne][im: n*Ne]
-  long long nInG, nOutG, nInGt, nOutGt;
-  const double *extDelta; // [Nc][2][N0^2]
-  const double *A;        // [2][N0^2] (scaled by lambda^2)
-  double *G;              // [Na][N1^2][4]: Re/Im of f + f~*, Re g, Re i(D* g - D g~)
-};
-
-struct SweepItemDev {
-  int angle, x0, ncols, ylo, yhi;
-};
-
-struct SweepArgs {
-  GrainView v;
-  const SweepItemDev *items;
-  int numItems;
-  int cols;   // columns per item (lanes per energy group)
-  int groups; // energy groups actually used (ceil(Ne / EL))
-  int *counter;
-  void *scratch;       // per slot: [maxLen][EL][64] complex T
-  long long slotStride; // elements of complex T per slot
-  int burnin;
-};
 
 // Order parameter and Doppler shift at site (x, y) of frame k, from the
 // extended order parameter and the vector potential on the N0 grid
@@ -113,9 +74,9 @@ __global__ void resetCounter(int *c) { *c = 0; }
 template <typename T, int EL>
 __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev &it,
                                           Cx<T> *scr) {
-  const GrainView &v = a.v;
+  const GrainView &v = a.views[it.grain];
   const int lane = threadIdx.x;
-  const int cols = a.cols;
+  const int cols = v.cols;
   const int gp = 64 / cols; // lanes per column
   const int g = lane / cols;
   const int c = lane - g * cols;
@@ -125,7 +86,7 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev
   const int k = it.angle;
   const bool colActive = c < it.ncols;
   const int x = it.x0 + (colActive ? c : it.ncols - 1);
-  const bool active = colActive && (g < a.groups);
+  const bool active = colActive && (g < v.groups);
   const int8_t *lab = v.labels + size_t(k) * n1 + x;
   T ser[EL], res[EL];
   bool eOk[EL];
@@ -371,7 +332,9 @@ __global__ __launch_bounds__(64, WAVES) void sweepKernel(SweepArgs a) {
   for (;;) {
     int item = 0;
     if (threadIdx.x == 0) item = atomicAdd(a.counter, 1);
-    item = __shfl(item, 0);
+    // uniform across the wavefront: the item and its grain view are read
+    // with scalar loads
+    item = __builtin_amdgcn_readfirstlane(__shfl(item, 0));
     if (item >= a.numItems) break;
     sweepItem<T, EL>(a, a.items[item], scr);
   }

@@ -1,0 +1,51 @@
+// sc_view.hpp -- the per-grain view the kernels read and the sweep's work
+// description (no kernels: safe to include from host-side translation units).
+#pragma once
+
+#include <cstdint>
+
+namespace scamd {
+
+// Per-grain constants and device pointers consumed by the kernels.
+struct GrainView {
+  int N0, N1, Ne, Na, Nc;
+  double h;
+  double lambdaInv2;  // 1 / lambda^2, 0 when the induced potential is off
+  double fluxDensity; // pi * Phi / area
+  double area;
+  int gauge;          // 0 landau, 1 symmetric, 2 solenoid
+  int charge;
+  int useInduced;     // lambda > 0
+  int useField;       // any vector potential (induced or external)
+  const double *energy, *residue;
+  const float *opCos, *opSin, *aCos, *aSin, *bkCos, *bkSin;
+  const double *saCos, *saSin, *vFx, *vFy, *weight, *basis;
+  const int8_t *labels;
+  const int *idxInG, *idxOutG, *idxInGt, *idxOutGt;
+  const double *distInG, *distOutG, *distInGt, *distOutGt;
+  double *bInG, *bOutG, *bInGt, *bOutGt; // planar [re: n*Ne][im: n*Ne]
+  long long nInG, nOutG, nInGt, nOutGt;
+  const double *extDelta; // [Nc][2][N0^2]
+  const double *A;        // [2][N0^2] (scaled by lambda^2)
+  double *G;              // [Na][N1^2][4]: Re/Im of f + f~*, Re g, Re i(D* g - D g~)
+  int cols;   // sweep lane layout: columns per item
+  int groups; // energy groups per column (ceil(Ne / EL))
+};
+
+struct SweepItemDev {
+  int grain, angle, x0, ncols, ylo, yhi;
+};
+
+// One launch sweeps the items of one or several grains (a batch); each item
+// names its grain's view.
+struct SweepArgs {
+  const GrainView *views;
+  const SweepItemDev *items;
+  int numItems;
+  int *counter;
+  void *scratch;       // per slot: [maxLen][EL][64] complex T
+  long long slotStride; // elements of complex T per slot
+  int burnin;
+};
+
+} // namespace scamd
