@@ -34,6 +34,7 @@ extern "C" {
 #define SC_PRECISION_MIXED_REFINE 2 /* sc_run only: MIXED to the FP32 floor, then FP64 */
 
 typedef struct sc_context sc_context;
+typedef struct sc_batch sc_batch;
 
 typedef struct {
   int n0;               /* reference lattice side (Parameters::getGridResolutionBase) */
@@ -160,6 +161,34 @@ int sc_get_labels(sc_context *ctx, int angle, signed char *labels);
  * null, the labels of frame `angle` that sc_create would build for a config. */
 int sc_describe(const char *config_json, int angle, sc_dims *out,
                 signed char *labels);
+
+/* ----- Batches: many grains of a parameter sweep iterated together -----
+ * No counterpart in the reference, whose sweeps are sequential processes
+ * warm-started from files (examples/parameter_sweeps/*.sh).  Every iteration
+ * runs the per-grain stages on one stream and sweeps all grains that are
+ * still iterating in one persistent launch over a shared work queue.  The
+ * handles returned by sc_batch_grain are borrowed: they work with every
+ * per-context accessor above and are destroyed with the batch. */
+int sc_batch_create(const char *const *config_json, int count, sc_batch **out);
+int sc_batch_destroy(sc_batch *batch);
+int sc_batch_size(sc_batch *batch, int *count);
+int sc_batch_grain(sc_batch *batch, int index, sc_context **out);
+int sc_batch_compute(sc_batch *batch, int precision);
+int sc_batch_burnin(sc_batch *batch, int precision);
+/* runSimulation per grain (limits of the first config when negative); each
+ * grain stops when converged with the minimum reached, or at the maximum.
+ * converged (count entries, may be null) receives the final status. */
+int sc_batch_run(sc_batch *batch, int num_burnin, int num_min, int num_max,
+                 int precision, int *converged);
+int sc_batch_set_sweep_shape(sc_batch *batch, int precision, int energies_per_lane,
+                             int waves_per_simd, int slots_per_cu);
+int sc_batch_get_sweep_shape(sc_batch *batch, int precision, int *energies_per_lane,
+                             int *waves_per_simd, int *slots_per_cu);
+int sc_batch_autotune(sc_batch *batch, int precision);
+/* Last batched iteration: sweep kernel time, whole iteration, and the
+ * trajectory steps it took. */
+int sc_batch_timing(sc_batch *batch, double *sweep_ms, double *iteration_ms,
+                    long long *traj_steps);
 
 #ifdef __cplusplus
 }

@@ -103,6 +103,20 @@ def lib():
         L.sc_bulk_gap.argtypes = [ctypes.c_double]
         L.sc_get_labels.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_byte)]
         L.sc_get_domain.argtypes = [vp, ctypes.POINTER(ctypes.c_byte)]
+        L.sc_batch_create.argtypes = [ctypes.POINTER(ctypes.c_char_p), ctypes.c_int,
+                                      ctypes.POINTER(vp)]
+        L.sc_batch_destroy.argtypes = [vp]
+        L.sc_batch_size.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+        L.sc_batch_grain.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp)]
+        L.sc_batch_compute.argtypes = [vp, ctypes.c_int]
+        L.sc_batch_burnin.argtypes = [vp, ctypes.c_int]
+        L.sc_batch_run.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.sc_batch_set_sweep_shape.argtypes = [vp] + [ctypes.c_int] * 4
+        L.sc_batch_get_sweep_shape.argtypes = [vp, ctypes.c_int] + [
+            ctypes.POINTER(ctypes.c_int)] * 3
+        L.sc_batch_autotune.argtypes = [vp, ctypes.c_int]
+        L.sc_batch_timing.argtypes = [vp, _DP, _DP, ctypes.POINTER(ctypes.c_longlong)]
         L.sc_describe.argtypes = [ctypes.c_char_p, ctypes.c_int,
                                   ctypes.POINTER(Dims), ctypes.POINTER(ctypes.c_byte)]
         _lib = L
@@ -128,11 +142,16 @@ def _precision(p):
 class Simulation:
     """One grain: Context<T> with its modules (Context.h:43-183)."""
 
-    def __init__(self, config):
-        text = config if isinstance(config, str) else json.dumps(config)
-        h = ctypes.c_void_p()
-        _check(lib().sc_create(text.encode(), ctypes.byref(h)))
-        self._h = h
+    def __init__(self, config=None, _borrow=None):
+        if _borrow is not None:
+            self._h, self._owner = _borrow
+            self._borrowed = True
+        else:
+            text = config if isinstance(config, str) else json.dumps(config)
+            h = ctypes.c_void_p()
+            _check(lib().sc_create(text.encode(), ctypes.byref(h)))
+            self._h = h
+            self._borrowed = False
         d = Dims()
         _check(lib().sc_get_dims(self._h, ctypes.byref(d)))
         self.dims = d
@@ -141,7 +160,8 @@ class Simulation:
 
     def close(self):
         if getattr(self, "_h", None):
-            lib().sc_destroy(self._h)
+            if not getattr(self, "_borrowed", False):
+                lib().sc_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -266,6 +286,82 @@ class Simulation:
         """Picks the fastest sweep variant for this grain and precision."""
         _check(lib().sc_autotune_sweep(self._h, _precision(precision)))
         return self.sweep_shape(precision)
+
+
+class Batch:
+    """Many grains iterated together; one sweep launch per iteration covers
+    every grain still iterating.  grain(i) gives the per-grain accessors of
+    Simulation (borrowed: they live as long as the batch)."""
+
+    def __init__(self, configs):
+        texts = [c if isinstance(c, str) else json.dumps(c) for c in configs]
+        arr = (ctypes.c_char_p * len(texts))(*[t.encode() for t in texts])
+        h = ctypes.c_void_p()
+        _check(lib().sc_batch_create(arr, len(texts), ctypes.byref(h)))
+        self._h = h
+        self._grains = []
+        for i in range(len(texts)):
+            g = ctypes.c_void_p()
+            _check(lib().sc_batch_grain(self._h, i, ctypes.byref(g)))
+            self._grains.append(Simulation(_borrow=(g, self)))
+
+    def __len__(self):
+        return len(self._grains)
+
+    def grain(self, i):
+        return self._grains[i]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for g in self._grains:
+                g._h = None
+            lib().sc_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def compute(self, precision="fp64"):
+        _check(lib().sc_batch_compute(self._h, _precision(precision)))
+
+    def burnin(self, precision="fp64"):
+        _check(lib().sc_batch_burnin(self._h, _precision(precision)))
+
+    def run(self, burnin=-1, nmin=-1, nmax=-1, precision="fp64"):
+        conv = (ctypes.c_int * len(self._grains))()
+        _check(lib().sc_batch_run(self._h, burnin, nmin, nmax, _precision(precision), conv))
+        return [bool(c) for c in conv]
+
+    def set_sweep_shape(self, energies_per_lane, waves_per_simd, slots_per_cu,
+                        precision="fp64"):
+        _check(lib().sc_batch_set_sweep_shape(self._h, _precision(precision),
+                                              energies_per_lane, waves_per_simd,
+                                              slots_per_cu))
+
+    def sweep_shape(self, precision="fp64"):
+        e, w, s = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().sc_batch_get_sweep_shape(self._h, _precision(precision),
+                                              ctypes.byref(e), ctypes.byref(w),
+                                              ctypes.byref(s)))
+        return e.value, w.value, s.value
+
+    def autotune(self, precision="fp64"):
+        _check(lib().sc_batch_autotune(self._h, _precision(precision)))
+        return self.sweep_shape(precision)
+
+    def timing(self):
+        """(sweep ms, iteration ms, trajectory steps) of the last iteration."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        n = ctypes.c_longlong()
+        _check(lib().sc_batch_timing(self._h, ctypes.byref(a), ctypes.byref(b),
+                                     ctypes.byref(n)))
+        return a.value, b.value, n.value
 
 
 def describe(config, angle=None):

@@ -12,7 +12,14 @@
 #include <vector>
 
 struct sc_context {
-  std::unique_ptr<scamd::Grain> grain;
+  std::unique_ptr<scamd::Grain> owned;
+  scamd::Grain *grain = nullptr;
+  bool borrowed = false; // a grain of a batch: the batch owns it
+};
+
+struct sc_batch {
+  std::unique_ptr<scamd::Batch> batch;
+  std::vector<std::unique_ptr<sc_context>> views;
 };
 
 namespace {
@@ -85,7 +92,8 @@ int sc_create(const char *config_json, sc_context **out) {
     scamd::Config cfg = scamd::parseConfig(config_json);
     auto setup = scamd::buildSetup(cfg);
     auto ctx = std::make_unique<sc_context>();
-    ctx->grain = std::make_unique<scamd::Grain>(std::move(setup));
+    ctx->owned = std::make_unique<scamd::Grain>(std::move(setup));
+    ctx->grain = ctx->owned.get();
     ctx->grain->initialGuess();
     *out = ctx.release();
     return SC_OK;
@@ -94,6 +102,8 @@ int sc_create(const char *config_json, sc_context **out) {
 
 int sc_destroy(sc_context *ctx) {
   if (!ctx) return SC_OK;
+  if (ctx->borrowed)
+    return fail(SC_ERR_ARGUMENT, "a grain of a batch is destroyed with its batch");
   return guarded([&] {
     delete ctx;
     return SC_OK;
@@ -394,6 +404,115 @@ int sc_get_labels(sc_context *ctx, int angle, signed char *labels) {
   if (angle < 0 || angle >= s.Na) return fail(SC_ERR_ARGUMENT, "angle out of range");
   const size_t n1 = size_t(s.N1) * s.N1;
   std::memcpy(labels, s.labels.data() + size_t(angle) * n1, n1);
+  return SC_OK;
+}
+
+// ----- batches -----
+int sc_batch_create(const char *const *config_json, int count, sc_batch **out) {
+  SC_CHECK_PTR(config_json);
+  SC_CHECK_PTR(out);
+  *out = nullptr;
+  if (count < 1) return fail(SC_ERR_ARGUMENT, "a batch needs at least one config");
+  return guarded([&] {
+    std::vector<std::unique_ptr<scamd::Setup>> setups;
+    for (int i = 0; i < count; ++i) {
+      if (!config_json[i]) throw std::invalid_argument("null config in the batch");
+      setups.push_back(scamd::buildSetup(scamd::parseConfig(config_json[i])));
+    }
+    auto b = std::make_unique<sc_batch>();
+    b->batch = std::make_unique<scamd::Batch>(std::move(setups));
+    for (int i = 0; i < count; ++i) {
+      auto v = std::make_unique<sc_context>();
+      v->grain = &b->batch->grain(i);
+      v->borrowed = true;
+      b->views.push_back(std::move(v));
+    }
+    *out = b.release();
+    return SC_OK;
+  });
+}
+int sc_batch_destroy(sc_batch *b) {
+  if (!b) return SC_OK;
+  return guarded([&] {
+    delete b;
+    return SC_OK;
+  });
+}
+int sc_batch_size(sc_batch *b, int *count) {
+  SC_CHECK_PTR(b);
+  SC_CHECK_PTR(count);
+  *count = b->batch->size();
+  return SC_OK;
+}
+int sc_batch_grain(sc_batch *b, int index, sc_context **out) {
+  SC_CHECK_PTR(b);
+  SC_CHECK_PTR(out);
+  if (index < 0 || index >= b->batch->size()) return fail(SC_ERR_ARGUMENT, "grain index out of range");
+  *out = b->views[size_t(index)].get();
+  return SC_OK;
+}
+int sc_batch_compute(sc_batch *b, int precision) {
+  SC_CHECK_PTR(b);
+  return guarded([&] {
+    checkPrecision(precision);
+    b->batch->compute(precision);
+    return SC_OK;
+  });
+}
+int sc_batch_burnin(sc_batch *b, int precision) {
+  SC_CHECK_PTR(b);
+  return guarded([&] {
+    checkPrecision(precision);
+    b->batch->burnin(precision);
+    return SC_OK;
+  });
+}
+int sc_batch_run(sc_batch *b, int nb, int nmin, int nmax, int precision, int *converged) {
+  SC_CHECK_PTR(b);
+  return guarded([&] {
+    checkPrecision(precision);
+    const scamd::Config &c = b->batch->grain(0).setup().cfg;
+    std::vector<int> conv;
+    b->batch->run(nb < 0 ? c.burnin : nb, nmin < 0 ? c.itMin : nmin, nmax < 0 ? c.itMax : nmax,
+                  precision, conv);
+    if (converged)
+      for (size_t i = 0; i < conv.size(); ++i) converged[i] = conv[i];
+    return SC_OK;
+  });
+}
+int sc_batch_set_sweep_shape(sc_batch *b, int precision, int el, int waves, int slots) {
+  SC_CHECK_PTR(b);
+  return guarded([&] {
+    checkPrecision(precision);
+    b->batch->setSweepShape(precision, el, waves, slots);
+    return SC_OK;
+  });
+}
+int sc_batch_get_sweep_shape(sc_batch *b, int precision, int *el, int *waves, int *slots) {
+  SC_CHECK_PTR(b);
+  SC_CHECK_PTR(el);
+  SC_CHECK_PTR(waves);
+  SC_CHECK_PTR(slots);
+  return guarded([&] {
+    checkPrecision(precision);
+    b->batch->sweepShape(precision, *el, *waves, *slots);
+    return SC_OK;
+  });
+}
+int sc_batch_autotune(sc_batch *b, int precision) {
+  SC_CHECK_PTR(b);
+  return guarded([&] {
+    checkPrecision(precision);
+    b->batch->autotune(precision);
+    return SC_OK;
+  });
+}
+int sc_batch_timing(sc_batch *b, double *sweep_ms, double *iteration_ms,
+                    long long *traj_steps) {
+  SC_CHECK_PTR(b);
+  if (sweep_ms) *sweep_ms = b->batch->lastSweepMs();
+  if (iteration_ms) *iteration_ms = b->batch->lastIterationMs();
+  if (traj_steps) *traj_steps = b->batch->lastTrajSteps();
   return SC_OK;
 }
 

@@ -51,11 +51,18 @@ inline unsigned blocksFor(long long n, int bs = 256) {
 
 } // namespace
 
-Grain::Grain(std::unique_ptr<Setup> setup) : s_(std::move(setup)) {
-  SC_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+Grain::Grain(std::unique_ptr<Setup> setup, hipStream_t stream) : s_(std::move(setup)) {
+  if (stream) {
+    stream_ = stream;
+    ownsStream_ = false;
+  } else {
+    SC_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  }
   SC_HIP(hipEventCreate(&ev0_));
   SC_HIP(hipEventCreate(&ev1_));
   SC_HIP(hipEventCreate(&ev2_));
+  SC_HIP(hipEventCreate(&evIterStart_));
+  SC_HIP(hipEventCreate(&evIterStop_));
   upload();
   // Allocation clears and uploads ran on the null stream, which a
   // non-blocking stream does not wait for.
@@ -125,7 +132,9 @@ Grain::~Grain() {
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
   if (ev2_) (void)hipEventDestroy(ev2_);
-  if (stream_) (void)hipStreamDestroy(stream_);
+  if (evIterStart_) (void)hipEventDestroy(evIterStart_);
+  if (evIterStop_) (void)hipEventDestroy(evIterStop_);
+  if (stream_ && ownsStream_) (void)hipStreamDestroy(stream_);
 }
 
 void Grain::upload() {
@@ -759,15 +768,12 @@ void Grain::boundaryCondition() {
   res_.resBoundary = boundaryResidual(hb);
 }
 
-void Grain::integrate(int precision, bool burnin) {
-  // IntegrationIteratorOzaki::compute (IntegrationIteratorOzaki.h:303-404):
-  // sweeps, back-rotation and angle integration, then the reductions of the
-  // compute objects and the boundary condition, queued for one read-back.
+void Grain::afterSweep(bool burnin) {
+  // Back-rotation and angle integration of the swept Green functions, the
+  // compute objects' reductions and the boundary condition, queued for one
+  // read-back.
   const Setup &s = *s_;
   const size_t n0 = size_t(s.N0) * s.N0, n1 = size_t(s.N1) * s.N1;
-  ensureBoundaryInit();
-  extendDelta();
-  sweep(precision, burnin);
   pendingIntegrate_ = !burnin;
   if (!burnin) {
     const int nt = static_cast<int>(s.extRot.target.size());
@@ -1080,25 +1086,29 @@ void Grain::acceleratorStep() {
   SC_HIP(hipGetLastError());
 }
 
-void Grain::compute(int precision) {
-  // Context::compute (Context.h:391-447)
+void Grain::beginCompute() {
+  // Context::compute (Context.h:391-447) up to the Riccati sweeps
+  SC_HIP(hipEventRecord(evIterStart_, stream_));
+  clearReductions();
+  if (!firstCompute_) acceleratorStep();
+  ensureBoundaryInit();
+  extendDelta();
+}
+
+void Grain::endCompute() {
+  // the rest of Context::compute after the sweeps
   const Setup &s = *s_;
   const size_t n0 = size_t(s.N0) * s.N0;
   const bool doA = s.cfg.computeCurrent && s.lambda > 0.0;
-  hipEvent_t start;
-  SC_HIP(hipEventCreate(&start));
-  SC_HIP(hipEventRecord(start, stream_));
-  clearReductions();
-  if (!firstCompute_) acceleratorStep();
-  integrate(precision, false);
+  afterSweep(false);
   int hA = -1;
   if (doA) {
     solveVectorPotential(dCurrent_, dANew_);
     hA = queueStats(dA_, dANew_, static_cast<long long>(2 * n0));
   }
+  SC_HIP(hipEventRecord(evIterStop_, stream_));
   flushReductions();
   finishIntegrate();
-  updateSweepTime();
   if (doA) res_.resVectorPotential = reductionResidual(hA);
   res_.iterations++;
   firstCompute_ = false;
@@ -1113,26 +1123,38 @@ void Grain::compute(int precision) {
   if (s.cfg.computeCurrent) ok = ok && convCurrent_;
   if (s.cfg.computeFreeEnergy) ok = ok && convFE_;
   res_.converged = ok ? 1 : 0;
-  hipEvent_t stop;
-  SC_HIP(hipEventCreate(&stop));
-  SC_HIP(hipEventRecord(stop, stream_));
-  SC_HIP(hipEventSynchronize(stop));
   float ms = 0.f;
-  SC_HIP(hipEventElapsedTime(&ms, start, stop));
+  SC_HIP(hipEventElapsedTime(&ms, evIterStart_, evIterStop_));
   tim_.iterMs = ms;
-  (void)hipEventDestroy(start);
-  (void)hipEventDestroy(stop);
+}
+
+void Grain::compute(int precision) {
+  beginCompute();
+  sweep(precision, false);
+  endCompute();
+  updateSweepTime();
+}
+
+void Grain::beginBurnin() {
+  clearReductions();
+  ensureBoundaryInit();
+  extendDelta();
+}
+
+void Grain::endBurnin() {
+  afterSweep(true);
+  flushReductions();
+  finishIntegrate();
+  res_.iterations++;
+  res_.residual = res_.resBoundary;
 }
 
 void Grain::burnin(int precision) {
   // Context::burnIn (Context.h:377-389)
-  clearReductions();
-  integrate(precision, true);
-  flushReductions();
-  finishIntegrate();
+  beginBurnin();
+  sweep(precision, true);
+  endBurnin();
   updateSweepTime();
-  res_.iterations++;
-  res_.residual = res_.resBoundary;
 }
 
 bool Grain::run(int nBurnin, int itMin, int itMax, int precision) {
@@ -1168,6 +1190,226 @@ bool Grain::run(int nBurnin, int itMin, int itMax, int precision) {
     if ((conv && minReached) || maxReached) break;
   }
   return conv;
+}
+
+// ---------------------------------------------------------------------------
+// Batch: many grains, one sweep launch per iteration
+// ---------------------------------------------------------------------------
+Batch::Batch(std::vector<std::unique_ptr<Setup>> setups) {
+  if (setups.empty()) throw std::runtime_error("a batch needs at least one grain");
+  SC_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  for (hipEvent_t *e : {&ev0_, &ev1_, &evIt0_, &evIt1_}) SC_HIP(hipEventCreate(e));
+  for (auto &st : setups) {
+    grains_.push_back(std::make_unique<Grain>(std::move(st), stream_));
+    grains_.back()->initialGuess();
+  }
+  done_.assign(grains_.size(), 0);
+  dCounter_ = devAlloc<int>(1);
+}
+
+Batch::~Batch() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  grains_.clear();
+  for (Plan &pl : plan_) {
+    if (pl.dItems) (void)hipFree(pl.dItems);
+    if (pl.dViews) (void)hipFree(pl.dViews);
+  }
+  if (dScratch_) (void)hipFree(dScratch_);
+  devFree(dCounter_);
+  for (hipEvent_t e : {ev0_, ev1_, evIt0_, evIt1_})
+    if (e) (void)hipEventDestroy(e);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+std::vector<int> Batch::active() const {
+  std::vector<int> a;
+  for (size_t i = 0; i < grains_.size(); ++i)
+    if (!done_[i]) a.push_back(static_cast<int>(i));
+  return a;
+}
+
+void Batch::setSweepShape(int precision, int el, int waves, int slotsPerCU) {
+  if (precision != PREC_FP64 && precision != PREC_MIXED)
+    throw std::runtime_error("unknown precision");
+  if (!sweepVariantExists(el, waves))
+    throw std::runtime_error("no sweep variant with " + std::to_string(el) +
+                             " energies per lane and " + std::to_string(waves) +
+                             " wavefronts per SIMD");
+  if (slotsPerCU < 1 || slotsPerCU > 32)
+    throw std::runtime_error("slots per CU must be in [1, 32]");
+  Plan &pl = plan_[precision];
+  if (el != pl.el || waves != pl.waves || slotsPerCU != pl.slotsPerCU) pl.ready = false;
+  pl.el = el;
+  pl.waves = waves;
+  pl.slotsPerCU = slotsPerCU;
+}
+
+void Batch::sweepShape(int precision, int &el, int &waves, int &slotsPerCU) const {
+  const Plan &pl = plan_[precision == PREC_FP64 ? 0 : 1];
+  el = pl.el;
+  waves = pl.waves;
+  slotsPerCU = pl.slotsPerCU;
+}
+
+void Batch::prepare(int precision, const std::vector<int> &act) {
+  Plan &pl = plan_[precision];
+  if (pl.ready && pl.key == act) return;
+  SC_HIP(hipStreamSynchronize(stream_));
+  // views of every grain with this lane layout (index = grain number)
+  std::vector<GrainView> views;
+  for (auto &g : grains_) views.push_back(g->sweepView(pl.el));
+  // the items of the active grains, longest first
+  std::vector<SweepItemDev> items;
+  std::vector<int> lens;
+  pl.maxLen = 0;
+  for (int gi : act) {
+    int maxLen = 0;
+    const std::vector<SweepItem> its =
+        sweepItems(grains_[size_t(gi)]->setup(), views[size_t(gi)].cols, &maxLen);
+    pl.maxLen = std::max(pl.maxLen, maxLen);
+    for (const SweepItem &it : its) {
+      items.push_back({gi, it.angle, it.x0, it.ncols, it.ylo, it.yhi});
+      lens.push_back(it.yhi - it.ylo + 1);
+    }
+  }
+  std::vector<size_t> order(items.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t a, size_t b) { return lens[a] > lens[b]; });
+  std::vector<SweepItemDev> sorted(items.size());
+  for (size_t i = 0; i < order.size(); ++i) sorted[i] = items[order[i]];
+  pl.numItems = static_cast<int>(sorted.size());
+  if (pl.dItems) (void)hipFree(pl.dItems);
+  if (pl.dViews) (void)hipFree(pl.dViews);
+  pl.dItems = pl.dViews = nullptr;
+  SC_HIP(hipMalloc(&pl.dItems, std::max<size_t>(1, sorted.size()) * sizeof(SweepItemDev)));
+  SC_HIP(hipMalloc(&pl.dViews, views.size() * sizeof(GrainView)));
+  if (!sorted.empty())
+    SC_HIP(hipMemcpyAsync(pl.dItems, sorted.data(), sorted.size() * sizeof(SweepItemDev),
+                          hipMemcpyHostToDevice, stream_));
+  SC_HIP(hipMemcpyAsync(pl.dViews, views.data(), views.size() * sizeof(GrainView),
+                        hipMemcpyHostToDevice, stream_));
+  int dev = 0, cus = 0;
+  SC_HIP(hipGetDevice(&dev));
+  SC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  pl.numSlots = std::max(1, std::min(pl.numItems, pl.slotsPerCU * std::max(cus, 1)));
+  const size_t elemBytes = precision == PREC_FP64 ? sizeof(Cx<double>) : sizeof(Cx<float>);
+  const size_t bytes = size_t(pl.numSlots) * size_t(pl.maxLen) * pl.el * 64 * elemBytes;
+  if (bytes > scratchBytes_) {
+    if (dScratch_) (void)hipFree(dScratch_);
+    dScratch_ = nullptr;
+    scratchBytes_ = 0;
+    SC_HIP(hipMalloc(&dScratch_, bytes));
+    scratchBytes_ = bytes;
+  }
+  SC_HIP(hipStreamSynchronize(stream_));
+  pl.key = act;
+  pl.ready = true;
+}
+
+void Batch::sweep(int precision, bool burnin, const std::vector<int> &act) {
+  prepare(precision, act);
+  const Plan &pl = plan_[precision];
+  SweepArgs a;
+  a.views = reinterpret_cast<const GrainView *>(pl.dViews);
+  a.items = reinterpret_cast<const SweepItemDev *>(pl.dItems);
+  a.numItems = pl.numItems;
+  a.counter = dCounter_;
+  a.scratch = dScratch_;
+  a.slotStride = static_cast<long long>(pl.maxLen) * pl.el * 64;
+  a.burnin = burnin ? 1 : 0;
+  resetCounter<<<1, 1, 0, stream_>>>(dCounter_);
+  SC_HIP(hipEventRecord(ev0_, stream_));
+  launchSweepVariant(precision, pl.el, pl.waves, a, pl.numSlots, stream_);
+  SC_HIP(hipEventRecord(ev1_, stream_));
+}
+
+void Batch::compute(int precision) {
+  if (precision != PREC_FP64 && precision != PREC_MIXED)
+    throw std::runtime_error("unknown precision");
+  const std::vector<int> act = active();
+  if (act.empty()) return;
+  SC_HIP(hipEventRecord(evIt0_, stream_));
+  for (int gi : act) grains_[size_t(gi)]->beginCompute();
+  sweep(precision, false, act);
+  for (int gi : act) grains_[size_t(gi)]->endCompute();
+  SC_HIP(hipEventRecord(evIt1_, stream_));
+  SC_HIP(hipEventSynchronize(evIt1_));
+  float ms = 0.f;
+  SC_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+  sweepMs_ = ms;
+  SC_HIP(hipEventElapsedTime(&ms, evIt0_, evIt1_));
+  iterMs_ = ms;
+  trajSteps_ = 0;
+  for (int gi : act) trajSteps_ += grains_[size_t(gi)]->setup().trajSteps;
+}
+
+void Batch::burnin(int precision) {
+  const std::vector<int> act = active();
+  if (act.empty()) return;
+  for (int gi : act) grains_[size_t(gi)]->beginBurnin();
+  sweep(precision, true, act);
+  for (int gi : act) grains_[size_t(gi)]->endBurnin();
+  SC_HIP(hipStreamSynchronize(stream_));
+}
+
+int Batch::run(int nBurnin, int itMin, int itMax, int precision, std::vector<int> &converged) {
+  // runSimulation (run_simulation.h:124-187) for every grain at once.
+  std::fill(done_.begin(), done_.end(), 0);
+  for (int i = 0; i < nBurnin; ++i) burnin(precision);
+  converged.assign(grains_.size(), 0);
+  if (itMin == 0 && itMax == 0) return 0;
+  for (int it = 1;; ++it) {
+    compute(precision);
+    for (size_t g = 0; g < grains_.size(); ++g) {
+      if (done_[g]) continue;
+      const bool conv = grains_[g]->results().converged != 0;
+      converged[g] = conv ? 1 : 0;
+      const bool minReached = it >= itMin;
+      const bool maxReached = itMax >= 0 && it >= itMax;
+      if ((conv && minReached) || maxReached) done_[g] = 1;
+    }
+    if (active().empty()) break;
+  }
+  std::fill(done_.begin(), done_.end(), 0);
+  int n = 0;
+  for (int c : converged) n += c;
+  return n;
+}
+
+void Batch::autotune(int precision) {
+  if (precision != PREC_FP64 && precision != PREC_MIXED)
+    throw std::runtime_error("unknown precision");
+  const std::vector<int> act = active();
+  for (int gi : act) grains_[size_t(gi)]->prepareSweepInputs();
+  int maxNe = 0;
+  for (auto &g : grains_) maxNe = std::max(maxNe, g->setup().Ne);
+  float best = -1.f;
+  int bestEl = plan_[precision].el, bestW = plan_[precision].waves,
+      bestSlots = plan_[precision].slotsPerCU;
+  const int slotChoices[] = {4, 8, 16};
+  auto tryShape = [&](int el, int w, int slots) {
+    int gp = 1;
+    while (gp < (maxNe + el - 1) / el) gp <<= 1;
+    if (gp > 64) return;
+    setSweepShape(precision, el, w, slots);
+    sweep(precision, false, act);
+    SC_HIP(hipEventSynchronize(ev1_));
+    float ms = 0.f;
+    SC_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+    if (best < 0.f || ms < best) {
+      best = ms;
+      bestEl = el;
+      bestW = w;
+      bestSlots = slots;
+    }
+  };
+#define SC_TRY(E, W)                                                                 \
+  for (int slots : slotChoices)                                                       \
+    if (slots >= 4 * W || slots == 16) tryShape(E, W, slots);
+  SC_SWEEP_VARIANTS(SC_TRY)
+#undef SC_TRY
+  setSweepShape(precision, bestEl, bestW, bestSlots);
 }
 
 void poissonSolve(int n, double h, int numGrain, bool enforce, const double *current,

@@ -33,7 +33,9 @@ struct Timing {
 
 class Grain {
 public:
-  explicit Grain(std::unique_ptr<Setup> setup);
+  // With a stream the grain runs on it (and does not own it); without, it
+  // creates its own.
+  explicit Grain(std::unique_ptr<Setup> setup, hipStream_t stream = nullptr);
   ~Grain();
   Grain(const Grain &) = delete;
   Grain &operator=(const Grain &) = delete;
@@ -54,6 +56,12 @@ public:
   // the self-consistency iteration (Context.h:377-447)
   void compute(int precision);
   void burnin(int precision);
+  // The same iteration in two halves around the sweep, for callers that
+  // sweep several grains in one launch: beginCompute(); <sweep>; endCompute().
+  void beginCompute();
+  void endCompute();
+  void beginBurnin();
+  void endBurnin();
   bool run(int burnin, int itMin, int itMax, int precision);
   // individual stages
   // Reads the HIP-event time of the last sweep launch into the timing.
@@ -75,6 +83,12 @@ public:
   // Times every compiled sweep variant on the current state and keeps the
   // fastest for `precision`; the sweeps it runs leave the state unchanged.
   void autotune(int precision);
+  // Boundary initial values (first call) and the extended order parameter:
+  // the inputs of a sweep.
+  void prepareSweepInputs() {
+    ensureBoundaryInit();
+    extendDelta();
+  }
   hipStream_t stream() const { return stream_; }
 
   // Device view of the grain, and with the lane layout of `el` energies
@@ -92,7 +106,7 @@ private:
   void fromDevice(void *dst, const void *src, size_t bytes);
   void extendDelta();
   void ensureBoundaryInit();
-  void integrate(int precision, bool burnin);
+  void afterSweep(bool burnin);
   void finishIntegrate();
   void acceleratorStep();
   double residualOf(const double *oldv, const double *newv, long long n);
@@ -114,7 +128,9 @@ private:
 
   std::unique_ptr<Setup> s_;
   hipStream_t stream_ = nullptr;
+  bool ownsStream_ = true;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr, ev2_ = nullptr;
+  hipEvent_t evIterStart_ = nullptr, evIterStop_ = nullptr;
   // constant tables
   double *dEnergy_ = nullptr, *dResidue_ = nullptr, *dSaCos_ = nullptr,
          *dSaSin_ = nullptr, *dVFx_ = nullptr, *dVFy_ = nullptr,
@@ -193,5 +209,60 @@ private:
 // Stand-alone ComputeVectorPotential solve with host buffers (2*n*n each).
 void poissonSolve(int n, double h, int numGrain, bool enforceMean,
                   const double *current, double *out);
+
+// Many grains iterated together (a parameter sweep).  Every iteration runs
+// the per-grain stages on one stream and sweeps all grains that are still
+// iterating in one persistent launch whose work queue holds the items of all
+// of them, longest first, so that small grains fill the gaps of large ones.
+class Batch {
+public:
+  explicit Batch(std::vector<std::unique_ptr<Setup>> setups);
+  ~Batch();
+  Batch(const Batch &) = delete;
+  Batch &operator=(const Batch &) = delete;
+
+  int size() const { return static_cast<int>(grains_.size()); }
+  Grain &grain(int i) { return *grains_.at(size_t(i)); }
+  // One iteration of every grain that is still iterating.
+  void compute(int precision);
+  void burnin(int precision);
+  // runSimulation for every grain; a grain stops once converged with the
+  // minimum reached, or at the maximum.  Returns the number converged.
+  int run(int nBurnin, int itMin, int itMax, int precision, std::vector<int> &converged);
+  void setSweepShape(int precision, int el, int waves, int slotsPerCU);
+  void sweepShape(int precision, int &el, int &waves, int &slotsPerCU) const;
+  void autotune(int precision);
+  double lastSweepMs() const { return sweepMs_; }
+  double lastIterationMs() const { return iterMs_; }
+  long long lastTrajSteps() const { return trajSteps_; }
+  std::vector<int> active() const;
+
+private:
+  struct Plan {
+    int el = 8, waves = 2, slotsPerCU = 8;
+    std::vector<int> key; // the active grains the items were built for
+    int numItems = 0, numSlots = 0, maxLen = 0;
+    void *dItems = nullptr, *dViews = nullptr;
+    bool ready = false;
+  };
+  void prepare(int precision, const std::vector<int> &act);
+  void sweep(int precision, bool burnin, const std::vector<int> &act);
+
+  hipStream_t stream_ = nullptr;
+  std::vector<std::unique_ptr<Grain>> grains_;
+  std::vector<char> done_;
+  Plan plan_[2];
+  void *dScratch_ = nullptr;
+  size_t scratchBytes_ = 0;
+  int *dCounter_ = nullptr;
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr, evIt0_ = nullptr, evIt1_ = nullptr;
+  double sweepMs_ = 0, iterMs_ = 0;
+  long long trajSteps_ = 0;
+};
+
+// One launch of the compiled sweep variant (precision, energies per lane,
+// wavefronts per SIMD) over the items of `args`.
+void launchSweepVariant(int precision, int el, int waves, const SweepArgs &args, int slots,
+                        hipStream_t stream);
 
 } // namespace scamd

@@ -77,5 +77,39 @@ def swave_disc(seed=1234, noise=0.01):
     }
 
 
+def parameter_sweep(count=48, seed=2024, ppxi=20.0):
+    """A parameter sweep of `count` grains drawn from a seeded generator:
+    each a disc of radius U[5, 20] xi0 or a square of side U[10, 40] xi0
+    (coin flip), at `ppxi` points per xi0, external flux U[0, 4] flux quanta
+    (no induced potential), T U[0.1, 0.9] Tc, s- or d_{x2-y2}-wave pairing
+    (coin flip), 64 angles, cutoff 16, FP32-friendly criterion 1e-8."""
+    import random
+    rng = random.Random(seed)
+    configs = []
+    for _ in range(count):
+        disc = rng.random() < 0.5
+        size = rng.uniform(5.0, 20.0) if disc else rng.uniform(10.0, 40.0)
+        flux = rng.uniform(0.0, 4.0)
+        temperature = rng.uniform(0.1, 0.9)
+        sym = "s" if rng.random() < 0.5 else "dx2-y2"
+        geometry = ({"disc": {"radius": size, "center_x": 0.0, "center_y": 0.0,
+                              "add": True}} if disc else
+                    {"regular_polygon": {"num_edges": 4, "side_length": size,
+                                         "rotation": 0.0, "center_x": 0.0,
+                                         "center_y": 0.0, "add": True}})
+        configs.append({
+            "physics": {"temperature": temperature, "external_flux_quanta": flux,
+                        "penetration_depth": -1.0, "gauge": "symmetric",
+                        "charge_sign": -1,
+                        "order_parameter": {sym: {"critical_temperature": 1.0,
+                                                  "initial_noise_stddev": 0.01}}},
+            "geometry": [geometry],
+            "numerics": _numerics(64, ppxi, 16.0, 1e-8),
+            "accelerator": {"name": "congacc"},
+            "misc": {"seed": seed},
+        })
+    return configs
+
+
 WORKLOADS = {"octagon_dwave": octagon_dwave, "vortex_disc": vortex_disc,
              "swave_disc": swave_disc}

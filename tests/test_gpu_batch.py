@@ -1,0 +1,87 @@
+"""Batched grains: one sweep launch per iteration over several grains must
+give every grain exactly what iterating it alone gives, and the oracle's
+results at the parity bar."""
+import numpy as np
+import pytest
+
+import cases
+from test_gpu_parity import TOL, compare_state, rel
+
+pytestmark = pytest.mark.gpu
+
+
+def batch_configs():
+    return [
+        cases.base(temperature=0.5, flux=1.0, lam=10.0, angles=16, ppxi=3.0),
+        cases.base(order_parameter={"dx2-y2": {"critical_temperature": 1.0}},
+                   temperature=0.3, flux=0.5, lam=-1.0, angles=24, ppxi=4.0, cutoff=12.0,
+                   rotation=0.125, components=[cases.regular_polygon_unit(4)], width=9.0),
+        cases.mixed_small(),
+    ]
+
+
+def test_batch_equals_individual_grains(product):
+    cfgs = batch_configs()
+    b = product.Batch(cfgs)
+    b.set_sweep_shape(8, 2, 8)
+    singles = [product.Simulation(c) for c in cfgs]
+    for s in singles:
+        s.set_sweep_shape(8, 2, 8)
+    for i, s in enumerate(singles):
+        b.grain(i).set_order_parameter(0, s.order_parameter(0)[0])
+        for c in range(1, s.Nc):
+            b.grain(i).set_order_parameter(c, s.order_parameter(0)[c])
+    for it in range(3):
+        b.compute("fp64")
+        for s in singles:
+            s.compute("fp64")
+        for i, s in enumerate(singles):
+            g = b.grain(i)
+            assert rel(g.order_parameter(1), s.order_parameter(1)) < 1e-14, (it, i)
+            assert rel(g.current(), s.current()) < 1e-14, (it, i)
+            assert abs(g.results()["free_energy"] - s.results()["free_energy"]) <= \
+                1e-14 * abs(s.results()["free_energy"])
+    sweep_ms, iter_ms, steps = b.timing()
+    assert sweep_ms > 0 and iter_ms >= sweep_ms
+    assert steps == sum(s.dims.traj_steps for s in singles)
+
+
+def test_batch_grain_matches_oracle(product, oracle):
+    cfgs = batch_configs()
+    b = product.Batch(cfgs)
+    o = oracle.Oracle(cfgs[2])
+    o.initial_bulk()
+    for c, d in enumerate(o.delta()):
+        b.grain(2).set_order_parameter(c, d)
+    # the other grains start from their own initial guesses
+    for it in range(2):
+        b.compute("fp64")
+        o.compute()
+        compare_state(b.grain(2), o, cfgs[2], what="batch grain 2 it %d" % it)
+
+
+def test_batch_run_stops_grains_independently(product):
+    cfgs = batch_configs()
+    for c in cfgs:
+        c["numerics"]["convergence_criterion"] = 1e-4
+    b = product.Batch(cfgs)
+    conv = b.run(0, 0, 60, "fp64")
+    its = [b.grain(i).results()["iterations"] for i in range(len(cfgs))]
+    print("batch run: converged", conv, "iterations", its)
+    for i, c in enumerate(conv):
+        r = b.grain(i).results()
+        assert c == bool(r["converged"])
+        if c:
+            assert r["residual"] < 1e-4
+    assert any(conv)
+
+
+def test_batch_autotune_and_mixed(product):
+    cfgs = batch_configs()
+    b = product.Batch(cfgs)
+    b.compute("fp64")
+    shape = b.autotune("mixed")
+    assert len(shape) == 3
+    b.compute("mixed")
+    for i in range(len(cfgs)):
+        assert np.isfinite(b.grain(i).results()["free_energy"])
