@@ -1160,10 +1160,12 @@ void Grain::burnin(int precision) {
 bool Grain::run(int nBurnin, int itMin, int itMax, int precision) {
   // runSimulation (run_simulation.h:124-187), save_frequency < 0.
   // PREC_MIXED_REFINE iterates with FP32 stepping until the residual reaches
-  // the FP32 floor (below kRefineSwitch, or no 10% gain over three
-  // iterations), then continues in FP64; convergence only counts on an FP64
-  // iteration, so the converged state is an FP64 fixed point.
-  constexpr double kRefineSwitch = 3e-6;
+  // the FP32 floor (below kRefineSwitch, or no new best residual for
+  // kRefineStall iterations), then continues in FP64; convergence only
+  // counts on an FP64 iteration, so the converged state is an FP64 fixed
+  // point.
+  constexpr double kRefineSwitch = 1e-5;
+  constexpr int kRefineStall = 20;
   const bool refine = precision == PREC_MIXED_REFINE;
   int prec = refine ? PREC_MIXED : precision;
   for (int i = 1; nBurnin > 0; ++i) {
@@ -1181,9 +1183,9 @@ bool Grain::run(int nBurnin, int itMin, int itMax, int precision) {
     conv = res_.converged != 0 && prec == PREC_FP64;
     if (refine && prec == PREC_MIXED) {
       const double r = res_.residual;
-      stall = (i > 1 && r > 0.9 * best) ? stall + 1 : 0;
+      stall = (i > 1 && r >= best) ? stall + 1 : 0;
       best = (i == 1) ? r : std::min(best, r);
-      if (r < kRefineSwitch || stall >= 3 || res_.converged) prec = PREC_FP64;
+      if (r < kRefineSwitch || stall >= kRefineStall || res_.converged) prec = PREC_FP64;
     }
     const bool minReached = i >= itMin;
     const bool maxReached = itMax < 0 ? false : i >= itMax;
