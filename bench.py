@@ -103,6 +103,76 @@ def copy_probe(mib):
     log("copy probe: %d MiB device-to-device" % mib)
 
 
+def converge(sa, cfg, precision, cap):
+    """Wall time of one solve to the config's criterion (sc_run) from the
+    initial state, with the sweep shape autotuned first (outside the timed
+    region)."""
+    sim = sa.Simulation(cfg)
+    # The autotuner's sweeps leave the state as it is.
+    shape64 = sim.autotune("fp64")
+    shape32 = sim.autotune("mixed")
+    t = time.perf_counter()
+    ok = sim.run(0, 0, cap, precision)
+    wall = time.perf_counter() - t
+    r = sim.results()
+    sim.close()
+    return {"converged": bool(ok), "iterations": r["iterations"],
+            "fp32_iterations": r["mixed_iterations"], "wall_s": round(wall, 4),
+            "residual": r["residual"], "free_energy_per_area": r["free_energy_per_area"],
+            "sweep_shapes": [list(shape64), list(shape32)]}
+
+
+def batch_bench(sa, workloads, args):
+    """Batched parameter sweep: one iteration of every grain, the sweep of
+    all of them in one persistent launch, against the same grains iterated
+    one after the other."""
+    cfgs = workloads.parameter_sweep(args.batch_count)
+    t = time.perf_counter()
+    b = sa.Batch(cfgs)
+    log("batch of %d grains set up in %.1fs" % (len(cfgs), time.perf_counter() - t))
+    b.compute("fp64")
+    shape = b.autotune("fp64")
+    log("batch autotune fp64: %s" % (shape,))
+    for _ in range(max(args.warmup - 1, 0)):
+        b.compute("fp64")
+    sweeps, steps = [], 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.compute("fp64")
+        sw, _it, n = b.timing()
+        sweeps.append(sw)
+        steps += n
+    wall = time.perf_counter() - t0
+    b.close()
+    # the same grains one at a time
+    seq = 0.0
+    seq_steps = 0
+    for c in cfgs:
+        s = sa.Simulation(c)
+        s.compute("fp64")
+        s.autotune("fp64")
+        t1 = time.perf_counter()
+        s.compute("fp64")
+        seq += time.perf_counter() - t1
+        seq_steps += s.dims.traj_steps
+        s.close()
+    value = steps / wall
+    return {
+        "metric": METRIC + " (batched parameter sweep)", "value": value,
+        "unit": "traj-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "parameter_sweep", "grains": len(cfgs),
+                   "traj_steps_per_iteration": steps // max(args.steps, 1),
+                   "step": "one self-consistency iteration of every grain"},
+        "roofline": roofline(steps / max(args.steps, 1), np.array(sweeps), PEAK_FP64_TFLOPS),
+        "sweep_shape": {"energies_per_lane": shape[0], "waves_per_simd": shape[1],
+                        "slots_per_cu": shape[2]},
+        "sequential": {"value": seq_steps / seq, "ms_per_step": 1e3 * seq,
+                       "speedup_of_batch": (seq_steps / seq and value / (seq_steps / seq))},
+    }
+
+
 def rel(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
 
@@ -156,6 +226,11 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one device-to-device copy of this size first "
                          "(calibrates HBM counters of a profiler run)")
+    ap.add_argument("--no-converge", action="store_true")
+    ap.add_argument("--converge-workload", default="swave_disc",
+                    help="workload of the time-to-criterion comparison")
+    ap.add_argument("--converge-cap", type=int, default=2000)
+    ap.add_argument("--batch-count", type=int, default=48)
     ap.add_argument("--cpu-threads", type=int,
                     default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
                     or min(16, os.cpu_count() or 1))
@@ -172,6 +247,10 @@ def main():
 
     if args.copy_probe_mib > 0:
         copy_probe(args.copy_probe_mib)
+
+    if args.workload == "parameter_sweep":
+        print(json.dumps(batch_bench(sa, workloads, args)), flush=True)
+        return
 
     cfg = workloads.WORKLOADS[args.workload]()
     t = time.perf_counter()
@@ -249,6 +328,17 @@ def main():
                                         "waves_per_simd": mshape[1],
                                         "slots_per_cu": mshape[2]}}
         sim.close()
+
+    if not args.no_converge:
+        ccfg = workloads.WORKLOADS[args.converge_workload]()
+        ccfg["numerics"]["convergence_criterion"] = 1e-8
+        conv = {"workload": args.converge_workload, "criterion": 1e-8}
+        for prec in ("fp64", "mixed_refine"):
+            conv[prec] = converge(sa, ccfg, prec, args.converge_cap)
+            log("converge %s %s: %s" % (args.converge_workload, prec, conv[prec]))
+        if conv["fp64"]["converged"] and conv["mixed_refine"]["converged"]:
+            conv["speedup_mixed_refine"] = conv["fp64"]["wall_s"] / conv["mixed_refine"]["wall_s"]
+        out["time_to_1e-8"] = conv
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds, args.cpu_threads)
