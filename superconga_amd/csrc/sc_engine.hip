@@ -928,13 +928,42 @@ void Grain::acceleratorStep() {
     const size_t m = accD_.size(), M = m + 1;
     // Only the replaced difference changes the Gram matrix: one pass of dot
     // products of it with every stored difference (itself included).
-    if (accGram_.size() != rank * rank) accGram_.assign(rank * rank, 0.0);
-    const std::vector<double> dots = multiDot(accD_[slot], accD_, n);
-    for (size_t i = 0; i < m; ++i) {
-      accGram_[slot * rank + i] = dots[i];
-      accGram_[i * rank + slot] = dots[i];
+    // Field blocks: each order-parameter component (two planes) and the
+    // vector potential (two planes).  Without block scaling one block spans
+    // the whole field (Anderson.h:300-329).
+    const int numBlocks = s.cfg.blockScaling ? (F / 2) : 1;
+    const long long blockLen = s.cfg.blockScaling ? 2 * static_cast<long long>(n0) : n;
+    if (accGram_.size() != rank * rank * size_t(numBlocks))
+      accGram_.assign(rank * rank * size_t(numBlocks), 0.0);
+    std::vector<double> weight(size_t(numBlocks), 1.0);
+    for (int bk = 0; bk < numBlocks; ++bk) {
+      std::vector<double *> ds(m);
+      for (size_t i = 0; i < m; ++i) ds[i] = accD_[i] + size_t(bk) * size_t(blockLen);
+      const std::vector<double> dots = multiDot(accD_[slot] + size_t(bk) * size_t(blockLen), ds,
+                                                blockLen);
+      double *gb = accGram_.data() + size_t(bk) * rank * rank;
+      for (size_t i = 0; i < m; ++i) {
+        gb[slot * rank + i] = dots[i];
+        gb[i * rank + slot] = dots[i];
+      }
+      if (s.cfg.blockScaling) {
+        const double xx =
+            deviceSum(X + size_t(bk) * size_t(blockLen), nullptr, nullptr, blockLen, 2);
+        weight[size_t(bk)] = xx > 0.0 ? 1.0 / xx : 0.0;
+      }
     }
-    accSq_[slot] = dots[slot];
+    for (size_t i = 0; i < m; ++i) {
+      double d = 0.0;
+      for (int bk = 0; bk < numBlocks; ++bk)
+        d += weight[size_t(bk)] * accGram_[size_t(bk) * rank * rank + i * rank + i];
+      accSq_[i] = d;
+    }
+    auto gram = [&](size_t i, size_t j) {
+      double d = 0.0;
+      for (int bk = 0; bk < numBlocks; ++bk)
+        d += weight[size_t(bk)] * accGram_[size_t(bk) * rank * rank + i * rank + j];
+      return d;
+    };
     std::vector<double> A(M * M, 1.0), b(M, 0.0);
     double sr = 0.0;
     for (double v : accSq_) sr += v;
@@ -942,8 +971,8 @@ void Grain::acceleratorStep() {
     for (size_t i = 0; i < m; ++i) {
       A[i * M + i] = accSq_[i] * scale;
       for (size_t j = i + 1; j < m; ++j) {
-        A[i * M + j] = accGram_[i * rank + j] * scale;
-        A[j * M + i] = accGram_[i * rank + j] * scale;
+        A[i * M + j] = gram(i, j) * scale;
+        A[j * M + i] = gram(i, j) * scale;
       }
     }
     A[(M - 1) * M + (M - 1)] = 0.0;

@@ -644,6 +644,7 @@ Config parseConfig(const std::string &text) {
   else throw std::runtime_error("unknown accelerator '" + an + "'");
   c.stepSize = ac["step_size"].asDouble(-1);
   c.rank = ac["rank"].asInt(-1);
+  c.blockScaling = ac["block_scaling"].asBool(false);
   c.drag = ac["drag"].asDouble(-1);
   c.stepFactor = ac["step_size_factor"].asDouble(-1);
   c.cosMin = ac["cos_similarity_min"].asDouble(-1);

@@ -73,6 +73,9 @@ struct Config {
   double stepSize = -1, drag = -1, stepFactor = -1, cosMin = -1, stepMax = -1,
          stepMin = -1;
   int rank = -1;
+  // Anderson extension: weight each field block (a component of Delta, the
+  // vector potential) by 1 / |x_block|^2 in the least squares
+  bool blockScaling = false;
   // misc
   std::string loadPath;
   unsigned long long seed = 0;
