@@ -117,6 +117,34 @@ int sc_run(sc_context *ctx, int num_burnin, int num_min, int num_max,
            int precision, int *converged);
 int sc_get_results(sc_context *ctx, sc_results *out);
 
+/* Outcome of sc_newton_krylov. */
+typedef struct {
+  int newton_iterations;  /* accepted Newton steps */
+  int krylov_iterations;  /* GMRES directions over all steps */
+  int map_evaluations;    /* plain iterations G evaluated */
+  int converged;          /* 1 when |G(u) - u|_W fell below tol */
+  double residual;        /* final |G(u) - u|_W */
+  double *history;        /* optional, caller-owned: |G(u) - u|_W per step */
+  int history_capacity;   /* entries history can hold */
+  int history_length;     /* entries written */
+} sc_newton_report;
+
+/* Jacobian-free Newton-Krylov on the fixed point u = G(u) of one plain
+ * iteration (no reference counterpart: the reference only has the fixed-point
+ * accelerators of accelerators/, Context.h:391-447).  u stacks the order
+ * parameter, the induced vector potential when the penetration depth is
+ * finite, and the boundary in-values of both coherence functions; G is one
+ * FP64 sweep + angle integration + gap equation + boundary condition +
+ * Poisson solve.  Each Newton step solves J d = -(G(u) - u) by restarted-free
+ * GMRES of at most krylov_dim (1..31) directions with finite-difference
+ * products, in the inner product weighted per block by 1 / |u_block|^2,
+ * followed by a backtracking line search; the forcing term follows
+ * Eisenstat-Walker.  On return the state holds the last accepted u and the
+ * accelerator restarts; a following sc_compute reports the reference's
+ * residuals for it. */
+int sc_newton_krylov(sc_context *ctx, int max_newton, int krylov_dim, double tol,
+                     sc_newton_report *report);
+
 /* Individual stages. */
 /* RiccatiSolver::computeCoherenceFunctions for every angle (RiccatiSolverConfined.h:513-671)
  * fused with ComputeProperty::computeGreensAngle; burnin = 1 only updates the

@@ -55,6 +55,14 @@ class Results(ctypes.Structure):
         ("mixed_iterations", ctypes.c_int)]
 
 
+class NewtonReport(ctypes.Structure):
+    _fields_ = [("newton_iterations", ctypes.c_int), ("krylov_iterations", ctypes.c_int),
+                ("map_evaluations", ctypes.c_int), ("converged", ctypes.c_int),
+                ("residual", ctypes.c_double),
+                ("history", ctypes.POINTER(ctypes.c_double)),
+                ("history_capacity", ctypes.c_int), ("history_length", ctypes.c_int)]
+
+
 _lib = None
 _DP = ctypes.POINTER(ctypes.c_double)
 
@@ -262,6 +270,21 @@ class Simulation:
         r = Results()
         _check(lib().sc_get_results(self._h, ctypes.byref(r)))
         return {n: getattr(r, n) for n, _ in Results._fields_}
+
+    def newton_krylov(self, max_newton=20, krylov_dim=20, tol=1e-9):
+        """Jacobian-free Newton-Krylov on the fixed point of one plain FP64
+        iteration; returns the report as a dict with the weighted residual
+        history.  A following compute() reports the usual residuals."""
+        hist = (ctypes.c_double * (max_newton + 1))()
+        r = NewtonReport(history=ctypes.cast(hist, ctypes.POINTER(ctypes.c_double)),
+                         history_capacity=max_newton + 1)
+        _check(lib().sc_newton_krylov(self._h, max_newton, krylov_dim, ctypes.c_double(tol),
+                                      ctypes.byref(r)))
+        return {"newton_iterations": r.newton_iterations,
+                "krylov_iterations": r.krylov_iterations,
+                "map_evaluations": r.map_evaluations, "converged": bool(r.converged),
+                "residual": r.residual,
+                "history": [hist[i] for i in range(r.history_length)]}
 
     def riccati_sweep(self, precision="fp64", burnin=False):
         _check(lib().sc_riccati_sweep(self._h, _precision(precision), int(burnin)))

@@ -4,6 +4,7 @@
 #include "sc_engine.hpp"
 #include "sc_device.hpp"
 
+#include <algorithm>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -310,6 +311,32 @@ int sc_autotune_sweep(sc_context *ctx, int precision) {
     checkPrecision(precision);
     ctx->grain->autotune(precision);
     hipCheck(hipStreamSynchronize(ctx->grain->stream()), "autotune");
+    return SC_OK;
+  });
+}
+
+int sc_newton_krylov(sc_context *ctx, int maxNewton, int krylovDim, double tol,
+                     sc_newton_report *report) {
+  SC_CHECK_CTX(ctx);
+  return guarded([&] {
+    if (maxNewton < 0) throw std::invalid_argument("max_newton must be >= 0");
+    if (krylovDim < 1 || krylovDim > 31)
+      throw std::invalid_argument("krylov_dim must be in [1, 31]");
+    if (!(tol > 0.0)) throw std::invalid_argument("tol must be positive");
+    if (report && report->history_capacity > 0 && !report->history)
+      throw std::invalid_argument("history is null with a positive capacity");
+    const scamd::NewtonReport r = ctx->grain->newtonKrylov(maxNewton, krylovDim, tol);
+    if (report) {
+      report->newton_iterations = r.newtonIterations;
+      report->krylov_iterations = r.krylovIterations;
+      report->map_evaluations = r.mapEvaluations;
+      report->converged = r.converged ? 1 : 0;
+      report->residual = r.residual;
+      const int n = std::min(static_cast<int>(r.history.size()),
+                             std::max(report->history_capacity, 0));
+      for (int i = 0; i < n; ++i) report->history[i] = r.history[static_cast<size_t>(i)];
+      report->history_length = n;
+    }
     return SC_OK;
   });
 }

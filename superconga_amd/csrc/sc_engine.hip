@@ -4,6 +4,7 @@
 // reads back scalars (residuals, the accelerator's Gram matrix).
 
 #include "sc_engine.hpp"
+#include "sc_util.hpp"
 #include "sc_kernels.hpp"
 
 #include <algorithm>
@@ -14,42 +15,6 @@
 
 namespace scamd {
 
-#define SC_HIP(expr)                                                            \
-  do {                                                                          \
-    const hipError_t err_ = (expr);                                             \
-    if (err_ != hipSuccess)                                                     \
-      throw std::runtime_error(std::string("HIP error in ") + #expr + ": " +    \
-                               hipGetErrorString(err_));                        \
-  } while (0)
-
-namespace {
-
-// Allocates and clears.  The clear runs on the null stream, which the
-// engine's non-blocking stream does not wait for, so it is completed here
-// before any stream work can touch the buffer.
-template <class T> T *devAlloc(size_t n) {
-  T *p = nullptr;
-  if (n == 0) n = 1;
-  SC_HIP(hipMalloc(reinterpret_cast<void **>(&p), n * sizeof(T)));
-  SC_HIP(hipMemset(p, 0, n * sizeof(T)));
-  SC_HIP(hipDeviceSynchronize());
-  return p;
-}
-template <class T> T *devUpload(const std::vector<T> &v) {
-  T *p = devAlloc<T>(v.size());
-  if (!v.empty())
-    SC_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-  return p;
-}
-template <class T> void devFree(T *&p) {
-  if (p) (void)hipFree(p);
-  p = nullptr;
-}
-inline unsigned blocksFor(long long n, int bs = 256) {
-  return static_cast<unsigned>((n + bs - 1) / bs);
-}
-
-} // namespace
 
 Grain::Grain(std::unique_ptr<Setup> setup, hipStream_t stream) : s_(std::move(setup)) {
   if (stream) {
@@ -120,6 +85,7 @@ Grain::~Grain() {
   devFree(dPartials_);
   devFree(dRed_);
   devFree(dMean_);
+  for (double *p : nkVecs_) devFree(p);
   devFree(dWork_);
   devFree(dWork2_);
   fft_.reset();
@@ -768,6 +734,27 @@ void Grain::boundaryCondition() {
   res_.resBoundary = boundaryResidual(hb);
 }
 
+void Grain::backRotate() {
+  // Extension of the Green-function planes of every frame, then their
+  // back-rotation, angle integration, gap equation and current
+  // (ComputeOrderParameter.h:174-262, ComputeCurrent.h:142-222).
+  const Setup &s = *s_;
+  const size_t n1 = size_t(s.N1) * s.N1;
+  const int nt = static_cast<int>(s.extRot.target.size());
+  if (nt > 0) {
+    extendKernel<<<blocksFor(nt), 256, 0, stream_>>>(
+        dG_, dExtRotT_, dExtRotO_, dExtRotS_, nt, 4, 1, static_cast<long long>(n1),
+        static_cast<long long>(4 * n1), 4);
+    SC_HIP(hipGetLastError());
+  }
+  GrainView v = view();
+  const int ns = static_cast<int>(s.domainSites.size());
+  const double jf = 2.0 * static_cast<double>(s.cfg.chargeSign) * s.temperature;
+  gatherKernel<<<blocksFor(ns), 256, 0, stream_>>>(v, dDomainSites_, ns, dCoupling_, jf,
+                                                   dDeltaNew_, dCurrent_, dEnergyDens_);
+  SC_HIP(hipGetLastError());
+}
+
 void Grain::afterSweep(bool burnin) {
   // Back-rotation and angle integration of the swept Green functions, the
   // compute objects' reductions and the boundary condition, queued for one
@@ -776,21 +763,9 @@ void Grain::afterSweep(bool burnin) {
   const size_t n0 = size_t(s.N0) * s.N0, n1 = size_t(s.N1) * s.N1;
   pendingIntegrate_ = !burnin;
   if (!burnin) {
-    const int nt = static_cast<int>(s.extRot.target.size());
-    if (nt > 0) {
-      extendKernel<<<blocksFor(nt), 256, 0, stream_>>>(
-          dG_, dExtRotT_, dExtRotO_, dExtRotS_, nt, 4, 1, static_cast<long long>(n1),
-          static_cast<long long>(4 * n1), 4);
-      SC_HIP(hipGetLastError());
-    }
     SC_HIP(hipMemcpyAsync(dCurrentOld_, dCurrent_, 2 * n0 * sizeof(double),
                           hipMemcpyDeviceToDevice, stream_));
-    GrainView v = view();
-    const int ns = static_cast<int>(s.domainSites.size());
-    const double jf = 2.0 * static_cast<double>(s.cfg.chargeSign) * s.temperature;
-    gatherKernel<<<blocksFor(ns), 256, 0, stream_>>>(v, dDomainSites_, ns, dCoupling_, jf,
-                                                     dDeltaNew_, dCurrent_, dEnergyDens_);
-    SC_HIP(hipGetLastError());
+    backRotate();
     SC_HIP(hipEventRecord(ev2_, stream_));
     // ComputeOrderParameter::computePropertyFromGreens (:212-262)
     hDelta_.assign(size_t(s.Nc), -1);
@@ -889,6 +864,21 @@ void Grain::solveVectorPotential(const double *dj, double *dout) {
     SC_HIP(hipGetLastError());
   }
   fft_->solve(dj, dMean_, dout);
+}
+
+void Grain::resetAccelerator() {
+  SC_HIP(hipStreamSynchronize(stream_));
+  for (double *p : accX_) devFree(p);
+  for (double *p : accD_) devFree(p);
+  accX_.clear();
+  accD_.clear();
+  accSq_.clear();
+  accIdx_.clear();
+  accGram_.clear();
+  accIter_ = 0;
+  accInit_ = false;
+  // the next iteration starts without an accelerator step, like the first
+  firstCompute_ = true;
 }
 
 void Grain::acceleratorStep() {

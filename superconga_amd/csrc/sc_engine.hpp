@@ -31,6 +31,16 @@ struct Timing {
   long long trajSteps = 0;
 };
 
+// Outcome of a Newton-Krylov solve: Newton steps, GMRES iterations, map
+// evaluations (plain iterations) and the block-weighted residual norm after
+// every Newton step.
+struct NewtonReport {
+  int newtonIterations = 0, krylovIterations = 0, mapEvaluations = 0;
+  double residual = 0;
+  bool converged = false;
+  std::vector<double> history;
+};
+
 class Grain {
 public:
   // With a stream the grain runs on it (and does not own it); without, it
@@ -83,6 +93,12 @@ public:
   // Times every compiled sweep variant on the current state and keeps the
   // fastest for `precision`; the sweeps it runs leave the state unchanged.
   void autotune(int precision);
+  // Jacobian-free Newton-Krylov solve of the fixed point of one iteration
+  // (state: order parameter, scaled vector potential, boundary in-values)
+  // to a block-weighted residual below tol; leaves the solution as the
+  // iterate and restarts the accelerators.
+  NewtonReport newtonKrylov(int maxNewton, int krylovDim, double tol);
+  void resetAccelerator();
   // Boundary initial values (first call) and the extended order parameter:
   // the inputs of a sweep.
   void prepareSweepInputs() {
@@ -107,6 +123,7 @@ private:
   void extendDelta();
   void ensureBoundaryInit();
   void afterSweep(bool burnin);
+  void backRotate();
   void finishIntegrate();
   void acceleratorStep();
   double residualOf(const double *oldv, const double *newv, long long n);
@@ -156,6 +173,18 @@ private:
   double *dG_ = nullptr;
   double *dPartials_ = nullptr, *dWork_ = nullptr, *dWork2_ = nullptr;
   std::unique_ptr<PoissonFFT> fft_;
+  // Newton-Krylov state
+  struct NkBlock {
+    size_t offset, length;
+  };
+  std::vector<NkBlock> nkBlocks() const;
+  size_t nkLength() const;
+  void nkGet(double *u);
+  void nkSet(const double *u);
+  void nkMap(double *gu);
+  std::vector<double *> nkVecs_;
+  size_t nkLen_ = 0;
+  int nkMaps_ = 0;
   struct Pending {
     int kind = 0; // 0: sum (one partial per block), 1: change statistics (six)
     long long blocks = 0, n = 0;

@@ -1,0 +1,300 @@
+// sc_newton.hip -- Jacobian-free Newton-Krylov solve of the self-consistency
+// fixed point (declared in sc_engine.hpp).
+//
+// The fixed point of one iteration is G(u) = u, where the state u holds the
+// order parameter, the scaled induced vector potential and the boundary
+// in-values of both coherence functions (the reference iterates all three:
+// Context.h:391-447 mixes the first two, the boundary condition updates the
+// third).  Newton's method solves F(u) = G(u) - u = 0; each step solves
+// J d = -F with restarted GMRES in the block-weighted inner product
+// <a, b> = sum_blocks <a_b, b_b> / |u_b|^2, so every block is measured
+// relative to its size like the reference's residuals.  Jacobian-vector
+// products are finite differences of G: J v = (G(u + e v) - G(u)) / e - v.
+// G is evaluated in FP64; the sweep and everything after it are the same
+// kernels as the plain iteration.
+#include "sc_engine.hpp"
+#include "sc_device.hpp"
+#include "sc_util.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#define SC_NK_HIP(call)                                                             \
+  do {                                                                              \
+    const hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess)                                                           \
+      throw std::runtime_error(std::string("HIP error in the Newton-Krylov solve: ") + \
+                               hipGetErrorString(e_));                              \
+  } while (0)
+
+namespace scamd {
+namespace {
+
+constexpr int kMaxTerms = 32;
+struct Terms {
+  const double *v[kMaxTerms];
+  double c[kMaxTerms];
+};
+
+// y = a * base + sum_i c_i v_i (base may alias y)
+__global__ void combineKernel(double *y, const double *base, double a, Terms t, int m,
+                              long long n) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double acc = base ? a * base[i] : 0.0;
+  for (int q = 0; q < m; ++q) acc += t.c[q] * t.v[q][i];
+  y[i] = acc;
+}
+
+unsigned blocks256(long long n) { return static_cast<unsigned>((n + 255) / 256); }
+
+} // namespace
+
+std::vector<Grain::NkBlock> Grain::nkBlocks() const {
+  const Setup &s = *s_;
+  const size_t n0 = size_t(s.N0) * s.N0;
+  const bool doA = s.cfg.computeCurrent && s.lambda > 0.0;
+  std::vector<NkBlock> b;
+  size_t off = 0;
+  for (int c = 0; c < s.Nc; ++c) {
+    b.push_back({off, 2 * n0});
+    off += 2 * n0;
+  }
+  if (doA) {
+    b.push_back({off, 2 * n0});
+    off += 2 * n0;
+  }
+  const size_t g = size_t(2 * s.nInG * s.Ne), gt = size_t(2 * s.nInGt * s.Ne);
+  if (g) b.push_back({off, g});
+  off += g;
+  if (gt) b.push_back({off, gt});
+  return b;
+}
+
+size_t Grain::nkLength() const {
+  const Setup &s = *s_;
+  const size_t n0 = size_t(s.N0) * s.N0;
+  const bool doA = s.cfg.computeCurrent && s.lambda > 0.0;
+  return size_t(2 * s.Nc + (doA ? 2 : 0)) * n0 + size_t(2 * s.nInG * s.Ne) +
+         size_t(2 * s.nInGt * s.Ne);
+}
+
+void Grain::nkGet(double *u) {
+  const Setup &s = *s_;
+  const size_t n0 = size_t(s.N0) * s.N0;
+  const bool doA = s.cfg.computeCurrent && s.lambda > 0.0;
+  const size_t nx = size_t(2 * s.Nc + (doA ? 2 : 0)) * n0;
+  const size_t g = size_t(2 * s.nInG * s.Ne), gt = size_t(2 * s.nInGt * s.Ne);
+  SC_NK_HIP(hipMemcpyAsync(u, dDelta_, nx * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  SC_NK_HIP(hipMemcpyAsync(u + nx, dBInG_, g * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  SC_NK_HIP(hipMemcpyAsync(u + nx + g, dBInGt_, gt * sizeof(double), hipMemcpyDeviceToDevice,
+                           stream_));
+}
+
+void Grain::nkSet(const double *u) {
+  const Setup &s = *s_;
+  const size_t n0 = size_t(s.N0) * s.N0;
+  const bool doA = s.cfg.computeCurrent && s.lambda > 0.0;
+  const size_t nx = size_t(2 * s.Nc + (doA ? 2 : 0)) * n0;
+  const size_t g = size_t(2 * s.nInG * s.Ne), gt = size_t(2 * s.nInGt * s.Ne);
+  SC_NK_HIP(hipMemcpyAsync(dDelta_, u, nx * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  SC_NK_HIP(hipMemcpyAsync(dBInG_, u + nx, g * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  SC_NK_HIP(hipMemcpyAsync(dBInGt_, u + nx + g, gt * sizeof(double), hipMemcpyDeviceToDevice,
+                           stream_));
+}
+
+void Grain::nkMap(double *gu) {
+  // One plain iteration of the current state without the accelerator and
+  // without the compute objects' reductions.
+  const Setup &s = *s_;
+  const size_t n0 = size_t(s.N0) * s.N0;
+  const bool doA = s.cfg.computeCurrent && s.lambda > 0.0;
+  ensureBoundaryInit();
+  extendDelta();
+  sweep(PREC_FP64, false);
+  backRotate();
+  int hb[2];
+  queueBoundaryCondition(hb); // new in-values, in place
+  if (doA) solveVectorPotential(dCurrent_, dANew_);
+  clearReductions();
+  const size_t nx = size_t(2 * s.Nc + (doA ? 2 : 0)) * n0;
+  const size_t g = size_t(2 * s.nInG * s.Ne), gt = size_t(2 * s.nInGt * s.Ne);
+  SC_NK_HIP(hipMemcpyAsync(gu, dDeltaNew_, nx * sizeof(double), hipMemcpyDeviceToDevice,
+                           stream_));
+  SC_NK_HIP(hipMemcpyAsync(gu + nx, dBInG_, g * sizeof(double), hipMemcpyDeviceToDevice,
+                           stream_));
+  SC_NK_HIP(hipMemcpyAsync(gu + nx + g, dBInGt_, gt * sizeof(double), hipMemcpyDeviceToDevice,
+                           stream_));
+  ++nkMaps_;
+}
+
+NewtonReport Grain::newtonKrylov(int maxNewton, int krylovDim, double tol) {
+  if (krylovDim < 1 || krylovDim > kMaxTerms - 1)
+    throw std::runtime_error("Krylov dimension must be in [1, " + std::to_string(kMaxTerms - 1) +
+                             "]");
+  const std::vector<NkBlock> blocks = nkBlocks();
+  const size_t L = nkLength();
+  const long long nL = static_cast<long long>(L);
+  // workspace: u, G(u), F, trial, w, and the Krylov basis
+  const int nv = 5 + krylovDim + 1;
+  if (nkVecs_.size() != size_t(nv) || nkLen_ != L) {
+    for (double *p : nkVecs_) devFree(p);
+    nkVecs_.clear();
+    for (int i = 0; i < nv; ++i) nkVecs_.push_back(devAlloc<double>(L));
+    nkLen_ = L;
+  }
+  double *u = nkVecs_[0], *gu = nkVecs_[1], *f = nkVecs_[2], *trial = nkVecs_[3],
+         *w = nkVecs_[4];
+  std::vector<double *> V(nkVecs_.begin() + 5, nkVecs_.end());
+  nkMaps_ = 0;
+  NewtonReport rep;
+
+  // block weights 1 / |u_b|^2 of the starting state (blocks at zero: 1 / size)
+  ensureBoundaryInit();
+  nkGet(u);
+  std::vector<double> wt(blocks.size(), 1.0);
+  for (size_t b = 0; b < blocks.size(); ++b) {
+    const double uu = multiDot(u + blocks[b].offset, {u + blocks[b].offset},
+                               static_cast<long long>(blocks[b].length))[0];
+    wt[b] = uu > 0.0 ? 1.0 / uu : 1.0 / static_cast<double>(blocks[b].length);
+  }
+  // weighted dots of a with every vector of bs
+  auto wdots = [&](const double *a, const std::vector<double *> &bs) {
+    std::vector<double> out(bs.size(), 0.0);
+    for (size_t b = 0; b < blocks.size(); ++b) {
+      std::vector<double *> part(bs.size());
+      for (size_t i = 0; i < bs.size(); ++i) part[i] = bs[i] + blocks[b].offset;
+      const std::vector<double> d = multiDot(a + blocks[b].offset, part,
+                                             static_cast<long long>(blocks[b].length));
+      for (size_t i = 0; i < bs.size(); ++i) out[i] += wt[b] * d[i];
+    }
+    return out;
+  };
+  auto wnorm = [&](double *a) { return std::sqrt(wdots(a, {a})[0]); };
+  auto combine = [&](double *y, const double *base, double a,
+                     const std::vector<const double *> &vs, const std::vector<double> &cs) {
+    if (vs.size() > size_t(kMaxTerms)) throw std::runtime_error("too many terms");
+    Terms t{};
+    for (size_t q = 0; q < vs.size(); ++q) {
+      t.v[q] = vs[q];
+      t.c[q] = cs[q];
+    }
+    combineKernel<<<blocks256(nL), 256, 0, stream_>>>(y, base, a, t, static_cast<int>(vs.size()),
+                                                      nL);
+    SC_NK_HIP(hipGetLastError());
+  };
+  // F(x) into f given x: writes G(x) into gout
+  auto residual = [&](const double *x, double *gout, double *fout) {
+    nkSet(x);
+    nkMap(gout);
+    combine(fout, gout, 1.0, {x}, {-1.0});
+    return wnorm(fout);
+  };
+
+  double fn = residual(u, gu, f);
+  double fnPrev = fn;
+  rep.history.push_back(fn);
+  double eta = 0.1;
+  for (int it = 0; it < maxNewton && fn >= tol; ++it) {
+    // GMRES(krylovDim) on J d = -F from d = 0
+    const double beta = fn;
+    combine(V[0], nullptr, 0.0, {f}, {-1.0 / beta});
+    std::vector<double> H(size_t(krylovDim + 1) * krylovDim, 0.0), g(size_t(krylovDim + 1), 0.0),
+        cs(size_t(krylovDim), 0.0), sn(size_t(krylovDim), 0.0);
+    g[0] = beta;
+    int k = 0;
+    const double unorm = wnorm(u);
+    for (; k < krylovDim; ++k) {
+      const double e = 1e-7 * (1.0 + unorm);
+      combine(trial, u, 1.0, {V[size_t(k)]}, {e});
+      nkSet(trial);
+      nkMap(w);
+      combine(w, w, 1.0 / e, {gu, V[size_t(k)]}, {-1.0 / e, -1.0});
+      // classical Gram-Schmidt, twice
+      std::vector<double *> basis(V.begin(), V.begin() + k + 1);
+      std::vector<const double *> cbasis(basis.begin(), basis.end());
+      std::vector<double> h(size_t(k) + 1, 0.0);
+      for (int pass = 0; pass < 2; ++pass) {
+        const std::vector<double> d = wdots(w, basis);
+        std::vector<double> neg(d.size());
+        for (size_t i = 0; i < d.size(); ++i) {
+          h[i] += d[i];
+          neg[i] = -d[i];
+        }
+        combine(w, w, 1.0, cbasis, neg);
+      }
+      const double hn = wnorm(w);
+      for (int i = 0; i <= k; ++i) H[size_t(i) * krylovDim + k] = h[size_t(i)];
+      H[size_t(k + 1) * krylovDim + k] = hn;
+      // Givens rotations
+      for (int i = 0; i < k; ++i) {
+        const double a = H[size_t(i) * krylovDim + k], b = H[size_t(i + 1) * krylovDim + k];
+        H[size_t(i) * krylovDim + k] = cs[size_t(i)] * a + sn[size_t(i)] * b;
+        H[size_t(i + 1) * krylovDim + k] = -sn[size_t(i)] * a + cs[size_t(i)] * b;
+      }
+      const double a = H[size_t(k) * krylovDim + k], b = H[size_t(k + 1) * krylovDim + k];
+      const double r = std::hypot(a, b);
+      cs[size_t(k)] = r > 0.0 ? a / r : 1.0;
+      sn[size_t(k)] = r > 0.0 ? b / r : 0.0;
+      H[size_t(k) * krylovDim + k] = r;
+      H[size_t(k + 1) * krylovDim + k] = 0.0;
+      g[size_t(k + 1)] = -sn[size_t(k)] * g[size_t(k)];
+      g[size_t(k)] = cs[size_t(k)] * g[size_t(k)];
+      if (hn > 0.0) combine(V[size_t(k + 1)], nullptr, 0.0, {w}, {1.0 / hn});
+      if (std::fabs(g[size_t(k + 1)]) <= eta * beta || hn == 0.0) {
+        ++k;
+        break;
+      }
+    }
+    // back substitution: H y = g on the first k rows
+    std::vector<double> y(size_t(k), 0.0);
+    for (int i = k - 1; i >= 0; --i) {
+      double acc = g[size_t(i)];
+      for (int q = i + 1; q < k; ++q) acc -= H[size_t(i) * krylovDim + q] * y[size_t(q)];
+      y[size_t(i)] = H[size_t(i) * krylovDim + i] != 0.0 ? acc / H[size_t(i) * krylovDim + i] : 0.0;
+    }
+    rep.krylovIterations += k;
+    // step d = V y, with backtracking on |F|
+    std::vector<const double *> vk(V.begin(), V.begin() + k);
+    double lambda = 1.0, fnew = 0.0;
+    bool accepted = false;
+    for (int ls = 0; ls < 6; ++ls) {
+      std::vector<double> ys(y);
+      for (double &q : ys) q *= lambda;
+      combine(trial, u, 1.0, vk, ys);
+      fnew = residual(trial, w, V[size_t(krylovDim)]);
+      if (fnew < (1.0 - 1e-4 * lambda) * fn) {
+        accepted = true;
+        break;
+      }
+      lambda *= 0.5;
+    }
+    if (!accepted) break; // no descent along the Newton direction
+    std::swap(u, trial);
+    std::swap(gu, w);
+    combine(f, V[size_t(krylovDim)], 1.0, {}, {});
+    fnPrev = fn;
+    fn = fnew;
+    rep.history.push_back(fn);
+    ++rep.newtonIterations;
+    // Eisenstat-Walker forcing term (choice 2)
+    eta = std::clamp(0.9 * (fn / fnPrev) * (fn / fnPrev), 1e-4, 0.3);
+    nkVecs_[0] = u;
+    nkVecs_[1] = gu;
+    nkVecs_[3] = trial;
+    nkVecs_[4] = w;
+  }
+  nkSet(u);
+  SC_NK_HIP(hipStreamSynchronize(stream_));
+  rep.residual = fn;
+  rep.converged = fn < tol;
+  rep.mapEvaluations = nkMaps_;
+  // The iterate now lives in the state; start the accelerators afresh.
+  resetAccelerator();
+  return rep;
+}
+
+} // namespace scamd

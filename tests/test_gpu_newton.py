@@ -1,0 +1,83 @@
+"""Jacobian-free Newton-Krylov (sc_newton_krylov) against the fixed point
+that plain self-consistency iterations converge to.
+
+The fixed point itself is the reference's: the iterations that reach it are
+parity-tested against the oracle iteration by iteration, and its free energy
+matches the accelerator known answer of tests/accelerators_test.cu:42-150.
+Newton's answer must be that same fixed point: the map G it solves u = G(u)
+for is one plain FP64 iteration, so its zero is the state every accelerator
+converges to.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import cases
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def converged_plain(product, cfg, criterion=1e-12):
+    c = json.loads(json.dumps(cfg))
+    c["numerics"]["convergence_criterion"] = criterion
+    s = product.Simulation(c)
+    assert s.run(0, 0, 3000, "fp64")
+    return s
+
+
+def test_newton_krylov_finds_the_fixed_point(product):
+    cfg = cases.accelerator_case("picard")
+    ref = converged_plain(product, cfg)
+    s = product.Simulation(cfg)
+    rep = s.newton_krylov(max_newton=20, krylov_dim=20, tol=1e-10)
+    print("newton-krylov:", {k: v for k, v in rep.items() if k != "history"},
+          "history", ["%.2e" % h for h in rep["history"]])
+    assert rep["converged"], rep
+    assert rep["history"][-1] < 1e-10
+    # residuals decrease monotonically (the line search only accepts descent)
+    assert all(b < a for a, b in zip(rep["history"], rep["history"][1:]))
+    assert rep["map_evaluations"] >= rep["krylov_iterations"] + rep["newton_iterations"] + 1
+    # a plain iteration from Newton's answer leaves it where it is
+    s.compute("fp64")
+    r = s.results()
+    assert r["residual"] < 1e-8, r
+    assert rel(s.order_parameter(1), ref.order_parameter(1)) < 1e-7
+    assert rel(s.current(), ref.current()) < 1e-6
+    assert rel(s.vector_potential(1), ref.vector_potential(1)) < 1e-6
+    with open(os.path.join(GOLDEN, "known_answers.json")) as f:
+        ka = json.load(f)["accelerator_free_energy_per_area"]
+    assert abs(r["free_energy_per_area"] - ka["value"]) < ka["tolerance_rel"] * abs(ka["value"])
+    assert abs(r["free_energy_per_area"] - ref.results()["free_energy_per_area"]) < 1e-9 * abs(
+        ka["value"])
+
+
+def test_newton_krylov_without_vector_potential(product):
+    # no induced potential: the state is the order parameter and the boundary
+    cfg = cases.base(temperature=0.5, angles=16, ppxi=3.0,
+                     order_parameter={"dx2-y2": {"critical_temperature": 1.0}},
+                     rotation=0.125, components=[cases.regular_polygon_unit(6)])
+    ref = converged_plain(product, cfg)
+    s = product.Simulation(cfg)
+    rep = s.newton_krylov(max_newton=20, krylov_dim=16, tol=1e-10)
+    assert rep["converged"], rep
+    s.compute("fp64")
+    assert s.results()["residual"] < 1e-8
+    assert rel(s.order_parameter(1), ref.order_parameter(1)) < 1e-7
+
+
+def test_newton_krylov_rejects_bad_arguments(product):
+    s = product.Simulation(cases.accelerator_case("picard"))
+    for kw in ({"krylov_dim": 0}, {"krylov_dim": 32}, {"tol": 0.0}, {"max_newton": -1}):
+        args = dict(max_newton=2, krylov_dim=4, tol=1e-8)
+        args.update(kw)
+        with pytest.raises(product.SupercongaError):
+            s.newton_krylov(**args)
