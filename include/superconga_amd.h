@@ -117,6 +117,48 @@ int sc_run(sc_context *ctx, int num_burnin, int num_min, int num_max,
            int precision, int *converged);
 int sc_get_results(sc_context *ctx, sc_results *out);
 
+/* ----- Keldysh spectroscopy (src/postprocess.cu) ----- */
+
+/* A postprocess context: the simulation config's physics and geometry, the
+ * postprocess config's energies (spectroscopy.energy_min/max, num_energies,
+ * energy_broadening), angles, criterion, norm, energies per block and
+ * iteration limits (src/postprocess.cu:64-330).  The order parameter starts
+ * at zero and the vector potential at zero: set them with
+ * sc_set_order_parameter / sc_set_vector_potential (the reference reads
+ * order_parameter.csv and vector_potential.csv of misc.load_path).  Such a
+ * context runs only the sc_ldos_* iteration and the sweep-shape calls. */
+int sc_ldos_create(const char *simulation_config_json, const char *postprocess_config_json,
+                   sc_context **out);
+/* Context<T>::compute / burnIn of the postprocess context: one iteration of
+ * IntegrationIteratorKeldysh::compute (IntegrationIteratorKeldysh.h:235-388)
+ * with ComputeLDOS (ComputeLDOS.h:110-192) on the current energy block. */
+int sc_ldos_compute(sc_context *ctx, int precision, int burnin);
+/* runPostprocess (run_simulation.h:44-108): for every energy block a burn-in
+ * of num_burnin iterations (-1: until the boundary converges) and then
+ * compute iterations between num_min and num_max (-1: no maximum) until the
+ * block converges; stops at the first block that does not.  Pass the
+ * postprocess config's num_iterations_* to reproduce the reference. */
+int sc_ldos_run(sc_context *ctx, int num_burnin, int num_min, int num_max, int precision,
+                int *converged);
+typedef struct {
+  int num_energies, energies_per_block;
+  int block_start, block_count; /* current energy block */
+  int iterations;
+  int angle_chunk;              /* angles whose spectral weights are held at once */
+  int partially_converged;      /* Context::isPartiallyConverged */
+  int converged;                /* Context::isConverged: every block done */
+  double residual;              /* max of the two below (Context::getResidual) */
+  double residual_boundary, residual_ldos;
+  double last_sweep_ms, last_iteration_ms;
+  long long traj_steps;         /* per iteration of the current block */
+} sc_ldos_status;
+int sc_ldos_get_status(sc_context *ctx, sc_ldos_status *out);
+/* The LDOS of every energy, ComputeLDOS::getResult: [num_energies][N0][N0]. */
+int sc_ldos_get(sc_context *ctx, double *out);
+/* Angles swept per chunk of spectral weights (0: as many as half the free
+ * device memory holds, at most 64 GiB). */
+int sc_ldos_set_angle_chunk(sc_context *ctx, int angles);
+
 /* Outcome of sc_newton_krylov. */
 typedef struct {
   int newton_iterations;  /* accepted Newton steps */

@@ -775,8 +775,20 @@ struct Oracle {
                                      // frame on every use)
   // Fermi surface
   std::vector<double> systemAngle, polar;
-  // energies
-  std::vector<double> energyIm, residue;
+  // energies: eps_e = energyRe[e] + i energyIm[e] (energyRe is zero for the
+  // Ozaki poles of the self-consistency)
+  std::vector<double> energyRe, energyIm, residue;
+  // Keldysh spectroscopy (IntegrationIteratorKeldysh.h, ComputeLDOS.h)
+  struct Spectroscopy {
+    bool on = false;
+    double eMin = 0, eMax = 0, broadening = 0;
+    int numEnergies = 0, perBlock = 0;
+    int blockStart = 0, lastBlock = -1;
+    std::vector<double> all;      // numEnergies x N0^2
+    std::vector<double> interval; // Ne x N0^2
+    double residual = 0;
+    bool partial = false, converged = false;
+  } kel;
   // order parameter
   std::vector<int> symmetry;
   std::vector<double> tc, basisNorm, coupling;
@@ -1113,7 +1125,7 @@ static long long riccatiAngle(Oracle &o, int k, const std::vector<double> &drot,
     const long long r = isGamma ? t : t - (long long)W * Ne;
     const int x = x0 + int(r % W);
     const int e = int(r / W);
-    const cplx e0(0.0, o.energyIm[e]);
+    const cplx e0(o.energyRe[e], o.energyIm[e]);
     int cIn = 0, cOut = 0;
     cplx eprev(0, 0), dprev(0, 0), coh(0, 0);
     double *gr = isGamma ? o.gamma.data() : o.gammat.data();
@@ -1183,7 +1195,7 @@ static void initialConditionAngle(Oracle &o, int k,
     for (int dir = 0; dir < 2; ++dir) {
       const bool isGamma = dir == 0;
       for (int e = 0; e < Ne; ++e) {
-        const cplx e0(0.0, o.energyIm[e]);
+        const cplx e0(o.energyRe[e], o.energyIm[e]);
         int cnt = 0;
         for (int i = 0; i < N1; ++i) {
           const int y = isGamma ? i : N1 - 1 - i;
@@ -1926,6 +1938,137 @@ static bool contextConverged(const Oracle &o) {
 }
 
 // ---------------------------------------------------------------------------
+// Keldysh spectroscopy (src/postprocess.cu:64-472): the LDOS at real
+// energies E + i delta, energy block by energy block.  The context holds no
+// order-parameter, current or vector-potential compute object, so there is
+// no accelerator step and no Poisson solve (Context.h:391-447).
+// ---------------------------------------------------------------------------
+// IntegrationIteratorKeldysh::initEnergies (:115-150)
+static double keldyshEnergy(const Oracle &o, int i) {
+  const int n = o.kel.numEnergies;
+  const double step = n > 1 ? (o.kel.eMax - o.kel.eMin) / static_cast<double>(n - 1) : 0.0;
+  return o.kel.eMin + step * static_cast<double>(i);
+}
+// A new energy interval: the boundary storage for its energies and the
+// homogeneous initial condition (IntegrationIteratorKeldysh.h:184-232,274-293).
+static void keldyshBeginBlock(Oracle &o) {
+  const int start = o.kel.blockStart;
+  o.Ne = std::min(o.kel.perBlock, o.kel.numEnergies - start);
+  o.energyRe.resize(size_t(o.Ne));
+  o.energyIm.assign(size_t(o.Ne), o.kel.broadening);
+  o.residue.assign(size_t(o.Ne), 1.0);
+  for (int e = 0; e < o.Ne; ++e) o.energyRe[size_t(e)] = keldyshEnergy(o, start + e);
+  o.bInG.assign(size_t(2 * o.nInG * o.Ne), 0.0);
+  o.bOutG.assign(size_t(2 * o.nOutG * o.Ne), 0.0);
+  o.bInGt.assign(size_t(2 * o.nInGt * o.Ne), 0.0);
+  o.bOutGt.assign(size_t(2 * o.nOutGt * o.Ne), 0.0);
+  o.gamma.assign(size_t(o.Ne) * 2 * o.n1sq(), 0.0);
+  o.gammat.assign(size_t(o.Ne) * 2 * o.n1sq(), 0.0);
+  o.kel.interval.assign(size_t(o.Ne) * o.n0sq(), 0.0);
+  setBoundaryInitialCondition(o);
+  o.kel.partial = false;
+  o.kel.lastBlock = start;
+}
+static bool keldyshDone(const Oracle &o) { return o.kel.blockStart >= o.kel.numEnergies; }
+
+// IntegrationIteratorKeldysh::compute (:235-388) with ComputeLDOS
+// (ComputeLDOS.h:110-192), then Context::compute / burnIn bookkeeping.
+static void keldyshCompute(Oracle &o, bool burnin) {
+  if (!keldyshDone(o)) {
+    if (o.kel.lastBlock != o.kel.blockStart) keldyshBeginBlock(o);
+    const size_t n0 = o.n0sq(), n1 = o.n1sq();
+    std::vector<double> drot, es;
+    if (burnin) {
+      for (int k = 0; k < o.Na; ++k) {
+        rotatedOrderParameter(o, k, drot);
+        energyShift(o, k, es);
+        riccatiAngle(o, k, drot, es, false);
+      }
+      boundaryCondition(o);
+    } else {
+      std::fill(o.kel.interval.begin(), o.kel.interval.end(), 0.0);
+      std::vector<double> gim(n1);
+      for (int k = 0; k < o.Na; ++k) {
+        rotatedOrderParameter(o, k, drot);
+        energyShift(o, k, es);
+        riccatiAngle(o, k, drot, es, true);
+        // ComputeLDOS::computeGreensAngle / integrateGreensAngle (:126-151):
+        // Greens per energy, extended at the boundary, times the integrand
+        // factor, imaginary part rotated back additively.
+        const signed char *lab = o.frames[size_t(k)].label.data();
+        const double wk = angleWeight(k, o.Na) / static_cast<double>(o.Na);
+        const float phi = static_cast<float>(o.systemAngle[size_t(k)]);
+        for (int e = 0; e < o.Ne; ++e) {
+#pragma omp parallel for schedule(static)
+          for (long long i = 0; i < (long long)n1; ++i) {
+            const size_t gi = size_t(e) * 2 * n1 + size_t(i);
+            const cplx ga(o.gamma[gi], o.gamma[gi + n1]);
+            const cplx gt(o.gammat[gi], o.gammat[gi + n1]);
+            gim[size_t(i)] = greensG(ga, gt).imag();
+          }
+          extendValues(lab, o.N1, gim.data());
+          for (double &v : gim) v *= wk;
+          insertRotated(o.kel.interval.data() + size_t(e) * n0, o.N0, o.N0, gim.data(), o.N1,
+                        o.N1, phi, true);
+        }
+      }
+      boundaryCondition(o);
+      // ComputeLDOS::computePropertyFromGreens (:153-192)
+      for (double &v : o.kel.interval) v *= -1.0;
+      for (int e = 0; e < o.Ne; ++e) zeroOutside(o, o.kel.interval.data() + size_t(e) * n0);
+      double *slot = o.kel.all.data() + size_t(o.kel.blockStart) * n0;
+      const size_t len = size_t(o.Ne) * n0;
+      o.kel.residual = residualArrays(slot, o.kel.interval.data(), len, o.cfg.norm);
+      std::copy(o.kel.interval.begin(), o.kel.interval.begin() + std::ptrdiff_t(len), slot);
+      o.kel.partial = o.kel.residual < o.cfg.convergence_criterion;
+      const bool boundaryConverged = o.resBoundary < o.cfg.convergence_criterion;
+      if (o.kel.partial && boundaryConverged) {
+        o.kel.blockStart += o.kel.perBlock;
+        if (keldyshDone(o)) o.kel.converged = true;
+      }
+    }
+  }
+  o.numIterations++;
+}
+// Context::getResidual / isPartiallyConverged / isConverged (Context.h:462-511)
+static double keldyshResidual(const Oracle &o) { return std::max(o.resBoundary, o.kel.residual); }
+static bool keldyshPartiallyConverged(const Oracle &o) {
+  return keldyshResidual(o) < o.cfg.convergence_criterion && o.kel.partial;
+}
+static bool keldyshConverged(const Oracle &o) {
+  return keldyshResidual(o) < o.cfg.convergence_criterion && o.kel.converged;
+}
+// run_iteration.h:85-141
+static bool keldyshIterations(Oracle &o, bool compute, int mn, int mx) {
+  if (mn == 0 && mx == 0) return false;
+  bool conv = false;
+  for (int i = 1;; ++i) {
+    if (compute) {
+      keldyshCompute(o, false);
+      conv = keldyshPartiallyConverged(o);
+    } else {
+      keldyshCompute(o, true);
+      conv = o.resBoundary < o.cfg.convergence_criterion;
+    }
+    const bool minReached = i >= mn;
+    const bool maxReached = mx < 0 ? false : i >= mx;
+    if ((conv && minReached) || maxReached) break;
+  }
+  return conv;
+}
+// runPostprocess (run_simulation.h:44-108)
+static bool keldyshRun(Oracle &o, int nb, int mn, int mx) {
+  bool converged = false;
+  while (!converged) {
+    keldyshIterations(o, false, nb, nb);
+    const bool blockConverged = keldyshIterations(o, true, mn, mx);
+    converged = keldyshConverged(o);
+    if (!blockConverged) break;
+  }
+  return converged;
+}
+
+// ---------------------------------------------------------------------------
 // Construction: simulation.cu:64-463 + Context::initialize (Context.h:285-375)
 // ---------------------------------------------------------------------------
 static Oracle *build(const oc_config &cfg) {
@@ -2085,6 +2228,7 @@ static Oracle *build(const oc_config &cfg) {
   std::vector<double> poles, res;
   ozakiPoles(o->Ne, poles, res);
   o->energyIm.resize(o->Ne);
+  o->energyRe.assign(o->Ne, 0.0);
   o->residue = res;
   for (int i = 0; i < o->Ne; ++i)
     o->energyIm[i] = poles[i] * o->T / static_cast<double>(2.0 * M_PI);
@@ -2169,6 +2313,9 @@ long long oc_get_int(void *hh, const char *key) {
   if (k == "num_storage_out_gammatilde") return o.nOutGt;
   if (k == "num_iterations") return o.numIterations;
   if (k == "converged") return contextConverged(o) ? 1 : 0;
+  if (k == "ldos_block_start") return o.kel.blockStart;
+  if (k == "ldos_converged") return keldyshConverged(o) ? 1 : 0;
+  if (k == "ldos_partially_converged") return keldyshPartiallyConverged(o) ? 1 : 0;
   if (k == "traj_steps") {
     long long s = 0;
     for (int a = 0; a < o.Na; ++a)
@@ -2186,7 +2333,8 @@ long long oc_get_int(void *hh, const char *key) {
 double oc_get_double(void *hh, const char *key) {
   const Oracle &o = *static_cast<Oracle *>(hh);
   const std::string k(key);
-  if (k == "residual") return contextResidual(o);
+  if (k == "residual") return o.kel.on ? keldyshResidual(o) : contextResidual(o);
+  if (k == "residual_ldos") return o.kel.residual;
   if (k == "residual_boundary") return o.resBoundary;
   if (k == "residual_order_parameter") return o.resOP;
   if (k == "residual_current") return o.resJ;
@@ -2234,6 +2382,8 @@ long long oc_get_array(void *hh, const char *name, double *out,
     src = &o.Anew;
   } else if (k == "free_energy_grid") {
     src = &o.feGrid;
+  } else if (k == "ldos") {
+    src = &o.kel.all;
   } else if (k == "domain") {
     tmp.assign(o.domainRef.begin(), o.domainRef.end());
     src = &tmp;
@@ -2350,6 +2500,46 @@ int oc_run(void *hh, int nb, int mn, int mx) {
   }
   return conv ? 1 : 0;
 }
+// Turns the context into the postprocess context of src/postprocess.cu:
+// energies min..max (num) with broadening, blocks of per_block energies
+// (< 1: all).  The order parameter and vector potential are the current ones.
+int oc_ldos_setup(void *hh, double emin, double emax, int num, double broadening,
+                  int perBlock) {
+  Oracle &o = *static_cast<Oracle *>(hh);
+  if (num < 1) {
+    g_error = "num_energies must be positive";
+    return -1;
+  }
+  o.kel = Oracle::Spectroscopy{};
+  o.kel.on = true;
+  o.kel.eMin = emin;
+  o.kel.eMax = emax;
+  o.kel.numEnergies = num;
+  o.kel.broadening = broadening;
+  o.kel.perBlock = perBlock < 1 ? num : std::min(perBlock, num);
+  o.kel.all.assign(size_t(num) * o.n0sq(), 0.0);
+  o.resBoundary = std::numeric_limits<double>::max();
+  o.numIterations = 0;
+  return 0;
+}
+int oc_ldos_compute(void *hh, int burnin) {
+  Oracle &o = *static_cast<Oracle *>(hh);
+  if (!o.kel.on) {
+    g_error = "oc_ldos_setup was not called";
+    return -1;
+  }
+  keldyshCompute(o, burnin != 0);
+  return 0;
+}
+int oc_ldos_run(void *hh, int nb, int mn, int mx) {
+  Oracle &o = *static_cast<Oracle *>(hh);
+  if (!o.kel.on) {
+    g_error = "oc_ldos_setup was not called";
+    return -1;
+  }
+  return keldyshRun(o, nb, mn, mx) ? 1 : 0;
+}
+
 long long oc_sweep_angles(void *hh, int a0, int a1) {
   Oracle &o = *static_cast<Oracle *>(hh);
   if (!o.boundaryInit) setBoundaryInitialCondition(o);

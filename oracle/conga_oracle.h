@@ -135,6 +135,18 @@ int oc_burnin(void *handle);  /* Context::burnIn */
 /* runSimulation(burnin, min, max): returns 1 if converged. */
 int oc_run(void *handle, int num_burnin, int min_iter, int max_iter);
 
+/* Keldysh spectroscopy, the postprocess context of src/postprocess.cu:
+ * energies emin..emax (num of them) with imaginary part `broadening`,
+ * iterated in blocks of per_block energies (< 1: all at once) by
+ * IntegrationIteratorKeldysh (IntegrationIteratorKeldysh.h:115-392) with
+ * ComputeLDOS (ComputeLDOS.h).  Uses the current order parameter and
+ * vector potential.  The LDOS is the array "ldos" (num x N0^2). */
+int oc_ldos_setup(void *handle, double emin, double emax, int num, double broadening,
+                  int per_block);
+int oc_ldos_compute(void *handle, int burnin); /* Context::compute / burnIn */
+/* runPostprocess (run_simulation.h:44-108): returns 1 if every block converged. */
+int oc_ldos_run(void *handle, int num_burnin, int min_iter, int max_iter);
+
 /* Timing helper for the CPU baseline: the per-angle work of one iteration
  * (rotation, energy shift, Riccati sweeps in both directions, Green
  * functions and their integration into the angle sums) for angles

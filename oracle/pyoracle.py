@@ -104,6 +104,10 @@ def lib():
         L.oc_compute.argtypes = [vp]
         L.oc_burnin.argtypes = [vp]
         L.oc_run.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.oc_ldos_setup.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                    ctypes.c_double, ctypes.c_int]
+        L.oc_ldos_compute.argtypes = [vp, ctypes.c_int]
+        L.oc_ldos_run.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.oc_sweep_angles.restype = ctypes.c_longlong
         L.oc_sweep_angles.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.oc_riccati_step.argtypes = [dp, dp, dp, ctypes.c_double, dp]
@@ -273,7 +277,7 @@ class Oracle:
         gamma/gamma-tilde (same convention as the product's C ABI)."""
         io = ("in", "out")[in_out]
         base = ("gamma", "gammatilde")[direction]
-        cnt = self.get_int("num_storage_%s_%s" % (io, base)) * self.Ne
+        cnt = self.get_int("num_storage_%s_%s" % (io, base)) * self.get_int("num_energies")
         a = self.array("boundary_%s_%s" % (io, base), 2 * cnt)
         return a[:cnt] + 1j * a[cnt:]
 
@@ -314,6 +318,37 @@ class Oracle:
 
     def sweep_angles(self, a0, a1):
         return lib().oc_sweep_angles(self.h, a0, a1)
+
+    # Keldysh spectroscopy (src/postprocess.cu)
+    def ldos_setup(self, spectroscopy, per_block=-1):
+        sp = spectroscopy
+        self.num_ldos_energies = sp["num_energies"]
+        if lib().oc_ldos_setup(self.h, sp["energy_min"], sp["energy_max"], sp["num_energies"],
+                               sp["energy_broadening"], per_block) != 0:
+            raise RuntimeError(lib().oc_last_error().decode())
+
+    def ldos_compute(self, burnin=False):
+        lib().oc_ldos_compute(self.h, 1 if burnin else 0)
+
+    def ldos_run(self, burnin, nmin, nmax):
+        return bool(lib().oc_ldos_run(self.h, burnin, nmin, nmax))
+
+    def ldos(self):
+        n = self.num_ldos_energies
+        return self.array("ldos", n * self.N0 ** 2).reshape(n, self.N0, self.N0)
+
+
+def postprocess_config(simulation, postprocess):
+    """The oracle config of the postprocess context (src/postprocess.cu:
+    118-330): physics and geometry of the simulation, angles, criterion and
+    norm of the postprocess."""
+    import copy
+    cfg = copy.deepcopy(simulation)
+    nu = postprocess["numerics"]
+    cfg["numerics"]["num_fermi_momenta"] = nu["num_fermi_momenta"]
+    cfg["numerics"]["convergence_criterion"] = nu["convergence_criterion"]
+    cfg["numerics"]["norm"] = nu.get("norm", "l2")
+    return cfg
 
 
 def riccati_step(delta, energy, init, step):

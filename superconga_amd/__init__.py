@@ -63,6 +63,15 @@ class NewtonReport(ctypes.Structure):
                 ("history_capacity", ctypes.c_int), ("history_length", ctypes.c_int)]
 
 
+class LdosStatus(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in (
+        "num_energies", "energies_per_block", "block_start", "block_count", "iterations",
+        "angle_chunk", "partially_converged", "converged")] + [
+        (n, ctypes.c_double) for n in (
+            "residual", "residual_boundary", "residual_ldos", "last_sweep_ms",
+            "last_iteration_ms")] + [("traj_steps", ctypes.c_longlong)]
+
+
 _lib = None
 _DP = ctypes.POINTER(ctypes.c_double)
 
@@ -127,6 +136,15 @@ def lib():
         L.sc_batch_timing.argtypes = [vp, _DP, _DP, ctypes.POINTER(ctypes.c_longlong)]
         L.sc_describe.argtypes = [ctypes.c_char_p, ctypes.c_int,
                                   ctypes.POINTER(Dims), ctypes.POINTER(ctypes.c_byte)]
+        L.sc_newton_krylov.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                       ctypes.POINTER(NewtonReport)]
+        L.sc_ldos_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(vp)]
+        L.sc_ldos_compute.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.sc_ldos_run.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_int)]
+        L.sc_ldos_get_status.argtypes = [vp, ctypes.POINTER(LdosStatus)]
+        L.sc_ldos_get.argtypes = [vp, _DP]
+        L.sc_ldos_set_angle_chunk.argtypes = [vp, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -309,6 +327,62 @@ class Simulation:
         """Picks the fastest sweep variant for this grain and precision."""
         _check(lib().sc_autotune_sweep(self._h, _precision(precision)))
         return self.sweep_shape(precision)
+
+
+class Spectroscopy(Simulation):
+    """The postprocess context of src/postprocess.cu: the LDOS at real
+    energies E + i delta of a given order parameter and vector potential
+    (set them with set_order_parameter / set_vector_potential)."""
+
+    def __init__(self, simulation_config, postprocess_config):
+        sim = simulation_config if isinstance(simulation_config, str) else json.dumps(
+            simulation_config)
+        post = postprocess_config if isinstance(postprocess_config, str) else json.dumps(
+            postprocess_config)
+        h = ctypes.c_void_p()
+        _check(lib().sc_ldos_create(sim.encode(), post.encode(), ctypes.byref(h)))
+        self.postprocess = json.loads(post)
+        super().__init__(_borrow=(h, None))
+        self._borrowed = False  # owned: destroyed on close()
+
+    def compute(self, precision="fp64", burnin=False):
+        _check(lib().sc_ldos_compute(self._h, _precision(precision), int(bool(burnin))))
+
+    def burnin(self, precision="fp64"):
+        self.compute(precision, burnin=True)
+
+    def run(self, burnin=None, nmin=None, nmax=None, precision="fp64"):
+        """runPostprocess; limits default to the postprocess config's."""
+        nu = self.postprocess.get("numerics", {})
+        burnin = nu.get("num_iterations_burnin", -1) if burnin is None else burnin
+        nmin = nu.get("num_iterations_min", 0) if nmin is None else nmin
+        nmax = nu.get("num_iterations_max", -1) if nmax is None else nmax
+        c = ctypes.c_int()
+        _check(lib().sc_ldos_run(self._h, burnin, nmin, nmax, _precision(precision),
+                                 ctypes.byref(c)))
+        return bool(c.value)
+
+    def status(self):
+        st = LdosStatus()
+        _check(lib().sc_ldos_get_status(self._h, ctypes.byref(st)))
+        return {n: getattr(st, n) for n, _ in LdosStatus._fields_}
+
+    def ldos(self):
+        n = self.status()["num_energies"]
+        out = np.empty((n, self.N0, self.N0))
+        _check(lib().sc_ldos_get(self._h, _dp(out)))
+        return out
+
+    def set_angle_chunk(self, angles):
+        _check(lib().sc_ldos_set_angle_chunk(self._h, angles))
+
+    def energies_ldos(self):
+        """The real energies of the whole grid (IntegrationIteratorKeldysh
+        initEnergies)."""
+        sp = self.postprocess["spectroscopy"]
+        n = sp["num_energies"]
+        step = (sp["energy_max"] - sp["energy_min"]) / (n - 1) if n > 1 else 0.0
+        return sp["energy_min"] + step * np.arange(n)
 
 
 class Batch:

@@ -48,6 +48,13 @@ template <class F> int guarded(F &&f) {
   if (!(ctx) || !(ctx)->grain) return fail(SC_ERR_ARGUMENT, "null context")
 #define SC_CHECK_PTR(p)                                                        \
   if (!(p)) return fail(SC_ERR_ARGUMENT, "null pointer argument: " #p)
+#define SC_CHECK_SIM(ctx)                                                      \
+  if ((ctx)->grain->keldysh())                                                 \
+  return fail(SC_ERR_STATE, "a spectroscopy context only runs sc_ldos_*")
+#define SC_CHECK_LDOS(ctx)                                                     \
+  if (!(ctx)->grain->keldysh())                                                \
+  return fail(SC_ERR_STATE, "not a spectroscopy context (use sc_ldos_create)")
+
 
 void checkPrecision(int p) {
   if (p != SC_PRECISION_FP64 && p != SC_PRECISION_MIXED)
@@ -97,6 +104,88 @@ int sc_create(const char *config_json, sc_context **out) {
     ctx->grain = ctx->owned.get();
     ctx->grain->initialGuess();
     *out = ctx.release();
+    return SC_OK;
+  });
+}
+
+int sc_ldos_create(const char *simulation_json, const char *postprocess_json, sc_context **out) {
+  SC_CHECK_PTR(simulation_json);
+  SC_CHECK_PTR(postprocess_json);
+  SC_CHECK_PTR(out);
+  *out = nullptr;
+  return guarded([&] {
+    scamd::Config cfg = scamd::parseConfig(simulation_json);
+    const scamd::Spectroscopy sp = scamd::parseSpectroscopy(postprocess_json);
+    auto setup = scamd::buildSpectroscopySetup(cfg, sp);
+    auto ctx = std::make_unique<sc_context>();
+    ctx->owned = std::make_unique<scamd::Grain>(std::move(setup));
+    ctx->grain = ctx->owned.get();
+    *out = ctx.release();
+    return SC_OK;
+  });
+}
+
+int sc_ldos_compute(sc_context *ctx, int precision, int burnin) {
+  SC_CHECK_CTX(ctx);
+  SC_CHECK_LDOS(ctx);
+  return guarded([&] {
+    checkPrecision(precision);
+    ctx->grain->ldosCompute(precision, burnin != 0);
+    return SC_OK;
+  });
+}
+
+int sc_ldos_run(sc_context *ctx, int nb, int nmin, int nmax, int precision, int *converged) {
+  SC_CHECK_CTX(ctx);
+  SC_CHECK_LDOS(ctx);
+  return guarded([&] {
+    checkPrecision(precision);
+    const bool ok = ctx->grain->ldosRun(nb, nmin, nmax, precision);
+    if (converged) *converged = ok ? 1 : 0;
+    return SC_OK;
+  });
+}
+
+int sc_ldos_get_status(sc_context *ctx, sc_ldos_status *out) {
+  SC_CHECK_CTX(ctx);
+  SC_CHECK_LDOS(ctx);
+  SC_CHECK_PTR(out);
+  return guarded([&] {
+    const scamd::LdosStatus st = ctx->grain->ldosStatus();
+    out->num_energies = st.numEnergies;
+    out->energies_per_block = st.perBlock;
+    out->block_start = st.blockStart;
+    out->block_count = st.blockCount;
+    out->iterations = st.iterations;
+    out->angle_chunk = st.angleChunk;
+    out->partially_converged = st.partiallyConverged ? 1 : 0;
+    out->converged = st.converged ? 1 : 0;
+    out->residual = st.residual;
+    out->residual_boundary = st.residualBoundary;
+    out->residual_ldos = st.residualLdos;
+    out->last_sweep_ms = st.sweepMs;
+    out->last_iteration_ms = st.iterationMs;
+    out->traj_steps = st.trajSteps;
+    return SC_OK;
+  });
+}
+
+int sc_ldos_get(sc_context *ctx, double *out) {
+  SC_CHECK_CTX(ctx);
+  SC_CHECK_LDOS(ctx);
+  SC_CHECK_PTR(out);
+  return guarded([&] {
+    ctx->grain->getLdos(out);
+    return SC_OK;
+  });
+}
+
+int sc_ldos_set_angle_chunk(sc_context *ctx, int angles) {
+  SC_CHECK_CTX(ctx);
+  SC_CHECK_LDOS(ctx);
+  return guarded([&] {
+    if (angles < 0) throw std::invalid_argument("angles must be >= 0");
+    ctx->grain->setLdosAngleChunk(angles);
     return SC_OK;
   });
 }
@@ -220,6 +309,7 @@ int sc_set_boundary(sc_context *ctx, int dir, const double *re, const double *im
 
 int sc_compute(sc_context *ctx, int precision) {
   SC_CHECK_CTX(ctx);
+  SC_CHECK_SIM(ctx);
   return guarded([&] {
     checkPrecision(precision);
     ctx->grain->compute(precision);
@@ -228,6 +318,7 @@ int sc_compute(sc_context *ctx, int precision) {
 }
 int sc_burnin(sc_context *ctx, int precision) {
   SC_CHECK_CTX(ctx);
+  SC_CHECK_SIM(ctx);
   return guarded([&] {
     checkPrecision(precision);
     ctx->grain->burnin(precision);
@@ -236,6 +327,7 @@ int sc_burnin(sc_context *ctx, int precision) {
 }
 int sc_run(sc_context *ctx, int nb, int nmin, int nmax, int precision, int *converged) {
   SC_CHECK_CTX(ctx);
+  SC_CHECK_SIM(ctx);
   return guarded([&] {
     if (precision != SC_PRECISION_MIXED_REFINE) checkPrecision(precision);
     const scamd::Config &c = ctx->grain->setup().cfg;
@@ -270,6 +362,7 @@ int sc_get_results(sc_context *ctx, sc_results *out) {
 
 int sc_riccati_sweep(sc_context *ctx, int precision, int burnin) {
   SC_CHECK_CTX(ctx);
+  SC_CHECK_SIM(ctx);
   return guarded([&] {
     checkPrecision(precision);
     ctx->grain->sweep(precision, burnin != 0);
@@ -280,6 +373,7 @@ int sc_riccati_sweep(sc_context *ctx, int precision, int burnin) {
 }
 int sc_boundary_condition(sc_context *ctx, double *residual) {
   SC_CHECK_CTX(ctx);
+  SC_CHECK_SIM(ctx);
   return guarded([&] {
     ctx->grain->boundaryCondition();
     if (residual) *residual = ctx->grain->results().resBoundary;
@@ -318,6 +412,7 @@ int sc_autotune_sweep(sc_context *ctx, int precision) {
 int sc_newton_krylov(sc_context *ctx, int maxNewton, int krylovDim, double tol,
                      sc_newton_report *report) {
   SC_CHECK_CTX(ctx);
+  SC_CHECK_SIM(ctx);
   return guarded([&] {
     if (maxNewton < 0) throw std::invalid_argument("max_newton must be >= 0");
     if (krylovDim < 1 || krylovDim > 31)

@@ -131,6 +131,20 @@ __device__ __forceinline__ Cx<T> stepEnergy(const StepSite<T> &p, T ser, Cx<T> y
   const Cx<T> sq = csqrt(disc);
   return cdiv(c + c, sq - b);
 }
+// Real-frequency energies eps = E + i delta: sej = h E joins the site's
+// Im(s e) (the Doppler shift enters the same way).
+template <typename T>
+__device__ __forceinline__ Cx<T> stepEnergy(const StepSite<T> &p, T ser, T sej, Cx<T> y0) {
+  const T sei = p.sei + sej;
+  const Cx<T> ay = p.a * y0;
+  const Cx<T> b = {T(2) * ay.re + (ser - T(1)), T(2) * ay.im + sei};
+  const Cx<T> t = {ay.re + (ser + T(1)), ay.im + sei};
+  const Cx<T> c = t * y0 + p.sD;
+  const Cx<T> ac4 = p.a4 * c;
+  const Cx<T> disc = {b.re * b.re - b.im * b.im - ac4.re, T(2) * b.re * b.im - ac4.im};
+  const Cx<T> sq = csqrt(disc);
+  return cdiv(c + c, sq - b);
+}
 template <typename T>
 __device__ __forceinline__ Cx<T> riccatiStep(Cx<T> D, Cx<T> e, Cx<T> y0, T h) {
   return stepEnergy(stepSite(D, e.re, h), -h * e.im, y0);

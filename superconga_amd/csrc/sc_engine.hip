@@ -44,6 +44,12 @@ Grain::~Grain() {
   devFree(dAccVel_);
   devFree(dEnergy_);
   devFree(dResidue_);
+  devFree(dEnergyRe_);
+  devFree(dLdosAll_);
+  devFree(dLdosAcc_);
+  devFree(dLdosInterval_);
+  devFree(dLdosG_);
+  for (hipEvent_t e : ldosEvents_) (void)hipEventDestroy(e);
   devFree(dSaCos_);
   devFree(dSaSin_);
   devFree(dVFx_);
@@ -106,8 +112,15 @@ Grain::~Grain() {
 void Grain::upload() {
   const Setup &s = *s_;
   const size_t n0 = size_t(s.N0) * s.N0, n1 = size_t(s.N1) * s.N1;
-  dEnergy_ = devUpload(s.energy);
-  dResidue_ = devUpload(s.residue);
+  {
+    // Keldysh grains change their energies block by block: room for the
+    // largest block
+    const size_t cap = size_t(s.keldysh ? s.spec.perBlock : s.Ne);
+    dEnergy_ = devAlloc<double>(cap);
+    dResidue_ = devAlloc<double>(cap);
+    if (s.keldysh) dEnergyRe_ = devAlloc<double>(cap);
+    uploadEnergies();
+  }
   dSaCos_ = devUpload(s.saCos);
   dSaSin_ = devUpload(s.saSin);
   dVFx_ = devUpload(s.vFx);
@@ -169,11 +182,26 @@ void Grain::upload() {
   dCurrent_ = devAlloc<double>(2 * n0);
   dCurrentOld_ = devAlloc<double>(2 * n0);
   dEnergyDens_ = devAlloc<double>(n0);
-  dBInG_ = devAlloc<double>(size_t(2 * s.nInG * s.Ne));
-  dBOutG_ = devAlloc<double>(size_t(2 * s.nOutG * s.Ne));
-  dBInGt_ = devAlloc<double>(size_t(2 * s.nInGt * s.Ne));
-  dBOutGt_ = devAlloc<double>(size_t(2 * s.nOutGt * s.Ne));
-  dG_ = devAlloc<double>(size_t(s.Na) * 4 * n1);
+  const long long neCap = s.keldysh ? s.spec.perBlock : s.Ne;
+  dBInG_ = devAlloc<double>(size_t(2 * s.nInG * neCap));
+  dBOutG_ = devAlloc<double>(size_t(2 * s.nOutG * neCap));
+  dBInGt_ = devAlloc<double>(size_t(2 * s.nInGt * neCap));
+  dBOutGt_ = devAlloc<double>(size_t(2 * s.nOutGt * neCap));
+  // Keldysh grains hold spectral weights of an angle chunk instead
+  if (!s.keldysh) dG_ = devAlloc<double>(size_t(s.Na) * 4 * n1);
+  {
+    // first extension entry of every frame (the list is built frame by frame)
+    extRotFrameStart_.assign(size_t(s.Na) + 1, 0);
+    const long long nt = static_cast<long long>(s.extRot.target.size());
+    long long t = 0;
+    for (int k = 0; k < s.Na; ++k) {
+      extRotFrameStart_[size_t(k)] = static_cast<int>(t);
+      while (t < nt && static_cast<long long>(s.extRot.target[size_t(t)]) / static_cast<long long>(n1) == k)
+        ++t;
+    }
+    extRotFrameStart_[size_t(s.Na)] = static_cast<int>(nt);
+    if (t != nt) throw std::runtime_error("extension list is not ordered by frame");
+  }
   redCap_ = size_t(64) * 1024 * 6;
   dRed_ = devAlloc<double>(redCap_);
   dMean_ = devAlloc<double>(2);
@@ -185,14 +213,28 @@ void Grain::upload() {
   dCounter_ = devAlloc<int>(1);
 }
 
-// Compiled sweep variants: (energies per lane, wavefronts per SIMD).
+void Grain::uploadEnergies() {
+  const Setup &s = *s_;
+  const size_t b = size_t(s.Ne) * sizeof(double);
+  SC_HIP(hipMemcpy(dEnergy_, s.energy.data(), b, hipMemcpyHostToDevice));
+  SC_HIP(hipMemcpy(dResidue_, s.residue.data(), b, hipMemcpyHostToDevice));
+  if (dEnergyRe_) SC_HIP(hipMemcpy(dEnergyRe_, s.energyRe.data(), b, hipMemcpyHostToDevice));
+}
+
+// Compiled sweep variants: (energies per lane, wavefronts per SIMD), for
+// Matsubara energies and for real-frequency (Keldysh) energies.
 #define SC_SWEEP_VARIANTS(X)                                                         \
   X(4, 2) X(4, 4) X(7, 1) X(7, 2) X(7, 4) X(8, 1) X(8, 2) X(8, 4) X(14, 1) X(14, 2)   \
   X(16, 1) X(16, 2)
+#define SC_KELDYSH_VARIANTS(X) X(4, 2) X(8, 1) X(8, 2) X(16, 1)
 
-static bool sweepVariantExists(int el, int waves) {
+bool sweepVariantExists(int el, int waves, bool keldysh) {
 #define SC_HAS(E, W) if (el == E && waves == W) return true;
-  SC_SWEEP_VARIANTS(SC_HAS)
+  if (keldysh) {
+    SC_KELDYSH_VARIANTS(SC_HAS)
+  } else {
+    SC_SWEEP_VARIANTS(SC_HAS)
+  }
 #undef SC_HAS
   return false;
 }
@@ -200,7 +242,7 @@ static bool sweepVariantExists(int el, int waves) {
 void Grain::setSweepShape(int precision, int el, int waves, int slotsPerCU) {
   if (precision != PREC_FP64 && precision != PREC_MIXED)
     throw std::runtime_error("unknown precision");
-  if (!sweepVariantExists(el, waves))
+  if (!sweepVariantExists(el, waves, s_->keldysh))
     throw std::runtime_error("no sweep variant with " + std::to_string(el) +
                              " energies per lane and " + std::to_string(waves) +
                              " wavefronts per SIMD");
@@ -231,15 +273,43 @@ void Grain::prepareSweep(int precision) {
   pl.cols = gv.cols;
   std::vector<SweepItem> items = sweepItems(s, pl.cols, &pl.maxLen);
   pl.numItems = static_cast<int>(items.size());
+  // Keldysh grains sweep by angle chunks: the items of a chunk are
+  // contiguous (still longest first within it) and name the view whose
+  // angle base is the chunk's first angle.
+  std::vector<GrainView> views(1, gv);
+  pl.chunkStart.assign(2, 0);
+  pl.chunkStart[1] = pl.numItems;
+  pl.chunkAngles = s.Na;
+  if (s.keldysh) {
+    int ca = ldosChunkAngles();
+    if (dLdosG_) {
+      // never more angles than the weight buffer holds
+      const size_t perAngle = size_t(s.N1) * s.N1 * size_t(s.spec.perBlock);
+      ca = std::max(1, std::min(ca, static_cast<int>(ldosGCap_ / perAngle)));
+    }
+    const int nch = (s.Na + ca - 1) / ca;
+    pl.chunkAngles = ca;
+    std::stable_sort(items.begin(), items.end(), [ca](const SweepItem &a, const SweepItem &b) {
+      return a.angle / ca < b.angle / ca;
+    });
+    pl.chunkStart.assign(size_t(nch) + 1, 0);
+    for (const SweepItem &it : items) ++pl.chunkStart[size_t(it.angle / ca) + 1];
+    for (int c = 0; c < nch; ++c) pl.chunkStart[size_t(c) + 1] += pl.chunkStart[size_t(c)];
+    views.assign(size_t(nch), gv);
+    for (int c = 0; c < nch; ++c) views[size_t(c)].angleBase = c * ca;
+  }
   std::vector<SweepItemDev> di(items.size());
   for (size_t i = 0; i < items.size(); ++i)
-    di[i] = {0, items[i].angle, items[i].x0, items[i].ncols, items[i].ylo, items[i].yhi};
+    di[i] = {s.keldysh ? items[i].angle / pl.chunkAngles : 0, items[i].angle, items[i].x0,
+             items[i].ncols, items[i].ylo, items[i].yhi};
   if (pl.dItems) (void)hipFree(pl.dItems);
   pl.dItems = nullptr;
   SC_HIP(hipMalloc(&pl.dItems, std::max<size_t>(1, di.size()) * sizeof(SweepItemDev)));
   if (!di.empty()) toDevice(pl.dItems, di.data(), di.size() * sizeof(SweepItemDev));
-  if (!pl.dView) SC_HIP(hipMalloc(&pl.dView, sizeof(GrainView)));
-  toDevice(pl.dView, &gv, sizeof(GrainView));
+  if (pl.dView) (void)hipFree(pl.dView);
+  pl.dView = nullptr;
+  SC_HIP(hipMalloc(&pl.dView, views.size() * sizeof(GrainView)));
+  toDevice(pl.dView, views.data(), views.size() * sizeof(GrainView));
   int dev = 0, cus = 0;
   SC_HIP(hipGetDevice(&dev));
   SC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -281,6 +351,8 @@ static GrainView makeView(const Setup &s, const float *trig, const int *idx,
   v.charge = s.cfg.chargeSign;
   v.energy = energy;
   v.residue = residue;
+  v.energyRe = nullptr;
+  v.angleBase = 0;
   const int Na = s.Na;
   v.opCos = trig;
   v.opSin = trig + Na;
@@ -589,15 +661,17 @@ void Grain::extendDelta() {
   if (nt > 0) {
     extendKernel<<<blocksFor(nt), 256, 0, stream_>>>(
         dExtDelta_, dExtRefT_, dExtRefO_, dExtRefS_, nt, 2 * s.Nc,
-        static_cast<long long>(n0), static_cast<long long>(n0), 0, 1);
+        static_cast<long long>(n0), static_cast<long long>(n0), 0, 1, 0);
     SC_HIP(hipGetLastError());
   }
 }
 
 GrainView Grain::view() const {
-  return makeView(*s_, dTrig_, dIdx_, dDist_, dBInG_, dBOutG_, dBInGt_, dBOutGt_,
-                  dExtDelta_, dA_, dG_, dEnergy_, dResidue_, dSaCos_, dSaSin_, dVFx_, dVFy_,
-                  dWeight_, dBasis_, dLabels_);
+  GrainView v = makeView(*s_, dTrig_, dIdx_, dDist_, dBInG_, dBOutG_, dBInGt_, dBOutGt_,
+                         dExtDelta_, dA_, s_->keldysh ? dLdosG_ : dG_, dEnergy_, dResidue_,
+                         dSaCos_, dSaSin_, dVFx_, dVFy_, dWeight_, dBasis_, dLabels_);
+  v.energyRe = dEnergyRe_;
+  return v;
 }
 
 GrainView Grain::sweepView(int el) const {
@@ -624,23 +698,30 @@ void Grain::ensureBoundaryInit() {
   boundaryInit_ = true;
 }
 
-template <typename T, int EL, int W>
+template <typename T, int EL, int W, bool KEL>
 static void launchSweep(const SweepArgs &a, int slots, hipStream_t st) {
-  sweepKernel<T, EL, W><<<slots, 64, 0, st>>>(a);
+  sweepKernel<T, EL, W, KEL><<<slots, 64, 0, st>>>(a);
 }
 
 void launchSweepVariant(int precision, int el, int waves, const SweepArgs &a, int slots,
-                        hipStream_t stream) {
+                        hipStream_t stream, bool keldysh) {
   const bool dp = precision == PREC_FP64;
 #define SC_LAUNCH(E, W)                                                              \
   if (el == E && waves == W) {                                                       \
-    if (dp) launchSweep<double, E, W>(a, slots, stream);                             \
-    else launchSweep<float, E, W>(a, slots, stream);                                 \
-  } else
-  SC_SWEEP_VARIANTS(SC_LAUNCH)
+    if (dp) launchSweep<double, E, W, KEL>(a, slots, stream);                        \
+    else launchSweep<float, E, W, KEL>(a, slots, stream);                            \
+    SC_HIP(hipGetLastError());                                                       \
+    return;                                                                          \
+  }
+  if (keldysh) {
+    constexpr bool KEL = true;
+    SC_KELDYSH_VARIANTS(SC_LAUNCH)
+  } else {
+    constexpr bool KEL = false;
+    SC_SWEEP_VARIANTS(SC_LAUNCH)
+  }
 #undef SC_LAUNCH
   throw std::runtime_error("sweep variant not compiled");
-  SC_HIP(hipGetLastError());
 }
 
 void Grain::sweep(int precision, bool burnin) {
@@ -656,13 +737,19 @@ void Grain::sweep(int precision, bool burnin) {
   a.burnin = burnin ? 1 : 0;
   resetCounter<<<1, 1, 0, stream_>>>(dCounter_);
   SC_HIP(hipEventRecord(ev0_, stream_));
-  launchSweepVariant(precision, pl.el, pl.waves, a, pl.numSlots, stream_);
+  launchSweepVariant(precision, pl.el, pl.waves, a, pl.numSlots, stream_, s_->keldysh);
   SC_HIP(hipEventRecord(ev1_, stream_));
 }
 
 void Grain::autotune(int precision) {
   if (precision != PREC_FP64 && precision != PREC_MIXED)
     throw std::runtime_error("unknown precision");
+  if (s_->keldysh) {
+    // the sweeps write spectral weights: the chunk buffer must exist
+    if (ldosLastBlock_ != ldosBlockStart_ && ldosBlockStart_ < s_->spec.numEnergies)
+      ldosBeginBlock();
+    ldosEnsureBuffers(precision);
+  }
   ensureBoundaryInit();
   extendDelta();
   SweepPlan &pl = plan_[precision];
@@ -691,7 +778,11 @@ void Grain::autotune(int precision) {
 #define SC_TRY(E, W)                                                                 \
   for (int slots : slotChoices)                                                       \
     if (slots >= 4 * W || slots == 16) tryShape(E, W, slots);
-  SC_SWEEP_VARIANTS(SC_TRY)
+  if (s_->keldysh) {
+    SC_KELDYSH_VARIANTS(SC_TRY)
+  } else {
+    SC_SWEEP_VARIANTS(SC_TRY)
+  }
 #undef SC_TRY
   setSweepShape(precision, bestEl, bestW, bestSlots);
   (void)keepEl;
@@ -734,6 +825,17 @@ void Grain::boundaryCondition() {
   res_.resBoundary = boundaryResidual(hb);
 }
 
+void Grain::extendFrames(double *base, int k0, int k1, int planes, long long planeStride,
+                         long long blockStride, long long siteScale) {
+  const size_t n1 = size_t(s_->N1) * s_->N1;
+  const int t0 = extRotFrameStart_[size_t(k0)], t1 = extRotFrameStart_[size_t(k1)];
+  if (t1 <= t0) return;
+  extendKernel<<<blocksFor(t1 - t0), 256, 0, stream_>>>(
+      base, dExtRotT_ + t0, dExtRotO_ + t0, dExtRotS_, t1 - t0, planes, planeStride,
+      static_cast<long long>(n1), blockStride, siteScale, k0);
+  SC_HIP(hipGetLastError());
+}
+
 void Grain::backRotate() {
   // Extension of the Green-function planes of every frame, then their
   // back-rotation, angle integration, gap equation and current
@@ -744,7 +846,7 @@ void Grain::backRotate() {
   if (nt > 0) {
     extendKernel<<<blocksFor(nt), 256, 0, stream_>>>(
         dG_, dExtRotT_, dExtRotO_, dExtRotS_, nt, 4, 1, static_cast<long long>(n1),
-        static_cast<long long>(4 * n1), 4);
+        static_cast<long long>(4 * n1), 4, 0);
     SC_HIP(hipGetLastError());
   }
   GrainView v = view();
@@ -1107,6 +1209,7 @@ void Grain::acceleratorStep() {
 
 void Grain::beginCompute() {
   // Context::compute (Context.h:391-447) up to the Riccati sweeps
+  if (s_->keldysh) throw std::runtime_error("a spectroscopy context only runs the LDOS iteration");
   SC_HIP(hipEventRecord(evIterStart_, stream_));
   clearReductions();
   if (!firstCompute_) acceleratorStep();
@@ -1155,6 +1258,7 @@ void Grain::compute(int precision) {
 }
 
 void Grain::beginBurnin() {
+  if (s_->keldysh) throw std::runtime_error("a spectroscopy context only runs the LDOS iteration");
   clearReductions();
   ensureBoundaryInit();
   extendDelta();

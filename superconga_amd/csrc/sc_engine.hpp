@@ -41,6 +41,16 @@ struct NewtonReport {
   std::vector<double> history;
 };
 
+// State of a Keldysh (LDOS) context.
+struct LdosStatus {
+  int numEnergies = 0, perBlock = 0, blockStart = 0, blockCount = 0;
+  int iterations = 0, angleChunk = 0;
+  bool partiallyConverged = false, converged = false;
+  double residual = 0, residualBoundary = 0, residualLdos = 0;
+  double sweepMs = 0, iterationMs = 0;
+  long long trajSteps = 0; // per iteration of the current block
+};
+
 class Grain {
 public:
   // With a stream the grain runs on it (and does not own it); without, it
@@ -99,6 +109,20 @@ public:
   // iterate and restarts the accelerators.
   NewtonReport newtonKrylov(int maxNewton, int krylovDim, double tol);
   void resetAccelerator();
+
+  // Keldysh spectroscopy (src/postprocess.cu), for a grain built by
+  // buildSpectroscopySetup: the LDOS at real energies of the current order
+  // parameter and vector potential.
+  bool keldysh() const { return s_->keldysh; }
+  // Context::compute / burnIn of the postprocess context
+  // (IntegrationIteratorKeldysh.h:235-388, ComputeLDOS.h:110-192).
+  void ldosCompute(int precision, bool burnin);
+  // runPostprocess (run_simulation.h:44-108); true when every block converged.
+  bool ldosRun(int burnin, int itMin, int itMax, int precision);
+  void getLdos(double *out); // [numEnergies][N0][N0]
+  LdosStatus ldosStatus() const;
+  // Angles whose spectral weights are held at once (0: from free memory).
+  void setLdosAngleChunk(int angles);
   // Boundary initial values (first call) and the extended order parameter:
   // the inputs of a sweep.
   void prepareSweepInputs() {
@@ -118,6 +142,15 @@ public:
 
 private:
   void upload();
+  void uploadEnergies();
+  // Keldysh helpers (sc_ldos.hip)
+  void ldosBeginBlock();
+  int ldosChunkAngles() const;
+  void ldosEnsureBuffers(int precision);
+  // extendValuesAtBoundary on `planes` values per site of frames [k0, k1)
+  // whose data start at base (frame k at base + (k - k0) * blockStride).
+  void extendFrames(double *base, int k0, int k1, int planes, long long planeStride,
+                    long long blockStride, long long siteScale);
   void toDevice(void *dst, const void *src, size_t bytes);
   void fromDevice(void *dst, const void *src, size_t bytes);
   void extendDelta();
@@ -173,6 +206,17 @@ private:
   double *dG_ = nullptr;
   double *dPartials_ = nullptr, *dWork_ = nullptr, *dWork2_ = nullptr;
   std::unique_ptr<PoissonFFT> fft_;
+  // Keldysh state
+  double *dEnergyRe_ = nullptr;
+  double *dLdosAll_ = nullptr, *dLdosAcc_ = nullptr, *dLdosInterval_ = nullptr;
+  double *dLdosG_ = nullptr;
+  size_t ldosGCap_ = 0;
+  int ldosChunkReq_ = 0;
+  int ldosBlockStart_ = 0, ldosLastBlock_ = -1, ldosIterations_ = 0;
+  double ldosRes_ = 0, ldosSweepMs_ = 0, ldosIterMs_ = 0;
+  bool ldosPartial_ = false, ldosConverged_ = false;
+  std::vector<int> extRotFrameStart_; // first extRot entry of each frame (+ end)
+  std::vector<hipEvent_t> ldosEvents_;
   // Newton-Krylov state
   struct NkBlock {
     size_t offset, length;
@@ -207,6 +251,10 @@ private:
   struct SweepPlan {
     int el = 8, waves = 2, slotsPerCU = 8;
     int cols = 0, groups = 0, maxLen = 0, numItems = 0, numSlots = 0;
+    // Keldysh: items grouped by angle chunk (chunkAngles frames each),
+    // chunk c's items at [chunkStart[c], chunkStart[c + 1])
+    int chunkAngles = 0;
+    std::vector<int> chunkStart;
     void *dItems = nullptr;
     void *dView = nullptr; // device copy of the grain view with this lane layout
     bool ready = false;
@@ -292,6 +340,7 @@ private:
 // One launch of the compiled sweep variant (precision, energies per lane,
 // wavefronts per SIMD) over the items of `args`.
 void launchSweepVariant(int precision, int el, int waves, const SweepArgs &args, int slots,
-                        hipStream_t stream);
+                        hipStream_t stream, bool keldysh = false);
+bool sweepVariantExists(int el, int waves, bool keldysh = false);
 
 } // namespace scamd

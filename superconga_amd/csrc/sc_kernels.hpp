@@ -60,6 +60,14 @@ __device__ __forceinline__ void siteFields(const GrainView &v, int k, int x,
 
 __global__ void resetCounter(int *c) { *c = 0; }
 
+// One step for one energy: Matsubara energies (i omega) or, with KEL,
+// real-frequency energies E + i delta.
+template <bool KEL, typename T>
+__device__ __forceinline__ Cx<T> stepE(const StepSite<T> &p, T ser, T sej, Cx<T> y0) {
+  if constexpr (KEL) return stepEnergy(p, ser, sej, y0);
+  else return stepEnergy(p, ser, y0);
+}
+
 // The Riccati sweep.  One wavefront handles an item: `cols` adjacent columns
 // of one frame; each column is carried by 64 / cols lanes, of which `groups`
 // hold EL Ozaki energies each.  Pass 1 integrates gamma~ from the top of the
@@ -71,7 +79,7 @@ __global__ void resetCounter(int *c) { *c = 0; }
 // The order parameter and Doppler shift of a row are computed once per
 // column: the lanes of a column take consecutive rows of a chunk and pass
 // the values round with __shfl.
-template <typename T, int EL>
+template <typename T, int EL, bool KEL>
 __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev &it,
                                           Cx<T> *scr) {
   const GrainView &v = a.views[it.grain];
@@ -88,13 +96,14 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev
   const int x = it.x0 + (colActive ? c : it.ncols - 1);
   const bool active = colActive && (g < v.groups);
   const int8_t *lab = v.labels + size_t(k) * n1 + x;
-  T ser[EL], res[EL];
+  T ser[EL], sej[EL], res[EL];
   bool eOk[EL];
 #pragma unroll
   for (int j = 0; j < EL; ++j) {
     const int e = g * EL + j;
     eOk[j] = active && e < Ne;
     ser[j] = eOk[j] ? -h * T(v.energy[e]) : T(0);
+    sej[j] = (KEL && eOk[j]) ? h * T(v.energyRe[e]) : T(0);
     res[j] = eOk[j] ? T(v.residue[e]) : T(0);
   }
   const int len = it.yhi - it.ylo + 1;
@@ -138,7 +147,7 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev
             const StepSite<T> ps = stepSite(Dm, (esPrev + es) * T(0.5), h);
             const T half = havePrev ? T(1) : T(0.5);
 #pragma unroll
-            for (int j = 0; j < EL; ++j) coh[j] = stepEnergy(ps, ser[j] * half, coh[j]);
+            for (int j = 0; j < EL; ++j) coh[j] = stepE<KEL>(ps, ser[j] * half, sej[j] * half, coh[j]);
             if (lcur == 2) {
               const int ci = baseOut + cntOut;
               const T dr = T(v.distOutGt[ci]), d = dr * h;
@@ -146,7 +155,7 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev
 #pragma unroll
               for (int j = 0; j < EL; ++j) {
                 if (!eOk[j]) continue;
-                const Cx<T> bv = stepEnergy(pe, ser[j] * dr, coh[j]);
+                const Cx<T> bv = stepE<KEL>(pe, ser[j] * dr, sej[j] * dr, coh[j]);
                 const size_t si = size_t(ci) * Ne + g * EL + j;
                 v.bOutGt[si] = double(bv.re);
                 v.bOutGt[size_t(v.nOutGt) * Ne + si] = double(bv.im);
@@ -163,7 +172,7 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev
               const size_t si = size_t(ci) * Ne + g * EL + j;
               const Cx<T> b0 = eOk[j] ? Cx<T>{T(v.bInGt[si]), T(v.bInGt[size_t(v.nInGt) * Ne + si])}
                                       : Cx<T>{T(0), T(0)};
-              coh[j] = stepEnergy(pe, ser[j] * dr, b0);
+              coh[j] = stepE<KEL>(pe, ser[j] * dr, sej[j] * dr, b0);
             }
             ++cntIn;
           } else {
@@ -238,7 +247,7 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev
             const StepSite<T> ps = stepSite(Dm, (esPrev + es) * T(0.5), h);
             const T half = havePrev ? T(1) : T(0.5);
 #pragma unroll
-            for (int j = 0; j < EL; ++j) coh[j] = stepEnergy(ps, ser[j] * half, coh[j]);
+            for (int j = 0; j < EL; ++j) coh[j] = stepE<KEL>(ps, ser[j] * half, sej[j] * half, coh[j]);
             if (lcur == 3) {
               const int ci = baseOut + cntOut;
               const T dr = T(v.distOutG[ci]), d = dr * h;
@@ -246,7 +255,7 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev
 #pragma unroll
               for (int j = 0; j < EL; ++j) {
                 if (!eOk[j]) continue;
-                const Cx<T> bv = stepEnergy(pe, ser[j] * dr, coh[j]);
+                const Cx<T> bv = stepE<KEL>(pe, ser[j] * dr, sej[j] * dr, coh[j]);
                 const size_t si = size_t(ci) * Ne + g * EL + j;
                 v.bOutG[si] = double(bv.re);
                 v.bOutG[size_t(v.nOutG) * Ne + si] = double(bv.im);
@@ -262,7 +271,7 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev
               const size_t si = size_t(ci) * Ne + g * EL + j;
               const Cx<T> b0 = eOk[j] ? Cx<T>{T(v.bInG[si]), T(v.bInG[size_t(v.nInG) * Ne + si])}
                                       : Cx<T>{T(0), T(0)};
-              coh[j] = stepEnergy(pe, ser[j] * dr, b0);
+              coh[j] = stepE<KEL>(pe, ser[j] * dr, sej[j] * dr, b0);
             }
             ++cntIn;
           } else {
@@ -277,7 +286,22 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev
         //   Re g = Im((1 - p) / (1 + p)) = -2 Im p / |1 + p|^2
         //   Re i(D* gamma - D gamma~) summed as D.re Su + D.im Sw with
         //   Su = sum r Im(gamma~ - gamma), Sw = sum r Re(gamma + gamma~).
-        if (!a.burnin && lcur > 0) {
+        if constexpr (KEL) {
+          // Spectral weight per energy (ComputeLDOS.h:126-151 with
+          // LocalGreens.h:32-45): -Im(i X) with X = -(1 - p) / (1 + p),
+          // i.e. Re((1 - p) / (1 + p)) = (1 - |p|^2) / |1 + p|^2.
+          if (!a.burnin && lcur > 0 && active) {
+            double *Gs = v.G + (size_t(k - v.angleBase) * n1 + size_t(y) * N1 + x) * Ne + g * EL;
+#pragma unroll
+            for (int j = 0; j < EL; ++j) {
+              if (!eOk[j]) continue;
+              const Cx<T> p = coh[j] * gtCur[j];
+              const T pr = T(1) + p.re;
+              const T den = pr * pr + p.im * p.im;
+              Gs[j] = double((T(1) - (p.re * p.re + p.im * p.im)) * rcpT(den));
+            }
+          }
+        } else if (!a.burnin && lcur > 0) {
           double gdr = 0.0, gdi = 0.0, gj = 0.0, su = 0.0, sw = 0.0;
           if (active) {
 #pragma unroll
@@ -326,7 +350,7 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev
 // Persistent wavefronts pulling items from the work counter.  The second
 // launch bound asks the compiler for a register budget that leaves room for
 // WAVES wavefronts per SIMD.
-template <typename T, int EL, int WAVES>
+template <typename T, int EL, int WAVES, bool KEL>
 __global__ __launch_bounds__(64, WAVES) void sweepKernel(SweepArgs a) {
   Cx<T> *scr = reinterpret_cast<Cx<T> *>(a.scratch) + size_t(blockIdx.x) * a.slotStride;
   for (;;) {
@@ -336,7 +360,7 @@ __global__ __launch_bounds__(64, WAVES) void sweepKernel(SweepArgs a) {
     // with scalar loads
     item = __builtin_amdgcn_readfirstlane(__shfl(item, 0));
     if (item >= a.numItems) break;
-    sweepItem<T, EL>(a, a.items[item], scr);
+    sweepItem<T, EL, KEL>(a, a.items[item], scr);
   }
 }
 
@@ -366,7 +390,8 @@ __global__ void initialBoundaryKernel(GrainView v, const int *xEnd) {
     double dre, dim, es;
     siteFields(v, k, x, y, dre, dim, es);
     Cx<double> ga, gt;
-    homogeneous(Cx<double>{dre, dim}, Cx<double>{es, v.energy[e]}, ga, gt);
+    const double ere = v.energyRe ? v.energyRe[e] : 0.0;
+    homogeneous(Cx<double>{dre, dim}, Cx<double>{ere + es, v.energy[e]}, ga, gt);
     if (l == 3) {
       const size_t si = size_t(baseG + cg) * Ne + e;
       v.bOutG[si] = ga.re;
@@ -385,17 +410,18 @@ __global__ void initialBoundaryKernel(GrainView v, const int *xEnd) {
 // (geometry/domain_ops.h:11-57): target <- mean of its listed sources, for
 // `planes` values per site.  A target index t addresses block t / siteStride
 // and site t % siteStride; value q of a site lives at
-// base + block * blockStride + site * siteScale + q * planeStride.
+// base + (block - blockBase) * blockStride + site * siteScale + q * planeStride.
 __global__ void extendKernel(double *base, const int *target, const int *offset,
                              const int *source, int numTargets, int planes,
                              long long planeStride, long long siteStride,
-                             long long blockStride, long long siteScale) {
+                             long long blockStride, long long siteScale,
+                             long long blockBase) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= numTargets) return;
   const long long tg = target[t];
   const long long blk = tg / siteStride;
   const long long site = tg - blk * siteStride;
-  double *p0 = base + blk * blockStride;
+  double *p0 = base + (blk - blockBase) * blockStride;
   const int o0 = offset[t], o1 = offset[t + 1];
   const double inv = double(o1 - o0);
   for (int q = 0; q < planes; ++q) {

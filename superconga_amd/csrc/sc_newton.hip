@@ -132,6 +132,7 @@ void Grain::nkMap(double *gu) {
 }
 
 NewtonReport Grain::newtonKrylov(int maxNewton, int krylovDim, double tol) {
+  if (s_->keldysh) throw std::runtime_error("a spectroscopy context only runs the LDOS iteration");
   if (krylovDim < 1 || krylovDim > kMaxTerms - 1)
     throw std::runtime_error("Krylov dimension must be in [1, " + std::to_string(kMaxTerms - 1) +
                              "]");

@@ -787,6 +787,64 @@ Config parseConfig(const std::string &text) {
 // ---------------------------------------------------------------------------
 // Setup
 // ---------------------------------------------------------------------------
+Spectroscopy parseSpectroscopy(const std::string &text) {
+  const json::Value root = json::parse(text);
+  Spectroscopy sp;
+  const json::Value &sc = root["spectroscopy"];
+  sp.eMin = sc["energy_min"].asDouble(0.0);
+  sp.eMax = sc["energy_max"].asDouble(0.0);
+  sp.broadening = sc["energy_broadening"].asDouble(0.0);
+  sp.numEnergies = sc["num_energies"].asInt(0);
+  const json::Value &nu = root["numerics"];
+  sp.convergence = nu["convergence_criterion"].asDouble(1e-4);
+  const std::string norm = nu["norm"].asString("l2");
+  if (norm == "l1") sp.norm = NORM_L1;
+  else if (norm == "l2") sp.norm = NORM_L2;
+  else if (norm == "linf") sp.norm = NORM_LINF;
+  else throw std::runtime_error("unknown norm '" + norm + "'");
+  sp.perBlock = nu["num_energies_per_block"].asInt(-1);
+  sp.numAngles = nu["num_fermi_momenta"].asInt(0);
+  sp.burnin = nu["num_iterations_burnin"].asInt(-1);
+  sp.itMin = nu["num_iterations_min"].asInt(0);
+  sp.itMax = nu["num_iterations_max"].asInt(-1);
+  if (sp.numEnergies < 1) throw std::runtime_error("num_energies must be positive");
+  if (sp.numAngles < 1) throw std::runtime_error("num_fermi_momenta must be positive");
+  if (!(sp.broadening > 0.0)) throw std::runtime_error("energy_broadening must be positive");
+  if (sp.perBlock == 0) throw std::runtime_error("num_energies_per_block must not be 0");
+  // IntegrationIteratorKeldysh::initIterators (:152-180)
+  sp.perBlock = sp.perBlock < 1 ? sp.numEnergies : std::min(sp.perBlock, sp.numEnergies);
+  return sp;
+}
+
+void setKeldyshBlock(Setup &s, int start) {
+  // IntegrationIteratorKeldysh::initEnergies (:115-150): E_i = min + i step
+  const Spectroscopy &sp = s.spec;
+  const int n = sp.numEnergies;
+  const double step = n > 1 ? (sp.eMax - sp.eMin) / static_cast<double>(n - 1) : 0.0;
+  s.Ne = std::min(sp.perBlock, n - start);
+  s.energyRe.resize(size_t(s.Ne));
+  s.energy.assign(size_t(s.Ne), sp.broadening);
+  s.residue.assign(size_t(s.Ne), 1.0);
+  for (int e = 0; e < s.Ne; ++e) s.energyRe[size_t(e)] = sp.eMin + step * static_cast<double>(start + e);
+  s.trajSteps = 2LL * s.Ne * s.insideSites;
+}
+
+std::unique_ptr<Setup> buildSpectroscopySetup(Config cfg, const Spectroscopy &sp) {
+  cfg.numAngles = sp.numAngles;
+  cfg.convergence = sp.convergence;
+  cfg.norm = sp.norm;
+  cfg.burnin = sp.burnin;
+  cfg.itMin = sp.itMin;
+  cfg.itMax = sp.itMax;
+  cfg.computeCurrent = false;
+  cfg.computeFreeEnergy = false;
+  std::unique_ptr<Setup> s = buildSetup(cfg);
+  s->keldysh = true;
+  s->spec = sp;
+  setKeldyshBlock(*s, 0);
+  return s;
+}
+
 std::unique_ptr<Setup> buildSetup(const Config &cfg) {
   auto sp = std::make_unique<Setup>();
   Setup &s = *sp;
@@ -1100,6 +1158,7 @@ std::unique_ptr<Setup> buildSetup(const Config &cfg) {
 
   long long inside = 0;
   for (int8_t l : s.labels) inside += l > 0;
+  s.insideSites = inside;
   s.trajSteps = 2LL * s.Ne * inside;
   return sp;
 }

@@ -112,8 +112,27 @@ struct SweepItem {
   int ylo, yhi;
 };
 
+// postprocess_config.json (src/postprocess.cu:118-160,
+// frontend/postprocess/arguments.py:309-345): real energies for the LDOS.
+struct Spectroscopy {
+  double eMin = 0.0, eMax = 0.0, broadening = 0.0;
+  int numEnergies = 0;
+  int perBlock = -1; // < 1: every energy in one block
+  int numAngles = 0;
+  double convergence = 1e-4;
+  int norm = NORM_L2;
+  int burnin = -1, itMin = 0, itMax = -1;
+};
+Spectroscopy parseSpectroscopy(const std::string &text);
+
 struct Setup {
   Config cfg;
+  // Keldysh spectroscopy: the energies are E + i broadening of one block
+  // (energyRe, energy), the residues are one; spec holds the whole range.
+  bool keldysh = false;
+  Spectroscopy spec;
+  std::vector<double> energyRe;
+  long long insideSites = 0; // #{sites with label > 0 over all frames}
   int N0 = 0, N1 = 0, Ne = 0, Na = 0, Nc = 0, numGrain = 0;
   double h = 0, grainFraction = 0, temperature = 0, lambda = -1;
   // energies (IntegrationIteratorOzaki.h:108-200)
@@ -146,6 +165,13 @@ struct Setup {
 };
 
 std::unique_ptr<Setup> buildSetup(const Config &cfg);
+// The postprocess context (src/postprocess.cu:118-330): the simulation's
+// physics and geometry with the spectroscopy's angles, criterion and norm,
+// energies set to the first block.
+std::unique_ptr<Setup> buildSpectroscopySetup(Config cfg, const Spectroscopy &sp);
+// Energies [start, start + count) of the Keldysh grid into s (energy,
+// energyRe, residue, Ne, trajSteps).
+void setKeldyshBlock(Setup &s, int start);
 
 // Work list of the sweep for a given number of columns per item, sorted by
 // decreasing length.

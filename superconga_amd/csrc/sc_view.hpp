@@ -17,7 +17,8 @@ struct GrainView {
   int charge;
   int useInduced;     // lambda > 0
   int useField;       // any vector potential (induced or external)
-  const double *energy, *residue;
+  const double *energy, *residue; // Im eps_e (Matsubara / broadening), weight R_e
+  const double *energyRe;         // Re eps_e for real-energy (Keldysh) sweeps, else null
   const float *opCos, *opSin, *aCos, *aSin, *bkCos, *bkSin;
   const double *saCos, *saSin, *vFx, *vFy, *weight, *basis;
   const int8_t *labels;
@@ -27,7 +28,9 @@ struct GrainView {
   long long nInG, nOutG, nInGt, nOutGt;
   const double *extDelta; // [Nc][2][N0^2]
   const double *A;        // [2][N0^2] (scaled by lambda^2)
-  double *G;              // [Na][N1^2][4]: Re/Im of f + f~*, Re g, Re i(D* g - D g~)
+  double *G;              // [Na][N1^2][4]: Re/Im of f + f~*, Re g, Re i(D* g - D g~);
+                          // Keldysh: [angle - angleBase][N1^2][Ne] spectral weight
+  int angleBase;          // first angle held in G (Keldysh sweeps by angle chunks)
   int cols;   // sweep lane layout: columns per item
   int groups; // energy groups per column (ceil(Ne / EL))
 };

@@ -15,7 +15,11 @@ work unchanged on the files this package writes.
   ``domain.csv``) and ``vector_potential.csv`` (``a_x``, ``a_y``).
 
 :func:`simulate` is the in-process counterpart of the reference's
-``simulation`` binary (src/simulation.cu:64-530).
+``simulation`` binary (src/simulation.cu:64-530), :func:`postprocess` of its
+``postprocess`` binary (src/postprocess.cu:64-472): ``ldos.csv`` (header
+``x_idx,y_idx,e_idx,ldos``, one row per site and energy, CSVGridWriter with
+one title for many fields), ``postprocess_config.json`` and
+``postprocess_convergence.csv`` (io/CSVPostprocessIterationWriter.h).
 """
 import json
 import math
@@ -259,3 +263,103 @@ def simulate(config, precision="fp64", verbose=False):
         return converged, results
     finally:
         sim.close()
+
+
+def write_ldos_csv(path, ldos):
+    """CSVGridWriter::write of the LDOS grid with the single title "ldos":
+    the energy index becomes a column (io/CSVGridWriter.h, fieldsAsIndex)."""
+    ne, n, _ = ldos.shape
+    yy, xx = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    xs, ys = xx.reshape(-1), yy.reshape(-1)
+    with open(path, "w") as f:
+        f.write("x_idx,y_idx,e_idx,ldos\n")
+        for e in range(ne):
+            vals = ldos[e].reshape(-1)
+            f.write("\n".join("%d,%d,%d,%s" % (xs[i], ys[i], e, _fmt(vals[i]))
+                              for i in range(n * n)) + "\n")
+
+
+class PostprocessConvergenceWriter:
+    """CSVPostprocessIterationWriter: iteration, boundary and LDOS residuals."""
+
+    HEADER = "iteration,boundary_coherence_residual,ldos_residual"
+
+    def __init__(self, path):
+        self.path = path
+
+    def write(self, iteration, st):
+        new = not os.path.exists(self.path)
+        with open(self.path, "a") as f:
+            if new:
+                f.write(self.HEADER + "\n")
+            f.write("%d,%s,%s\n" % (iteration, _fmt(st["residual_boundary"]),
+                                     _fmt(st["residual_ldos"])))
+
+
+def postprocess(config, precision="fp64", verbose=False):
+    """The reference's postprocess binary in process: read the postprocess
+    config and misc.load_path/simulation_config.json, load the order
+    parameter and vector potential written there, run the Keldysh iteration
+    block by block and write misc.save_path.  Returns (converged, ldos)."""
+    import superconga_amd as sa
+    if isinstance(config, str):
+        with open(config) as f:
+            config = json.load(f)
+    misc = config.get("misc", {})
+    load_path = misc.get("load_path", "")
+    sim_path = os.path.join(load_path, "simulation_config.json")
+    if not os.path.exists(sim_path):
+        raise FileNotFoundError("-- ERROR! Could not find/open config file '%s'." % sim_path)
+    with open(sim_path) as f:
+        sim_cfg = json.load(f)
+    nu = config["numerics"]
+    spec = sa.Spectroscopy(sim_cfg, config)
+    try:
+        load_state(spec, sim_cfg, load_path)
+        save_path = misc.get("save_path", "")
+        writer = None
+        if save_path:
+            os.makedirs(save_path, exist_ok=True)
+            conv_path = os.path.join(save_path, "postprocess_convergence.csv")
+            if os.path.exists(conv_path):
+                os.remove(conv_path)
+            writer = PostprocessConvergenceWriter(conv_path)
+
+        def iterations(compute, mn, mx):
+            # runIterations (run_iteration.h:85-141)
+            if mn == 0 and mx == 0:
+                return False
+            conv = False
+            i = 1
+            while True:
+                spec.compute(precision, burnin=not compute)
+                st = spec.status()
+                conv = bool(st["partially_converged"]) if compute else (
+                    st["residual_boundary"] < nu["convergence_criterion"])
+                if writer:
+                    writer.write(st["iterations"], st)
+                if verbose:
+                    print("Iteration: %d | Residual: %e" % (st["iterations"], st["residual"]))
+                if (conv and i >= mn) or (mx >= 0 and i >= mx):
+                    break
+                i += 1
+            return conv
+
+        # runPostprocess (run_simulation.h:44-108)
+        nb = nu.get("num_iterations_burnin", -1)
+        converged = False
+        while not converged:
+            iterations(False, nb, nb)
+            block_converged = iterations(True, nu.get("num_iterations_min", 0),
+                                         nu.get("num_iterations_max", -1))
+            converged = bool(spec.status()["converged"])
+            if not block_converged:
+                break
+        ldos = spec.ldos()
+        if save_path:
+            with open(os.path.join(save_path, "postprocess_config.json"), "w") as f:
+                json.dump(config, f, indent=4)
+            write_ldos_csv(os.path.join(save_path, "ldos.csv"), ldos)
+        return converged, ldos
+    finally:
+        spec.close()

@@ -92,3 +92,30 @@ def test_load_rejects_other_lattice(tmp_path):
     cfg = {"physics": {"order_parameter": {"s": {}}}}
     with pytest.raises(NotImplementedError):
         scio.load_state(Fake(), cfg, str(tmp_path))
+
+
+def test_ldos_csv_energy_index_layout(tmp_path):
+    # CSVGridWriter with one title for a multi-field grid: e_idx column,
+    # fields outermost, then rows, x fastest (io/CSVGridWriter.h)
+    ne, n = 3, 4
+    ldos = np.arange(ne * n * n, dtype=float).reshape(ne, n, n) / 7.0
+    p = tmp_path / "ldos.csv"
+    scio.write_ldos_csv(str(p), ldos)
+    lines = p.read_text().splitlines()
+    assert lines[0] == "x_idx,y_idx,e_idx,ldos"
+    assert len(lines) == 1 + ne * n * n
+    assert lines[1] == "0,0,0,%.8e" % ldos[0, 0, 0]
+    assert lines[2].startswith("1,0,0,")
+    assert lines[1 + n].startswith("0,1,0,")
+    assert lines[1 + n * n] == "0,0,1,%.8e" % ldos[1, 0, 0]
+    back = scio.read_csv(str(p))
+    assert np.allclose(back["ldos"].reshape(ne, n, n), ldos, rtol=1e-8, atol=0)
+
+
+def test_postprocess_convergence_writer(tmp_path):
+    p = tmp_path / "postprocess_convergence.csv"
+    w = scio.PostprocessConvergenceWriter(str(p))
+    w.write(5, {"residual_boundary": 1e-3, "residual_ldos": 2e-4})
+    lines = p.read_text().splitlines()
+    assert lines == ["iteration,boundary_coherence_residual,ldos_residual",
+                     "5,1.00000000e-03,2.00000000e-04"]
