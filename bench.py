@@ -38,6 +38,12 @@ sys.path.insert(0, ROOT)
 STEP_FLOPS = 71.0
 GREEN_FLOPS_PER_SITE_ENERGY = 59.0
 FLOPS_PER_TRAJ_STEP = STEP_FLOPS + 0.5 * GREEN_FLOPS_PER_SITE_ENERGY
+# Real-frequency (Keldysh) sweep of the LDOS: the step plus 2 for the real
+# energy part (scale, add), and the spectral weight (1 - |p|^2) / |1 + p|^2
+# per site and energy: 15 add/mul + 1 rcp = 16, shared by the two steps.
+KELDYSH_STEP_FLOPS = STEP_FLOPS + 2.0
+SPECTRAL_FLOPS_PER_SITE_ENERGY = 16.0
+KELDYSH_FLOPS_PER_TRAJ_STEP = KELDYSH_STEP_FLOPS + 0.5 * SPECTRAL_FLOPS_PER_SITE_ENERGY
 PEAK_FP64_TFLOPS = 78.6   # MI355X vector FP64
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 (packed)
 PEAK_HBM_GBS = 8000.0
@@ -63,13 +69,13 @@ def timed_iterations(sim, precision, warmup, steps):
     return (t1 - t0), np.array(sweep)
 
 
-def roofline(traj_steps, sweep_ms, peak):
+def roofline(traj_steps, sweep_ms, peak, flops_per_step=FLOPS_PER_TRAJ_STEP):
     t = float(np.mean(sweep_ms)) * 1e-3
-    achieved = traj_steps * FLOPS_PER_TRAJ_STEP / t / 1e12
+    achieved = traj_steps * flops_per_step / t / 1e12
     return {"bound": "valu", "achieved": round(achieved, 3), "peak": peak,
             "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
             "traffic": None, "kernel": "sweepKernel",
-            "flops_per_traj_step": FLOPS_PER_TRAJ_STEP,
+            "flops_per_traj_step": flops_per_step,
             "traj_steps_per_launch": int(traj_steps),
             "avg_launch_ms": round(float(np.mean(sweep_ms)), 4)}
 
@@ -120,6 +126,80 @@ def converge(sa, cfg, precision, cap):
             "fp32_iterations": r["mixed_iterations"], "wall_s": round(wall, 4),
             "residual": r["residual"], "free_energy_per_area": r["free_energy_per_area"],
             "sweep_shapes": [list(shape64), list(shape32)]}
+
+
+def converge_newton(sa, cfg, cap, warm_residual=1e-3, warm_max=30):
+    """Wall time to the config's criterion with Newton-Krylov: FP32-stepping
+    iterations until the residual is below warm_residual (at most warm_max),
+    Jacobian-free Newton-Krylov on the FP64 map, then FP64 iterations until
+    the reference's convergence test passes."""
+    sim = sa.Simulation(cfg)
+    sim.autotune("fp64")
+    sim.autotune("mixed")
+    t = time.perf_counter()
+    warm = 0
+    for warm in range(1, warm_max + 1):
+        sim.compute("mixed")
+        if sim.results()["residual"] < warm_residual:
+            break
+    rep = sim.newton_krylov(max_newton=20, krylov_dim=30, tol=1e-11)
+    ok = sim.run(0, 0, cap, "fp64")
+    wall = time.perf_counter() - t
+    r = sim.results()
+    sim.close()
+    return {"converged": bool(ok), "fp32_iterations": warm,
+            "newton_iterations": rep["newton_iterations"],
+            "krylov_iterations": rep["krylov_iterations"],
+            "map_evaluations": rep["map_evaluations"], "newton_residual": rep["residual"],
+            "fp64_iterations_after": r["iterations"] - warm, "wall_s": round(wall, 4),
+            "residual": r["residual"], "free_energy_per_area": r["free_energy_per_area"]}
+
+
+def ldos_bench(sa, workloads, args):
+    """The LDOS postprocess of the workload's grain with the reference's
+    example spectroscopy settings (examples/*/postprocess_config.json: 720
+    angles, 128 energies in [0, 0.5] with broadening 0.008, blocks of 32):
+    iterations of the first energy block, FP64."""
+    cfg = workloads.WORKLOADS[args.workload]()
+    post = workloads.postprocess_config()
+    t = time.perf_counter()
+    sp = sa.Spectroscopy(cfg, post)
+    setup_s = time.perf_counter() - t
+    sim = sa.Simulation(cfg)  # the initial order parameter of the grain
+    delta = sim.order_parameter(0)
+    sim.close()
+    for c, d in enumerate(delta):
+        sp.set_order_parameter(c, d)
+    sp.compute("fp64")
+    if args.ldos_shape:
+        sp.set_sweep_shape(*(int(v) for v in args.ldos_shape.split(",")), precision="fp64")
+        shape = sp.sweep_shape("fp64")
+    else:
+        shape = sp.autotune("fp64")
+    log("ldos sweep shape fp64: %s" % (shape,))
+    for _ in range(max(args.warmup - 1, 0)):
+        sp.compute("fp64")
+    sweeps = []
+    t0 = time.perf_counter()
+    for _ in range(args.ldos_steps):
+        sp.compute("fp64")
+        sweeps.append(sp.status()["last_sweep_ms"])
+    wall = time.perf_counter() - t0
+    st = sp.status()
+    sp.close()
+    steps = st["traj_steps"]
+    rf = roofline(steps, sweeps, PEAK_FP64_TFLOPS, KELDYSH_FLOPS_PER_TRAJ_STEP)
+    rf["kernel"] = "sweepKernel (Keldysh, summed over angle chunks)"
+    return {"workload": args.workload + " LDOS postprocess",
+            "num_angles": post["numerics"]["num_fermi_momenta"],
+            "num_energies": post["spectroscopy"]["num_energies"],
+            "energies_per_block": post["numerics"]["num_energies_per_block"],
+            "angle_chunk": st["angle_chunk"], "setup_s": round(setup_s, 2),
+            "value": steps * args.ldos_steps / wall, "unit": "traj-steps/s",
+            "ms_per_iteration": 1e3 * wall / args.ldos_steps,
+            "traj_steps_per_iteration": steps, "dtype": "f64", "roofline": rf,
+            "sweep_shape": {"energies_per_lane": shape[0], "waves_per_simd": shape[1],
+                            "slots_per_cu": shape[2]}}
 
 
 def batch_bench(sa, workloads, args):
@@ -223,6 +303,10 @@ def main():
     ap.add_argument("--shape", default="",
                     help="fixed sweep shape 'energies,waves,slots' for both "
                          "precisions instead of the autotuner")
+    ap.add_argument("--shape-mixed", default="",
+                    help="fixed sweep shape of the mixed path (default: --shape)")
+    ap.add_argument("--ldos-only", action="store_true",
+                    help="only the LDOS postprocess measurement")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one device-to-device copy of this size first "
                          "(calibrates HBM counters of a profiler run)")
@@ -231,6 +315,10 @@ def main():
                     help="workload of the time-to-criterion comparison")
     ap.add_argument("--converge-cap", type=int, default=2000)
     ap.add_argument("--batch-count", type=int, default=48)
+    ap.add_argument("--no-ldos", action="store_true")
+    ap.add_argument("--ldos-steps", type=int, default=3)
+    ap.add_argument("--ldos-shape", default="", help="fixed LDOS sweep shape")
+    ap.add_argument("--no-newton", action="store_true")
     ap.add_argument("--cpu-threads", type=int,
                     default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
                     or min(16, os.cpu_count() or 1))
@@ -250,6 +338,9 @@ def main():
 
     if args.workload == "parameter_sweep":
         print(json.dumps(batch_bench(sa, workloads, args)), flush=True)
+        return
+    if args.ldos_only:
+        print(json.dumps(ldos_bench(sa, workloads, args)), flush=True)
         return
 
     cfg = workloads.WORKLOADS[args.workload]()
@@ -304,8 +395,10 @@ def main():
         sim.compute("mixed")
         got = (sim.order_parameter(1), sim.current(), sim.vector_potential(1),
                sim.results()["free_energy"])
-        if fixed:
-            sim.set_sweep_shape(*fixed, precision="mixed")
+        mfixed = (tuple(int(v) for v in args.shape_mixed.split(","))
+                  if args.shape_mixed else fixed)
+        if mfixed:
+            sim.set_sweep_shape(*mfixed, precision="mixed")
             mshape = sim.sweep_shape("mixed")
         else:
             mshape = sim.autotune("mixed")
@@ -338,7 +431,17 @@ def main():
             log("converge %s %s: %s" % (args.converge_workload, prec, conv[prec]))
         if conv["fp64"]["converged"] and conv["mixed_refine"]["converged"]:
             conv["speedup_mixed_refine"] = conv["fp64"]["wall_s"] / conv["mixed_refine"]["wall_s"]
+        if not args.no_newton:
+            conv["newton_krylov"] = converge_newton(sa, ccfg, args.converge_cap)
+            log("converge %s newton-krylov: %s" % (args.converge_workload, conv["newton_krylov"]))
+            if conv["fp64"]["converged"] and conv["newton_krylov"]["converged"]:
+                conv["speedup_newton_krylov"] = (conv["fp64"]["wall_s"]
+                                                 / conv["newton_krylov"]["wall_s"])
         out["time_to_1e-8"] = conv
+
+    if not args.no_ldos:
+        out["ldos"] = ldos_bench(sa, workloads, args)
+        log("ldos: %s" % out["ldos"])
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds, args.cpu_threads)

@@ -177,9 +177,10 @@ typedef struct {
  * parameter, the induced vector potential when the penetration depth is
  * finite, and the boundary in-values of both coherence functions; G is one
  * FP64 sweep + angle integration + gap equation + boundary condition +
- * Poisson solve.  Each Newton step solves J d = -(G(u) - u) by restarted-free
- * GMRES of at most krylov_dim (1..31) directions with finite-difference
- * products, in the inner product weighted per block by 1 / |u_block|^2,
+ * Poisson solve.  Each Newton step solves J d = -(G(u) - u) by GMRES
+ * restarted every krylov_dim (1..31) directions, at most 4 cycles, with
+ * finite-difference products, in the inner product weighted per block by
+ * 1 / |u_block|^2,
  * followed by a backtracking line search; the forcing term follows
  * Eisenstat-Walker.  On return the state holds the last accepted u and the
  * accelerator restarts; a following sc_compute reports the reference's

@@ -34,6 +34,8 @@ namespace scamd {
 namespace {
 
 constexpr int kMaxTerms = 32;
+// GMRES cycles per Newton step (each of at most krylovDim directions)
+constexpr int kRestarts = 4;
 struct Terms {
   const double *v[kMaxTerms];
   double c[kMaxTerms];
@@ -139,8 +141,9 @@ NewtonReport Grain::newtonKrylov(int maxNewton, int krylovDim, double tol) {
   const std::vector<NkBlock> blocks = nkBlocks();
   const size_t L = nkLength();
   const long long nL = static_cast<long long>(L);
-  // workspace: u, G(u), F, trial, w, and the Krylov basis
-  const int nv = 5 + krylovDim + 1;
+  // workspace: u, G(u), F, trial, w, the step d, the linear residual r and
+  // the Krylov basis
+  const int nv = 7 + krylovDim + 1;
   if (nkVecs_.size() != size_t(nv) || nkLen_ != L) {
     for (double *p : nkVecs_) devFree(p);
     nkVecs_.clear();
@@ -148,8 +151,8 @@ NewtonReport Grain::newtonKrylov(int maxNewton, int krylovDim, double tol) {
     nkLen_ = L;
   }
   double *u = nkVecs_[0], *gu = nkVecs_[1], *f = nkVecs_[2], *trial = nkVecs_[3],
-         *w = nkVecs_[4];
-  std::vector<double *> V(nkVecs_.begin() + 5, nkVecs_.end());
+         *w = nkVecs_[4], *dstep = nkVecs_[5], *rlin = nkVecs_[6];
+  std::vector<double *> V(nkVecs_.begin() + 7, nkVecs_.end());
   nkMaps_ = 0;
   NewtonReport rep;
 
@@ -187,7 +190,7 @@ NewtonReport Grain::newtonKrylov(int maxNewton, int krylovDim, double tol) {
                                                       nL);
     SC_NK_HIP(hipGetLastError());
   };
-  // F(x) into f given x: writes G(x) into gout
+  // F(x) into fout given x: writes G(x) into gout
   auto residual = [&](const double *x, double *gout, double *fout) {
     nkSet(x);
     nkMap(gout);
@@ -195,78 +198,98 @@ NewtonReport Grain::newtonKrylov(int maxNewton, int krylovDim, double tol) {
     return wnorm(fout);
   };
 
+  const int m = krylovDim;
   double fn = residual(u, gu, f);
   double fnPrev = fn;
   rep.history.push_back(fn);
   double eta = 0.1;
   for (int it = 0; it < maxNewton && fn >= tol; ++it) {
-    // GMRES(krylovDim) on J d = -F from d = 0
-    const double beta = fn;
-    combine(V[0], nullptr, 0.0, {f}, {-1.0 / beta});
-    std::vector<double> H(size_t(krylovDim + 1) * krylovDim, 0.0), g(size_t(krylovDim + 1), 0.0),
-        cs(size_t(krylovDim), 0.0), sn(size_t(krylovDim), 0.0);
-    g[0] = beta;
-    int k = 0;
+    // restarted GMRES(m) on J d = -F from d = 0
     const double unorm = wnorm(u);
-    for (; k < krylovDim; ++k) {
-      const double e = 1e-7 * (1.0 + unorm);
-      combine(trial, u, 1.0, {V[size_t(k)]}, {e});
-      nkSet(trial);
-      nkMap(w);
-      combine(w, w, 1.0 / e, {gu, V[size_t(k)]}, {-1.0 / e, -1.0});
-      // classical Gram-Schmidt, twice
-      std::vector<double *> basis(V.begin(), V.begin() + k + 1);
-      std::vector<const double *> cbasis(basis.begin(), basis.end());
-      std::vector<double> h(size_t(k) + 1, 0.0);
-      for (int pass = 0; pass < 2; ++pass) {
-        const std::vector<double> d = wdots(w, basis);
-        std::vector<double> neg(d.size());
-        for (size_t i = 0; i < d.size(); ++i) {
-          h[i] += d[i];
-          neg[i] = -d[i];
+    const double e = 1e-7 * (1.0 + unorm);
+    combine(dstep, nullptr, 0.0, {}, {});
+    combine(rlin, nullptr, 0.0, {f}, {-1.0});
+    double rn = fn;
+    for (int cycle = 0; cycle < kRestarts; ++cycle) {
+      const double beta = rn;
+      combine(V[0], nullptr, 0.0, {rlin}, {1.0 / beta});
+      // H: Hessenberg matrix as built (for the restart residual), R: with
+      // the Givens rotations applied
+      std::vector<double> H(size_t(m + 1) * m, 0.0), R(size_t(m + 1) * m, 0.0),
+          g(size_t(m + 1), 0.0), cs(size_t(m), 0.0), sn(size_t(m), 0.0);
+      g[0] = beta;
+      int k = 0;
+      bool breakdown = false;
+      for (; k < m; ++k) {
+        combine(trial, u, 1.0, {V[size_t(k)]}, {e});
+        nkSet(trial);
+        nkMap(w);
+        combine(w, w, 1.0 / e, {gu, V[size_t(k)]}, {-1.0 / e, -1.0});
+        // classical Gram-Schmidt, twice
+        std::vector<double *> basis(V.begin(), V.begin() + k + 1);
+        std::vector<const double *> cbasis(basis.begin(), basis.end());
+        std::vector<double> h(size_t(k) + 1, 0.0);
+        for (int pass = 0; pass < 2; ++pass) {
+          const std::vector<double> d = wdots(w, basis);
+          std::vector<double> neg(d.size());
+          for (size_t i = 0; i < d.size(); ++i) {
+            h[i] += d[i];
+            neg[i] = -d[i];
+          }
+          combine(w, w, 1.0, cbasis, neg);
         }
-        combine(w, w, 1.0, cbasis, neg);
+        const double hn = wnorm(w);
+        for (int i = 0; i <= k; ++i) H[size_t(i) * m + k] = R[size_t(i) * m + k] = h[size_t(i)];
+        H[size_t(k + 1) * m + k] = R[size_t(k + 1) * m + k] = hn;
+        for (int i = 0; i < k; ++i) {
+          const double a = R[size_t(i) * m + k], b = R[size_t(i + 1) * m + k];
+          R[size_t(i) * m + k] = cs[size_t(i)] * a + sn[size_t(i)] * b;
+          R[size_t(i + 1) * m + k] = -sn[size_t(i)] * a + cs[size_t(i)] * b;
+        }
+        const double a = R[size_t(k) * m + k], b = R[size_t(k + 1) * m + k];
+        const double r = std::hypot(a, b);
+        cs[size_t(k)] = r > 0.0 ? a / r : 1.0;
+        sn[size_t(k)] = r > 0.0 ? b / r : 0.0;
+        R[size_t(k) * m + k] = r;
+        R[size_t(k + 1) * m + k] = 0.0;
+        g[size_t(k + 1)] = -sn[size_t(k)] * g[size_t(k)];
+        g[size_t(k)] = cs[size_t(k)] * g[size_t(k)];
+        breakdown = !(hn > 0.0);
+        if (!breakdown) combine(V[size_t(k + 1)], nullptr, 0.0, {w}, {1.0 / hn});
+        if (std::fabs(g[size_t(k + 1)]) <= eta * fn || breakdown) {
+          ++k;
+          break;
+        }
       }
-      const double hn = wnorm(w);
-      for (int i = 0; i <= k; ++i) H[size_t(i) * krylovDim + k] = h[size_t(i)];
-      H[size_t(k + 1) * krylovDim + k] = hn;
-      // Givens rotations
-      for (int i = 0; i < k; ++i) {
-        const double a = H[size_t(i) * krylovDim + k], b = H[size_t(i + 1) * krylovDim + k];
-        H[size_t(i) * krylovDim + k] = cs[size_t(i)] * a + sn[size_t(i)] * b;
-        H[size_t(i + 1) * krylovDim + k] = -sn[size_t(i)] * a + cs[size_t(i)] * b;
+      // back substitution: R y = g on the first k rows
+      std::vector<double> y(size_t(k), 0.0);
+      for (int i = k - 1; i >= 0; --i) {
+        double acc = g[size_t(i)];
+        for (int q = i + 1; q < k; ++q) acc -= R[size_t(i) * m + q] * y[size_t(q)];
+        y[size_t(i)] = R[size_t(i) * m + i] != 0.0 ? acc / R[size_t(i) * m + i] : 0.0;
       }
-      const double a = H[size_t(k) * krylovDim + k], b = H[size_t(k + 1) * krylovDim + k];
-      const double r = std::hypot(a, b);
-      cs[size_t(k)] = r > 0.0 ? a / r : 1.0;
-      sn[size_t(k)] = r > 0.0 ? b / r : 0.0;
-      H[size_t(k) * krylovDim + k] = r;
-      H[size_t(k + 1) * krylovDim + k] = 0.0;
-      g[size_t(k + 1)] = -sn[size_t(k)] * g[size_t(k)];
-      g[size_t(k)] = cs[size_t(k)] * g[size_t(k)];
-      if (hn > 0.0) combine(V[size_t(k + 1)], nullptr, 0.0, {w}, {1.0 / hn});
-      if (std::fabs(g[size_t(k + 1)]) <= eta * beta || hn == 0.0) {
-        ++k;
-        break;
+      rep.krylovIterations += k;
+      std::vector<const double *> vk(V.begin(), V.begin() + k);
+      combine(dstep, dstep, 1.0, vk, y);
+      rn = std::fabs(g[size_t(k)]);
+      if (rn <= eta * fn || breakdown || cycle + 1 == kRestarts) break;
+      // the linear residual -F - J d = V_(k+1) (beta e1 - H y)
+      std::vector<double> c(size_t(k) + 1, 0.0);
+      std::vector<const double *> vk1(V.begin(), V.begin() + k + 1);
+      for (int i = 0; i <= k; ++i) {
+        double acc = i == 0 ? beta : 0.0;
+        for (int q = 0; q < k; ++q) acc -= H[size_t(i) * m + q] * y[size_t(q)];
+        c[size_t(i)] = acc;
       }
+      combine(rlin, nullptr, 0.0, vk1, c);
+      rn = wnorm(rlin);
     }
-    // back substitution: H y = g on the first k rows
-    std::vector<double> y(size_t(k), 0.0);
-    for (int i = k - 1; i >= 0; --i) {
-      double acc = g[size_t(i)];
-      for (int q = i + 1; q < k; ++q) acc -= H[size_t(i) * krylovDim + q] * y[size_t(q)];
-      y[size_t(i)] = H[size_t(i) * krylovDim + i] != 0.0 ? acc / H[size_t(i) * krylovDim + i] : 0.0;
-    }
-    rep.krylovIterations += k;
-    // step d = V y, with backtracking on |F|
-    std::vector<const double *> vk(V.begin(), V.begin() + k);
+    // backtracking line search on |F| along d
     double lambda = 1.0, fnew = 0.0;
     bool accepted = false;
     for (int ls = 0; ls < 6; ++ls) {
-      std::vector<double> ys(y);
-      for (double &q : ys) q *= lambda;
-      combine(trial, u, 1.0, vk, ys);
-      fnew = residual(trial, w, V[size_t(krylovDim)]);
+      combine(trial, u, 1.0, {dstep}, {lambda});
+      fnew = residual(trial, w, rlin);
       if (fnew < (1.0 - 1e-4 * lambda) * fn) {
         accepted = true;
         break;
@@ -276,7 +299,7 @@ NewtonReport Grain::newtonKrylov(int maxNewton, int krylovDim, double tol) {
     if (!accepted) break; // no descent along the Newton direction
     std::swap(u, trial);
     std::swap(gu, w);
-    combine(f, V[size_t(krylovDim)], 1.0, {}, {});
+    std::swap(f, rlin);
     fnPrev = fn;
     fn = fnew;
     rep.history.push_back(fn);
@@ -285,8 +308,10 @@ NewtonReport Grain::newtonKrylov(int maxNewton, int krylovDim, double tol) {
     eta = std::clamp(0.9 * (fn / fnPrev) * (fn / fnPrev), 1e-4, 0.3);
     nkVecs_[0] = u;
     nkVecs_[1] = gu;
+    nkVecs_[2] = f;
     nkVecs_[3] = trial;
     nkVecs_[4] = w;
+    nkVecs_[6] = rlin;
   }
   nkSet(u);
   SC_NK_HIP(hipStreamSynchronize(stream_));

@@ -111,5 +111,18 @@ def parameter_sweep(count=48, seed=2024, ppxi=20.0):
     return configs
 
 
+def postprocess_config(num_angles=720, num_energies=128, per_block=32):
+    """The spectroscopy settings of the reference's examples
+    (examples/*/postprocess_config.json): energies [0, 0.5] with broadening
+    0.008, criterion 1e-4 in the L2 norm."""
+    return {"spectroscopy": {"energy_min": 0.0, "energy_max": 0.5,
+                             "energy_broadening": 0.008, "num_energies": num_energies},
+            "numerics": {"convergence_criterion": 1e-4, "norm": "l2",
+                         "num_energies_per_block": per_block,
+                         "num_fermi_momenta": num_angles, "num_iterations_burnin": -1,
+                         "num_iterations_max": 1000, "num_iterations_min": 0},
+            "misc": {"data_format": "csv", "load_path": "", "save_path": ""}}
+
+
 WORKLOADS = {"octagon_dwave": octagon_dwave, "vortex_disc": vortex_disc,
              "swave_disc": swave_disc}
