@@ -62,6 +62,8 @@ Grain::~Grain() {
   devFree(dDomain_);
   devFree(dDomainSites_);
   devFree(dIdx_);
+  devFree(dOutCross_);
+  devFree(dFieldTiles_);
   devFree(dDist_);
   devFree(dBcOffG_);
   devFree(dBcSrcG_);
@@ -88,6 +90,8 @@ Grain::~Grain() {
   devFree(dBInGt_);
   devFree(dBOutGt_);
   devFree(dG_);
+  devFree(dFields_);
+  devFree(dLdosFields_);
   devFree(dPartials_);
   devFree(dRed_);
   devFree(dMean_);
@@ -152,6 +156,13 @@ void Grain::upload() {
   for (const auto *v : {&s.idxInG, &s.idxOutG, &s.idxInGt, &s.idxOutGt})
     idx.insert(idx.end(), v->begin(), v->end());
   dIdx_ = devUpload(idx);
+  {
+    std::vector<int> oc;
+    for (const auto *v : {&s.colOutG, &s.coordG, &s.colOutGt, &s.coordGt})
+      oc.insert(oc.end(), v->begin(), v->end());
+    dOutCross_ = devUpload(oc);
+  }
+  dFieldTiles_ = devUpload(s.fieldTiles);
   std::vector<double> dist;
   for (const auto *v : {&s.distInG, &s.distOutG, &s.distInGt, &s.distOutGt})
     dist.insert(dist.end(), v->begin(), v->end());
@@ -188,7 +199,10 @@ void Grain::upload() {
   dBInGt_ = devAlloc<double>(size_t(2 * s.nInGt * neCap));
   dBOutGt_ = devAlloc<double>(size_t(2 * s.nOutGt * neCap));
   // Keldysh grains hold spectral weights of an angle chunk instead
-  if (!s.keldysh) dG_ = devAlloc<double>(size_t(s.Na) * 4 * n1);
+  if (!s.keldysh) {
+    dG_ = devAlloc<double>(size_t(s.Na) * 4 * n1);
+    dFields_ = devAlloc<double>(size_t(3) * s.Na * n1);
+  }
   {
     // first extension entry of every frame (the list is built frame by frame)
     extRotFrameStart_.assign(size_t(s.Na) + 1, 0);
@@ -283,9 +297,10 @@ void Grain::prepareSweep(int precision) {
   if (s.keldysh) {
     int ca = ldosChunkAngles();
     if (dLdosG_) {
-      // never more angles than the weight buffer holds
+      // never more angles than the weight and field buffers hold
       const size_t perAngle = size_t(s.N1) * s.N1 * size_t(s.spec.perBlock);
       ca = std::max(1, std::min(ca, static_cast<int>(ldosGCap_ / perAngle)));
+      ca = std::min(ca, ldosFieldFrames_);
     }
     const int nch = (s.Na + ca - 1) / ca;
     pl.chunkAngles = ca;
@@ -296,7 +311,10 @@ void Grain::prepareSweep(int precision) {
     for (const SweepItem &it : items) ++pl.chunkStart[size_t(it.angle / ca) + 1];
     for (int c = 0; c < nch; ++c) pl.chunkStart[size_t(c) + 1] += pl.chunkStart[size_t(c)];
     views.assign(size_t(nch), gv);
-    for (int c = 0; c < nch; ++c) views[size_t(c)].angleBase = c * ca;
+    for (int c = 0; c < nch; ++c) {
+      views[size_t(c)].angleBase = c * ca;
+      views[size_t(c)].fieldBase = c * ca;
+    }
   }
   std::vector<SweepItemDev> di(items.size());
   for (size_t i = 0; i < items.size(); ++i)
@@ -664,6 +682,18 @@ void Grain::extendDelta() {
         static_cast<long long>(n0), static_cast<long long>(n0), 0, 1, 0);
     SC_HIP(hipGetLastError());
   }
+  // the site fields every sweep of this iteration reads (Keldysh grains
+  // compute them per angle chunk)
+  if (!s.keldysh) computeFields(0, s.Na, static_cast<long long>(s.Na) * s.N1 * s.N1, dFields_);
+}
+
+void Grain::computeFields(int k0, int frames, long long plane, double *F) {
+  const Setup &s = *s_;
+  const int t0 = s.fieldTileStart[size_t(k0)], t1 = s.fieldTileStart[size_t(k0 + frames)];
+  if (t1 <= t0) return;
+  GrainView v = view();
+  fieldsKernel<<<t1 - t0, 256, 0, stream_>>>(v, dFieldTiles_ + 4 * size_t(t0), k0, plane, F);
+  SC_HIP(hipGetLastError());
 }
 
 GrainView Grain::view() const {
@@ -671,6 +701,10 @@ GrainView Grain::view() const {
                          dExtDelta_, dA_, s_->keldysh ? dLdosG_ : dG_, dEnergy_, dResidue_,
                          dSaCos_, dSaSin_, dVFx_, dVFy_, dWeight_, dBasis_, dLabels_);
   v.energyRe = dEnergyRe_;
+  const long long n1 = static_cast<long long>(s_->N1) * s_->N1;
+  v.fields = s_->keldysh ? dLdosFields_ : dFields_;
+  v.fieldPlane = (s_->keldysh ? ldosFieldFrames_ : s_->Na) * n1;
+  v.fieldBase = 0;
   return v;
 }
 
@@ -691,9 +725,19 @@ void Grain::ensureBoundaryInit() {
   const Setup &s = *s_;
   extendDelta();
   GrainView v = view();
-  const long long n = static_cast<long long>(s.Na) * s.N1 * s.Ne;
-  initialBoundaryKernel<<<blocksFor(n), 256, 0, stream_>>>(v, nullptr);
-  SC_HIP(hipGetLastError());
+  // homogeneous out-values at every exit crossing, then the boundary condition
+  const int *colG = dOutCross_, *yG = dOutCross_ + s.nOutG;
+  const int *colGt = dOutCross_ + 2 * s.nOutG, *yGt = colGt + s.nOutGt;
+  if (s.nOutG > 0) {
+    initialOutKernel<<<blocksFor(s.nOutG * s.Ne), 256, 0, stream_>>>(v, colG, yG, s.nOutG, 0,
+                                                                      dBOutG_);
+    SC_HIP(hipGetLastError());
+  }
+  if (s.nOutGt > 0) {
+    initialOutKernel<<<blocksFor(s.nOutGt * s.Ne), 256, 0, stream_>>>(v, colGt, yGt, s.nOutGt,
+                                                                        1, dBOutGt_);
+    SC_HIP(hipGetLastError());
+  }
   boundaryCondition();
   boundaryInit_ = true;
 }
@@ -725,6 +769,10 @@ void launchSweepVariant(int precision, int el, int waves, const SweepArgs &a, in
 }
 
 void Grain::sweep(int precision, bool burnin) {
+  if (s_->keldysh) {
+    ldosSweepAll(precision, burnin);
+    return;
+  }
   prepareSweep(precision);
   const SweepPlan &pl = plan_[precision];
   SweepArgs a;

@@ -151,6 +151,11 @@ private:
   // whose data start at base (frame k at base + (k - k0) * blockStride).
   void extendFrames(double *base, int k0, int k1, int planes, long long planeStride,
                     long long blockStride, long long siteScale);
+  // Site fields of frames [k0, k0 + frames) into F (plane stride `plane`).
+  void computeFields(int k0, int frames, long long plane, double *F);
+  // Keldysh sweeps chunk by chunk: site fields of the chunk, then its items.
+  void ldosSweepChunk(int precision, int chunk, bool burnin);
+  void ldosSweepAll(int precision, bool burnin);
   void toDevice(void *dst, const void *src, size_t bytes);
   void fromDevice(void *dst, const void *src, size_t bytes);
   void extendDelta();
@@ -190,6 +195,8 @@ private:
   uint8_t *dDomain_ = nullptr;
   int *dDomainSites_ = nullptr;
   int *dIdx_ = nullptr; // idxInG, idxOutG, idxInGt, idxOutGt
+  int *dOutCross_ = nullptr; // colOutG, coordG, colOutGt, coordGt
+  int *dFieldTiles_ = nullptr;
   double *dDist_ = nullptr;
   int *dBcOffG_ = nullptr, *dBcSrcG_ = nullptr, *dBcOffGt_ = nullptr,
       *dBcSrcGt_ = nullptr;
@@ -204,6 +211,7 @@ private:
   double *dBInG_ = nullptr, *dBOutG_ = nullptr, *dBInGt_ = nullptr,
          *dBOutGt_ = nullptr;
   double *dG_ = nullptr;
+  double *dFields_ = nullptr; // site fields of every frame, [3][Na][N1^2]
   double *dPartials_ = nullptr, *dWork_ = nullptr, *dWork2_ = nullptr;
   std::unique_ptr<PoissonFFT> fft_;
   // Keldysh state
@@ -211,6 +219,8 @@ private:
   double *dLdosAll_ = nullptr, *dLdosAcc_ = nullptr, *dLdosInterval_ = nullptr;
   double *dLdosG_ = nullptr;
   size_t ldosGCap_ = 0;
+  double *dLdosFields_ = nullptr; // site fields of one angle chunk
+  int ldosFieldFrames_ = 0;
   int ldosChunkReq_ = 0;
   int ldosBlockStart_ = 0, ldosLastBlock_ = -1, ldosIterations_ = 0;
   double ldosRes_ = 0, ldosSweepMs_ = 0, ldosIterMs_ = 0;

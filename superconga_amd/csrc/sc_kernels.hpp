@@ -60,6 +60,28 @@ __device__ __forceinline__ void siteFields(const GrainView &v, int k, int x,
 
 __global__ void resetCounter(int *c) { *c = 0; }
 
+// The site fields a sweep reads, for the tiles of frames [k0, ...): each
+// block covers one tile (16 adjacent columns of one frame and the rows a
+// sweep may read there), 16 columns x 16 rows per pass, so that the writes
+// of a row of the tile are contiguous.  F is planar [3][frames][N1^2] with
+// plane stride `plane`; frame k is at (k - k0) * N1^2.
+__global__ void fieldsKernel(GrainView v, const int *tiles, int k0, long long plane,
+                             double *F) {
+  const int *tl = tiles + 4 * size_t(blockIdx.x);
+  const int k = tl[0], x = tl[1] + int(threadIdx.x & 15), y0 = tl[2], y1 = tl[3];
+  const int N1 = v.N1;
+  if (x >= N1) return;
+  const size_t base = size_t(k - k0) * N1 * N1 + x;
+  for (int y = y0 + int(threadIdx.x >> 4); y <= y1; y += 16) {
+    double re, im, es;
+    siteFields(v, k, x, y, re, im, es);
+    const size_t i = base + size_t(y) * N1;
+    F[i] = re;
+    F[plane + i] = im;
+    F[2 * plane + i] = es;
+  }
+}
+
 // One step for one energy: Matsubara energies (i omega) or, with KEL,
 // real-frequency energies E + i delta.
 template <bool KEL, typename T>
@@ -126,9 +148,10 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev
         const int tt = t0 + g;
         if (colActive && tt < len) {
           const int y = it.yhi - tt;
-          const int8_t l0 = lab[size_t(y) * N1];
-          const int8_t l1 = (tt + 1 < len) ? lab[size_t(y - 1) * N1] : int8_t(0);
-          if (l0 != 0 || l1 != 0) siteFields(v, k, x, y, fre, fim, fes);
+          const size_t fi = size_t(k - v.fieldBase) * n1 + size_t(y) * N1 + x;
+          fre = v.fields[fi];
+          fim = v.fields[v.fieldPlane + fi];
+          fes = v.fields[2 * v.fieldPlane + fi];
         }
       }
       const int tEnd = min(gp, len - t0);
@@ -219,9 +242,10 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev
         const int tt = t0 + g;
         if (colActive && tt < len) {
           const int y = it.ylo + tt;
-          const int8_t l0 = lab[size_t(y) * N1];
-          const int8_t l1 = (tt + 1 < len) ? lab[size_t(y + 1) * N1] : int8_t(0);
-          if (l0 != 0 || l1 != 0) siteFields(v, k, x, y, fre, fim, fes);
+          const size_t fi = size_t(k - v.fieldBase) * n1 + size_t(y) * N1 + x;
+          fre = v.fields[fi];
+          fim = v.fields[v.fieldPlane + fi];
+          fes = v.fields[2 * v.fieldPlane + fi];
         }
       }
       const int tEnd = min(gp, len - t0);
@@ -364,46 +388,27 @@ __global__ __launch_bounds__(64, WAVES) void sweepKernel(SweepArgs a) {
   }
 }
 
-// Homogeneous coherence functions at every exit crossing
-// (RiccatiSolverConfined.h:400-468): one thread per (frame, column, energy).
-__global__ void initialBoundaryKernel(GrainView v, const int *xEnd) {
+// Homogeneous coherence functions at the exit crossings of one direction
+// (RiccatiSolverConfined.h:400-468, 673-767): one thread per (crossing,
+// energy); crossing i lies in column col[i] = k * N1 + x at row y[i].  For
+// gamma the out storage takes gamma, for gamma~ it takes gamma~.
+__global__ void initialOutKernel(GrainView v, const int *col, const int *yOf,
+                                 long long nOut, int tilde, double *bOut) {
   const long long tid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  const int N1 = v.N1, Ne = v.Ne;
-  if (tid >= (long long)v.Na * N1 * Ne) return;
-  const int e = int(tid % Ne);
-  const int x = int((tid / Ne) % N1);
-  const int k = int(tid / ((long long)Ne * N1));
-  (void)xEnd;
-  const size_t n1 = size_t(N1) * N1;
-  const int8_t *lab = v.labels + size_t(k) * n1 + x;
-  const size_t col = size_t(k) * N1 + x;
-  int cg = 0, cgt = 0;
-  const int baseG = v.idxOutG[col], baseGt = v.idxOutGt[col];
-  // exits of gamma (label 3) in ascending y and of gamma~ (label 2) in
-  // descending y; the sparse index only depends on the count so a single
-  // ascending pass with a reversed count for gamma~ suffices.
-  int nGt = 0;
-  for (int y = 0; y < N1; ++y) nGt += lab[size_t(y) * N1] == 2;
-  for (int y = 0; y < N1; ++y) {
-    const int8_t l = lab[size_t(y) * N1];
-    if (l != 2 && l != 3) continue;
-    double dre, dim, es;
-    siteFields(v, k, x, y, dre, dim, es);
-    Cx<double> ga, gt;
-    const double ere = v.energyRe ? v.energyRe[e] : 0.0;
-    homogeneous(Cx<double>{dre, dim}, Cx<double>{ere + es, v.energy[e]}, ga, gt);
-    if (l == 3) {
-      const size_t si = size_t(baseG + cg) * Ne + e;
-      v.bOutG[si] = ga.re;
-      v.bOutG[size_t(v.nOutG) * Ne + si] = ga.im;
-      ++cg;
-    } else {
-      const size_t si = size_t(baseGt + (nGt - 1 - cgt)) * Ne + e;
-      v.bOutGt[si] = gt.re;
-      v.bOutGt[size_t(v.nOutGt) * Ne + si] = gt.im;
-      ++cgt;
-    }
-  }
+  const int Ne = v.Ne;
+  if (tid >= nOut * Ne) return;
+  const long long i = tid / Ne;
+  const int e = int(tid - i * Ne);
+  const int c = col[i];
+  const int k = c / v.N1, x = c - k * v.N1, y = yOf[i];
+  double dre, dim, es;
+  siteFields(v, k, x, y, dre, dim, es);
+  Cx<double> ga, gt;
+  const double ere = v.energyRe ? v.energyRe[e] : 0.0;
+  homogeneous(Cx<double>{dre, dim}, Cx<double>{ere + es, v.energy[e]}, ga, gt);
+  const Cx<double> b = tilde ? gt : ga;
+  bOut[tid] = b.re;
+  bOut[nOut * Ne + tid] = b.im;
 }
 
 // Extension of values outwards from the boundary

@@ -102,15 +102,51 @@ void Grain::ldosEnsureBuffers(int precision) {
   if (!dLdosInterval_) dLdosInterval_ = devAlloc<double>(cap * n0);
   const int ca = plan_[precision].ready ? plan_[precision].chunkAngles : ldosChunkAngles();
   const size_t need = size_t(ca) * n1 * cap;
-  if (need > ldosGCap_) {
+  if (need > ldosGCap_ || ca > ldosFieldFrames_) {
     SC_HIP(hipStreamSynchronize(stream_));
     devFree(dLdosG_);
+    devFree(dLdosFields_);
     ldosGCap_ = 0;
+    ldosFieldFrames_ = 0;
     dLdosG_ = devAlloc<double>(need); // cleared: every weight is finite
     ldosGCap_ = need;
-    // the device views hold the old address
+    dLdosFields_ = devAlloc<double>(size_t(3) * size_t(ca) * n1);
+    ldosFieldFrames_ = ca;
+    // the device views hold the old addresses
     plan_[0].ready = plan_[1].ready = false;
   }
+}
+
+void Grain::ldosSweepChunk(int precision, int c, bool burnin) {
+  // site fields of the chunk's frames, then the sweep of its items
+  const Setup &s = *s_;
+  const SweepPlan &pl = plan_[precision];
+  const int k0 = c * pl.chunkAngles, k1 = std::min(s.Na, k0 + pl.chunkAngles);
+  if (k1 - k0 > ldosFieldFrames_)
+    throw std::runtime_error("angle chunk exceeds the site-field buffer");
+  computeFields(k0, k1 - k0, static_cast<long long>(ldosFieldFrames_) * s.N1 * s.N1,
+                dLdosFields_);
+  SweepArgs a;
+  a.views = reinterpret_cast<const GrainView *>(pl.dView);
+  a.items = reinterpret_cast<const SweepItemDev *>(pl.dItems) + pl.chunkStart[size_t(c)];
+  a.numItems = pl.chunkStart[size_t(c) + 1] - pl.chunkStart[size_t(c)];
+  a.counter = dCounter_;
+  a.scratch = dScratch_;
+  a.slotStride = static_cast<long long>(pl.maxLen) * pl.el * 64;
+  a.burnin = burnin ? 1 : 0;
+  ldosResetCounter<<<1, 1, 0, stream_>>>(dCounter_);
+  SC_HIP(hipGetLastError());
+  if (a.numItems > 0) launchSweepVariant(precision, pl.el, pl.waves, a, pl.numSlots, stream_, true);
+}
+
+void Grain::ldosSweepAll(int precision, bool burnin) {
+  // every chunk in turn (burn-in, autotuning)
+  ldosEnsureBuffers(precision);
+  prepareSweep(precision);
+  const int nch = static_cast<int>(plan_[precision].chunkStart.size()) - 1;
+  SC_HIP(hipEventRecord(ev0_, stream_));
+  for (int c = 0; c < nch; ++c) ldosSweepChunk(precision, c, burnin);
+  SC_HIP(hipEventRecord(ev1_, stream_));
 }
 
 void Grain::ldosBeginBlock() {
@@ -164,18 +200,8 @@ void Grain::ldosCompute(int precision, bool burnin) {
       GrainView v = view();
       for (int c = 0; c < nch; ++c) {
         const int k0 = c * pl.chunkAngles, k1 = std::min(s.Na, k0 + pl.chunkAngles);
-        SweepArgs a;
-        a.views = reinterpret_cast<const GrainView *>(pl.dView);
-        a.items = reinterpret_cast<const SweepItemDev *>(pl.dItems) + pl.chunkStart[size_t(c)];
-        a.numItems = pl.chunkStart[size_t(c) + 1] - pl.chunkStart[size_t(c)];
-        a.counter = dCounter_;
-        a.scratch = dScratch_;
-        a.slotStride = static_cast<long long>(pl.maxLen) * pl.el * 64;
-        a.burnin = 0;
-        ldosResetCounter<<<1, 1, 0, stream_>>>(dCounter_);
         SC_HIP(hipEventRecord(ldosEvents_[size_t(2 * c)], stream_));
-        if (a.numItems > 0)
-          launchSweepVariant(precision, pl.el, pl.waves, a, pl.numSlots, stream_, true);
+        ldosSweepChunk(precision, c, false);
         SC_HIP(hipEventRecord(ldosEvents_[size_t(2 * c + 1)], stream_));
         // extendValuesAtBoundary on the chunk's weights, then back-rotation
         extendFrames(dLdosG_, k0, k1, s.Ne, 1, static_cast<long long>(n1) * s.Ne, s.Ne);

@@ -1011,11 +1011,13 @@ std::unique_ptr<Setup> buildSetup(const Config &cfg) {
       for (const Crossing &cr : x.outG[col]) {
         s.distOutG.push_back(std::fabs(cr.dist));
         s.coordG.push_back(cr.y);
+        s.colOutG.push_back(static_cast<int>(q));
       }
       for (const Crossing &cr : x.inGt[col]) s.distInGt.push_back(std::fabs(cr.dist));
       for (const Crossing &cr : x.outGt[col]) {
         s.distOutGt.push_back(std::fabs(cr.dist));
         s.coordGt.push_back(cr.y);
+        s.colOutGt.push_back(static_cast<int>(q));
       }
     }
   s.nInG = cIn;
@@ -1155,6 +1157,26 @@ std::unique_ptr<Setup> buildSetup(const Config &cfg) {
       for (int v : per[k].source) s.extRot.source.push_back(static_cast<int>(v + shift));
     }
   }
+
+  // site-field tiles: per frame, groups of 16 columns from xmin with the
+  // union of the columns' [first - 1, last + 1]
+  s.fieldTileStart.assign(size_t(Na) + 1, 0);
+  for (int k = 0; k < Na; ++k) {
+    s.fieldTileStart[size_t(k)] = static_cast<int>(s.fieldTiles.size() / 4);
+    for (int x0 = s.xmin[k]; x0 <= s.xmax[k]; x0 += 16) {
+      int lo = N1, hi = -1;
+      for (int c = 0; c < 16 && x0 + c < N1; ++c) {
+        const size_t q = size_t(k) * N1 + x0 + c;
+        if (s.colFirst[q] < 0) continue;
+        lo = std::min(lo, s.colFirst[q]);
+        hi = std::max(hi, s.colLast[q]);
+      }
+      if (hi < 0) continue;
+      s.fieldTiles.insert(s.fieldTiles.end(),
+                          {k, x0, std::max(lo - 1, 0), std::min(hi + 1, N1 - 1)});
+    }
+  }
+  s.fieldTileStart[size_t(Na)] = static_cast<int>(s.fieldTiles.size() / 4);
 
   long long inside = 0;
   for (int8_t l : s.labels) inside += l > 0;

@@ -155,10 +155,15 @@ struct Setup {
   // boundary storage (BoundaryStorage.h:298-527)
   std::vector<int> idxInG, idxOutG, idxInGt, idxOutGt, numIsG, numIsGt;
   long long nInG = 0, nOutG = 0, nInGt = 0, nOutGt = 0;
-  std::vector<int> coordG, coordGt;
+  std::vector<int> coordG, coordGt;       // y of every out crossing (storage order)
+  std::vector<int> colOutG, colOutGt;     // k * N1 + column of every out crossing
   std::vector<double> distInG, distOutG, distInGt, distOutGt;
   std::vector<int> xmin, xmax, ymin, ymax;
   std::vector<int> colFirst, colLast; // [Na][N1], -1 where empty
+  // Site-field tiles: 16 adjacent columns of one frame and the rows a sweep
+  // may read there (k, x0, y0, y1 per tile, frames in order);
+  // fieldTileStart[k] is the first tile of frame k.
+  std::vector<int> fieldTiles, fieldTileStart;
   BcTable bcG, bcGt;
   ExtensionList extRef, extRot;
   long long trajSteps = 0; // 2 * Ne * #{sites with label > 0 over all frames}

@@ -31,6 +31,11 @@ struct GrainView {
   double *G;              // [Na][N1^2][4]: Re/Im of f + f~*, Re g, Re i(D* g - D g~);
                           // Keldysh: [angle - angleBase][N1^2][Ne] spectral weight
   int angleBase;          // first angle held in G (Keldysh sweeps by angle chunks)
+  // Site fields the sweep reads (Re D, Im D, Doppler shift), planar
+  // [3][frames][N1^2] from frame fieldBase on, fieldPlane = frames * N1^2.
+  const double *fields;
+  long long fieldPlane;
+  int fieldBase;
   int cols;   // sweep lane layout: columns per item
   int groups; // energy groups per column (ceil(Ne / EL))
 };
