@@ -238,8 +238,8 @@ void Grain::uploadEnergies() {
 // Compiled sweep variants: (energies per lane, wavefronts per SIMD), for
 // Matsubara energies and for real-frequency (Keldysh) energies.
 #define SC_SWEEP_VARIANTS(X)                                                         \
-  X(4, 2) X(4, 4) X(7, 1) X(7, 2) X(7, 4) X(8, 1) X(8, 2) X(8, 4) X(14, 1) X(14, 2)   \
-  X(16, 1) X(16, 2)
+  X(1, 4) X(2, 4) X(4, 2) X(4, 4) X(7, 1) X(7, 2) X(7, 4) X(8, 1) X(8, 2) X(8, 4)      \
+  X(14, 1) X(14, 2) X(16, 1) X(16, 2)
 #define SC_KELDYSH_VARIANTS(X) X(4, 2) X(4, 4) X(8, 1) X(8, 2) X(8, 4) X(16, 1)
 
 bool sweepVariantExists(int el, int waves, bool keldysh) {
@@ -810,8 +810,6 @@ void Grain::autotune(int precision) {
     int gp = 1;
     while (gp < groups) gp <<= 1;
     if (gp > 64) return;
-    // skip layouts that leave more than a quarter of the energy slots idle
-    if (4 * (gp * el - s_->Ne) > gp * el) return;
     setSweepShape(precision, el, w, slots);
     sweep(precision, false);
     updateSweepTime();
