@@ -126,6 +126,45 @@ def test_ldos_bcs_closed_form(product):
     sp.close()
 
 
+@pytest.mark.parametrize("symmetry", ["s", "dx2-y2"])
+def test_reference_dos_known_answer(product, symmetry):
+    # tests/physics/dos_test.cu:32-241 on the product (see test_oracle_ldos.py)
+    from test_oracle_ldos import boole_weight
+    n_angles = 160
+    comps = [{"disc": {"radius": 0.5}}] if symmetry == "s" else [
+        {"regular_polygon": {"num_edges": 4}}]
+    cfg = cases.base(temperature=0.5, angles=n_angles, ppxi=2.0, criterion=1e-8,
+                     norm="linf", width=5.0, components=comps,
+                     order_parameter={symmetry: {"critical_temperature": 1.0}})
+    spec = {"energy_min": 0.0, "energy_max": 0.5, "num_energies": 32,
+            "energy_broadening": 0.008}
+    post = postprocess_config(per_block=-1, criterion=1e-8, angles=n_angles, spec=spec)
+    post["numerics"]["norm"] = "linf"
+    sim = product.Simulation(cfg)  # its initial order parameter is the bulk one
+    delta = sim.order_parameter(0)[0]
+    sim.close()
+    sp = product.Spectroscopy(cfg, post)
+    sp.set_order_parameter(0, delta)
+    assert sp.run(0, 0, 42, "fp64")
+    ldos = sp.ldos()
+    inside = sp.domain() > 0
+    d0 = float(np.max(np.abs(delta)))
+    eps = np.linspace(0.0, 0.5, 32) + 0.008j
+    phi = np.pi * 2 * np.arange(n_angles) / n_angles + 0.5 * np.pi
+    w = np.array([boole_weight(k, n_angles) for k in range(n_angles)]) / n_angles
+    if symmetry == "s":
+        eta = np.ones(n_angles)
+    else:
+        raw = np.cos(phi) ** 2 - np.sin(phi) ** 2
+        eta = raw / np.sqrt(np.sum(w * raw * raw))
+    for e in range(32):
+        omega = np.sqrt(np.abs(d0 * eta) ** 2 - eps[e] ** 2)
+        expected = float(np.sum(w * (eps[e] / omega).imag))
+        got = float(np.sum(ldos[e][inside]) / np.count_nonzero(inside))
+        assert abs(got - expected) <= 1e-8 * abs(expected), (e, got, expected)
+    sp.close()
+
+
 def test_ldos_chunking_and_shapes_agree(product):
     cfg = CASES["dwave_hexagon"]()
     cfg["numerics"]["norm"] = "l2"

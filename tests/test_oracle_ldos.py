@@ -13,6 +13,9 @@ mathematics fixes.
 * Energies never couple: each energy's trajectories, boundary condition and
   fixed point are its own, so the converged LDOS does not depend on how the
   energies are split into blocks (IntegrationIteratorKeldysh.h:152-180).
+* The reference's own DOS test (tests/physics/dos_test.cu): s-wave disc and
+  d-wave square at the bulk gap, grain-averaged LDOS equal to the angle
+  average of the bulk density of states to 1e-8.
 """
 import numpy as np
 import pytest
@@ -109,3 +112,50 @@ def test_ldos_setup_rejects_bad_energies():
     o = uniform_oracle(0.2)
     with pytest.raises(RuntimeError):
         o.ldos_setup(dict(SPEC, num_energies=0), -1)
+
+
+# tests/physics/dos_test.cu:32-241: a grain of width 5 at 2 points per
+# coherence length, 160 angles, 32 energies in [0, 0.5] with broadening
+# 0.008, T = 0.5, L-infinity norm, at most 42 iterations, the order parameter
+# at its bulk value; the grain-averaged LDOS must equal the angle average of
+# the bulk density of states Im(eps / sqrt(|Delta eta|^2 - eps^2)) to 1e-8.
+DOS_SPEC = {"energy_min": 0.0, "energy_max": 0.5, "num_energies": 32,
+            "energy_broadening": 0.008}
+
+
+def boole_weight(k, n):
+    # FermiSurface::getAngleWeight (FermiSurface.h:119-138) for n % 4 == 0
+    assert n % 4 == 0
+    return (28.0 if k % 4 == 0 else 24.0 if k % 2 == 0 else 64.0) / 45.0
+
+
+@pytest.mark.parametrize("symmetry", ["s", "dx2-y2"])
+def test_reference_dos_known_answer(symmetry):
+    n_angles = 160
+    comps = [{"disc": {"radius": 0.5}}] if symmetry == "s" else [
+        {"regular_polygon": {"num_edges": 4}}]
+    cfg = cases.base(temperature=0.5, angles=n_angles, ppxi=2.0, criterion=1e-8,
+                     norm="linf", width=5.0, components=comps,
+                     order_parameter={symmetry: {"critical_temperature": 1.0}})
+    o = po.Oracle(cfg)
+    o.initial_bulk()
+    o.set_vector_potential(np.zeros((o.N0, o.N0)), np.zeros((o.N0, o.N0)))
+    o.ldos_setup(DOS_SPEC, -1)
+    assert o.ldos_run(0, 0, 42)
+    ldos = o.ldos()
+    inside = o.domain() > 0
+    d0 = o.get("delta_bulk_0")
+    eps = np.linspace(0.0, 0.5, 32) + 0.008j
+    phi = np.pi * 2 * np.arange(n_angles) / n_angles + 0.5 * np.pi
+    if symmetry == "s":
+        eta = np.ones(n_angles)
+    else:
+        raw = np.cos(phi) ** 2 - np.sin(phi) ** 2
+        w = np.array([boole_weight(k, n_angles) for k in range(n_angles)]) / n_angles
+        eta = raw / np.sqrt(np.sum(w * raw * raw))  # BasisFunction::normalize
+    w = np.array([boole_weight(k, n_angles) for k in range(n_angles)]) / n_angles
+    for e in range(32):
+        omega = np.sqrt(np.abs(d0 * eta) ** 2 - eps[e] ** 2)
+        expected = float(np.sum(w * (eps[e] / omega).imag))
+        got = float(np.sum(ldos[e][inside]) / np.count_nonzero(inside))
+        assert abs(got - expected) <= 1e-8 * abs(expected), (e, got, expected)
