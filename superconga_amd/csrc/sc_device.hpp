@@ -131,6 +131,21 @@ __device__ __forceinline__ Cx<T> stepEnergy(const StepSite<T> &p, T ser, Cx<T> y
   const Cx<T> sq = csqrt(disc);
   return cdiv(c + c, sq - b);
 }
+// Workgroups are dispatched round robin over the eight XCDs, each with its
+// own L2.  A grid of 8 * ceil(n / 8) blocks read through this mapping gives
+// every XCD a contiguous range of the n logical blocks (spatially coherent
+// sites or tiles share their L2 lines); -1 for the padding blocks.
+constexpr int kNumXcd = 8;
+__device__ __forceinline__ long long xcdBlock(long long nLogical) {
+  const long long per = (nLogical + kNumXcd - 1) / kNumXcd;
+  const long long b = blockIdx.x;
+  const long long l = (b % kNumXcd) * per + b / kNumXcd;
+  return l < nLogical ? l : -1;
+}
+inline unsigned xcdGrid(long long nLogical) {
+  return static_cast<unsigned>(((nLogical + kNumXcd - 1) / kNumXcd) * kNumXcd);
+}
+
 // Real-frequency energies eps = E + i delta: sej = h E joins the site's
 // Im(s e) (the Doppler shift enters the same way).
 template <typename T>

@@ -692,7 +692,8 @@ void Grain::computeFields(int k0, int frames, long long plane, double *F) {
   const int t0 = s.fieldTileStart[size_t(k0)], t1 = s.fieldTileStart[size_t(k0 + frames)];
   if (t1 <= t0) return;
   GrainView v = view();
-  fieldsKernel<<<t1 - t0, 256, 0, stream_>>>(v, dFieldTiles_ + 4 * size_t(t0), k0, plane, F);
+  fieldsKernel<<<xcdGrid(t1 - t0), 256, 0, stream_>>>(v, dFieldTiles_ + 4 * size_t(t0), t1 - t0,
+                                                       k0, plane, F);
   SC_HIP(hipGetLastError());
 }
 
@@ -898,7 +899,7 @@ void Grain::backRotate() {
   GrainView v = view();
   const int ns = static_cast<int>(s.domainSites.size());
   const double jf = 2.0 * static_cast<double>(s.cfg.chargeSign) * s.temperature;
-  gatherKernel<<<blocksFor(ns), 256, 0, stream_>>>(v, dDomainSites_, ns, dCoupling_, jf,
+  gatherKernel<<<xcdGrid(blocksFor(ns)), 256, 0, stream_>>>(v, dDomainSites_, ns, dCoupling_, jf,
                                                    dDeltaNew_, dCurrent_, dEnergyDens_);
   SC_HIP(hipGetLastError());
 }

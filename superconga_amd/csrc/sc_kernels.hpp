@@ -65,9 +65,11 @@ __global__ void resetCounter(int *c) { *c = 0; }
 // sweep may read there), 16 columns x 16 rows per pass, so that the writes
 // of a row of the tile are contiguous.  F is planar [3][frames][N1^2] with
 // plane stride `plane`; frame k is at (k - k0) * N1^2.
-__global__ void fieldsKernel(GrainView v, const int *tiles, int k0, long long plane,
-                             double *F) {
-  const int *tl = tiles + 4 * size_t(blockIdx.x);
+__global__ void fieldsKernel(GrainView v, const int *tiles, int numTiles, int k0,
+                             long long plane, double *F) {
+  const long long tile = xcdBlock(numTiles);
+  if (tile < 0) return;
+  const int *tl = tiles + 4 * size_t(tile);
   const int k = tl[0], x = tl[1] + int(threadIdx.x & 15), y0 = tl[2], y1 = tl[3];
   const int N1 = v.N1;
   if (x >= N1) return;
@@ -448,7 +450,9 @@ __global__ void gatherKernel(GrainView v, const int *sites, int numSites,
                              const double *coupling, double currentFactor,
                              double *deltaOut, double *currentOut,
                              double *energyDensity) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const long long blk = xcdBlock((numSites + blockDim.x - 1) / blockDim.x);
+  if (blk < 0) return;
+  const int t = int(blk * blockDim.x + threadIdx.x);
   if (t >= numSites) return;
   const int N0 = v.N0, N1 = v.N1, Na = v.Na, Nc = v.Nc;
   const int site = sites[t];

@@ -32,9 +32,12 @@ namespace scamd {
 // in increasing order; the first chunk starts the sums.
 __global__ void ldosGatherKernel(GrainView v, const int *sites, int numSites, int k0, int k1,
                                  int first, double *acc) {
-  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   const int Ne = v.Ne;
-  if (t >= (long long)numSites * Ne) return;
+  const long long total = (long long)numSites * Ne;
+  const long long blk = xcdBlock((total + blockDim.x - 1) / blockDim.x);
+  if (blk < 0) return;
+  const long long t = blk * blockDim.x + threadIdx.x;
+  if (t >= total) return;
   const int i = int(t / Ne);
   const int e = int(t - (long long)i * Ne);
   const int site = sites[i];
@@ -208,7 +211,7 @@ void Grain::ldosCompute(int precision, bool burnin) {
         v.angleBase = k0;
         const long long nt = static_cast<long long>(ns) * s.Ne;
         if (nt > 0) {
-          ldosGatherKernel<<<blocksFor(nt), 256, 0, stream_>>>(v, dDomainSites_, ns, k0, k1,
+          ldosGatherKernel<<<xcdGrid(blocksFor(nt)), 256, 0, stream_>>>(v, dDomainSites_, ns, k0, k1,
                                                               c == 0 ? 1 : 0, dLdosAcc_);
           SC_HIP(hipGetLastError());
         }
