@@ -146,3 +146,27 @@ def test_reference_example_configs_plan_like_oracle(product, oracle, name):
     for k in sorted({0, o.Na // 8, o.Na // 3, o.Na - 1}):
         _, lab = product.describe(cfg, angle=k)
         assert np.array_equal(lab, o.geometry(k)[0]), k
+
+
+def test_spectroscopy_config_errors(product):
+    # parseSpectroscopy rejects a postprocess config before touching a device
+    L = product.lib()
+    sim = json.dumps(cases.base(angles=8)).encode()
+    good = {"spectroscopy": {"energy_min": 0.0, "energy_max": 0.5, "num_energies": 8,
+                             "energy_broadening": 0.01},
+            "numerics": {"convergence_criterion": 1e-4, "norm": "l2",
+                         "num_energies_per_block": 4, "num_fermi_momenta": 8}}
+    h = ctypes.c_void_p()
+    for key, sub, bad in (("spectroscopy", "num_energies", 0),
+                          ("spectroscopy", "energy_broadening", 0.0),
+                          ("numerics", "num_energies_per_block", 0),
+                          ("numerics", "num_fermi_momenta", 0),
+                          ("numerics", "norm", "l3")):
+        cfg = json.loads(json.dumps(good))
+        cfg[key][sub] = bad
+        code = L.sc_ldos_create(sim, json.dumps(cfg).encode(), ctypes.byref(h))
+        assert code == 2, (sub, code)  # SC_ERR_CONFIG
+        assert L.sc_last_error()
+    assert L.sc_ldos_create(sim, b"{broken", ctypes.byref(h)) != 0
+    assert L.sc_ldos_compute(None, 0, 0) == 1      # SC_ERR_ARGUMENT: null context
+    assert L.sc_newton_krylov(None, 1, 4, ctypes.c_double(1e-8), None) == 1
