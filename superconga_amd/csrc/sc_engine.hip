@@ -1163,13 +1163,21 @@ void Grain::acceleratorStep() {
       accInit_ = true;
     }
     diffKernel<<<blocksFor(n), 256, 0, stream_>>>(dWork_, G, X, n);
+    // the norms and dot products of every field, read back together
+    std::vector<int> hD(static_cast<size_t>(F)), hM(static_cast<size_t>(F)), hDot(static_cast<size_t>(F));
     for (int f = 0; f < F; ++f) {
       const double *Df = dWork_ + size_t(f) * n0, *Mf = dAccMean_ + size_t(f) * n0;
-      const double d = std::sqrt(deviceSum(Df, nullptr, nullptr, static_cast<long long>(n0), 2));
-      const double m = std::sqrt(deviceSum(Mf, nullptr, nullptr, static_cast<long long>(n0), 2));
+      hD[size_t(f)] = queueSum(Df, nullptr, nullptr, static_cast<long long>(n0), 2);
+      hM[size_t(f)] = queueSum(Mf, nullptr, nullptr, static_cast<long long>(n0), 2);
+      hDot[size_t(f)] = queueSum(Df, Mf, nullptr, static_cast<long long>(n0), 1);
+    }
+    flushReductions();
+    for (int f = 0; f < F; ++f) {
+      const double d = std::sqrt(reductionSum(hD[size_t(f)]));
+      const double m = std::sqrt(reductionSum(hM[size_t(f)]));
       const double den = d * m;
       if (den > 0.0) {
-        const double dot = deviceSum(Df, Mf, nullptr, static_cast<long long>(n0), 1);
+        const double dot = reductionSum(hDot[size_t(f)]);
         const double cs = dot / den;
         accStep_[f] *= std::pow(fac, cs);
         if (cs < minCos) {
@@ -1222,14 +1230,17 @@ void Grain::acceleratorStep() {
     diffKernel<<<blocksFor(n), 256, 0, stream_>>>(dWork_, G, X, n);
     if (accIter_ > 0) {
       diffKernel<<<blocksFor(n), 256, 0, stream_>>>(dWork2_, dWork_, dAccPrev_, n);
-      double dot = 0.0;
+      std::vector<int> hDot(static_cast<size_t>(F));
       for (int f = 0; f < F; ++f)
-        dot += deviceSum(dAccPrev_ + size_t(f) * n0, dWork2_ + size_t(f) * n0, nullptr,
-                         static_cast<long long>(n0), 1);
+        hDot[size_t(f)] = queueSum(dAccPrev_ + size_t(f) * n0, dWork2_ + size_t(f) * n0,
+                                   nullptr, static_cast<long long>(n0), 1);
+      const int hS = (accIter_ % 2 == 0) ? queueSum(dWork2_, nullptr, nullptr, n, 2)
+                                         : queueSum(dAccPrev_, nullptr, nullptr, n, 2);
+      flushReductions();
+      double dot = 0.0;
+      for (int f = 0; f < F; ++f) dot += reductionSum(hDot[size_t(f)]);
       dot = std::fabs(dot);
-      const double sc = (accIter_ % 2 == 0)
-                            ? dot / deviceSum(dWork2_, nullptr, nullptr, n, 2)
-                            : deviceSum(dAccPrev_, nullptr, nullptr, n, 2) / dot;
+      const double sc = (accIter_ % 2 == 0) ? dot / reductionSum(hS) : reductionSum(hS) / dot;
       bbStep_ *= sc;
     }
     bbStep_ = std::clamp(bbStep_, smin, smax);
@@ -1370,9 +1381,19 @@ bool Grain::run(int nBurnin, int itMin, int itMax, int precision) {
 Batch::Batch(std::vector<std::unique_ptr<Setup>> setups) {
   if (setups.empty()) throw std::runtime_error("a batch needs at least one grain");
   SC_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-  for (hipEvent_t *e : {&ev0_, &ev1_, &evIt0_, &evIt1_}) SC_HIP(hipEventCreate(e));
-  for (auto &st : setups) {
-    grains_.push_back(std::make_unique<Grain>(std::move(st), stream_));
+  for (hipEvent_t *e : {&ev0_, &ev1_, &evIt0_, &evIt1_, &evFork_}) SC_HIP(hipEventCreate(e));
+  // three lanes: with the main stream, the four hardware queues a process has
+  const size_t numLanes = std::min<size_t>(3, setups.size());
+  for (size_t i = 0; i < numLanes; ++i) {
+    hipStream_t l = nullptr;
+    SC_HIP(hipStreamCreateWithFlags(&l, hipStreamNonBlocking));
+    lanes_.push_back(l);
+    hipEvent_t e = nullptr;
+    SC_HIP(hipEventCreate(&e));
+    laneEv_.push_back(e);
+  }
+  for (size_t i = 0; i < setups.size(); ++i) {
+    grains_.push_back(std::make_unique<Grain>(std::move(setups[i]), lanes_[i % numLanes]));
     grains_.back()->initialGuess();
   }
   done_.assign(grains_.size(), 0);
@@ -1381,6 +1402,7 @@ Batch::Batch(std::vector<std::unique_ptr<Setup>> setups) {
 
 Batch::~Batch() {
   if (stream_) (void)hipStreamSynchronize(stream_);
+  for (hipStream_t l : lanes_) (void)hipStreamSynchronize(l);
   grains_.clear();
   for (Plan &pl : plan_) {
     if (pl.dItems) (void)hipFree(pl.dItems);
@@ -1388,9 +1410,23 @@ Batch::~Batch() {
   }
   if (dScratch_) (void)hipFree(dScratch_);
   devFree(dCounter_);
-  for (hipEvent_t e : {ev0_, ev1_, evIt0_, evIt1_})
+  for (hipEvent_t e : {ev0_, ev1_, evIt0_, evIt1_, evFork_})
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : laneEv_) (void)hipEventDestroy(e);
+  for (hipStream_t l : lanes_) (void)hipStreamDestroy(l);
   if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Batch::joinLanes() {
+  for (size_t i = 0; i < lanes_.size(); ++i) {
+    SC_HIP(hipEventRecord(laneEv_[i], lanes_[i]));
+    SC_HIP(hipStreamWaitEvent(stream_, laneEv_[i], 0));
+  }
+}
+
+void Batch::forkLanes() {
+  SC_HIP(hipEventRecord(evFork_, stream_));
+  for (hipStream_t l : lanes_) SC_HIP(hipStreamWaitEvent(l, evFork_, 0));
 }
 
 std::vector<int> Batch::active() const {
@@ -1480,6 +1516,8 @@ void Batch::prepare(int precision, const std::vector<int> &act) {
 }
 
 void Batch::sweep(int precision, bool burnin, const std::vector<int> &act) {
+  // the sweep reads what the grains' stages queued on their lanes
+  joinLanes();
   prepare(precision, act);
   const Plan &pl = plan_[precision];
   SweepArgs a;
@@ -1494,6 +1532,8 @@ void Batch::sweep(int precision, bool burnin, const std::vector<int> &act) {
   SC_HIP(hipEventRecord(ev0_, stream_));
   launchSweepVariant(precision, pl.el, pl.waves, a, pl.numSlots, stream_);
   SC_HIP(hipEventRecord(ev1_, stream_));
+  // and the stages after it read what the sweep wrote
+  forkLanes();
 }
 
 void Batch::compute(int precision) {
@@ -1505,6 +1545,7 @@ void Batch::compute(int precision) {
   for (int gi : act) grains_[size_t(gi)]->beginCompute();
   sweep(precision, false, act);
   for (int gi : act) grains_[size_t(gi)]->endCompute();
+  joinLanes();
   SC_HIP(hipEventRecord(evIt1_, stream_));
   SC_HIP(hipEventSynchronize(evIt1_));
   float ms = 0.f;
@@ -1522,6 +1563,7 @@ void Batch::burnin(int precision) {
   for (int gi : act) grains_[size_t(gi)]->beginBurnin();
   sweep(precision, true, act);
   for (int gi : act) grains_[size_t(gi)]->endBurnin();
+  joinLanes();
   SC_HIP(hipStreamSynchronize(stream_));
 }
 

@@ -334,8 +334,17 @@ private:
   };
   void prepare(int precision, const std::vector<int> &act);
   void sweep(int precision, bool burnin, const std::vector<int> &act);
+  // the main stream waits for every lane, every lane for the main stream
+  void joinLanes();
+  void forkLanes();
 
+  // The sweep runs on the main stream; the per-grain stages run on a few
+  // lanes (grain i on lane i % lanes) so that the small kernels of
+  // different grains overlap.
   hipStream_t stream_ = nullptr;
+  std::vector<hipStream_t> lanes_;
+  std::vector<hipEvent_t> laneEv_;
+  hipEvent_t evFork_ = nullptr;
   std::vector<std::unique_ptr<Grain>> grains_;
   std::vector<char> done_;
   Plan plan_[2];
