@@ -81,6 +81,9 @@ PLAN_CASES = {
     "polygon": cases.base(components=[{"polygon": {
         "vertices_x": [-0.5, 0.5, 0.3, -0.4], "vertices_y": [-0.45, -0.5, 0.5, 0.2]}}],
         angles=12, ppxi=4.0, width=12.0),
+    "disc_7_angles": cases.base(angles=7, ppxi=3.0),
+    "square_18_angles": cases.base(angles=18, ppxi=3.0,
+                                   components=[cases.regular_polygon_unit(4)]),
     "two_discs": cases.base(components=[
         {"disc": {"radius": 0.25, "center_x": -0.22, "center_y": 0.0}},
         {"disc": {"radius": 0.25, "center_x": 0.22, "center_y": 0.1}}],

@@ -162,6 +162,17 @@ ITERATION_CASES = {
         order_parameter={"dxy": {"critical_temperature": 1.0}}, temperature=0.4,
         flux=0.5, lam=-1.0, angles=20, ppxi=4.0, gauge="landau",
         components=[cases.regular_polygon_unit(4)], accelerator="anderson", rank=3),
+    # angle counts off the Boole rule (N % 3, N % 2 and plain weights,
+    # FermiSurface.h:119-138) and a high temperature with a low cutoff (two
+    # Ozaki poles)
+    "disc_7_angles_hot": cases.base(temperature=0.95, angles=7, ppxi=3.0, flux=0.5,
+                                    cutoff=3.0),
+    "square_10_angles": cases.base(
+        order_parameter={"dx2-y2": {"critical_temperature": 1.0}}, temperature=0.4,
+        angles=10, ppxi=3.0, rotation=0.1, components=[cases.regular_polygon_unit(4)]),
+    "hexagon_18_angles_field": cases.base(
+        temperature=0.5, angles=18, ppxi=3.0, flux=1.0, lam=6.0,
+        components=[cases.regular_polygon_unit(6)]),
     "polygon_g_solenoid": cases.base(
         order_parameter={"g": {"critical_temperature": 1.0}}, temperature=0.3,
         flux=1.5, lam=3.0, angles=16, ppxi=3.0, gauge="solenoid", width=12.0,
