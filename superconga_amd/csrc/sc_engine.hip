@@ -28,6 +28,7 @@ Grain::Grain(std::unique_ptr<Setup> setup, hipStream_t stream) : s_(std::move(se
   SC_HIP(hipEventCreate(&ev2_));
   SC_HIP(hipEventCreate(&evIterStart_));
   SC_HIP(hipEventCreate(&evIterStop_));
+  SC_HIP(hipEventCreateWithFlags(&evReadback_, hipEventDisableTiming));
   upload();
   // Allocation clears and uploads ran on the null stream, which a
   // non-blocking stream does not wait for.
@@ -110,6 +111,8 @@ Grain::~Grain() {
   if (ev2_) (void)hipEventDestroy(ev2_);
   if (evIterStart_) (void)hipEventDestroy(evIterStart_);
   if (evIterStop_) (void)hipEventDestroy(evIterStop_);
+  if (evReadback_) (void)hipEventDestroy(evReadback_);
+  if (redPinned_) (void)hipHostFree(redPinned_);
   if (stream_ && ownsStream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -552,9 +555,13 @@ double residualFromStats(const double *st, long long numElements, int norm) {
 
 // Deferred reductions.  Each queued reduction writes its block partials to
 // its own range of dRed_; flushReductions() brings every pending range back
-// with one transfer and combines the partials in block order.
+// with one transfer into pinned host memory and combines the partials in
+// block order.  queueReadback() starts that transfer without waiting, so
+// that a batch can start the transfers of all its grains before it waits on
+// the first.
 double *Grain::reserveReduction(int kind, long long blocks, long long n, int &handle) {
   const size_t need = size_t(blocks) * (kind == 0 ? 1 : 6);
+  if (readbackQueued_) flushReductions(); // the queued transfer would miss this one
   if (redUsed_ + need > redCap_) {
     flushReductions(); // keeps the results of everything queued so far
     if (need > redCap_) {
@@ -596,13 +603,30 @@ int Grain::queueStats(const double *oldv, const double *newv, long long n) {
   return h;
 }
 
+void Grain::queueReadback() {
+  if (pend_.empty() || readbackQueued_) return;
+  if (redUsed_ > redPinnedCap_) {
+    if (redPinned_) SC_HIP(hipHostFree(redPinned_));
+    redPinned_ = nullptr;
+    redPinnedCap_ = 0;
+    SC_HIP(hipHostMalloc(reinterpret_cast<void **>(&redPinned_), redCap_ * sizeof(double),
+                         hipHostMallocDefault));
+    redPinnedCap_ = redCap_;
+  }
+  SC_HIP(hipMemcpyAsync(redPinned_, dRed_, redUsed_ * sizeof(double), hipMemcpyDeviceToHost,
+                        stream_));
+  SC_HIP(hipEventRecord(evReadback_, stream_));
+  readbackQueued_ = true;
+}
+
 void Grain::flushReductions() {
   if (pend_.empty()) return;
-  redHost_.resize(redUsed_);
-  fromDevice(redHost_.data(), dRed_, redUsed_ * sizeof(double));
+  queueReadback();
+  SC_HIP(hipEventSynchronize(evReadback_));
+  readbackQueued_ = false;
   for (const Pending &p : pend_) {
     double *out = redOut_[size_t(p.handle)].data();
-    const double *h = redHost_.data() + p.offset;
+    const double *h = redPinned_ + p.offset;
     if (p.kind == 0) {
       double sum = 0.0;
       for (long long b = 0; b < p.blocks; ++b) sum += h[b];
@@ -711,13 +735,13 @@ GrainView Grain::view() const {
 
 GrainView Grain::sweepView(int el) const {
   GrainView v = view();
+  // each column takes one lane per energy group; a wavefront carries as many
+  // whole columns as fit in its 64 lanes (the rest of the lanes idle)
   v.groups = (s_->Ne + el - 1) / el;
-  int gp = 1;
-  while (gp < v.groups) gp <<= 1;
-  if (gp > 64)
+  if (v.groups > 64)
     throw std::runtime_error("too many Ozaki energies for the lane layout; "
                              "raise the energies per lane");
-  v.cols = 64 / gp;
+  v.cols = 64 / v.groups;
   return v;
 }
 
@@ -807,10 +831,7 @@ void Grain::autotune(int precision) {
   int bestEl = keepEl, bestW = keepW, bestSlots = keepSlots;
   const int slotChoices[] = {4, 8, 16};
   auto tryShape = [&](int el, int w, int slots) {
-    const int groups = (s_->Ne + el - 1) / el;
-    int gp = 1;
-    while (gp < groups) gp <<= 1;
-    if (gp > 64) return;
+    if ((s_->Ne + el - 1) / el > 64) return;
     setSweepShape(precision, el, w, slots);
     sweep(precision, false);
     updateSweepTime();
@@ -1019,6 +1040,10 @@ void Grain::resetAccelerator() {
   SC_HIP(hipStreamSynchronize(stream_));
   for (double *p : accX_) devFree(p);
   for (double *p : accD_) devFree(p);
+  // the running mean, previous step and velocity start from zero again
+  devFree(dAccMean_);
+  devFree(dAccPrev_);
+  devFree(dAccVel_);
   accX_.clear();
   accD_.clear();
   accSq_.clear();
@@ -1031,6 +1056,68 @@ void Grain::resetAccelerator() {
 }
 
 void Grain::acceleratorStep() {
+  acceleratorQueue();
+  acceleratorFinish();
+}
+
+void Grain::acceleratorQueue() {
+  // The device work of an accelerator step up to its read-back (the norms
+  // and dot products of CongAcc and Barzilai-Borwein), queued so that a
+  // batch can queue every grain's before it waits on one.
+  const Setup &s = *s_;
+  const size_t n0 = size_t(s.N0) * s.N0;
+  const bool doA = s.cfg.computeCurrent && s.lambda > 0.0;
+  const int F = 2 * s.Nc + (doA ? 2 : 0);
+  const long long n = static_cast<long long>(F) * static_cast<long long>(n0);
+  const double *X = dDelta_;
+  const double *G = dDeltaNew_;
+  accH_.clear();
+  switch (s.cfg.accel) {
+  case ACC_CONGACC: {
+    if (!accInit_) {
+      const double st0 = s.cfg.stepSize > 0 ? s.cfg.stepSize : 0.5;
+      dAccMean_ = devAlloc<double>(size_t(n));
+      accStep_.assign(F, st0);
+      accInertia_.assign(F, 0.0);
+      accInit_ = true;
+    }
+    diffKernel<<<blocksFor(n), 256, 0, stream_>>>(dWork_, G, X, n);
+    SC_HIP(hipGetLastError());
+    // |D_f|^2, |M_f|^2 and D_f . M_f of every field f
+    for (int f = 0; f < F; ++f) {
+      const double *Df = dWork_ + size_t(f) * n0, *Mf = dAccMean_ + size_t(f) * n0;
+      accH_.push_back(queueSum(Df, nullptr, nullptr, static_cast<long long>(n0), 2));
+      accH_.push_back(queueSum(Mf, nullptr, nullptr, static_cast<long long>(n0), 2));
+      accH_.push_back(queueSum(Df, Mf, nullptr, static_cast<long long>(n0), 1));
+    }
+    break;
+  }
+  case ACC_BB: {
+    if (!accInit_) {
+      bbStep_ = s.cfg.stepSize > 0 ? s.cfg.stepSize : 1.0;
+      dAccPrev_ = devAlloc<double>(size_t(n));
+      accInit_ = true;
+    }
+    diffKernel<<<blocksFor(n), 256, 0, stream_>>>(dWork_, G, X, n);
+    SC_HIP(hipGetLastError());
+    if (accIter_ > 0) {
+      diffKernel<<<blocksFor(n), 256, 0, stream_>>>(dWork2_, dWork_, dAccPrev_, n);
+      SC_HIP(hipGetLastError());
+      // the dot product per field, then the squared norm of the step
+      for (int f = 0; f < F; ++f)
+        accH_.push_back(queueSum(dAccPrev_ + size_t(f) * n0, dWork2_ + size_t(f) * n0,
+                                 nullptr, static_cast<long long>(n0), 1));
+      accH_.push_back((accIter_ % 2 == 0) ? queueSum(dWork2_, nullptr, nullptr, n, 2)
+                                          : queueSum(dAccPrev_, nullptr, nullptr, n, 2));
+    }
+    break;
+  }
+  default:
+    break;
+  }
+}
+
+void Grain::acceleratorFinish() {
   const Setup &s = *s_;
   const size_t n0 = size_t(s.N0) * s.N0;
   const bool doA = s.cfg.computeCurrent && s.lambda > 0.0;
@@ -1151,33 +1238,18 @@ void Grain::acceleratorStep() {
   }
   case ACC_CONGACC: {
     // CongAcc.h:271-419
-    const double st0 = s.cfg.stepSize > 0 ? s.cfg.stepSize : 0.5;
     const double fac = s.cfg.stepFactor > 0 ? s.cfg.stepFactor : 1.234;
     const double minCos = s.cfg.cosMin > 0 ? s.cfg.cosMin : 0.7;
     const double smax = s.cfg.stepMax > 0 ? s.cfg.stepMax : 100.0;
     const double smin = s.cfg.stepMin > 0 ? s.cfg.stepMin : 0.001;
-    if (!accInit_) {
-      dAccMean_ = devAlloc<double>(size_t(n));
-      accStep_.assign(F, st0);
-      accInertia_.assign(F, 0.0);
-      accInit_ = true;
-    }
-    diffKernel<<<blocksFor(n), 256, 0, stream_>>>(dWork_, G, X, n);
-    // the norms and dot products of every field, read back together
-    std::vector<int> hD(static_cast<size_t>(F)), hM(static_cast<size_t>(F)), hDot(static_cast<size_t>(F));
-    for (int f = 0; f < F; ++f) {
-      const double *Df = dWork_ + size_t(f) * n0, *Mf = dAccMean_ + size_t(f) * n0;
-      hD[size_t(f)] = queueSum(Df, nullptr, nullptr, static_cast<long long>(n0), 2);
-      hM[size_t(f)] = queueSum(Mf, nullptr, nullptr, static_cast<long long>(n0), 2);
-      hDot[size_t(f)] = queueSum(Df, Mf, nullptr, static_cast<long long>(n0), 1);
-    }
+    if (F > kMaxFields) throw std::runtime_error("too many fields for CongAcc");
     flushReductions();
     for (int f = 0; f < F; ++f) {
-      const double d = std::sqrt(reductionSum(hD[size_t(f)]));
-      const double m = std::sqrt(reductionSum(hM[size_t(f)]));
+      const double d = std::sqrt(reductionSum(accH_[size_t(3 * f)]));
+      const double m = std::sqrt(reductionSum(accH_[size_t(3 * f + 1)]));
       const double den = d * m;
       if (den > 0.0) {
-        const double dot = reductionSum(hDot[size_t(f)]);
+        const double dot = reductionSum(accH_[size_t(3 * f + 2)]);
         const double cs = dot / den;
         accStep_[f] *= std::pow(fac, cs);
         if (cs < minCos) {
@@ -1208,37 +1280,25 @@ void Grain::acceleratorStep() {
       }
       accStep_[f] = std::clamp(accStep_[f], smin, smax);
     }
-    double *stepDev = dWork2_, *inDev = dWork2_ + F;
-    SC_HIP(hipMemcpyAsync(stepDev, accStep_.data(), F * sizeof(double),
-                          hipMemcpyHostToDevice, stream_));
-    SC_HIP(hipMemcpyAsync(inDev, accInertia_.data(), F * sizeof(double),
-                          hipMemcpyHostToDevice, stream_));
-    congaccKernel<<<blocksFor(n), 256, 0, stream_>>>(X, dAccMean_, dWork_, stepDev, inDev,
+    // the per-field step and inertia travel in the kernel arguments
+    FieldCoefficients coef{};
+    for (int f = 0; f < F; ++f) {
+      coef.step[f] = accStep_[size_t(f)];
+      coef.inertia[f] = accInertia_[size_t(f)];
+    }
+    congaccKernel<<<blocksFor(n), 256, 0, stream_>>>(X, dAccMean_, dWork_, coef,
                                                      static_cast<long long>(n0), n);
-    SC_HIP(hipStreamSynchronize(stream_));
     break;
   }
   case ACC_BB: {
     // BarzilaiBorwein.h:209-243
     const double smax = s.cfg.stepMax > 0 ? s.cfg.stepMax : 100.0;
     const double smin = s.cfg.stepMin > 0 ? s.cfg.stepMin : 0.001;
-    if (!accInit_) {
-      bbStep_ = s.cfg.stepSize > 0 ? s.cfg.stepSize : 1.0;
-      dAccPrev_ = devAlloc<double>(size_t(n));
-      accInit_ = true;
-    }
-    diffKernel<<<blocksFor(n), 256, 0, stream_>>>(dWork_, G, X, n);
     if (accIter_ > 0) {
-      diffKernel<<<blocksFor(n), 256, 0, stream_>>>(dWork2_, dWork_, dAccPrev_, n);
-      std::vector<int> hDot(static_cast<size_t>(F));
-      for (int f = 0; f < F; ++f)
-        hDot[size_t(f)] = queueSum(dAccPrev_ + size_t(f) * n0, dWork2_ + size_t(f) * n0,
-                                   nullptr, static_cast<long long>(n0), 1);
-      const int hS = (accIter_ % 2 == 0) ? queueSum(dWork2_, nullptr, nullptr, n, 2)
-                                         : queueSum(dAccPrev_, nullptr, nullptr, n, 2);
       flushReductions();
+      const int hS = accH_[size_t(F)];
       double dot = 0.0;
-      for (int f = 0; f < F; ++f) dot += reductionSum(hDot[size_t(f)]);
+      for (int f = 0; f < F; ++f) dot += reductionSum(accH_[size_t(f)]);
       dot = std::fabs(dot);
       const double sc = (accIter_ % 2 == 0) ? dot / reductionSum(hS) : reductionSum(hS) / dot;
       bbStep_ *= sc;
@@ -1266,30 +1326,52 @@ void Grain::acceleratorStep() {
 }
 
 void Grain::beginCompute() {
-  // Context::compute (Context.h:391-447) up to the Riccati sweeps
+  queueBeginCompute();
+  finishBeginCompute();
+}
+
+void Grain::queueBeginCompute() {
+  // Context::compute (Context.h:391-447) up to the Riccati sweeps: the
+  // accelerator's device reductions
   if (s_->keldysh) throw std::runtime_error("a spectroscopy context only runs the LDOS iteration");
-  SC_HIP(hipEventRecord(evIterStart_, stream_));
   clearReductions();
-  if (!firstCompute_) acceleratorStep();
+  SC_HIP(hipEventRecord(evIterStart_, stream_));
+  if (!firstCompute_) acceleratorQueue();
+}
+
+void Grain::finishBeginCompute() {
+  // the accelerator's update once its reductions are back, then the
+  // extended order parameter and the site fields
+  if (!firstCompute_) acceleratorFinish();
   ensureBoundaryInit();
   extendDelta();
 }
 
-void Grain::endCompute() {
-  // the rest of Context::compute after the sweeps
+void Grain::queueEndCompute() {
+  // the rest of Context::compute after the sweeps: device work only
   const Setup &s = *s_;
   const size_t n0 = size_t(s.N0) * s.N0;
-  const bool doA = s.cfg.computeCurrent && s.lambda > 0.0;
   afterSweep(false);
-  int hA = -1;
-  if (doA) {
+  hA_ = -1;
+  if (s.cfg.computeCurrent && s.lambda > 0.0) {
     solveVectorPotential(dCurrent_, dANew_);
-    hA = queueStats(dA_, dANew_, static_cast<long long>(2 * n0));
+    hA_ = queueStats(dA_, dANew_, static_cast<long long>(2 * n0));
   }
   SC_HIP(hipEventRecord(evIterStop_, stream_));
+}
+
+void Grain::endCompute() {
+  queueEndCompute();
+  finishEndCompute();
+}
+
+void Grain::finishEndCompute() {
+  // the read-back of the iteration's reductions and the residuals
+  const Setup &s = *s_;
+  const bool doA = s.cfg.computeCurrent && s.lambda > 0.0;
   flushReductions();
   finishIntegrate();
-  if (doA) res_.resVectorPotential = reductionResidual(hA);
+  if (doA) res_.resVectorPotential = reductionResidual(hA_);
   res_.iterations++;
   firstCompute_ = false;
   // Context::getResidual / isPartiallyConverged (Context.h:462-497)
@@ -1323,7 +1405,13 @@ void Grain::beginBurnin() {
 }
 
 void Grain::endBurnin() {
-  afterSweep(true);
+  queueEndBurnin();
+  finishEndBurnin();
+}
+
+void Grain::queueEndBurnin() { afterSweep(true); }
+
+void Grain::finishEndBurnin() {
   flushReductions();
   finishIntegrate();
   res_.iterations++;
@@ -1542,9 +1630,15 @@ void Batch::compute(int precision) {
   const std::vector<int> act = active();
   if (act.empty()) return;
   SC_HIP(hipEventRecord(evIt0_, stream_));
-  for (int gi : act) grains_[size_t(gi)]->beginCompute();
+  for (int gi : act) grains_[size_t(gi)]->queueBeginCompute();
+  for (int gi : act) grains_[size_t(gi)]->queueReadback();
+  for (int gi : act) grains_[size_t(gi)]->finishBeginCompute();
   sweep(precision, false, act);
-  for (int gi : act) grains_[size_t(gi)]->endCompute();
+  // every grain's stages are queued before the first read-back waits, so
+  // the lanes run them side by side
+  for (int gi : act) grains_[size_t(gi)]->queueEndCompute();
+  for (int gi : act) grains_[size_t(gi)]->queueReadback();
+  for (int gi : act) grains_[size_t(gi)]->finishEndCompute();
   joinLanes();
   SC_HIP(hipEventRecord(evIt1_, stream_));
   SC_HIP(hipEventSynchronize(evIt1_));
@@ -1562,7 +1656,9 @@ void Batch::burnin(int precision) {
   if (act.empty()) return;
   for (int gi : act) grains_[size_t(gi)]->beginBurnin();
   sweep(precision, true, act);
-  for (int gi : act) grains_[size_t(gi)]->endBurnin();
+  for (int gi : act) grains_[size_t(gi)]->queueEndBurnin();
+  for (int gi : act) grains_[size_t(gi)]->queueReadback();
+  for (int gi : act) grains_[size_t(gi)]->finishEndBurnin();
   joinLanes();
   SC_HIP(hipStreamSynchronize(stream_));
 }
@@ -1603,9 +1699,7 @@ void Batch::autotune(int precision) {
       bestSlots = plan_[precision].slotsPerCU;
   const int slotChoices[] = {4, 8, 16};
   auto tryShape = [&](int el, int w, int slots) {
-    int gp = 1;
-    while (gp < (maxNe + el - 1) / el) gp <<= 1;
-    if (gp > 64) return;
+    if ((maxNe + el - 1) / el > 64) return;
     setSweepShape(precision, el, w, slots);
     sweep(precision, false, act);
     SC_HIP(hipEventSynchronize(ev1_));

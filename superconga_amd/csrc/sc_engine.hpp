@@ -78,10 +78,20 @@ public:
   void burnin(int precision);
   // The same iteration in two halves around the sweep, for callers that
   // sweep several grains in one launch: beginCompute(); <sweep>; endCompute().
+  // endCompute = queueEndCompute (device work) + finishEndCompute (read-back),
+  // so that a batch can queue every grain before it waits on any.
   void beginCompute();
+  void queueBeginCompute();
+  void finishBeginCompute();
   void endCompute();
+  void queueEndCompute();
+  void finishEndCompute();
   void beginBurnin();
   void endBurnin();
+  void queueEndBurnin();
+  void finishEndBurnin();
+  // starts the transfer of the queued reductions (flushReductions waits)
+  void queueReadback();
   bool run(int burnin, int itMin, int itMax, int precision);
   // individual stages
   // Reads the HIP-event time of the last sweep launch into the timing.
@@ -164,6 +174,8 @@ private:
   void backRotate();
   void finishIntegrate();
   void acceleratorStep();
+  void acceleratorQueue();  // device work up to the read-back
+  void acceleratorFinish(); // the update, once the reductions are back
   double residualOf(const double *oldv, const double *newv, long long n);
   double deviceSum(const double *a, const double *b, const uint8_t *mask,
                    long long n, int op);
@@ -248,7 +260,10 @@ private:
   std::vector<Pending> pend_;
   std::vector<std::array<double, 6>> redOut_;
   std::vector<long long> redN_;
-  std::vector<double> redHost_;
+  double *redPinned_ = nullptr; // pinned host copy of dRed_
+  size_t redPinnedCap_ = 0;
+  bool readbackQueued_ = false;
+  hipEvent_t evReadback_ = nullptr;
   double *dRed_ = nullptr;
   size_t redCap_ = 0, redUsed_ = 0;
   double *dMean_ = nullptr; // grain means of the current (Poisson solve)
@@ -256,6 +271,7 @@ private:
   std::vector<int> hDelta_, hDeltaSq_;
   int hCurrent_ = -1, hEnergy_ = -1, hCurl_ = -1, hBoundary_[2] = {-1, -1};
   bool pendingIntegrate_ = false;
+  int hA_ = -1; // queued residual of the vector potential
   long long partialsCap_ = 0;
   // sweep plans, one per precision
   struct SweepPlan {
@@ -281,6 +297,7 @@ private:
   std::vector<long long> accIdx_;
   double *dAccMean_ = nullptr, *dAccPrev_ = nullptr, *dAccVel_ = nullptr;
   std::vector<double> accStep_, accInertia_;
+  std::vector<int> accH_; // reductions queued by acceleratorQueue
   double bbStep_ = 0;
   bool accInit_ = false;
   // bookkeeping

@@ -93,8 +93,9 @@ __device__ __forceinline__ Cx<T> stepE(const StepSite<T> &p, T ser, T sej, Cx<T>
 }
 
 // The Riccati sweep.  One wavefront handles an item: `cols` adjacent columns
-// of one frame; each column is carried by 64 / cols lanes, of which `groups`
-// hold EL Ozaki energies each.  Pass 1 integrates gamma~ from the top of the
+// of one frame; each column is carried by `groups` lanes holding EL Ozaki
+// energies each (lane = group * cols + column; the 64 - groups * cols lanes
+// past the last group idle).  Pass 1 integrates gamma~ from the top of the
 // item to the bottom and keeps it in the slot's scratch; pass 2 integrates
 // gamma upwards and forms the energy-summed Green functions at every inside
 // site.  Boundary in-values are read and out-values written as in
@@ -109,7 +110,7 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev
   const GrainView &v = a.views[it.grain];
   const int lane = threadIdx.x;
   const int cols = v.cols;
-  const int gp = 64 / cols; // lanes per column
+  const int gp = v.groups; // lanes per column
   const int g = lane / cols;
   const int c = lane - g * cols;
   const int N1 = v.N1, Ne = v.Ne;
@@ -148,7 +149,7 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev
       double fre = 0.0, fim = 0.0, fes = 0.0;
       {
         const int tt = t0 + g;
-        if (colActive && tt < len) {
+        if (colActive && g < gp && tt < len) {
           const int y = it.yhi - tt;
           const size_t fi = size_t(k - v.fieldBase) * n1 + size_t(y) * N1 + x;
           fre = v.fields[fi];
@@ -242,7 +243,7 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev
       double fre = 0.0, fim = 0.0, fes = 0.0;
       {
         const int tt = t0 + g;
-        if (colActive && tt < len) {
+        if (colActive && g < gp && tt < len) {
           const int y = it.ylo + tt;
           const size_t fi = size_t(k - v.fieldBase) * n1 + size_t(y) * N1 + x;
           fre = v.fields[fi];
@@ -351,11 +352,24 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev
           }
           double gf = dre * su + dim * sw;
           // sum the energy groups of this column (lanes c, c + cols, ...)
-          for (int m = cols; m < 64; m <<= 1) {
-            gdr += __shfl_xor(gdr, m);
-            gdi += __shfl_xor(gdi, m);
-            gj += __shfl_xor(gj, m);
-            gf += __shfl_xor(gf, m);
+          // into lane c: a butterfly when the groups fill the wavefront,
+          // else the other groups' lanes read in turn
+          if (gp * cols == 64) {
+            for (int m = cols; m < 64; m <<= 1) {
+              gdr += __shfl_xor(gdr, m);
+              gdi += __shfl_xor(gdi, m);
+              gj += __shfl_xor(gj, m);
+              gf += __shfl_xor(gf, m);
+            }
+          } else {
+            const double g0r = gdr, g0i = gdi, g0j = gj, g0f = gf;
+            for (int m = 1; m < gp; ++m) {
+              const int src = c + m * cols;
+              gdr += __shfl(g0r, src);
+              gdi += __shfl(g0i, src);
+              gj += __shfl(g0j, src);
+              gf += __shfl(g0f, src);
+            }
           }
           if (g == 0 && colActive) {
             double2 *Gk = reinterpret_cast<double2 *>(
@@ -641,16 +655,22 @@ __global__ void andersonCombine(double *x, AndersonTerms t, int m, long long n, 
   for (int q = 0; q < m; ++q) acc += (t.X[q][i] + t.D[q][i] * s) * t.w[q];
   x[i] = acc;
 }
-// running mean and step per field (CongAcc.h:271-297)
-__global__ void congaccKernel(double *x, double *mean, const double *d,
-                              const double *step, const double *inertia,
+// running mean and step per field (CongAcc.h:271-297); the per-field step
+// and inertia travel by value (at most 4 order-parameter components and the
+// vector potential: 10 fields)
+constexpr int kMaxFields = 16;
+struct FieldCoefficients {
+  double step[kMaxFields];
+  double inertia[kMaxFields];
+};
+__global__ void congaccKernel(double *x, double *mean, const double *d, FieldCoefficients c,
                               long long fieldSize, long long n) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int f = int(i / fieldSize);
-  const double m = mean[i] * inertia[f] + d[i] * (1.0 - inertia[f]);
+  const double m = mean[i] * c.inertia[f] + d[i] * (1.0 - c.inertia[f]);
   mean[i] = m;
-  x[i] += step[f] * m;
+  x[i] += c.step[f] * m;
 }
 // x += d * s (BarzilaiBorwein.h:237-240)
 __global__ void axpyKernel(double *x, const double *d, long long n, double s) {

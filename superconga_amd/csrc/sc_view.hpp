@@ -36,8 +36,8 @@ struct GrainView {
   const double *fields;
   long long fieldPlane;
   int fieldBase;
-  int cols;   // sweep lane layout: columns per item
-  int groups; // energy groups per column (ceil(Ne / EL))
+  int cols;   // sweep lane layout: columns per item (64 / groups)
+  int groups; // energy groups per column (ceil(Ne / EL)), one lane each
 };
 
 struct SweepItemDev {

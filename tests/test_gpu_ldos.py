@@ -243,3 +243,25 @@ def test_postprocess_drop_in_writes_reference_files(product, tmp_path):
     conv = (pp / "postprocess_convergence.csv").read_text().splitlines()
     assert conv[0] == "iteration,boundary_coherence_residual,ldos_residual"
     assert len(conv) > 2
+
+
+def test_ldos_short_last_block_matches_oracle(product, oracle):
+    # 7 energies in blocks of 3: the last block has one energy, so the lane
+    # layout and the boundary storage change between blocks.  The block's
+    # residual is taken over its own energies in both codes.
+    cfg = CASES["dwave_hexagon"]()
+    cfg["numerics"]["norm"] = "l2"
+    delta, a = converged_state(product, cfg, iterations=3)
+    spec = dict(SPEC, num_energies=7)
+    post = postprocess_config(per_block=3, spec=spec)
+    sp, o = ldos_pair(product, oracle, cfg, post, delta, a)
+    conv = sp.run(1, 0, 200, "fp64")
+    oconv = o.ldos_run(1, 0, 200)
+    st = sp.status()
+    assert conv and oconv
+    assert st["iterations"] == o.get_int("num_iterations")
+    assert st["block_count"] == 1
+    e = rel(sp.ldos(), o.ldos())
+    print("short last block: %d iterations, rel err %.2e" % (st["iterations"], e))
+    assert e < TOL
+    sp.close()

@@ -206,9 +206,13 @@ def test_burnin_then_iterations(product, oracle):
 
 
 def test_sweep_shapes_agree(product, oracle):
+    # 10 Ozaki energies: 1, 2 and 4 energies per lane give 10, 5 and 3
+    # groups per column (lanes past the last whole column idle, the groups
+    # summed lane by lane), 7 and 8 give two groups, 14 and 16 one
     cfg = cases.mixed_small()
     out = []
-    for el, waves, slots in ((4, 4, 16), (7, 2, 8), (8, 1, 4), (14, 2, 8), (16, 1, 2)):
+    for el, waves, slots in ((1, 4, 16), (2, 4, 16), (4, 4, 16), (7, 2, 8), (8, 1, 4),
+                             (14, 2, 8), (16, 1, 2)):
         s, o = pair(product, oracle, cfg)
         s.set_sweep_shape(el, waves, slots)
         s.compute("fp64")
