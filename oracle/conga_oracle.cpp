@@ -1653,6 +1653,266 @@ static void integrate(Oracle &o, bool burnin) {
 }
 
 // ---------------------------------------------------------------------------
+// Discrete adjoint of the free energy (no counterpart in the reference).
+// Omega(Delta) is the free energy one iteration reports (ComputeFreeEnergy.h:
+// 281-414 on the Green functions of the sweep), with the vector potential and
+// the boundary in-values held fixed.  Reverse mode, written from the
+// mathematics of each forward stage: the back-rotation and extension of the
+// free-energy density, the Green sums, the two Riccati recurrences, and the
+// rotation and extension of the order parameter.  For a complex variable z
+// the adjoint is zbar = dOmega/dRe z + i dOmega/dIm z; for y = f(x) with f
+// holomorphic, xbar = conj(f'(x)) ybar.
+// ---------------------------------------------------------------------------
+
+// Adjoint of riccatiStep(delta, energy, y0, h).  Its result y is the root of
+// F(y) = a (y + y0)^2 + (se - 1) y + (se + 1) y0 + s delta = 0 with s = i h,
+// a = s conj(delta) / 4 and se = s energy, so by implicit differentiation
+// dy = -(F_y0 dy0 + F_a da + F_sd d(s delta)) / F_y with
+// F_y = 2 a (y + y0) + se - 1, F_y0 = 2 a (y + y0) + se + 1, F_a = (y + y0)^2.
+// Returns y0bar and adds the adjoint of delta to dbar.
+static cplx riccatiStepAdjoint(cplx delta, cplx energy, cplx y0, cplx y, double h,
+                               cplx ybar, cplx &dbar) {
+  const cplx s(0.0, h);
+  const cplx a = 0.25 * cmul(s, std::conj(delta));
+  const cplx se = cmul(s, energy);
+  const cplx w = y + y0;
+  const cplx aw2 = 2.0 * cmul(a, w);
+  const cplx q = -cdiv(cplx(1.0, 0.0), aw2 + se - 1.0);
+  const cplx abar = cmul(std::conj(cmul(q, cmul(w, w))), ybar);
+  const cplx sdbar = cmul(std::conj(q), ybar);
+  // a = (s / 4) conj(delta): deltabar += conj(abar) s / 4; s delta: conj(s) sdbar
+  dbar += cmul(std::conj(abar), 0.25 * s) + cmul(std::conj(s), sdbar);
+  return cmul(std::conj(cmul(q, aw2 + se + 1.0)), ybar);
+}
+
+// Adjoint of insertRotated: the adjoint of every written destination site is
+// scattered to its four source taps with the bilinear weights.
+static void insertRotatedAdjoint(const double *dstbar, int Nx, int Ny, double *srcbar, int Nxo,
+                                 int Nyo, float phi) {
+  const double xt = (Nxo - Nx) / 2.0, yt = (Nyo - Ny) / 2.0;
+  const double sx = (Nxo - 1) / 2.0, sy = (Nyo - 1) / 2.0;
+  const double c = static_cast<double>(std::cos(-phi));
+  const double s = static_cast<double>(std::sin(-phi));
+  for (int y = 0; y < Ny; ++y)
+    for (int x = 0; x < Nx; ++x) {
+      const double v = dstbar[size_t(y) * Nx + x];
+      if (v == 0.0) continue;
+      const double X = x + xt - sx;
+      const double Y = y + yt - sy;
+      const double Xr = X * c - Y * s + sx;
+      const double Yr = X * s + Y * c + sy;
+      const int Xi = static_cast<int>(Xr), Yi = static_cast<int>(Yr);
+      if (Xi >= 0 && Xi + 1 < Nxo && Yi >= 0 && Yi + 1 < Nyo) {
+        const double fx = Xr - (double)Xi, fy = Yr - (double)Yi;
+        srcbar[size_t(Yi) * Nxo + Xi] += (1.0 - fy) * (1.0 - fx) * v;
+        srcbar[size_t(Yi) * Nxo + Xi + 1] += (1.0 - fy) * fx * v;
+        srcbar[size_t(Yi + 1) * Nxo + Xi] += fy * (1.0 - fx) * v;
+        srcbar[size_t(Yi + 1) * Nxo + Xi + 1] += fy * fx * v;
+      }
+    }
+}
+
+// Adjoint of extendValues: a label-0 site that took the mean of its nearest
+// inside sites hands its adjoint to them in equal parts and keeps none.
+static void extendValuesAdjoint(const signed char *labels, int N, double *bar) {
+  const int ext = 2;
+  for (int y = 0; y < N; ++y)
+    for (int x = 0; x < N; ++x) {
+      if (labels[size_t(y) * N + x] != 0) continue;
+      const int x0 = std::max(x - ext, 0), x1 = std::min(x + ext, N - 1);
+      const int y0 = std::max(y - ext, 0), y1 = std::min(y + ext, N - 1);
+      int best = 1000, np = 0;
+      for (int ky = y0; ky <= y1; ++ky)
+        for (int kx = x0; kx <= x1; ++kx)
+          if (labels[size_t(ky) * N + kx] > 0) {
+            const int dx = x - kx, dy = y - ky, d2 = dx * dx + dy * dy;
+            if (d2 < best) {
+              best = d2;
+              np = 1;
+            } else if (d2 == best) {
+              np++;
+            }
+          }
+      if (np == 0) continue;
+      const double v = bar[size_t(y) * N + x] / static_cast<double>(np);
+      bar[size_t(y) * N + x] = 0.0;
+      for (int ky = y0; ky <= y1; ++ky)
+        for (int kx = x0; kx <= x1; ++kx)
+          if (labels[size_t(ky) * N + kx] > 0) {
+            const int dx = x - kx, dy = y - ky;
+            if (dx * dx + dy * dy == best) bar[size_t(ky) * N + kx] += v;
+          }
+    }
+}
+
+// Reverse sweep of one column (x) and energy (e) of frame k for one
+// coherence function: grid holds the forward values at every site of the
+// column, rho the adjoint seed of the free-energy density per frame site.
+// Adds the adjoint of the frame's order parameter to dbar (planar complex).
+template <bool GAMMA>
+static void reverseColumn(const Oracle &o, int k, int x, int e, const std::vector<double> &drot,
+                          const std::vector<double> &es, const std::vector<double> &rho,
+                          std::vector<double> &dbar) {
+  const int N1 = o.N1, Ne = o.Ne;
+  const size_t n1 = o.n1sq();
+  const FrameGeometry &g = o.frames[k];
+  const int ENTRY = GAMMA ? 2 : 3, EXIT = GAMMA ? 3 : 2;
+  const int ya = o.ymin[k], yb = o.ymax[k], H = yb - ya + 1;
+  const double step = o.h;
+  const double *grid = (GAMMA ? o.gamma.data() : o.gammat.data()) + size_t(e) * 2 * n1;
+  const int *idxIn = GAMMA ? o.idxInG.data() : o.idxInGt.data();
+  const double *bin = GAMMA ? o.bInG.data() : o.bInGt.data();
+  const long long nin = GAMMA ? o.nInG : o.nInGt;
+  const cplx e0(o.energyRe[e], o.energyIm[e]);
+  auto siteOf = [&](int i) { return size_t(GAMMA ? ya + i : yb - i) * N1 + x; };
+  auto dlOf = [&](size_t idx) {
+    return GAMMA ? cplx(drot[idx], drot[n1 + idx]) : cplx(-drot[idx], drot[n1 + idx]);
+  };
+  // entry counts in forward order (the sparse index of each entry)
+  std::vector<int> entryCount(size_t(H), 0);
+  for (int i = 0, cnt = 0; i < H; ++i) {
+    entryCount[size_t(i)] = cnt;
+    if (g.label[siteOf(i)] == ENTRY) ++cnt;
+  }
+  // the adjoint of the argument dl of a step, returned to the true order
+  // parameter D (dl = D for gamma, dl = -conj(D) for gamma~)
+  auto addDbar = [&](size_t idx, cplx b) {
+    const cplx d = GAMMA ? b : -std::conj(b);
+    dbar[idx] += d.real();
+    dbar[n1 + idx] += d.imag();
+  };
+  cplx carry(0.0, 0.0);
+  for (int i = H - 1; i >= 0; --i) {
+    const size_t idx = siteOf(i);
+    const cplx D(drot[idx], drot[n1 + idx]);
+    // dOmega/dgamma at the site through the free-energy density
+    // gF = sum_e r_e [D.re (gt.im - g.im) + D.im (g.re + gt.re)]
+    const double rr = rho[idx] * o.residue[e];
+    const cplx local = GAMMA ? cplx(rr * D.imag(), -rr * D.real())
+                             : cplx(rr * D.imag(), rr * D.real());
+    const cplx ybar = carry + local;
+    const signed char lab = g.label[idx];
+    const cplx y(grid[idx], grid[n1 + idx]);
+    const cplx en = e0 + es[idx];
+    if (lab == 1 || lab == EXIT) {
+      cplx y0(0.0, 0.0), dprev(0.0, 0.0), eprev(0.0, 0.0);
+      size_t pidx = 0;
+      if (i > 0) {
+        pidx = siteOf(i - 1);
+        y0 = cplx(grid[pidx], grid[n1 + pidx]);
+        dprev = dlOf(pidx);
+        eprev = e0 + es[pidx];
+      }
+      cplx db(0.0, 0.0);
+      carry = riccatiStepAdjoint((dprev + dlOf(idx)) * 0.5, (eprev + en) * 0.5, y0, y, step, ybar,
+                                 db);
+      addDbar(idx, 0.5 * db);
+      if (i > 0) addDbar(pidx, 0.5 * db);
+    } else if (lab == ENTRY) {
+      const size_t sp = sparseIdx(idxIn, N1, k, x, entryCount[size_t(i)], Ne, e);
+      const cplx bc(bin[sp], bin[size_t(nin * Ne) + sp]);
+      const double d = std::fabs(g.dist[idx]);
+      cplx db(0.0, 0.0);
+      (void)riccatiStepAdjoint(dlOf(idx), en, bc, y, d * step, ybar, db);
+      addDbar(idx, db);
+      carry = cplx(0.0, 0.0);
+    } else {
+      carry = cplx(0.0, 0.0);
+    }
+  }
+}
+
+// Omega and dOmega/dRe Delta_c, dOmega/dIm Delta_c (grad: component c at
+// [c * 2 N0^2, re plane then im plane]).  Leaves the state as it was (the
+// out-values the sweeps write are restored).
+static double freeEnergyGradient(Oracle &o, std::vector<double> &grad) {
+  if (!o.boundaryInit) setBoundaryInitialCondition(o);
+  const std::vector<double> keepG = o.bOutG, keepGt = o.bOutGt;
+  const size_t n0 = o.n0sq(), n1 = o.n1sq();
+  const int N1 = o.N1, Ne = o.Ne;
+  const double area = o.h * o.h;
+  grad.assign(size_t(o.Nc) * 2 * n0, 0.0);
+  std::vector<double> drot, es, gD, gJ, gF, fe(n0, 0.0), rho0(n0), rho(n1), dbar(2 * n1),
+      sumbar(2 * n0);
+  for (int k = 0; k < o.Na; ++k) {
+    rotatedOrderParameter(o, k, drot);
+    energyShift(o, k, es);
+    riccatiAngle(o, k, drot, es, true);
+    greensAngle(o, drot, gD, gJ, gF);
+    const signed char *lab = o.frames[k].label.data();
+    const float phi = static_cast<float>(o.systemAngle[k]);
+    const double wk = angleWeight(k, o.Na) / static_cast<double>(o.Na);
+    // forward: the free-energy density of this angle on the reference grid
+    extendValues(lab, N1, gF.data());
+    for (size_t i = 0; i < n1; ++i) gF[i] *= wk;
+    insertRotated(fe.data(), o.N0, o.N0, gF.data(), N1, N1, phi, true);
+    // adjoint seed: Omega = area T sum over the inside sites of fe
+    for (size_t i = 0; i < n0; ++i) rho0[i] = o.domainRef[i] != 0 ? area * o.T * wk : 0.0;
+    std::fill(rho.begin(), rho.end(), 0.0);
+    insertRotatedAdjoint(rho0.data(), o.N0, o.N0, rho.data(), N1, N1, phi);
+    extendValuesAdjoint(lab, N1, rho.data());
+    // the Green sums' own dependence on D, then the two recurrences
+    std::fill(dbar.begin(), dbar.end(), 0.0);
+    for (size_t i = 0; i < n1; ++i) {
+      if (rho[i] == 0.0) continue;
+      double su = 0.0, sw = 0.0;
+      for (int e = 0; e < Ne; ++e) {
+        const size_t gi = size_t(e) * 2 * n1 + i;
+        su += o.residue[e] * (o.gammat[gi + n1] - o.gamma[gi + n1]);
+        sw += o.residue[e] * (o.gamma[gi] + o.gammat[gi]);
+      }
+      dbar[i] += rho[i] * su;
+      dbar[n1 + i] += rho[i] * sw;
+    }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int x = o.xmin[k]; x <= o.xmax[k]; ++x)
+      for (int e = 0; e < Ne; ++e) {
+        reverseColumn<true>(o, k, x, e, drot, es, rho, dbar);
+        reverseColumn<false>(o, k, x, e, drot, es, rho, dbar);
+      }
+    // the rotation and extension of the total order parameter
+    std::fill(sumbar.begin(), sumbar.end(), 0.0);
+    const int kk = wrapToRangeInt(-k, 0, o.Na);
+    const float phiD = static_cast<float>(o.systemAngle[kk]);
+    insertRotatedAdjoint(dbar.data(), N1, N1, sumbar.data(), o.N0, o.N0, phiD);
+    insertRotatedAdjoint(dbar.data() + n1, N1, N1, sumbar.data() + n0, o.N0, o.N0, phiD);
+    extendValuesAdjoint(o.domainRef.data(), o.N0, sumbar.data());
+    extendValuesAdjoint(o.domainRef.data(), o.N0, sumbar.data() + n0);
+    for (int c = 0; c < o.Nc; ++c) {
+      const double b = basisMomentum(o, c, k);
+      double *gc = grad.data() + size_t(c) * 2 * n0;
+      for (size_t i = 0; i < 2 * n0; ++i) gc[i] += b * sumbar[i];
+    }
+  }
+  // the explicit terms sum_c |Delta_c|^2 (ln(T / Tc_c) + T sum_e r_e / w_e)
+  double sumInv = 0.0;
+  for (int e = 0; e < o.Ne; ++e) sumInv += o.residue[e] / o.energyIm[e];
+  sumInv *= o.T;
+  double sre = 0.0;
+  for (size_t i = 0; i < n0; ++i)
+    if (o.domainRef[i] != 0) sre += fe[i];
+  double nonMag = o.T * sre;
+  for (int c = 0; c < o.Nc; ++c) {
+    const double factor = std::log(o.T / o.tc[c]) + sumInv;
+    double s2 = 0.0;
+    double *gc = grad.data() + size_t(c) * 2 * n0;
+    for (size_t i = 0; i < n0; ++i) {
+      if (o.domainRef[i] == 0) continue;
+      const double re = o.delta[c][i], im = o.delta[c][n0 + i];
+      s2 += re * re + im * im;
+      gc[i] += 2.0 * area * factor * re;
+      gc[n0 + i] += 2.0 * area * factor * im;
+    }
+    nonMag += s2 * factor;
+  }
+  double mag = 0.0;
+  if (o.lambda > 0.0) mag = 0.5 / (o.lambda * o.lambda) * sumCurlSquare(o);
+  o.bOutG = keepG;
+  o.bOutGt = keepGt;
+  return (nonMag + mag) * area;
+}
+
+// ---------------------------------------------------------------------------
 // Accelerators -- include/conga/accelerators/*.h
 // ---------------------------------------------------------------------------
 static void concatenate(const Oracle &o, bool withA,
@@ -2538,6 +2798,20 @@ int oc_ldos_run(void *hh, int nb, int mn, int mx) {
     return -1;
   }
   return keldyshRun(o, nb, mn, mx) ? 1 : 0;
+}
+
+int oc_free_energy_gradient(void *hh, double *grad, double *free_energy) {
+  try {
+    Oracle &o = *static_cast<Oracle *>(hh);
+    std::vector<double> g;
+    const double f = freeEnergyGradient(o, g);
+    std::copy(g.begin(), g.end(), grad);
+    if (free_energy) *free_energy = f;
+    return 0;
+  } catch (const std::exception &e) {
+    g_error = e.what();
+    return -1;
+  }
 }
 
 long long oc_sweep_angles(void *hh, int a0, int a1) {

@@ -153,6 +153,11 @@ int oc_ldos_run(void *handle, int num_burnin, int min_iter, int max_iter);
  * [angle_begin, angle_end) with the current state.  Leaves the angle sums
  * partial; returns the number of trajectory steps taken. */
 long long oc_sweep_angles(void *handle, int angle_begin, int angle_end);
+/* The free energy of one iteration from the current state and its gradient
+ * with respect to the order parameter (vector potential and boundary
+ * in-values held fixed): grad holds num_components x 2 x N0^2 values,
+ * dF/dRe Delta_c then dF/dIm Delta_c.  No counterpart in the reference. */
+int oc_free_energy_gradient(void *handle, double *grad, double *free_energy);
 
 /* ----- unit-level restatements (closed-form checks) ----- */
 void oc_riccati_step(const double *delta, const double *energy,

@@ -108,6 +108,7 @@ def lib():
                                     ctypes.c_double, ctypes.c_int]
         L.oc_ldos_compute.argtypes = [vp, ctypes.c_int]
         L.oc_ldos_run.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.oc_free_energy_gradient.argtypes = [vp, dp, dp]
         L.oc_sweep_angles.restype = ctypes.c_longlong
         L.oc_sweep_angles.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.oc_riccati_step.argtypes = [dp, dp, dp, ctypes.c_double, dp]
@@ -318,6 +319,18 @@ class Oracle:
 
     def sweep_angles(self, a0, a1):
         return lib().oc_sweep_angles(self.h, a0, a1)
+
+    def free_energy_gradient(self):
+        """(F, grad): the free energy one iteration reports from the current
+        state and dF/dRe Delta_c + i dF/dIm Delta_c as an (Nc, N0, N0)
+        complex array (vector potential and boundary in-values fixed)."""
+        n = self.N0
+        g = np.zeros(self.Nc * 2 * n * n)
+        f = np.zeros(1)
+        if lib().oc_free_energy_gradient(self.h, _dp(g), _dp(f)) != 0:
+            raise RuntimeError(lib().oc_last_error().decode())
+        g = g.reshape(self.Nc, 2, n, n)
+        return float(f[0]), g[:, 0] + 1j * g[:, 1]
 
     # Keldysh spectroscopy (src/postprocess.cu)
     def ldos_setup(self, spectroscopy, per_block=-1):

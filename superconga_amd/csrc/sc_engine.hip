@@ -241,7 +241,8 @@ void Grain::uploadEnergies() {
 // Compiled sweep variants: (energies per lane, wavefronts per SIMD), for
 // Matsubara energies and for real-frequency (Keldysh) energies.
 #define SC_SWEEP_VARIANTS(X)                                                         \
-  X(1, 4) X(2, 4) X(4, 2) X(4, 4) X(7, 1) X(7, 2) X(7, 4) X(8, 1) X(8, 2) X(8, 4)      \
+  X(1, 4) X(2, 4) X(3, 3) X(4, 2) X(4, 3) X(4, 4) X(5, 2) X(6, 2) X(7, 1) X(7, 2) X(7, 4)  \
+  X(8, 1) X(8, 2) X(8, 4)                                                            \
   X(14, 1) X(14, 2) X(16, 1) X(16, 2)
 #define SC_KELDYSH_VARIANTS(X) X(4, 2) X(4, 4) X(8, 1) X(8, 2) X(8, 4) X(16, 1)
 

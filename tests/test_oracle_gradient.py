@@ -1,0 +1,126 @@
+"""The oracle's discrete adjoint of the free energy (no counterpart in the
+reference) against what the mathematics fixes: central finite differences of
+the oracle's own free energy, step 1e-6, at 64 seeded grid points per case in
+FP64 (re and im parts of every component), and the free energy itself equal
+to what one iteration of the oracle reports from the same state.
+
+The function differentiated is F(Delta) of ComputeFreeEnergy.h:281-414 on
+the Green functions of one sweep, with the vector potential and the boundary
+in-values held fixed.  Finite differences of a forward sum over ~1e5 sites
+resolve a derivative to ~1e-14 / step, so the bound is 2e-6 of the largest
+gradient entry at step 1e-6 (measured 1e-7 to 3e-7) and 1e-7 at step 1e-5
+(measured 1e-8): a dropped term, a wrong sign or a transposed tap moves
+entries by O(1) of the maximum.
+"""
+import numpy as np
+import pytest
+
+import cases
+from oracle import pyoracle as po
+
+CASES = {
+    # pair-breaking d-wave edges (rotated crystal axes)
+    "dwave_hexagon": lambda: cases.base(
+        temperature=0.3, angles=16, ppxi=3.0, order_parameter={"dx2-y2": {"critical_temperature": 1.0}},
+        rotation=0.125, components=[cases.regular_polygon_unit(6)]),
+    # s-wave vortex with external flux and induced vector potential (Doppler shift)
+    "vortex_field": lambda: cases.base(
+        temperature=0.4, flux=1.5, lam=4.0, angles=16, ppxi=3.0,
+        order_parameter={"s": {"critical_temperature": 1.0,
+                               "vortices": [{"center_x": 0.0, "center_y": 0.0,
+                                             "winding_number": 1}]}}),
+    # two components (d + s) on an octagon with a hole
+    "d_plus_s_hole": cases.mixed_small,
+}
+
+
+def state(name, iterations=2):
+    o = po.Oracle(CASES[name]())
+    o.initial_bulk()
+    for _ in range(iterations):
+        o.compute()
+    return o
+
+
+def sample_points(o, n, seed):
+    """n seeded sites: mostly inside, a few outside next to the edge (the
+    extension of the order parameter), a few on the edge itself."""
+    rng = np.random.default_rng(seed)
+    dom = o.domain()
+    inside = np.argwhere(dom > 0)
+    near = np.zeros_like(dom, dtype=bool)
+    for dy in (-2, -1, 0, 1, 2):
+        for dx in (-2, -1, 0, 1, 2):
+            near |= np.roll(np.roll(dom > 0, dy, 0), dx, 1)
+    outside = np.argwhere(near & (dom == 0))
+    k_out = n // 8
+    pts = [tuple(p) for p in inside[rng.choice(len(inside), n - k_out, replace=False)]]
+    pts += [tuple(p) for p in outside[rng.choice(len(outside), k_out, replace=False)]]
+    return pts
+
+
+def fd_errors(o, pts, step):
+    f0, grad = o.free_energy_gradient()
+    d = o.delta()
+    worst = 0.0
+    for c in range(o.Nc):
+        for (y, x) in pts:
+            for unit in (1.0, 1j):
+                dd = d[c].copy()
+                dd[y, x] += step * unit
+                o.set_order_parameter(c, dd)
+                fp, _ = o.free_energy_gradient()
+                dd[y, x] -= 2 * step * unit
+                o.set_order_parameter(c, dd)
+                fm, _ = o.free_energy_gradient()
+                o.set_order_parameter(c, d[c])
+                fd = (fp - fm) / (2 * step)
+                an = grad[c][y, x].real if unit == 1.0 else grad[c][y, x].imag
+                worst = max(worst, abs(fd - an))
+    return worst / np.max(np.abs(grad)), f0, grad
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_gradient_matches_central_differences(name):
+    o = state(name)
+    pts = sample_points(o, 64, seed=5)
+    err, f0, grad = fd_errors(o, pts, 1e-6)
+    print("%s: F %.10f, |grad| max %.3e, FD error %.2e of max" % (
+        name, f0, np.max(np.abs(grad)), err))
+    assert err < 2e-6
+    # a larger step resolves it further (rounding of the differences ~ 1/step)
+    err5, _, _ = fd_errors(o, pts[:8], 1e-5)
+    assert err5 < 1e-7
+
+
+def test_gradient_free_energy_is_the_iterations():
+    # F from the gradient evaluation equals what compute() reports from the
+    # same state, and the evaluation leaves the state untouched
+    # (a fresh context, so that compute() takes no accelerator step first)
+    src = state("vortex_field")
+    cfg = CASES["vortex_field"]()
+    o = po.Oracle(cfg)
+    o.set_order_parameter(0, src.delta()[0])
+    # the oracle stores lambda^2 A and takes A
+    a = src.vector_potential() / cfg["physics"]["penetration_depth"] ** 2
+    o.set_vector_potential(a[0], a[1])
+    f, _ = o.free_energy_gradient()
+    f2, _ = o.free_energy_gradient()
+    assert f == f2
+    o.compute()
+    assert abs(f - o.get("free_energy")) <= 1e-12 * abs(f)
+
+
+def test_gradient_vanishes_where_delta_is_overwritten():
+    # sites outside the grain that take the mean of the nearest inside sites
+    # (domain_ops.h:11-57) do not enter F
+    o = state("dwave_hexagon")
+    _, grad = o.free_energy_gradient()
+    dom = o.domain()
+    near = np.zeros_like(dom, dtype=bool)
+    for dy in (-1, 0, 1):
+        for dx in (-1, 0, 1):
+            near |= np.roll(np.roll(dom > 0, dy, 0), dx, 1)
+    ring = near & (dom == 0)
+    assert np.all(grad[0][ring] == 0.0)
+    assert np.max(np.abs(grad[0][dom > 0])) > 1e-3
