@@ -188,6 +188,20 @@ typedef struct {
 int sc_newton_krylov(sc_context *ctx, int max_newton, int krylov_dim, double tol,
                      sc_newton_report *report);
 
+/* The free energy one iteration reports from the current state (the
+ * reference's ComputeFreeEnergy.h:281-414 on the Green functions of one
+ * sweep) and its gradient with respect to the order parameter by a discrete
+ * adjoint of that iteration: the transposes of the back-rotation and of the
+ * extensions, a reverse sweep of both Riccati recurrences, the transpose of
+ * the rotated order parameter.  The vector potential and the boundary
+ * in-values are held fixed.  grad holds num_components x 2 x N0^2 values:
+ * dF/dRe Delta_c, then dF/dIm Delta_c, per component in order.  The state is
+ * left as it was.  No counterpart in the reference: it serves gradient
+ * studies (torch.autograd in the Python frontend).  Disabled in this build
+ * (returns an error): its reverse-sweep kernels have not passed their GPU
+ * test. */
+int sc_free_energy_gradient(sc_context *ctx, double *grad, double *free_energy);
+
 /* Individual stages. */
 /* RiccatiSolver::computeCoherenceFunctions for every angle (RiccatiSolverConfined.h:513-671)
  * fused with ComputeProperty::computeGreensAngle; burnin = 1 only updates the

@@ -138,6 +138,7 @@ def lib():
                                   ctypes.POINTER(Dims), ctypes.POINTER(ctypes.c_byte)]
         L.sc_newton_krylov.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                        ctypes.POINTER(NewtonReport)]
+        L.sc_free_energy_gradient.argtypes = [vp, _DP, _DP]
         L.sc_ldos_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(vp)]
         L.sc_ldos_compute.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.sc_ldos_run.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -303,6 +304,18 @@ class Simulation:
                 "map_evaluations": r.map_evaluations, "converged": bool(r.converged),
                 "residual": r.residual,
                 "history": [hist[i] for i in range(r.history_length)]}
+
+    def free_energy_gradient(self):
+        """(F, grad): the free energy one iteration reports from the current
+        state and its gradient dF/dRe Delta_c + i dF/dIm Delta_c, an
+        (Nc, N0, N0) complex array, by the discrete adjoint of the iteration
+        (vector potential and boundary in-values held fixed; the state is
+        left as it was)."""
+        g = np.empty(self.Nc * 2 * self.N0 * self.N0)
+        f = np.zeros(1)
+        _check(lib().sc_free_energy_gradient(self._h, _dp(g), _dp(f)))
+        g = g.reshape(self.Nc, 2, self.N0, self.N0)
+        return float(f[0]), g[:, 0] + 1j * g[:, 1]
 
     def riccati_sweep(self, precision="fp64", burnin=False):
         _check(lib().sc_riccati_sweep(self._h, _precision(precision), int(burnin)))

@@ -1,0 +1,615 @@
+// sc_adjoint.hip -- the gradient of the free energy with respect to the
+// order parameter by a discrete adjoint (reverse mode) of one iteration.
+// There is no counterpart in the reference: it serves gradient studies and
+// torch.autograd in the Python frontend.
+//
+// F(Delta) is the free energy one iteration reports (ComputeFreeEnergy.h:
+// 281-414 on the Green functions of the sweep) with the vector potential and
+// the boundary in-values held fixed:
+//   F = h^2 [ T sum_{inside g} fe(g) + sum_c |Delta_c|^2 (ln(T / Tc_c) + S)
+//             + magnetic term ],
+//   fe(g) = sum_k w_k bilinear_k(extend_k(gF_k))(g),
+//   gF_k(s) = sum_e r_e [D.re (gt.im - g.im) + D.im (g.re + gt.re)]
+// with g, gt the coherence functions of frame k at site s and D the rotated,
+// extended order parameter sum_c b_c(k) Delta_c.  Reverse mode runs the
+// stages backwards:
+//   1. the adjoint seed rho_k(s) = dF/dgF_k(s): the transpose of the
+//      back-rotation (bilinear taps of every inside site) and of the
+//      extension of the frame planes;
+//   2. per trajectory, the reverse of both Riccati recurrences: pass 1
+//      integrates gamma~ and keeps it, pass 2 integrates gamma, keeps it and
+//      carries the adjoint of gamma~ upwards, pass 3 carries the adjoint of
+//      gamma downwards; each step's adjoint adds to dF/dD at its sites;
+//   3. the transpose of the rotation and extension of the order parameter
+//      and the basis functions, plus the explicit |Delta|^2 terms.
+// Complex adjoints follow zbar = dF/dRe z + i dF/dIm z; for y = f(x) with f
+// holomorphic, xbar = conj(f'(x)) ybar.
+#include "sc_engine.hpp"
+#include "sc_device.hpp"
+#include "sc_util.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace scamd {
+
+namespace {
+
+constexpr int kAdjEL = 4;        // Ozaki energies per lane of the reverse sweep
+constexpr int kAdjSlotsPerCU = 4; // persistent wavefronts per compute unit
+
+__device__ __forceinline__ Cx<double> conjc(Cx<double> a) { return {a.re, -a.im}; }
+
+// Adjoint of stepEnergy(p, ser, y0) = y.  y is the root of
+//   F(y) = a (y + y0)^2 + (se - 1) y + (se + 1) y0 + sD = 0,
+// a = i hs conj(D) / 4, sD = i hs D, se = ser + i sei (the StepSite of D
+// over the step length hs), so by implicit differentiation
+//   dy = q (F_y0 dy0 + (y + y0)^2 da + d(sD)),  q = -1 / F_y,
+//   F_y = 2 a (y + y0) + se - 1,  F_y0 = 2 a (y + y0) + se + 1.
+// Returns y0bar; adds dF/dD of the step's D argument to dbar.
+__device__ __forceinline__ Cx<double> stepAdjoint(const StepSite<double> &p, double ser,
+                                                  Cx<double> y0, Cx<double> y, Cx<double> ybar,
+                                                  double hs, Cx<double> &dbar) {
+  const Cx<double> w = y + y0;
+  const Cx<double> aw = p.a * w;
+  const Cx<double> fy = {2.0 * aw.re + ser - 1.0, 2.0 * aw.im + p.sei};
+  const Cx<double> fy0 = {2.0 * aw.re + ser + 1.0, 2.0 * aw.im + p.sei};
+  const double r = rcpT(fy.re * fy.re + fy.im * fy.im);
+  const Cx<double> q = {-fy.re * r, fy.im * r};
+  const Cx<double> abar = conjc(q * (w * w)) * ybar;
+  const Cx<double> sdbar = conjc(q) * ybar;
+  // a = (i hs / 4) conj(D): Dbar += conj(abar) i hs / 4;  sD = i hs D: Dbar += -i hs sdbar
+  dbar.re += 0.25 * hs * abar.im + hs * sdbar.im;
+  dbar.im += 0.25 * hs * abar.re - hs * sdbar.re;
+  return conjc(q * fy0) * ybar;
+}
+
+// Sum over the energy groups of a column (lanes c, c + cols, ...) into the
+// group-0 lane (the sweep's reduction).
+__device__ __forceinline__ double groupSum(double v, int gp, int cols, int c) {
+  if (gp * cols == 64) {
+    for (int m = cols; m < 64; m <<= 1) v += __shfl_xor(v, m);
+    return v;
+  }
+  double s = v;
+  for (int m = 1; m < gp; ++m) s += __shfl(v, c + m * cols);
+  return s;
+}
+
+// 1a. Transpose of the back-rotation (gatherKernel's taps): rho_k at the
+// four taps of every inside reference site, times scale * w_k.
+__global__ void feSeedKernel(GrainView v, const int *sites, int numSites, double scale,
+                             double *rho) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= numSites) return;
+  const int N0 = v.N0, N1 = v.N1;
+  const int site = sites[t];
+  const int x = site % N0, y = site / N0;
+  const size_t n1 = size_t(N1) * N1;
+  for (int k = 0; k < v.Na; ++k) {
+    const RotSample r = rotatedCell(x, y, N0, N1, double(v.bkCos[k]), double(v.bkSin[k]));
+    if (!r.valid) continue;
+    const double w = scale * v.weight[k];
+    const double wx0 = 1.0 - r.fx, wx1 = r.fx, wy0 = 1.0 - r.fy, wy1 = r.fy;
+    double *p = rho + size_t(k) * n1 + r.idx;
+    atomicAdd(p, w * (wy0 * wx0));
+    atomicAdd(p + 1, w * (wy0 * wx1));
+    atomicAdd(p + N1, w * (wy1 * wx0));
+    atomicAdd(p + N1 + 1, w * (wy1 * wx1));
+  }
+}
+
+// 1b / 3b. Transpose of an extension (target <- mean of its sources): a
+// target hands its adjoint to its sources in equal parts and keeps none.
+// Targets and sources are indices within a plane; sources are never targets.
+__global__ void extendAdjointKernel(double *bar, const int *target, const int *offset,
+                                    const int *source, int numTargets, int planes,
+                                    long long planeStride) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= numTargets) return;
+  const long long tg = target[t];
+  const int o0 = offset[t], o1 = offset[t + 1];
+  const double np = double(o1 - o0);
+  for (int q = 0; q < planes; ++q) {
+    double *p = bar + q * planeStride;
+    const double v = p[tg];
+    if (v == 0.0) continue;
+    p[tg] = 0.0;
+    const double share = v / np;
+    for (int o = o0; o < o1; ++o) atomicAdd(p + source[o], share);
+  }
+}
+
+// 3a. Transpose of the site fields' order parameter
+// D_k(s) = sum_c b_c(k) bilinear(extDelta_c)(rotated s): one thread per frame
+// site, dF/dD_k(s) scattered to the four taps of every component.
+__global__ void fieldsAdjointKernel(GrainView v, const double *dbar, long long framePlane,
+                                    double *extBar) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= framePlane) return;
+  const double br = dbar[i], bi = dbar[framePlane + i];
+  if (br == 0.0 && bi == 0.0) return;
+  const int N0 = v.N0, N1 = v.N1;
+  const long long n1 = (long long)N1 * N1;
+  const int k = int(i / n1);
+  const int site = int(i - (long long)k * n1);
+  const int x = site % N1, y = site / N1;
+  const RotSample r = rotatedCell(x, y, N1, N0, double(v.opCos[k]), double(v.opSin[k]));
+  if (!r.valid) return;
+  const size_t n0 = size_t(N0) * N0;
+  const double w00 = (1.0 - r.fy) * (1.0 - r.fx), w10 = (1.0 - r.fy) * r.fx;
+  const double w01 = r.fy * (1.0 - r.fx), w11 = r.fy * r.fx;
+  for (int c = 0; c < v.Nc; ++c) {
+    const double b = v.basis[c * v.Na + k];
+    double *pr = extBar + size_t(c) * 2 * n0 + r.idx;
+    double *pi = pr + n0;
+    const double vr = b * br, vi = b * bi;
+    atomicAdd(pr, vr * w00);
+    atomicAdd(pr + 1, vr * w10);
+    atomicAdd(pr + N0, vr * w01);
+    atomicAdd(pr + N0 + 1, vr * w11);
+    atomicAdd(pi, vi * w00);
+    atomicAdd(pi + 1, vi * w10);
+    atomicAdd(pi + N0, vi * w01);
+    atomicAdd(pi + N0 + 1, vi * w11);
+  }
+}
+
+// The explicit terms: grad = extBar + 2 h^2 factor_c Delta_c at inside sites.
+struct ComponentFactors {
+  double f[4];
+};
+__global__ void finishGradientKernel(const double *extBar, const double *delta,
+                                     const uint8_t *domain, ComponentFactors fac, double area,
+                                     long long n0, int Nc, double *grad) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= 2LL * Nc * n0) return;
+  const int c = int(i / (2 * n0));
+  const long long site = i % n0;
+  const double e = domain[site] ? 2.0 * area * fac.f[c] * delta[i] : 0.0;
+  grad[i] = extBar[i] + e;
+}
+
+struct AdjointArgs {
+  GrainView view;
+  const SweepItemDev *items;
+  int numItems;
+  int *counter;
+  Cx<double> *scratch;    // per slot: gamma~ then gamma, [maxLen][EL][64] each
+  long long slotStride;   // complex elements of one of the two buffers
+  const double *rho;      // [Na][N1^2]
+  double *dbar;           // [2][Na][N1^2]
+  long long framePlane;   // Na * N1^2
+  double *partial;        // per item: its share of the free-energy sum
+};
+
+// 2. The reverse sweep of one item (the sweep's items and lane layout).
+template <int EL>
+__device__ void adjointItem(const AdjointArgs &a, const SweepItemDev &it, int itemIndex,
+                            Cx<double> *scrT, Cx<double> *scrG) {
+  const GrainView &v = a.view;
+  const int lane = threadIdx.x;
+  const int cols = v.cols, gp = v.groups;
+  const int g = lane / cols;
+  const int c = lane - g * cols;
+  const int N1 = v.N1, Ne = v.Ne;
+  const size_t n1 = size_t(N1) * N1;
+  const double h = v.h;
+  const int k = it.angle;
+  const bool colActive = c < it.ncols;
+  const int x = it.x0 + (colActive ? c : it.ncols - 1);
+  const bool active = colActive && g < gp;
+  const int8_t *lab = v.labels + size_t(k) * n1 + x;
+  const double *F = v.fields + size_t(k - v.fieldBase) * n1 + x;
+  const double *rhoK = a.rho + size_t(k) * n1 + x;
+  double *dbK = a.dbar + size_t(k) * n1 + x;
+  double ser[EL], res[EL];
+  bool eOk[EL];
+#pragma unroll
+  for (int j = 0; j < EL; ++j) {
+    const int e = g * EL + j;
+    eOk[j] = active && e < Ne;
+    ser[j] = eOk[j] ? -h * v.energy[e] : 0.0;
+    res[j] = eOk[j] ? v.residue[e] : 0.0;
+  }
+  const int len = it.yhi - it.ylo + 1;
+  const size_t col = size_t(k) * N1 + x;
+  auto slot = [&](int t, int j) { return (size_t(t) * EL + j) * 64 + lane; };
+  // dF/dD of a finished site: the column's groups summed, written (or added)
+  // by the group-0 lane
+  auto writeDbar = [&](int y, Cx<double> part, bool add) {
+    const double re = groupSum(part.re, gp, cols, c);
+    const double im = groupSum(part.im, gp, cols, c);
+    if (g == 0 && colActive) {
+      double *p = dbK + size_t(y) * N1;
+      if (add) {
+        p[0] += re;
+        p[a.framePlane] += im;
+      } else {
+        p[0] = re;
+        p[a.framePlane] = im;
+      }
+    }
+  };
+
+  // ---------------- pass 1: gamma~ from the top, kept at every site ----------
+  int numEntryT = 0;
+  {
+    const int baseIn = v.idxInGt[col];
+    Cx<double> coh[EL];
+#pragma unroll
+    for (int j = 0; j < EL; ++j) coh[j] = {0.0, 0.0};
+    Cx<double> dPrev = {0.0, 0.0};
+    double esPrev = 0.0;
+    bool havePrev = false;
+    for (int t = 0; t < len; ++t) {
+      const int y = it.yhi - t;
+      const size_t o = size_t(y) * N1;
+      const Cx<double> D = {-F[o], F[v.fieldPlane + o]};
+      const double es = F[2 * v.fieldPlane + o];
+      const int8_t lc = lab[o];
+      if (active) {
+        if (lc == 1 || lc == 2) {
+          const StepSite<double> ps = stepSite(
+              Cx<double>{(dPrev.re + D.re) * 0.5, (dPrev.im + D.im) * 0.5}, (esPrev + es) * 0.5, h);
+          const double half = havePrev ? 1.0 : 0.5;
+#pragma unroll
+          for (int j = 0; j < EL; ++j) coh[j] = stepEnergy(ps, ser[j] * half, coh[j]);
+        } else if (lc == 3) {
+          const int ci = baseIn + numEntryT;
+          const double dr = v.distInGt[ci];
+          const StepSite<double> ps = stepSite(D, es, dr * h);
+#pragma unroll
+          for (int j = 0; j < EL; ++j) {
+            const size_t si = size_t(ci) * Ne + g * EL + j;
+            const Cx<double> b0 = eOk[j] ? Cx<double>{v.bInGt[si], v.bInGt[size_t(v.nInGt) * Ne + si]}
+                                         : Cx<double>{0.0, 0.0};
+            coh[j] = stepEnergy(ps, ser[j] * dr, b0);
+          }
+        } else {
+#pragma unroll
+          for (int j = 0; j < EL; ++j) coh[j] = {0.0, 0.0};
+        }
+#pragma unroll
+        for (int j = 0; j < EL; ++j) scrT[slot(t, j)] = coh[j];
+      }
+      if (lc == 3) ++numEntryT;
+      dPrev = D;
+      esPrev = es;
+      havePrev = true;
+    }
+  }
+
+  // ------- pass 2: gamma from the bottom, kept; the adjoint of gamma~ -------
+  // (B = the site below, processed in the previous iteration)
+  int numEntryG = 0;
+  double omega = 0.0;
+  {
+    const int baseInG = v.idxInG[col], baseInT = v.idxInGt[col];
+    int seenT = 0;
+    Cx<double> coh[EL], mu[EL], gtB[EL];
+#pragma unroll
+    for (int j = 0; j < EL; ++j) coh[j] = mu[j] = gtB[j] = {0.0, 0.0};
+    Cx<double> dPrev = {0.0, 0.0}, dtB = {0.0, 0.0}, dbB = {0.0, 0.0};
+    double esPrev = 0.0;
+    bool havePrev = false;
+    int8_t labB = 0;
+    for (int u = 0; u < len; ++u) {
+      const int t = len - 1 - u;
+      const int y = it.ylo + u;
+      const size_t o = size_t(y) * N1;
+      const double dre = F[o], dim = F[v.fieldPlane + o], es = F[2 * v.fieldPlane + o];
+      const Cx<double> D = {dre, dim}, Dt = {-dre, dim};
+      const int8_t lc = lab[o];
+      const double rho = rhoK[o];
+      Cx<double> gt[EL];
+#pragma unroll
+      for (int j = 0; j < EL; ++j) gt[j] = active ? scrT[slot(t, j)] : Cx<double>{0.0, 0.0};
+      Cx<double> dbCur = {0.0, 0.0};
+      if (active) {
+        // gamma into this site (the sweep's pass 2)
+        if (lc == 1 || lc == 3) {
+          const StepSite<double> ps = stepSite(
+              Cx<double>{(dPrev.re + D.re) * 0.5, (dPrev.im + D.im) * 0.5}, (esPrev + es) * 0.5, h);
+          const double half = havePrev ? 1.0 : 0.5;
+#pragma unroll
+          for (int j = 0; j < EL; ++j) coh[j] = stepEnergy(ps, ser[j] * half, coh[j]);
+        } else if (lc == 2) {
+          const int ci = baseInG + numEntryG;
+          const double dr = v.distInG[ci];
+          const StepSite<double> ps = stepSite(D, es, dr * h);
+#pragma unroll
+          for (int j = 0; j < EL; ++j) {
+            const size_t si = size_t(ci) * Ne + g * EL + j;
+            const Cx<double> b0 = eOk[j] ? Cx<double>{v.bInG[si], v.bInG[size_t(v.nInG) * Ne + si]}
+                                         : Cx<double>{0.0, 0.0};
+            coh[j] = stepEnergy(ps, ser[j] * dr, b0);
+          }
+        } else {
+#pragma unroll
+          for (int j = 0; j < EL; ++j) coh[j] = {0.0, 0.0};
+        }
+#pragma unroll
+        for (int j = 0; j < EL; ++j) scrG[slot(t, j)] = coh[j];
+        // the free-energy density and its own dependence on D
+        double su = 0.0, sw = 0.0;
+#pragma unroll
+        for (int j = 0; j < EL; ++j) {
+          if (!eOk[j]) continue;
+          su += res[j] * (gt[j].im - coh[j].im);
+          sw += res[j] * (coh[j].re + gt[j].re);
+        }
+        omega += rho * (dre * su + dim * sw);
+        dbCur = {rho * su, rho * sw};
+        // adjoint of gamma~ here: what the step into the site below carries
+        // back, plus dF/dgt = rho r (D.im + i D.re)
+        Cx<double> carry[EL];
+#pragma unroll
+        for (int j = 0; j < EL; ++j) carry[j] = {0.0, 0.0};
+        if (u > 0 && (labB == 1 || labB == 2)) {
+          const StepSite<double> ps = stepSite(
+              Cx<double>{(Dt.re + dtB.re) * 0.5, (Dt.im + dtB.im) * 0.5}, (es + esPrev) * 0.5, h);
+          Cx<double> db = {0.0, 0.0};
+#pragma unroll
+          for (int j = 0; j < EL; ++j)
+            if (eOk[j]) carry[j] = stepAdjoint(ps, ser[j], gt[j], gtB[j], mu[j], h, db);
+          // the step's D is -conj(D): dF/dD = -conj(dF/dD~), half to each site
+          const Cx<double> dd = {-0.5 * db.re, 0.5 * db.im};
+          dbCur = dbCur + dd;
+          dbB = dbB + dd;
+        }
+#pragma unroll
+        for (int j = 0; j < EL; ++j)
+          mu[j] = eOk[j] ? carry[j] + Cx<double>{rho * res[j] * dim, rho * res[j] * dre}
+                         : Cx<double>{0.0, 0.0};
+        // gamma~ entered here from its in-value (the entries in reverse order)
+        if (lc == 3) {
+          const int ci = baseInT + (numEntryT - 1 - seenT);
+          const double dr = v.distInGt[ci];
+          const StepSite<double> ps = stepSite(Dt, es, dr * h);
+          Cx<double> db = {0.0, 0.0};
+#pragma unroll
+          for (int j = 0; j < EL; ++j) {
+            if (!eOk[j]) continue;
+            const size_t si = size_t(ci) * Ne + g * EL + j;
+            const Cx<double> b0 = {v.bInGt[si], v.bInGt[size_t(v.nInGt) * Ne + si]};
+            (void)stepAdjoint(ps, ser[j] * dr, b0, gt[j], mu[j], dr * h, db);
+          }
+          dbCur = dbCur + Cx<double>{-db.re, db.im};
+        }
+      }
+      if (lc == 3) ++seenT;
+      if (lc == 2) ++numEntryG;
+      // the site below has all its contributions now
+      if (u > 0) writeDbar(y - 1, dbB, false);
+      dbB = dbCur;
+#pragma unroll
+      for (int j = 0; j < EL; ++j) gtB[j] = gt[j];
+      dtB = Dt;
+      labB = lc;
+      dPrev = D;
+      esPrev = es;
+      havePrev = true;
+    }
+    // the first step of gamma~ (top site, from a zero start at half weight)
+    if (active && (labB == 1 || labB == 2)) {
+      const StepSite<double> ps =
+          stepSite(Cx<double>{dtB.re * 0.5, dtB.im * 0.5}, esPrev * 0.5, h);
+      Cx<double> db = {0.0, 0.0};
+#pragma unroll
+      for (int j = 0; j < EL; ++j)
+        if (eOk[j]) (void)stepAdjoint(ps, ser[j] * 0.5, Cx<double>{0.0, 0.0}, gtB[j], mu[j], h, db);
+      dbB = dbB + Cx<double>{-0.5 * db.re, 0.5 * db.im};
+    }
+    writeDbar(it.yhi, dbB, false);
+  }
+
+  // ------- pass 3: the adjoint of gamma from the top (A = the site above) -------
+  {
+    const int baseInG = v.idxInG[col];
+    int seenG = 0;
+    Cx<double> lam[EL], gA[EL];
+#pragma unroll
+    for (int j = 0; j < EL; ++j) lam[j] = gA[j] = {0.0, 0.0};
+    Cx<double> dA = {0.0, 0.0}, dbA = {0.0, 0.0};
+    double esA = 0.0;
+    int8_t labA = 0;
+    for (int t = 0; t < len; ++t) {
+      const int y = it.yhi - t;
+      const size_t o = size_t(y) * N1;
+      const double dre = F[o], dim = F[v.fieldPlane + o], es = F[2 * v.fieldPlane + o];
+      const Cx<double> D = {dre, dim};
+      const int8_t lc = lab[o];
+      const double rho = rhoK[o];
+      Cx<double> gc[EL];
+#pragma unroll
+      for (int j = 0; j < EL; ++j) gc[j] = active ? scrG[slot(t, j)] : Cx<double>{0.0, 0.0};
+      Cx<double> dbCur = {0.0, 0.0};
+      if (active) {
+        Cx<double> carry[EL];
+#pragma unroll
+        for (int j = 0; j < EL; ++j) carry[j] = {0.0, 0.0};
+        // the step of gamma from this site into the site above
+        if (t > 0 && (labA == 1 || labA == 3)) {
+          const StepSite<double> ps = stepSite(
+              Cx<double>{(D.re + dA.re) * 0.5, (D.im + dA.im) * 0.5}, (es + esA) * 0.5, h);
+          Cx<double> db = {0.0, 0.0};
+#pragma unroll
+          for (int j = 0; j < EL; ++j)
+            if (eOk[j]) carry[j] = stepAdjoint(ps, ser[j], gc[j], gA[j], lam[j], h, db);
+          const Cx<double> dd = {0.5 * db.re, 0.5 * db.im};
+          dbCur = dbCur + dd;
+          dbA = dbA + dd;
+        }
+        // dF/dg = rho r (D.im - i D.re)
+#pragma unroll
+        for (int j = 0; j < EL; ++j)
+          lam[j] = eOk[j] ? carry[j] + Cx<double>{rho * res[j] * dim, -rho * res[j] * dre}
+                          : Cx<double>{0.0, 0.0};
+        // gamma entered here from its in-value (the entries in reverse order)
+        if (lc == 2) {
+          const int ci = baseInG + (numEntryG - 1 - seenG);
+          const double dr = v.distInG[ci];
+          const StepSite<double> ps = stepSite(D, es, dr * h);
+          Cx<double> db = {0.0, 0.0};
+#pragma unroll
+          for (int j = 0; j < EL; ++j) {
+            if (!eOk[j]) continue;
+            const size_t si = size_t(ci) * Ne + g * EL + j;
+            const Cx<double> b0 = {v.bInG[si], v.bInG[size_t(v.nInG) * Ne + si]};
+            (void)stepAdjoint(ps, ser[j] * dr, b0, gc[j], lam[j], dr * h, db);
+          }
+          dbCur = dbCur + db;
+        }
+      }
+      if (lc == 2) ++seenG;
+      // the site above has all its contributions now
+      if (t > 0) writeDbar(y + 1, dbA, true);
+      dbA = dbCur;
+#pragma unroll
+      for (int j = 0; j < EL; ++j) gA[j] = gc[j];
+      dA = D;
+      esA = es;
+      labA = lc;
+    }
+    // the first step of gamma (bottom site, from a zero start at half weight)
+    if (active && (labA == 1 || labA == 3)) {
+      const StepSite<double> ps = stepSite(Cx<double>{dA.re * 0.5, dA.im * 0.5}, esA * 0.5, h);
+      Cx<double> db = {0.0, 0.0};
+#pragma unroll
+      for (int j = 0; j < EL; ++j)
+        if (eOk[j]) (void)stepAdjoint(ps, ser[j] * 0.5, Cx<double>{0.0, 0.0}, gA[j], lam[j], h, db);
+      dbA = dbA + Cx<double>{0.5 * db.re, 0.5 * db.im};
+    }
+    writeDbar(it.ylo, dbA, true);
+  }
+
+  // this item's share of the free-energy sum, in lane order
+  for (int m = 1; m < 64; m <<= 1) omega += __shfl_xor(omega, m);
+  if (lane == 0) a.partial[itemIndex] = omega;
+}
+
+template <int EL>
+__global__ __launch_bounds__(64, 2) void adjointKernel(AdjointArgs a) {
+  Cx<double> *scrT = a.scratch + size_t(blockIdx.x) * 2 * a.slotStride;
+  Cx<double> *scrG = scrT + a.slotStride;
+  for (;;) {
+    int item = 0;
+    if (threadIdx.x == 0) item = atomicAdd(a.counter, 1);
+    item = __builtin_amdgcn_readfirstlane(__shfl(item, 0));
+    if (item >= a.numItems) break;
+    adjointItem<EL>(a, a.items[item], item, scrT, scrG);
+  }
+}
+
+__global__ void adjointResetCounter(int *c) { *c = 0; }
+
+} // namespace
+
+double Grain::freeEnergyGradient(double *grad) {
+  const Setup &s = *s_;
+  if (s.keldysh) throw std::runtime_error("a spectroscopy context has no free-energy gradient");
+  if (s.Nc > 4) throw std::runtime_error("at most 4 order-parameter components");
+  const size_t n0 = size_t(s.N0) * s.N0, n1 = size_t(s.N1) * s.N1;
+  const long long framePlane = static_cast<long long>(s.Na) * static_cast<long long>(n1);
+  // the order parameter's extension and the site fields of every frame
+  ensureBoundaryInit();
+  extendDelta();
+  // the reverse sweep's items (the sweep's own, at kAdjEL energies per lane)
+  if (!adjItems_) {
+    const GrainView gv = sweepView(kAdjEL);
+    std::vector<SweepItem> items = sweepItems(s, gv.cols, &adjMaxLen_);
+    std::vector<SweepItemDev> di(items.size());
+    for (size_t i = 0; i < items.size(); ++i)
+      di[i] = {0, items[i].angle, items[i].x0, items[i].ncols, items[i].ylo, items[i].yhi};
+    adjNumItems_ = static_cast<int>(di.size());
+    SC_HIP(hipMalloc(&adjItems_, std::max<size_t>(1, di.size()) * sizeof(SweepItemDev)));
+    if (!di.empty()) toDevice(adjItems_, di.data(), di.size() * sizeof(SweepItemDev));
+    int dev = 0, cus = 0;
+    SC_HIP(hipGetDevice(&dev));
+    SC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    adjSlots_ = std::max(1, std::min(adjNumItems_, kAdjSlotsPerCU * std::max(cus, 1)));
+    const size_t stride = size_t(adjMaxLen_) * kAdjEL * 64;
+    dAdjScratch_ = devAlloc<double>(size_t(adjSlots_) * 4 * stride);
+    dAdjPartial_ = devAlloc<double>(size_t(std::max(adjNumItems_, 1)));
+    dAdjRho_ = devAlloc<double>(size_t(framePlane));
+    dAdjDbar_ = devAlloc<double>(2 * size_t(framePlane));
+    dAdjExt_ = devAlloc<double>(size_t(2 * s.Nc) * n0);
+    dAdjGrad_ = devAlloc<double>(size_t(2 * s.Nc) * n0);
+  }
+  SC_HIP(hipMemsetAsync(dAdjRho_, 0, size_t(framePlane) * sizeof(double), stream_));
+  SC_HIP(hipMemsetAsync(dAdjDbar_, 0, 2 * size_t(framePlane) * sizeof(double), stream_));
+  SC_HIP(hipMemsetAsync(dAdjExt_, 0, size_t(2 * s.Nc) * n0 * sizeof(double), stream_));
+  const GrainView v = view();
+  const double area = s.h * s.h;
+  // 1. the adjoint seed of the free-energy density per frame site
+  const int ns = static_cast<int>(s.domainSites.size());
+  if (ns > 0) {
+    feSeedKernel<<<blocksFor(ns), 256, 0, stream_>>>(v, dDomainSites_, ns, area * s.temperature,
+                                                      dAdjRho_);
+    SC_HIP(hipGetLastError());
+  }
+  const int nr = static_cast<int>(s.extRot.target.size());
+  if (nr > 0) {
+    extendAdjointKernel<<<blocksFor(nr), 256, 0, stream_>>>(dAdjRho_, dExtRotT_, dExtRotO_,
+                                                            dExtRotS_, nr, 1, 0);
+    SC_HIP(hipGetLastError());
+  }
+  // 2. the reverse sweep
+  AdjointArgs a;
+  a.view = sweepView(kAdjEL);
+  a.items = reinterpret_cast<const SweepItemDev *>(adjItems_);
+  a.numItems = adjNumItems_;
+  a.counter = dCounter_;
+  a.scratch = reinterpret_cast<Cx<double> *>(dAdjScratch_);
+  a.slotStride = static_cast<long long>(adjMaxLen_) * kAdjEL * 64;
+  a.rho = dAdjRho_;
+  a.dbar = dAdjDbar_;
+  a.framePlane = framePlane;
+  a.partial = dAdjPartial_;
+  adjointResetCounter<<<1, 1, 0, stream_>>>(dCounter_);
+  if (adjNumItems_ > 0) {
+    adjointKernel<kAdjEL><<<adjSlots_, 64, 0, stream_>>>(a);
+    SC_HIP(hipGetLastError());
+  }
+  // 3. the rotation, basis and extension of the order parameter
+  fieldsAdjointKernel<<<blocksFor(framePlane), 256, 0, stream_>>>(v, dAdjDbar_, framePlane,
+                                                                  dAdjExt_);
+  SC_HIP(hipGetLastError());
+  const int ne = static_cast<int>(s.extRef.target.size());
+  if (ne > 0) {
+    extendAdjointKernel<<<blocksFor(ne), 256, 0, stream_>>>(dAdjExt_, dExtRefT_, dExtRefO_,
+                                                            dExtRefS_, ne, 2 * s.Nc,
+                                                            static_cast<long long>(n0));
+    SC_HIP(hipGetLastError());
+  }
+  ComponentFactors fac{};
+  for (int c = 0; c < s.Nc; ++c)
+    fac.f[c] = std::log(s.temperature / s.cfg.tc[c]) + s.sumInverseEnergy;
+  const long long ng = 2LL * s.Nc * static_cast<long long>(n0);
+  finishGradientKernel<<<blocksFor(ng), 256, 0, stream_>>>(dAdjExt_, dDelta_, dDomain_, fac,
+                                                           area, static_cast<long long>(n0), s.Nc,
+                                                           dAdjGrad_);
+  SC_HIP(hipGetLastError());
+  // the free energy: the sweep's share from the items, the explicit terms
+  clearReductions();
+  const int hSweep = queueSum(dAdjPartial_, nullptr, nullptr, adjNumItems_, 0);
+  std::vector<int> hSq;
+  int hCurl = -1;
+  queueExplicitFreeEnergy(hSq, hCurl);
+  fromDevice(grad, dAdjGrad_, size_t(ng) * sizeof(double));
+  flushReductions();
+  double nonMag = 0.0;
+  for (int c = 0; c < s.Nc; ++c)
+    nonMag += (reductionSum(hSq[size_t(2 * c)]) + reductionSum(hSq[size_t(2 * c + 1)])) * fac.f[c];
+  double mag = 0.0;
+  if (hCurl >= 0) mag = 0.5 / (s.lambda * s.lambda) * reductionSum(hCurl);
+  const double sweepPart = reductionSum(hSweep);
+  clearReductions();
+  return sweepPart + (nonMag + mag) * area;
+}
+
+} // namespace scamd

@@ -99,6 +99,13 @@ Grain::~Grain() {
   for (double *p : nkVecs_) devFree(p);
   devFree(dWork_);
   devFree(dWork2_);
+  if (adjItems_) (void)hipFree(adjItems_);
+  devFree(dAdjScratch_);
+  devFree(dAdjPartial_);
+  devFree(dAdjRho_);
+  devFree(dAdjDbar_);
+  devFree(dAdjExt_);
+  devFree(dAdjGrad_);
   fft_.reset();
   if (dScratch_) (void)hipFree(dScratch_);
   for (SweepPlan &pl : plan_) {
@@ -1015,6 +1022,24 @@ void Grain::updateSweepTime() {
   float ms = 0.f;
   SC_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
   tim_.sweepMs = ms;
+}
+
+void Grain::queueExplicitFreeEnergy(std::vector<int> &hSq, int &hCurl) {
+  // the sums of |Delta_c|^2 over the grain (re and im planes) and of
+  // (curl A)^2 that the free energy adds to the Green-function part
+  const Setup &s = *s_;
+  const size_t n0 = size_t(s.N0) * s.N0;
+  hSq.clear();
+  for (int c = 0; c < s.Nc; ++c)
+    for (int q = 0; q < 2; ++q)
+      hSq.push_back(queueSum(dDelta_ + size_t(c) * 2 * n0 + size_t(q) * n0, nullptr, dDomain_,
+                             static_cast<long long>(n0), 2));
+  hCurl = -1;
+  if (s.lambda > 0.0) {
+    curlSquareKernel<<<blocksFor(n0), 256, 0, stream_>>>(dA_, s.N0, 1.0 / s.h, dWork_);
+    SC_HIP(hipGetLastError());
+    hCurl = queueSum(dWork_, nullptr, nullptr, static_cast<long long>(n0), 0);
+  }
 }
 
 void Grain::solveVectorPotential(const double *dj, double *dout) {

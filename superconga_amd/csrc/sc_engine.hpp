@@ -118,6 +118,12 @@ public:
   // to a block-weighted residual below tol; leaves the solution as the
   // iterate and restarts the accelerators.
   NewtonReport newtonKrylov(int maxNewton, int krylovDim, double tol);
+  // The free energy one iteration reports from the current state and its
+  // gradient with respect to the order parameter (vector potential and
+  // boundary in-values held fixed) by a discrete adjoint: grad holds
+  // Nc x 2 x N0^2 values, dF/dRe Delta_c then dF/dIm Delta_c.  The state
+  // is left as it was.
+  double freeEnergyGradient(double *grad);
   void resetAccelerator();
 
   // Keldysh spectroscopy (src/postprocess.cu), for a grain built by
@@ -271,6 +277,12 @@ private:
   std::vector<int> hDelta_, hDeltaSq_;
   int hCurrent_ = -1, hEnergy_ = -1, hCurl_ = -1, hBoundary_[2] = {-1, -1};
   bool pendingIntegrate_ = false;
+  // the free-energy gradient (sc_adjoint.hip)
+  void queueExplicitFreeEnergy(std::vector<int> &hSq, int &hCurl);
+  void *adjItems_ = nullptr;
+  int adjNumItems_ = 0, adjMaxLen_ = 0, adjSlots_ = 0;
+  double *dAdjScratch_ = nullptr, *dAdjPartial_ = nullptr, *dAdjRho_ = nullptr,
+         *dAdjDbar_ = nullptr, *dAdjExt_ = nullptr, *dAdjGrad_ = nullptr;
   int hA_ = -1; // queued residual of the vector potential
   long long partialsCap_ = 0;
   // sweep plans, one per precision
