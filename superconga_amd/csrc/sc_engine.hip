@@ -841,9 +841,14 @@ void Grain::autotune(int precision) {
   auto tryShape = [&](int el, int w, int slots) {
     if ((s_->Ne + el - 1) / el > 64) return;
     setSweepShape(precision, el, w, slots);
-    sweep(precision, false);
-    updateSweepTime();
-    const float ms = static_cast<float>(tim_.sweepMs);
+    // the faster of two launches: one launch varies by a few per cent
+    float ms = -1.f;
+    for (int rep = 0; rep < 2; ++rep) {
+      sweep(precision, false);
+      updateSweepTime();
+      const float t = static_cast<float>(tim_.sweepMs);
+      if (ms < 0.f || t < ms) ms = t;
+    }
     if (best < 0.f || ms < best) {
       best = ms;
       bestEl = el;
@@ -1727,10 +1732,15 @@ void Batch::autotune(int precision) {
   auto tryShape = [&](int el, int w, int slots) {
     if ((maxNe + el - 1) / el > 64) return;
     setSweepShape(precision, el, w, slots);
-    sweep(precision, false, act);
-    SC_HIP(hipEventSynchronize(ev1_));
-    float ms = 0.f;
-    SC_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+    // the faster of two launches: one launch varies by a few per cent
+    float ms = -1.f;
+    for (int rep = 0; rep < 2; ++rep) {
+      sweep(precision, false, act);
+      SC_HIP(hipEventSynchronize(ev1_));
+      float t = 0.f;
+      SC_HIP(hipEventElapsedTime(&t, ev0_, ev1_));
+      if (ms < 0.f || t < ms) ms = t;
+    }
     if (best < 0.f || ms < best) {
       best = ms;
       bestEl = el;
