@@ -213,9 +213,10 @@ int sc_boundary_condition(sc_context *ctx, double *residual);
  * reference's shape rule is getKernelDimensions, GridFunctions_GPU.h:1121-1189):
  * Ozaki energies per lane, the register budget the kernel variant is compiled
  * for (wavefronts per SIMD) and persistent wavefronts per compute unit.
- * Compiled variants (energies, waves): (1,4) (2,4) (4,2) (4,4) (7,1) (7,2)
- * (7,4) (8,1) (8,2) (8,4) (14,1) (14,2) (16,1) (16,2); for spectroscopy
- * contexts (4,2) (4,4) (8,1) (8,2) (8,4) (16,1). */
+ * Compiled variants (energies, waves): (1,4) (2,4) (3,3) (4,2) (4,3) (4,4)
+ * (5,2) (6,2) (7,1) (7,2) (7,4) (8,1) (8,2) (8,4) (14,1) (14,2) (16,1)
+ * (16,2); for spectroscopy contexts (2,4) (4,2) (4,3) (4,4) (8,1) (8,2)
+ * (8,4) (16,1). */
 int sc_set_sweep_shape(sc_context *ctx, int precision, int energies_per_lane,
                        int waves_per_simd, int slots_per_cu);
 int sc_get_sweep_shape(sc_context *ctx, int precision, int *energies_per_lane,

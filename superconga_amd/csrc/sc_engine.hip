@@ -251,7 +251,7 @@ void Grain::uploadEnergies() {
   X(1, 4) X(2, 4) X(3, 3) X(4, 2) X(4, 3) X(4, 4) X(5, 2) X(6, 2) X(7, 1) X(7, 2) X(7, 4)  \
   X(8, 1) X(8, 2) X(8, 4)                                                            \
   X(14, 1) X(14, 2) X(16, 1) X(16, 2)
-#define SC_KELDYSH_VARIANTS(X) X(4, 2) X(4, 4) X(8, 1) X(8, 2) X(8, 4) X(16, 1)
+#define SC_KELDYSH_VARIANTS(X) X(2, 4) X(4, 2) X(4, 3) X(4, 4) X(8, 1) X(8, 2) X(8, 4) X(16, 1)
 
 bool sweepVariantExists(int el, int waves, bool keldysh) {
 #define SC_HAS(E, W) if (el == E && waves == W) return true;
