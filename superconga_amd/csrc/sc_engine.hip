@@ -29,6 +29,7 @@ Grain::Grain(std::unique_ptr<Setup> setup, hipStream_t stream) : s_(std::move(se
   SC_HIP(hipEventCreate(&evIterStart_));
   SC_HIP(hipEventCreate(&evIterStop_));
   SC_HIP(hipEventCreateWithFlags(&evReadback_, hipEventDisableTiming));
+  SC_HIP(hipEventCreateWithFlags(&evDots_, hipEventDisableTiming));
   upload();
   // Allocation clears and uploads ran on the null stream, which a
   // non-blocking stream does not wait for.
@@ -119,7 +120,9 @@ Grain::~Grain() {
   if (evIterStart_) (void)hipEventDestroy(evIterStart_);
   if (evIterStop_) (void)hipEventDestroy(evIterStop_);
   if (evReadback_) (void)hipEventDestroy(evReadback_);
+  if (evDots_) (void)hipEventDestroy(evDots_);
   if (redPinned_) (void)hipHostFree(redPinned_);
+  if (dotPinned_) (void)hipHostFree(dotPinned_);
   if (stream_ && ownsStream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -678,25 +681,90 @@ double Grain::deviceSum(const double *a, const double *b, const uint8_t *mask,
 
 std::vector<double> Grain::multiDot(const double *a, const std::vector<double *> &bs,
                                     long long n) {
-  std::vector<double> out(bs.size(), 0.0);
-  const long long blocks = std::min<long long>(blocksFor(n), 512);
-  for (size_t first = 0; first < bs.size(); first += kMaxDots) {
-    const int m = static_cast<int>(std::min<size_t>(kMaxDots, bs.size() - first));
-    if (blocks * m > partialsCap_ * 6) {
-      devFree(dPartials_);
-      partialsCap_ = (blocks * m + 5) / 6;
-      dPartials_ = devAlloc<double>(size_t(partialsCap_) * 6);
+  return multiDotGroups({DotGroup{a, bs, n}})[0];
+}
+
+std::vector<std::vector<double>> Grain::multiDotGroups(const std::vector<DotGroup> &groups) {
+  // every group's kernels write their block partials to their own range of
+  // dPartials_; one transfer into pinned host memory brings them all back,
+  // and each dot product sums its partials in block order
+  struct Launch {
+    size_t group, first;
+    int m;
+    long long blocks;
+    size_t offset;
+  };
+  std::vector<Launch> launches;
+  size_t total = 0;
+  for (size_t gi = 0; gi < groups.size(); ++gi) {
+    const long long blocks = std::min<long long>(blocksFor(groups[gi].n), 512);
+    for (size_t first = 0; first < groups[gi].bs.size(); first += kMaxDots) {
+      const int m = static_cast<int>(std::min<size_t>(kMaxDots, groups[gi].bs.size() - first));
+      launches.push_back({gi, first, m, blocks, total});
+      total += size_t(blocks) * m;
     }
-    DotOperands ops{};
-    for (int i = 0; i < m; ++i) ops.b[i] = bs[first + i];
-    multiDotKernel<<<blocks, 256, 0, stream_>>>(a, ops, m, n, dPartials_);
+  }
+  if (static_cast<long long>(total) > partialsCap_ * 6) {
+    SC_HIP(hipStreamSynchronize(stream_));
+    devFree(dPartials_);
+    partialsCap_ = static_cast<long long>((total + 5) / 6);
+    dPartials_ = devAlloc<double>(size_t(partialsCap_) * 6);
+  }
+  if (total > dotPinnedCap_) {
+    SC_HIP(hipStreamSynchronize(stream_));
+    if (dotPinned_) SC_HIP(hipHostFree(dotPinned_));
+    dotPinned_ = nullptr;
+    dotPinnedCap_ = 0;
+    SC_HIP(hipHostMalloc(reinterpret_cast<void **>(&dotPinned_), total * sizeof(double),
+                         hipHostMallocDefault));
+    dotPinnedCap_ = total;
+  }
+  // up to kMaxDotGroups launches' worth of work per kernel
+  for (size_t first = 0; first < launches.size(); first += kMaxDotGroups) {
+    const size_t count = std::min<size_t>(kMaxDotGroups, launches.size() - first);
+    DotGroupTable t{};
+    t.numGroups = static_cast<int>(count);
+    int blocks = 0;
+    for (size_t q = 0; q < count; ++q) {
+      const Launch &l = launches[first + q];
+      const DotGroup &g = groups[l.group];
+      t.a[q] = g.a;
+      for (int i = 0; i < l.m; ++i) t.b[q][i] = g.bs[l.first + size_t(i)];
+      t.m[q] = l.m;
+      t.n[q] = g.n;
+      t.offset[q] = static_cast<long long>(l.offset);
+      t.blockStart[q] = blocks;
+      blocks += static_cast<int>(l.blocks);
+    }
+    t.blockStart[count] = blocks;
+    if (blocks == 0) continue; // empty vectors: their dots stay zero
+    multiDotGroupsKernel<<<blocks, 256, 0, stream_>>>(t, dPartials_);
     SC_HIP(hipGetLastError());
-    std::vector<double> h(size_t(blocks) * m);
-    fromDevice(h.data(), dPartials_, h.size() * sizeof(double));
-    for (long long blk = 0; blk < blocks; ++blk)
-      for (int i = 0; i < m; ++i) out[first + i] += h[size_t(blk) * m + i];
+  }
+  std::vector<std::vector<double>> out(groups.size());
+  for (size_t gi = 0; gi < groups.size(); ++gi) out[gi].assign(groups[gi].bs.size(), 0.0);
+  if (total == 0) return out;
+  SC_HIP(hipMemcpyAsync(dotPinned_, dPartials_, total * sizeof(double), hipMemcpyDeviceToHost,
+                        stream_));
+  SC_HIP(hipEventRecord(evDots_, stream_));
+  SC_HIP(hipEventSynchronize(evDots_));
+  for (const Launch &l : launches) {
+    const double *h = dotPinned_ + l.offset;
+    std::vector<double> &o = out[l.group];
+    for (long long blk = 0; blk < l.blocks; ++blk)
+      for (int i = 0; i < l.m; ++i) o[l.first + size_t(i)] += h[size_t(blk) * l.m + i];
   }
   return out;
+}
+
+void Grain::discardReductions() {
+  // the queued partials are never read; the next reservations reuse dRed_
+  // behind them in stream order
+  if (readbackQueued_) flushReductions();
+  pend_.clear();
+  redUsed_ = 0;
+  redOut_.clear();
+  redN_.clear();
 }
 
 // ---------------------------------------------------------------------------

@@ -198,6 +198,15 @@ private:
   void prepareSweep(int precision);
   std::vector<double> multiDot(const double *a, const std::vector<double *> &bs,
                                long long n);
+  // Several multiDot groups (a, bs, n) with one read-back for all of them.
+  struct DotGroup {
+    const double *a;
+    std::vector<double *> bs;
+    long long n;
+  };
+  std::vector<std::vector<double>> multiDotGroups(const std::vector<DotGroup> &groups);
+  // drops the queued reductions without reading them back
+  void discardReductions();
 
   std::unique_ptr<Setup> s_;
   hipStream_t stream_ = nullptr;
@@ -266,6 +275,9 @@ private:
   std::vector<Pending> pend_;
   std::vector<std::array<double, 6>> redOut_;
   std::vector<long long> redN_;
+  double *dotPinned_ = nullptr; // pinned host copy of dPartials_ (multiDotGroups)
+  size_t dotPinnedCap_ = 0;
+  hipEvent_t evDots_ = nullptr;
   double *redPinned_ = nullptr; // pinned host copy of dRed_
   size_t redPinnedCap_ = 0;
   bool readbackQueued_ = false;

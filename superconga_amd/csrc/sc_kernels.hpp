@@ -716,6 +716,49 @@ __global__ __launch_bounds__(256) void multiDotKernel(const double *a, DotOperan
     __syncthreads();
   }
 }
+// The same over several groups in one launch: block b belongs to the group
+// g with blockStart[g] <= b < blockStart[g + 1] and writes its partials at
+// offset[g] + (b - blockStart[g]) * m[g].
+constexpr int kMaxDotGroups = 8;
+struct DotGroupTable {
+  const double *a[kMaxDotGroups];
+  const double *b[kMaxDotGroups][kMaxDots];
+  int m[kMaxDotGroups];
+  long long n[kMaxDotGroups];
+  long long offset[kMaxDotGroups];
+  int blockStart[kMaxDotGroups + 1];
+  int numGroups;
+};
+__global__ __launch_bounds__(256) void multiDotGroupsKernel(DotGroupTable t, double *partials) {
+  int g = 0;
+  while (g + 1 < t.numGroups && int(blockIdx.x) >= t.blockStart[g + 1]) ++g;
+  const int blk = int(blockIdx.x) - t.blockStart[g];
+  const int nb = t.blockStart[g + 1] - t.blockStart[g];
+  const int m = t.m[g];
+  const long long n = t.n[g];
+  const double *a = t.a[g];
+  double s[kMaxDots];
+#pragma unroll
+  for (int i = 0; i < kMaxDots; ++i) s[i] = 0.0;
+  for (long long e = blk * (long long)blockDim.x + threadIdx.x; e < n;
+       e += (long long)nb * blockDim.x) {
+    const double x = a[e];
+#pragma unroll
+    for (int i = 0; i < kMaxDots; ++i)
+      if (i < m) s[i] += x * t.b[g][i][e];
+  }
+  __shared__ double sh[256];
+  for (int i = 0; i < m; ++i) {
+    sh[threadIdx.x] = s[i];
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+      if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[t.offset[g] + size_t(blk) * m + i] = sh[0];
+    __syncthreads();
+  }
+}
 // heavy ball (Polyak.h:216-240)
 __global__ void polyakKernel(double *x, double *vel, const double *g,
                              long long n, double s, double drag) {

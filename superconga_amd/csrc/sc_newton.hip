@@ -121,7 +121,8 @@ void Grain::nkMap(double *gu) {
   int hb[2];
   queueBoundaryCondition(hb); // new in-values, in place
   if (doA) solveVectorPotential(dCurrent_, dANew_);
-  clearReductions();
+  // the map needs none of the residual statistics: no read-back
+  discardReductions();
   const size_t nx = size_t(2 * s.Nc + (doA ? 2 : 0)) * n0;
   const size_t g = size_t(2 * s.nInG * s.Ne), gt = size_t(2 * s.nInGt * s.Ne);
   SC_NK_HIP(hipMemcpyAsync(gu, dDeltaNew_, nx * sizeof(double), hipMemcpyDeviceToDevice,
@@ -166,15 +167,18 @@ NewtonReport Grain::newtonKrylov(int maxNewton, int krylovDim, double tol) {
     wt[b] = uu > 0.0 ? 1.0 / uu : 1.0 / static_cast<double>(blocks[b].length);
   }
   // weighted dots of a with every vector of bs
+  // (the dots of every block with one read-back)
   auto wdots = [&](const double *a, const std::vector<double *> &bs) {
-    std::vector<double> out(bs.size(), 0.0);
+    std::vector<DotGroup> groups;
     for (size_t b = 0; b < blocks.size(); ++b) {
       std::vector<double *> part(bs.size());
       for (size_t i = 0; i < bs.size(); ++i) part[i] = bs[i] + blocks[b].offset;
-      const std::vector<double> d = multiDot(a + blocks[b].offset, part,
-                                             static_cast<long long>(blocks[b].length));
-      for (size_t i = 0; i < bs.size(); ++i) out[i] += wt[b] * d[i];
+      groups.push_back({a + blocks[b].offset, part, static_cast<long long>(blocks[b].length)});
     }
+    const std::vector<std::vector<double>> d = multiDotGroups(groups);
+    std::vector<double> out(bs.size(), 0.0);
+    for (size_t b = 0; b < blocks.size(); ++b)
+      for (size_t i = 0; i < bs.size(); ++i) out[i] += wt[b] * d[b][i];
     return out;
   };
   auto wnorm = [&](double *a) { return std::sqrt(wdots(a, {a})[0]); };
@@ -225,20 +229,32 @@ NewtonReport Grain::newtonKrylov(int maxNewton, int krylovDim, double tol) {
         nkSet(trial);
         nkMap(w);
         combine(w, w, 1.0 / e, {gu, V[size_t(k)]}, {-1.0 / e, -1.0});
-        // classical Gram-Schmidt, twice
+        // classical Gram-Schmidt with the norm of w read back with the
+        // projections: |w - sum d_i v_i|^2 = |w|^2 - sum d_i^2 for the
+        // orthonormal basis.  A second pass only when the first removed
+        // more than half of |w|^2 (Kahan-Parlett: then the difference has
+        // lost digits and the projection must be repeated).
         std::vector<double *> basis(V.begin(), V.begin() + k + 1);
         std::vector<const double *> cbasis(basis.begin(), basis.end());
+        std::vector<double *> withW(basis);
+        withW.push_back(w);
         std::vector<double> h(size_t(k) + 1, 0.0);
+        double hn = 0.0;
         for (int pass = 0; pass < 2; ++pass) {
-          const std::vector<double> d = wdots(w, basis);
-          std::vector<double> neg(d.size());
-          for (size_t i = 0; i < d.size(); ++i) {
+          const std::vector<double> d = wdots(w, withW);
+          const double ww = d[size_t(k) + 1];
+          double removed = 0.0;
+          std::vector<double> neg(size_t(k) + 1);
+          for (size_t i = 0; i <= size_t(k); ++i) {
             h[i] += d[i];
             neg[i] = -d[i];
+            removed += d[i] * d[i];
           }
           combine(w, w, 1.0, cbasis, neg);
+          const double rem = ww - removed;
+          hn = std::sqrt(std::max(rem, 0.0));
+          if (rem > 0.5 * ww) break;
         }
-        const double hn = wnorm(w);
         for (int i = 0; i <= k; ++i) H[size_t(i) * m + k] = R[size_t(i) * m + k] = h[size_t(i)];
         H[size_t(k + 1) * m + k] = R[size_t(k + 1) * m + k] = hn;
         for (int i = 0; i < k; ++i) {
