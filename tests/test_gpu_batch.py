@@ -20,13 +20,17 @@ def batch_configs():
     ]
 
 
-def test_batch_equals_individual_grains(product):
+@pytest.mark.parametrize("shape", [(8, 2, 8), (3, 3, 16)])
+def test_batch_equals_individual_grains(product, shape):
+    # (3, 3, 16): 9 energies give 3 groups per column (21 columns, the last
+    # lane idle, groups summed lane by lane), 10 energies 4 groups (16
+    # columns, butterfly), both layouts in one launch
     cfgs = batch_configs()
     b = product.Batch(cfgs)
-    b.set_sweep_shape(8, 2, 8)
+    b.set_sweep_shape(*shape)
     singles = [product.Simulation(c) for c in cfgs]
     for s in singles:
-        s.set_sweep_shape(8, 2, 8)
+        s.set_sweep_shape(*shape)
     for i, s in enumerate(singles):
         b.grain(i).set_order_parameter(0, s.order_parameter(0)[0])
         for c in range(1, s.Nc):
