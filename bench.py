@@ -155,6 +155,26 @@ def converge_newton(sa, cfg, cap, warm_residual=1e-3, warm_max=30):
             "residual": r["residual"], "free_energy_per_area": r["free_energy_per_area"]}
 
 
+def convergence(sa, workloads, args):
+    """Wall time to a 1e-8 residual from the same initial state: plain FP64
+    iterations, mixed refinement, Newton-Krylov."""
+    ccfg = workloads.WORKLOADS[args.converge_workload]()
+    ccfg["numerics"]["convergence_criterion"] = 1e-8
+    conv = {"workload": args.converge_workload, "criterion": 1e-8}
+    for prec in ("fp64", "mixed_refine"):
+        conv[prec] = converge(sa, ccfg, prec, args.converge_cap)
+        log("converge %s %s: %s" % (args.converge_workload, prec, conv[prec]))
+    if conv["fp64"]["converged"] and conv["mixed_refine"]["converged"]:
+        conv["speedup_mixed_refine"] = conv["fp64"]["wall_s"] / conv["mixed_refine"]["wall_s"]
+    if not args.no_newton:
+        conv["newton_krylov"] = converge_newton(sa, ccfg, args.converge_cap)
+        log("converge %s newton-krylov: %s" % (args.converge_workload, conv["newton_krylov"]))
+        if conv["fp64"]["converged"] and conv["newton_krylov"]["converged"]:
+            conv["speedup_newton_krylov"] = (conv["fp64"]["wall_s"]
+                                             / conv["newton_krylov"]["wall_s"])
+    return conv
+
+
 def ldos_bench(sa, workloads, args):
     """The LDOS postprocess of the workload's grain with the reference's
     example spectroscopy settings (examples/*/postprocess_config.json: 720
@@ -319,6 +339,8 @@ def main():
     ap.add_argument("--ldos-steps", type=int, default=3)
     ap.add_argument("--ldos-shape", default="", help="fixed LDOS sweep shape")
     ap.add_argument("--no-newton", action="store_true")
+    ap.add_argument("--converge-only", action="store_true",
+                    help="only the time-to-1e-8 legs (for profiling them)")
     ap.add_argument("--cpu-threads", type=int,
                     default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
                     or min(16, os.cpu_count() or 1))
@@ -341,6 +363,9 @@ def main():
         return
     if args.ldos_only:
         print(json.dumps(ldos_bench(sa, workloads, args)), flush=True)
+        return
+    if args.converge_only:
+        print(json.dumps(convergence(sa, workloads, args)), flush=True)
         return
 
     cfg = workloads.WORKLOADS[args.workload]()
@@ -423,21 +448,7 @@ def main():
         sim.close()
 
     if not args.no_converge:
-        ccfg = workloads.WORKLOADS[args.converge_workload]()
-        ccfg["numerics"]["convergence_criterion"] = 1e-8
-        conv = {"workload": args.converge_workload, "criterion": 1e-8}
-        for prec in ("fp64", "mixed_refine"):
-            conv[prec] = converge(sa, ccfg, prec, args.converge_cap)
-            log("converge %s %s: %s" % (args.converge_workload, prec, conv[prec]))
-        if conv["fp64"]["converged"] and conv["mixed_refine"]["converged"]:
-            conv["speedup_mixed_refine"] = conv["fp64"]["wall_s"] / conv["mixed_refine"]["wall_s"]
-        if not args.no_newton:
-            conv["newton_krylov"] = converge_newton(sa, ccfg, args.converge_cap)
-            log("converge %s newton-krylov: %s" % (args.converge_workload, conv["newton_krylov"]))
-            if conv["fp64"]["converged"] and conv["newton_krylov"]["converged"]:
-                conv["speedup_newton_krylov"] = (conv["fp64"]["wall_s"]
-                                                 / conv["newton_krylov"]["wall_s"])
-        out["time_to_1e-8"] = conv
+        out["time_to_1e-8"] = convergence(sa, workloads, args)
 
     if not args.no_ldos:
         out["ldos"] = ldos_bench(sa, workloads, args)
