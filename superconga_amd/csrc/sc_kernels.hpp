@@ -145,25 +145,38 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev
     T esPrev = T(0);
     bool havePrev = false;
     int8_t lcur = lab[size_t(it.yhi) * N1];
-    for (int t0 = 0; t0 < len; t0 += gp) {
-      double fre = 0.0, fim = 0.0, fes = 0.0;
-      {
-        const int tt = t0 + g;
-        if (colActive && g < gp && tt < len) {
-          const int y = it.yhi - tt;
-          const size_t fi = size_t(k - v.fieldBase) * n1 + size_t(y) * N1 + x;
-          fre = v.fields[fi];
-          fim = v.fields[v.fieldPlane + fi];
-          fes = v.fields[2 * v.fieldPlane + fi];
-        }
+    // the fields and label of the lane's row of the next chunk, loaded while
+    // the current chunk is swept
+    double nre = 0.0, nim = 0.0, nes = 0.0;
+    int nlab = 0;
+    auto loadChunk = [&](int t0) {
+      const int tt = t0 + g;
+      if (colActive && g < gp && tt < len) {
+        const int y = it.yhi - tt;
+        const size_t fi = size_t(k - v.fieldBase) * n1 + size_t(y) * N1 + x;
+        nre = v.fields[fi];
+        nim = v.fields[v.fieldPlane + fi];
+        nes = v.fields[2 * v.fieldPlane + fi];
+        nlab = lab[size_t(y) * N1];
+      } else {
+        nre = nim = nes = 0.0;
+        nlab = 0;
       }
+    };
+    loadChunk(0);
+    for (int t0 = 0; t0 < len; t0 += gp) {
+      const double fre = nre, fim = nim, fes = nes;
+      const int flab = nlab;
+      const bool more = t0 + gp < len;
+      if (more) loadChunk(t0 + gp);
       const int tEnd = min(gp, len - t0);
       for (int i = 0; i < tEnd; ++i) {
         const int src = i * cols + c;
         const double dre = __shfl(fre, src), dim = __shfl(fim, src), e = __shfl(fes, src);
         const int t = t0 + i;
-        const int y = it.yhi - t;
-        const int8_t lnext = (t + 1 < len) ? lab[size_t(y - 1) * N1] : int8_t(0);
+        // the next site's label: in this chunk, or the first row of the next
+        const int ln = i + 1 < tEnd ? __shfl(flab, src + cols) : (more ? __shfl(nlab, c) : 0);
+        const int8_t lnext = int8_t(ln);
         const Cx<T> D = {T(-dre), T(dim)}; // -conj(Delta) for gamma~
         const T es = T(e);
         if (active) {
@@ -239,25 +252,36 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev
       for (int j = 0; j < EL; ++j)
         gtNext[j] = loadStream(scr + (size_t(len - 1) * EL + j) * 64 + lane);
     }
-    for (int t0 = 0; t0 < len; t0 += gp) {
-      double fre = 0.0, fim = 0.0, fes = 0.0;
-      {
-        const int tt = t0 + g;
-        if (colActive && g < gp && tt < len) {
-          const int y = it.ylo + tt;
-          const size_t fi = size_t(k - v.fieldBase) * n1 + size_t(y) * N1 + x;
-          fre = v.fields[fi];
-          fim = v.fields[v.fieldPlane + fi];
-          fes = v.fields[2 * v.fieldPlane + fi];
-        }
+    double nre = 0.0, nim = 0.0, nes = 0.0;
+    int nlab = 0;
+    auto loadChunk = [&](int t0) {
+      const int tt = t0 + g;
+      if (colActive && g < gp && tt < len) {
+        const int y = it.ylo + tt;
+        const size_t fi = size_t(k - v.fieldBase) * n1 + size_t(y) * N1 + x;
+        nre = v.fields[fi];
+        nim = v.fields[v.fieldPlane + fi];
+        nes = v.fields[2 * v.fieldPlane + fi];
+        nlab = lab[size_t(y) * N1];
+      } else {
+        nre = nim = nes = 0.0;
+        nlab = 0;
       }
+    };
+    loadChunk(0);
+    for (int t0 = 0; t0 < len; t0 += gp) {
+      const double fre = nre, fim = nim, fes = nes;
+      const int flab = nlab;
+      const bool more = t0 + gp < len;
+      if (more) loadChunk(t0 + gp);
       const int tEnd = min(gp, len - t0);
       for (int i = 0; i < tEnd; ++i) {
         const int src = i * cols + c;
         const double dre = __shfl(fre, src), dim = __shfl(fim, src), e = __shfl(fes, src);
         const int t = t0 + i;
         const int y = it.ylo + t;
-        const int8_t lnext = (t + 1 < len) ? lab[size_t(y + 1) * N1] : int8_t(0);
+        const int ln = i + 1 < tEnd ? __shfl(flab, src + cols) : (more ? __shfl(nlab, c) : 0);
+        const int8_t lnext = int8_t(ln);
         const Cx<T> D = {T(dre), T(dim)};
         const T es = T(e);
         Cx<T> gtCur[EL];
