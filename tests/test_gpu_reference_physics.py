@@ -1,0 +1,91 @@
+"""The reference's own physics known-answer tests, run on the product in
+FP64 with the reference's parameters, iteration limits and tolerances:
+
+* tests/physics/swave_disc_test.cu: an s-wave disc (R = 20 xi0) in 5 flux
+  quanta with penetration depth 5 xi0 converges within 100
+  Barzilai-Borwein iterations, and its free energy per area is the London
+  result F_bulk + F_kinetic + F_magnetic (modified Bessel functions of R/k,
+  k = lambda / sqrt(Yoshida)) to 1e-3;
+* tests/physics/swave_annulus_solenoid_gauge_test.cu: an s-wave annulus
+  (R = 20, r = 10 xi0) with phase winding 1 in the solenoid gauge converges
+  at flux -1.5, -1, -0.5, and its free energy per area is
+  F_bulk + Y ln(R/r) / (4 (R^2 - r^2)) (Phi - q w)^2 to 1e-3, lowest at
+  Phi = q w.
+
+The bulk gap, bulk free energy and Yoshida function come from the oracle,
+whose values tests/test_oracle_closed_forms.py pins on the reference's
+tests/order_parameter_bulk_test.cu.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy import special
+
+import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def bulk_terms(oracle, temperature, cutoff):
+    d = oracle.bulk_gap(temperature)
+    return oracle.bulk_free_energy(temperature, d), oracle.yoshida(temperature, d, cutoff)
+
+
+def test_reference_swave_disc_free_energy(product, oracle):
+    # tests/physics/swave_disc_test.cu:60-160
+    T, cutoff, lam, flux, R = 0.2, 16.0, 5.0, 5.0, 20.0
+    cfg = cases.base(temperature=T, flux=flux, lam=lam, angles=16, ppxi=2.7, cutoff=cutoff,
+                     norm="linf", criterion=1e-3, accelerator="barzilai-borwein",
+                     width=2 * R)
+    s = product.Simulation(cfg)
+    converged = s.run(0, 0, 100, "fp64")
+    f = s.results()["free_energy_per_area"]
+    # the analytic London result (helper::computeFreeEnergyAnalytic)
+    f_bulk, y = bulk_terms(oracle, T, cutoff)
+    k = lam / math.sqrt(y)
+    area = math.pi * R * R
+    b = math.pi * flux / area
+    factor = 2 * math.pi * y * (k * b) ** 2 / (2 * area)
+    i0, i1 = special.iv(0, R / k), special.iv(1, R / k)
+    tmp = R * i1 / i0
+    f_mag = factor * (R * R - 2 * k * tmp - 0.5 * tmp * tmp)
+    f_kin = factor * 0.5 * R * (-R * i0 * i0 + 2 * k * i0 * i1 + R * i1 * i1) / (i0 * i0)
+    expected = f_bulk + f_kin + f_mag
+    print("s-wave disc: %d iterations, F/area %.6f, London %.6f, rel %.2e" % (
+        s.results()["iterations"], f, expected, abs(f - expected) / abs(expected)))
+    assert converged
+    assert abs(f - expected) <= 1e-3 * abs(expected)
+    s.close()
+
+
+def test_reference_swave_annulus_solenoid_free_energy(product, oracle):
+    # tests/physics/swave_annulus_solenoid_gauge_test.cu:38-170
+    T, cutoff, R, r = 0.2, 16.0, 20.0, 10.0
+    winding, charge = 1.0, -1
+    qw = charge * winding
+    fluxes = [qw - 0.5, qw, qw + 0.5]
+    f_bulk, y = bulk_terms(oracle, T, cutoff)
+    factor = 0.25 * y * math.log(R / r) / (R * R - r * r)
+    energies = []
+    for flux in fluxes:
+        cfg = cases.base(
+            order_parameter={"s": {"critical_temperature": 1.0,
+                                   "vortices": [{"center_x": 0.0, "center_y": 0.0,
+                                                 "winding_number": winding}]}},
+            temperature=T, flux=flux, lam=-1.0, angles=16, ppxi=3.5, cutoff=cutoff,
+            norm="linf", criterion=1e-4, accelerator="barzilai-borwein", step_size=1.0,
+            gauge="solenoid", width=2 * R,
+            components=[{"disc": {"radius": 0.5}},
+                        {"disc": {"radius": 0.5 * r / R, "add": False}}])
+        s = product.Simulation(cfg)
+        converged = s.run(0, 0, 100, "fp64")
+        f = s.results()["free_energy_per_area"]
+        expected = f_bulk + factor * (flux - qw) ** 2
+        print("annulus, flux %+.1f: %d iterations, F/area %.6f, expected %.6f, rel %.2e" % (
+            flux, s.results()["iterations"], f, expected, abs(f - expected) / abs(expected)))
+        assert converged
+        assert abs(f - expected) <= 1e-3 * abs(expected)
+        energies.append(f)
+        s.close()
+    assert energies[1] < energies[0] and energies[1] < energies[2]
