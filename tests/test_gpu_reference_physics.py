@@ -10,7 +10,10 @@ FP64 with the reference's parameters, iteration limits and tolerances:
   (R = 20, r = 10 xi0) with phase winding 1 in the solenoid gauge converges
   at flux -1.5, -1, -0.5, and its free energy per area is
   F_bulk + Y ln(R/r) / (4 (R^2 - r^2)) (Phi - q w)^2 to 1e-3, lowest at
-  Phi = q w.
+  Phi = q w;
+* tests/physics/current_phase_gradient_test.cu: after one iteration of a
+  disc with phase winding 1 and no field, j_i d_i phase has the sign of
+  q w everywhere (zero allowed).
 
 The bulk gap, bulk free energy and Yoshida function come from the oracle,
 whose values tests/test_oracle_closed_forms.py pins on the reference's
@@ -89,3 +92,66 @@ def test_reference_swave_annulus_solenoid_free_energy(product, oracle):
         energies.append(f)
         s.close()
     assert energies[1] < energies[0] and energies[1] < energies[2]
+
+
+def phase_gradient(delta, h=1.0):
+    """GridFunctions::complexPhaseGradient (GridFunctions_CPU.h:343-410):
+    central differences of the phase from Re and Im, zero on the border and
+    where 2 h |delta|^2 <= 1e-5."""
+    re, im = delta.real, delta.imag
+    gx = np.zeros_like(re)
+    gy = np.zeros_like(re)
+    denom = 2.0 * h * (re * re + im * im)
+    c = (slice(1, -1), slice(1, -1))
+    ok = denom[c] > 1e-5
+    inv = np.where(ok, 1.0 / np.where(ok, denom[c], 1.0), 0.0)
+    dx_im = im[1:-1, 2:] - im[1:-1, :-2]
+    dx_re = re[1:-1, 2:] - re[1:-1, :-2]
+    dy_im = im[2:, 1:-1] - im[:-2, 1:-1]
+    dy_re = re[2:, 1:-1] - re[:-2, 1:-1]
+    gx[c] = (dx_im * re[c] - im[c] * dx_re) * inv
+    gy[c] = (dy_im * re[c] - im[c] * dy_re) * inv
+    return gx, gy
+
+
+def phase_gradient_case():
+    # tests/physics/current_phase_gradient_test.cu:33-75
+    return cases.base(order_parameter={"s": {"critical_temperature": 1.0,
+                                             "vortices": [{"center_x": 0.0, "center_y": 0.0,
+                                                           "winding_number": 1.0}]}},
+                      temperature=0.9, flux=0.0, lam=-1.0, angles=16, ppxi=2.0, cutoff=16.0,
+                      norm="linf", criterion=1e-5, accelerator="barzilai-borwein", width=10.0)
+
+
+def check_overlap_sign(j, delta, expected_sign):
+    gx, gy = phase_gradient(delta)
+    for comp, g in ((j[0], gx), (j[1], gy)):
+        overlap = comp * g
+        # the reference's check: neither extreme has the wrong sign
+        assert np.sign(overlap.min()) != -expected_sign
+        assert np.sign(overlap.max()) != -expected_sign
+        # and the current does flow along the phase gradient
+        assert np.max(np.abs(overlap)) > 0.0
+
+
+def test_reference_current_follows_phase_gradient(product):
+    s = product.Simulation(phase_gradient_case())
+    s.compute("fp64")
+    check_overlap_sign(s.current(), s.order_parameter(0)[0], expected_sign=-1)
+    s.close()
+
+
+def test_reference_penetration_depth(product):
+    # tests/physics/penetration_depth_test.cu: the flux through a disc in a
+    # weak field fitted with the London expression gives the penetration
+    # depth back to 10% (the reference's tolerance); the oracle's fit is
+    # 1.107 (tests/test_oracle_reference_physics.py)
+    from test_oracle_reference_physics import penetration_case, penetration_depth_from
+    s = product.Simulation(penetration_case())
+    converged = s.run(0, 0, 100, "fp64")
+    d = s.dims
+    k = penetration_depth_from(s.vector_potential(0), s.domain(), d.h, d.num_grain_sites)
+    print("penetration depth fit %.4f after %d iterations" % (k, s.results()["iterations"]))
+    assert converged
+    assert abs(k - 1.0) <= 0.1 * max(k, 1.0)
+    s.close()

@@ -6,7 +6,10 @@ checks run on the product in tests/test_gpu_reference_physics.py):
   5 flux quanta, 1e-3;
 * tests/physics/swave_annulus_solenoid_gauge_test.cu: free energy of an
   annulus with phase winding 1, parabolic in the flux with its minimum at
-  Phi = q w, 1e-3.
+  Phi = q w, 1e-3;
+* tests/physics/current_phase_gradient_test.cu: after one iteration of a
+  disc with phase winding 1 and no field, the current never points along
+  q grad(phase) with the wrong sign (j_i d_i phase <= 0 for charge -1).
 """
 import math
 
@@ -72,3 +75,110 @@ def test_swave_annulus_solenoid_parabola():
         assert abs(f - (f_bulk + factor * (flux - qw) ** 2)) <= 1e-3 * abs(f)
         energies.append(f)
     assert energies[1] < energies[0] and energies[1] < energies[2]
+
+
+def phase_gradient(delta, h=1.0):
+    """GridFunctions::complexPhaseGradient (GridFunctions_CPU.h:343-410):
+    central differences of the phase from Re and Im, zero on the border and
+    where 2 h |delta|^2 <= 1e-5."""
+    re, im = delta.real, delta.imag
+    gx = np.zeros_like(re)
+    gy = np.zeros_like(re)
+    denom = 2.0 * h * (re * re + im * im)
+    c = (slice(1, -1), slice(1, -1))
+    ok = denom[c] > 1e-5
+    inv = np.where(ok, 1.0 / np.where(ok, denom[c], 1.0), 0.0)
+    dx_im = im[1:-1, 2:] - im[1:-1, :-2]
+    dx_re = re[1:-1, 2:] - re[1:-1, :-2]
+    dy_im = im[2:, 1:-1] - im[:-2, 1:-1]
+    dy_re = re[2:, 1:-1] - re[:-2, 1:-1]
+    gx[c] = (dx_im * re[c] - im[c] * dx_re) * inv
+    gy[c] = (dy_im * re[c] - im[c] * dy_re) * inv
+    return gx, gy
+
+
+def phase_gradient_case():
+    # tests/physics/current_phase_gradient_test.cu:33-75
+    return cases.base(order_parameter={"s": {"critical_temperature": 1.0,
+                                             "vortices": [{"center_x": 0.0, "center_y": 0.0,
+                                                           "winding_number": 1.0}]}},
+                      temperature=0.9, flux=0.0, lam=-1.0, angles=16, ppxi=2.0, cutoff=16.0,
+                      norm="linf", criterion=1e-5, accelerator="barzilai-borwein", width=10.0)
+
+
+def check_overlap_sign(j, delta, expected_sign):
+    gx, gy = phase_gradient(delta)
+    for comp, g in ((j[0], gx), (j[1], gy)):
+        overlap = comp * g
+        # the reference's check: neither extreme has the wrong sign
+        assert np.sign(overlap.min()) != -expected_sign
+        assert np.sign(overlap.max()) != -expected_sign
+        # and the current does flow along the phase gradient
+        assert np.max(np.abs(overlap)) > 0.0
+
+
+def test_current_follows_phase_gradient():
+    o = po.Oracle(phase_gradient_case())
+    o.initial_bulk()
+    n = o.N0
+    c = (n - 1) / 2.0
+    yy, xx = np.mgrid[0:n, 0:n]
+    delta = o.delta()[0] * np.exp(1j * np.arctan2(yy - c, xx - c))
+    o.set_order_parameter(0, delta)
+    o.compute()
+    # the order parameter the current was computed from (the accelerator
+    # steps at the next iteration)
+    check_overlap_sign(o.current(), o.delta()[0], expected_sign=-1)
+
+
+def fit_penetration_depth(lam0, b_ext, width, flux):
+    """fitPenetrationDepth (tests/physics/penetration_depth_test.cu:38-70):
+    Newton on the London flux 2 pi B k ((R - k) + k exp(-R / k))."""
+    R = 0.5 * width
+    factor = 2.0 * math.pi * b_ext
+    k = lam0
+    for _ in range(200):
+        r = factor * k * ((R - k) + k * math.exp(-R / k)) - flux
+        drdk = factor * ((R + 2 * k) * math.exp(-R / k) + R - 2 * k)
+        dk = r * np.sign(drdk) / max(abs(drdk), np.finfo(float).eps)
+        k -= dk
+        if abs(dk / (k + dk)) < 1e-6:
+            break
+    return k
+
+
+def curl(ax, ay, h):
+    """curl_kernel (GridFunctions_GPU.h:327-353): central differences, one
+    sided on the border, divided by h."""
+    n = ax.shape[0]
+    idx = np.arange(n)
+    xn, xp = np.minimum(idx + 1, n - 1), np.maximum(idx - 1, 0)
+    dAy = (ay[:, xn] - ay[:, xp]) / (xn - xp)[None, :]
+    dAx = (ax[xn, :] - ax[xp, :]) / (xn - xp)[:, None]
+    return (dAy - dAx) / h
+
+
+def penetration_case():
+    # tests/physics/penetration_depth_test.cu:90-125
+    return cases.base(temperature=0.1, flux=0.5, lam=1.0, angles=16, ppxi=5.0, cutoff=16.0,
+                      norm="linf", criterion=1e-3, accelerator="congacc", step_size=0.001,
+                      width=20.0)
+
+
+def penetration_depth_from(a, domain, h, num_grain, flux_quanta=0.5, width=20.0):
+    # the induced potential (lambda = 1: the stored lambda^2 A is A)
+    area = h * h * num_grain
+    b_ext = math.pi * flux_quanta / area
+    b = np.where(domain > 0, curl(a[0], a[1], h), 0.0)
+    flux = area * (b_ext + b.sum() / num_grain)
+    return fit_penetration_depth(1.0, b_ext, width, flux)
+
+
+def test_penetration_depth_london_fit():
+    o = po.Oracle(penetration_case())
+    o.initial_bulk()
+    assert o.run(0, 0, 100)
+    k = penetration_depth_from(o.vector_potential(), o.domain(), o.get("h"),
+                               o.get_int("num_grain_sites"))
+    print("oracle penetration depth fit %.4f" % k)
+    assert abs(k - 1.0) <= 0.1 * max(k, 1.0)
