@@ -13,7 +13,13 @@ FP64 with the reference's parameters, iteration limits and tolerances:
   Phi = q w;
 * tests/physics/current_phase_gradient_test.cu: after one iteration of a
   disc with phase winding 1 and no field, j_i d_i phase has the sign of
-  q w everywhere (zero allowed).
+  q w everywhere (zero allowed);
+* tests/physics/penetration_depth_test.cu: the London fit of the flux
+  through a disc in a weak field gives the penetration depth to 10%;
+* tests/physics/low_temp_free_energy_test.cu: at T = 0.001 (281 Ozaki
+  poles) an s-wave disc, an s-wave square and a d-wave square keep the
+  T = 0 bulk gap on average to 1e-5, carry no current (below machine
+  epsilon), and have F / area = -Delta_bulk^2 / 2 to 1e-4.
 
 The bulk gap, bulk free energy and Yoshida function come from the oracle,
 whose values tests/test_oracle_closed_forms.py pins on the reference's
@@ -154,4 +160,17 @@ def test_reference_penetration_depth(product):
     print("penetration depth fit %.4f after %d iterations" % (k, s.results()["iterations"]))
     assert converged
     assert abs(k - 1.0) <= 0.1 * max(k, 1.0)
+    s.close()
+
+
+@pytest.mark.parametrize("symmetry,shape", [("s", "disc"), ("s", "square"),
+                                            ("dx2-y2", "square")])
+def test_reference_low_temperature_bulk_values(product, symmetry, shape):
+    from test_oracle_reference_physics import check_low_temperature, low_temperature_case
+    s = product.Simulation(low_temperature_case(symmetry, shape))
+    converged = s.run(0, 1, 50, "fp64")
+    r = s.results()
+    print("%s %s: %d iterations" % (symmetry, shape, r["iterations"]))
+    check_low_temperature(symmetry, s.order_parameter(0)[0], s.domain(), s.current(),
+                          s.dims.num_grain_sites, r["free_energy_per_area"], converged)
     s.close()

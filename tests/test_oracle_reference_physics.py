@@ -9,11 +9,18 @@ checks run on the product in tests/test_gpu_reference_physics.py):
   Phi = q w, 1e-3;
 * tests/physics/current_phase_gradient_test.cu: after one iteration of a
   disc with phase winding 1 and no field, the current never points along
-  q grad(phase) with the wrong sign (j_i d_i phase <= 0 for charge -1).
+  q grad(phase) with the wrong sign (j_i d_i phase <= 0 for charge -1);
+* tests/physics/penetration_depth_test.cu: the London fit of the flux
+  through a disc in a weak field gives the penetration depth to 10%;
+* tests/physics/low_temp_free_energy_test.cu: at T = 0.001 an s-wave disc,
+  an s-wave square and a d-wave square (crystal axes along the edges, so
+  specular edges break no pairs) keep the T = 0 bulk gap on average to
+  1e-5, carry no current, and have F / area = -Delta_bulk^2 / 2 to 1e-4.
 """
 import math
 
 import numpy as np
+import pytest
 from scipy import special
 
 import cases
@@ -182,3 +189,51 @@ def test_penetration_depth_london_fit():
                                o.get_int("num_grain_sites"))
     print("oracle penetration depth fit %.4f" % k)
     assert abs(k - 1.0) <= 0.1 * max(k, 1.0)
+
+
+# include/conga/defines.h:25-46: the T = 0 bulk gaps in units of 2 pi kB Tc
+EULER_MASCHERONI = 0.57721566490153286060651209008240243
+S_WAVE_BULK = 0.5 * math.exp(-EULER_MASCHERONI)
+D_WAVE_BULK = math.exp(-EULER_MASCHERONI - 0.5) / math.sqrt(2.0)
+
+LOW_TEMPERATURE_CASES = [("s", "disc"), ("s", "square"), ("dx2-y2", "square")]
+
+
+def low_temperature_case(symmetry, shape):
+    """tests/physics/low_temp_free_energy_test.cu:28-120: T = 0.001, grain
+    side 10 xi0, 1 point per xi0, 32 angles for s-wave and 360 for d-wave,
+    cutoff 32, L-infinity criterion 1e-3, Barzilai-Borwein, bulk start."""
+    comp = {"disc": {"radius": 0.5}} if shape == "disc" else {"regular_polygon": {"num_edges": 4}}
+    return cases.base(order_parameter={symmetry: {"critical_temperature": 1.0}},
+                      temperature=0.001, flux=0.0, lam=-1.0,
+                      angles=32 if symmetry == "s" else 360, ppxi=1.0, cutoff=32.0,
+                      norm="linf", criterion=1e-3, accelerator="barzilai-borwein",
+                      width=10.0, components=[comp])
+
+
+def check_low_temperature(symmetry, delta, domain, current, num_grain, free_energy, converged):
+    """The reference's checks (low_temp_free_energy_test.cu:170-219): the
+    order parameter zeroed outside the grain, its summed magnitude over the
+    grain's site count; the current's likewise; doctest Approx tolerances."""
+    expected = S_WAVE_BULK if symmetry == "s" else D_WAVE_BULK
+    avg_delta = np.abs(np.where(domain > 0, delta, 0.0)).sum() / num_grain
+    avg_j = np.hypot(current[0], current[1]).sum() / num_grain
+    f_expected = -0.5 * expected * expected
+    print("%s: <|Delta|> %.10f (rel %.1e), <|j|> %.1e, F/area %.8f (rel %.1e)" % (
+        symmetry, avg_delta, abs(avg_delta - expected) / expected, avg_j, free_energy,
+        abs(free_energy - f_expected) / abs(f_expected)))
+    assert abs(avg_delta - expected) <= 1e-5 * expected
+    # Approx(0).epsilon(eps): |<|j|>| < eps (1 + |<|j|>|)
+    assert avg_j < np.finfo(float).eps * (1.0 + avg_j)
+    assert abs(free_energy - f_expected) <= 1e-4 * abs(f_expected)
+    assert converged
+
+
+@pytest.mark.parametrize("symmetry,shape", LOW_TEMPERATURE_CASES)
+def test_low_temperature_bulk_values(symmetry, shape):
+    o = po.Oracle(low_temperature_case(symmetry, shape))
+    o.initial_bulk()
+    converged = o.run(0, 1, 50)
+    check_low_temperature(symmetry, o.delta()[0], o.domain(), o.current(),
+                          o.get_int("num_grain_sites"), o.get("free_energy_per_area"),
+                          converged)
