@@ -19,7 +19,12 @@ FP64 with the reference's parameters, iteration limits and tolerances:
 * tests/physics/low_temp_free_energy_test.cu: at T = 0.001 (281 Ozaki
   poles) an s-wave disc, an s-wave square and a d-wave square keep the
   T = 0 bulk gap on average to 1e-5, carry no current (below machine
-  epsilon), and have F / area = -Delta_bulk^2 / 2 to 1e-4.
+  epsilon), and have F / area = -Delta_bulk^2 / 2 to 1e-4;
+* tests/physics/swave_annulus_radial_gauge_test.cu: an annulus (R = 20,
+  r = 10, lambda = 30) with phase winding -2 in one flux quantum has the
+  London free energy (1e-3), magnetic free energy (2e-2) and magnetic
+  moment (1e-3), the hole's field energy written as that of its uniform
+  field (tests/test_oracle_reference_physics.py:annulus_radial_expected).
 
 The bulk gap, bulk free energy and Yoshida function come from the oracle,
 whose values tests/test_oracle_closed_forms.py pins on the reference's
@@ -173,4 +178,18 @@ def test_reference_low_temperature_bulk_values(product, symmetry, shape):
     print("%s %s: %d iterations" % (symmetry, shape, r["iterations"]))
     check_low_temperature(symmetry, s.order_parameter(0)[0], s.domain(), s.current(),
                           s.dims.num_grain_sites, r["free_energy_per_area"], converged)
+    s.close()
+
+
+def test_reference_swave_annulus_radial_free_energy_and_moment(product):
+    from test_oracle_reference_physics import (annulus_radial_case, check_annulus_radial,
+                                               magnetic_moment_per_area)
+    s = product.Simulation(annulus_radial_case())
+    converged = s.run(0, 0, 100, "fp64")
+    r = s.results()
+    d = s.dims
+    print("annulus (radial): %d iterations" % r["iterations"])
+    check_annulus_radial(r["free_energy_per_area"], r["magnetic_free_energy_per_area"],
+                         magnetic_moment_per_area(s.current(), d.h, d.num_grain_sites),
+                         converged)
     s.close()

@@ -15,7 +15,12 @@ checks run on the product in tests/test_gpu_reference_physics.py):
 * tests/physics/low_temp_free_energy_test.cu: at T = 0.001 an s-wave disc,
   an s-wave square and a d-wave square (crystal axes along the edges, so
   specular edges break no pairs) keep the T = 0 bulk gap on average to
-  1e-5, carry no current, and have F / area = -Delta_bulk^2 / 2 to 1e-4.
+  1e-5, carry no current, and have F / area = -Delta_bulk^2 / 2 to 1e-4;
+* tests/physics/swave_annulus_radial_gauge_test.cu: an annulus (R = 20,
+  r = 10, lambda = 30) with phase winding -2 in one flux quantum: free
+  energy to 1e-3, magnetic free energy to 2e-2 and magnetic moment to 1e-3
+  of the London (Bessel-function) result, with the field energy of the hole
+  written as that of its uniform field (see annulus_radial_expected).
 """
 import math
 
@@ -237,3 +242,99 @@ def test_low_temperature_bulk_values(symmetry, shape):
     check_low_temperature(symmetry, o.delta()[0], o.domain(), o.current(),
                           o.get_int("num_grain_sites"), o.get("free_energy_per_area"),
                           converged)
+
+
+def annulus_radial_expected(R, r, lam, flux, winding, charge, T, cutoff):
+    """The London result for an s-wave annulus
+    (tests/physics/swave_annulus_radial_gauge_test.cu:76-330): (a, b, c)
+    from the boundary conditions, the grain's kinetic and magnetic terms by
+    32-point Gauss-Legendre quadrature over [r, R], the magnetic moment in
+    closed form.  Returns (F / area, F_magnetic / area, moment / area).
+
+    The hole's field energy per grain area is written as that of its
+    uniform induced field B_h, factor * B_h^2 r^2 / 2 with
+    factor = 2 pi lambda^2 / (2 A_grain) -- the grain integrand's own
+    normalisation at x = r, where the field is continuous (B_h equals the
+    grain's induced field at r).  evalFreeEnergy writes (B_h r)^2, which
+    equals this only where pi lambda^2 = 2 A_grain; for this annulus
+    (lambda = 30, A_grain = 300 pi) the two differ by a factor 1.5."""
+    d = po.bulk_gap(T)
+    f_bulk = po.bulk_free_energy(T, d)
+    k = lam / math.sqrt(po.yoshida(T, d, cutoff))
+    hole = math.pi * r * r
+    grain = math.pi * R * R - hole
+    Rk, rk = R / k, r / k
+    iv, kv = special.iv, special.kv
+    qw = charge * winding
+    inv_m = math.pi / (iv(0, Rk) * kv(2, rk) - iv(2, rk) * kv(0, Rk))
+    a = inv_m * (flux * kv(2, rk) / grain - qw * kv(0, Rk) / hole)
+    b = inv_m * (flux * iv(2, rk) / grain - qw * iv(0, Rk) / hole)
+    c = inv_m * (2.0 * math.pi * k * k * flux / grain
+                 + qw * (iv(0, Rk) * kv(0, rk) - iv(0, rk) * kv(0, Rk)))
+    factor = 2.0 * math.pi * lam * lam / (2.0 * grain)
+    b_ext = math.pi * flux / grain
+    xg, wg = np.polynomial.legendre.leggauss(32)
+    x = 0.5 * (R - r) * xg + 0.5 * (R + r)
+    wq = 0.5 * (R - r) * wg
+    f_mag_grain = factor * np.dot(wq, x * (a * iv(0, x / k) - b * kv(0, x / k) - b_ext) ** 2)
+    b_hole = c / hole - b_ext
+    f_mag = f_mag_grain + factor * 0.5 * (b_hole * r) ** 2
+    f_kin = factor * np.dot(wq, x * (a * iv(1, x / k) + b * kv(1, x / k)) ** 2)
+    moment = -factor * (R * R * (a * iv(2, Rk) - b * kv(2, Rk))
+                        - r * r * (a * iv(2, rk) - b * kv(2, rk)))
+    return f_bulk + f_kin + f_mag, f_mag, moment
+
+
+# tests/physics/swave_annulus_radial_gauge_test.cu:346-400, with the exact
+# lattice Green function for the induced potential (vector_potential_error
+# 0) where the reference test approximates it to 1e-3
+ANNULUS_RADIAL = dict(R=20.0, r=10.0, lam=30.0, flux=1.0, winding=-2.0, charge=-1,
+                      T=0.2, cutoff=16.0)
+
+
+def annulus_radial_case(p=ANNULUS_RADIAL):
+    return cases.base(order_parameter={"s": {"critical_temperature": 1.0,
+                                             "vortices": [{"center_x": 0.0, "center_y": 0.0,
+                                                           "winding_number": p["winding"]}]}},
+                      temperature=p["T"], flux=p["flux"], lam=p["lam"], angles=32, ppxi=6.2,
+                      cutoff=p["cutoff"], norm="linf", criterion=1e-4,
+                      accelerator="barzilai-borwein", step_size=1.0, gauge="symmetric",
+                      width=2 * p["R"],
+                      components=[{"disc": {"radius": 0.5}},
+                                  {"disc": {"radius": 0.5 * p["r"] / p["R"], "add": False}}])
+
+
+def magnetic_moment_per_area(j, h, num_grain):
+    """GridFunctions::moment (GridFunctions_GPU.h:602-616,726-752):
+    h^3 / 2 sum (x j_y - y j_x), x and y in sites from the grid centre,
+    over GeometryGroup::area() = h^2 (grain sites)."""
+    n = j.shape[-1]
+    x = np.arange(n) - 0.5 * (n - 1)
+    return 0.5 * h ** 3 * (x[None, :] * j[1] - x[:, None] * j[0]).sum() / (h * h * num_grain)
+
+
+def check_annulus_radial(f, f_mag, moment, converged, p=ANNULUS_RADIAL):
+    ef, ef_mag, em = annulus_radial_expected(**p)
+    print("annulus (radial): F/area %.8f vs %.8f (rel %.1e), F_mag/area %.4e vs %.4e "
+          "(rel %.1e), moment %.6f vs %.6f (rel %.1e)" % (
+              f, ef, abs(f - ef) / abs(ef), f_mag, ef_mag, abs(f_mag - ef_mag) / ef_mag,
+              moment, em, abs(moment - em) / abs(em)))
+    assert abs(f - ef) <= 1e-3 * abs(ef)
+    assert abs(f_mag - ef_mag) <= 2e-2 * abs(ef_mag)
+    assert abs(moment - em) <= 1e-3 * abs(em)
+    assert converged
+
+
+def test_swave_annulus_radial_free_energy_and_moment():
+    o = po.Oracle(annulus_radial_case())
+    o.initial_bulk()
+    # Initial<T>::phaseWinding: the bulk gap with winding -2 about the centre
+    n = o.N0
+    c = (n - 1) / 2.0
+    yy, xx = np.mgrid[0:n, 0:n]
+    w = ANNULUS_RADIAL["winding"]
+    o.set_order_parameter(0, o.delta()[0] * np.exp(1j * w * np.arctan2(yy - c, xx - c)))
+    converged = o.run(0, 0, 100)
+    m = magnetic_moment_per_area(o.current(), o.get("h"), o.get_int("num_grain_sites"))
+    check_annulus_radial(o.get("free_energy_per_area"), o.get("magnetic_free_energy_per_area"),
+                         m, converged)
