@@ -20,7 +20,10 @@ checks run on the product in tests/test_gpu_reference_physics.py):
   r = 10, lambda = 30) with phase winding -2 in one flux quantum: free
   energy to 1e-3, magnetic free energy to 2e-2 and magnetic moment to 1e-3
   of the London (Bessel-function) result, with the field energy of the hole
-  written as that of its uniform field (see annulus_radial_expected).
+  written as that of its uniform field (see annulus_radial_expected);
+* tests/physics/current_linear_response_test.cu: an s-wave square with a
+  uniform vector potential A = (0.01, 0.01) carries j = -Y(T) A at its
+  centre after one iteration, to 1e-2, at T = 0.1 and 0.9.
 """
 import math
 
@@ -338,3 +341,39 @@ def test_swave_annulus_radial_free_energy_and_moment():
     m = magnetic_moment_per_area(o.current(), o.get("h"), o.get_int("num_grain_sites"))
     check_annulus_radial(o.get("free_energy_per_area"), o.get("magnetic_free_energy_per_area"),
                          m, converged)
+
+
+def linear_response_case(temperature):
+    """tests/physics/current_linear_response_test.cu:18-160: a square of side
+    20 xi0 (PolygonGeometry with 4 vertices and side length 1), 1 point per
+    xi0, 32 angles, cutoff 16, lambda 1 so that the induced potential enters."""
+    return cases.base(temperature=temperature, flux=0.0, lam=1.0, angles=32, ppxi=1.0,
+                      cutoff=16.0, width=20.0,
+                      components=[{"regular_polygon": {"num_edges": 4, "side_length": 1.0}}])
+
+
+LINEAR_RESPONSE_A = 0.01
+
+
+def check_linear_response(j, temperature, yoshida):
+    """j / j0 = -Y_{3/2} A / A0 (the test's computeCurrentDensity) at the
+    grid's middle site (dimX / 2, dimY / 2), doctest epsilon 1e-2, scale 0."""
+    n = j.shape[-1]
+    m = n // 2
+    expected = -yoshida * LINEAR_RESPONSE_A
+    print("T %.1f: j(middle) (%.8f, %.8f), linear response %.8f" % (
+        temperature, j[0][m, m], j[1][m, m], expected))
+    for comp in (j[0][m, m], j[1][m, m]):
+        assert abs(comp - expected) < 1e-2 * max(abs(comp), abs(expected))
+
+
+@pytest.mark.parametrize("temperature", [0.1, 0.9])
+def test_current_linear_response(temperature):
+    o = po.Oracle(linear_response_case(temperature))
+    o.initial_bulk()
+    n = o.N0
+    a = np.full((n, n), LINEAR_RESPONSE_A)
+    o.set_vector_potential(a, a)
+    o.compute()
+    d = po.bulk_gap(temperature)
+    check_linear_response(o.current(), temperature, po.yoshida(temperature, d, 16.0))
