@@ -80,16 +80,22 @@ __device__ __forceinline__ float drcp(float x) { return rcpT(x); }
 // (|y| / 2t, copysign(t, y)) otherwise.  Both square roots come from
 // reciprocal square roots: |z| = n rsq(n) with n = |z|^2, and with
 // u = (|x| + |z|) / 2, t = u rsq(u) and y / 2t = y rsq(u) / 2.
+// z = 0 gives (0, y) by a select at the end, computed on n = 1 meanwhile:
+// no branch, so the square roots of a lane's energies stay in one basic
+// block and their latencies overlap.
 template <typename T> __device__ __forceinline__ Cx<T> csqrt(Cx<T> z) {
-  const T n = z.re * z.re + z.im * z.im;
-  if (n == T(0)) return {T(0), z.im};
+  const T n0 = z.re * z.re + z.im * z.im;
+  const bool zero = n0 == T(0);
+  const T n = zero ? T(1) : n0;
   const T m = n * rsqT(n);
   const T u = T(0.5) * (fabs(z.re) + m);
   const T r = rsqT(u);
   const T t = u * r;
   const T q = (T(0.5) * z.im) * r;
-  if (z.re >= T(0)) return {t, q};
-  return {fabs(q), copysign(t, z.im)};
+  const bool right = z.re >= T(0);
+  const T re = right ? t : fabs(q);
+  const T im = right ? q : copysign(t, z.im);
+  return {zero ? T(0) : re, zero ? z.im : im};
 }
 
 // a / b through the reciprocal of |b|^2 (|b| is of order one here).
