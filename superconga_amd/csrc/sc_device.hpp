@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <limits>
 
 namespace scamd {
 
@@ -63,11 +64,14 @@ __device__ __forceinline__ double rcpT(double x) {
   return r;
 }
 __device__ __forceinline__ double rsqT(double x) {
+  // r += r (1 - x r^2) / 2, with the halving applied to x once: scaling by
+  // 0.5 is exact, so this is the same arithmetic one multiply shorter
+  const double hx = 0.5 * x;
   double r = __builtin_amdgcn_rsq(x);
 #pragma unroll
   for (int i = 0; i < kNewtonSteps; ++i) {
-    const double e = fma(-(x * r), r, 1.0); // 1 - x r^2
-    r = fma(0.5 * r, e, r);
+    const double he = fma(-(hx * r), r, 0.5); // (1 - x r^2) / 2
+    r = fma(r, he, r);
   }
   return r;
 }
@@ -80,22 +84,19 @@ __device__ __forceinline__ float drcp(float x) { return rcpT(x); }
 // (|y| / 2t, copysign(t, y)) otherwise.  Both square roots come from
 // reciprocal square roots: |z| = n rsq(n) with n = |z|^2, and with
 // u = (|x| + |z|) / 2, t = u rsq(u) and y / 2t = y rsq(u) / 2.
-// z = 0 gives (0, y) by a select at the end, computed on n = 1 meanwhile:
-// no branch, so the square roots of a lane's energies stay in one basic
-// block and their latencies overlap.
+// n is held at or above the smallest normal number, so z = 0 gives a root
+// of order 1e-77 (1e-10 in FP32) instead of 0 -- negligible beside the O(1)
+// terms it is added to -- with no branch and no select: the square roots of
+// a lane's energies stay in one basic block and their latencies overlap.
 template <typename T> __device__ __forceinline__ Cx<T> csqrt(Cx<T> z) {
-  const T n0 = z.re * z.re + z.im * z.im;
-  const bool zero = n0 == T(0);
-  const T n = zero ? T(1) : n0;
+  const T n = fmax(z.re * z.re + z.im * z.im, std::numeric_limits<T>::min());
   const T m = n * rsqT(n);
   const T u = T(0.5) * (fabs(z.re) + m);
   const T r = rsqT(u);
   const T t = u * r;
   const T q = (T(0.5) * z.im) * r;
   const bool right = z.re >= T(0);
-  const T re = right ? t : fabs(q);
-  const T im = right ? q : copysign(t, z.im);
-  return {zero ? T(0) : re, zero ? z.im : im};
+  return {right ? t : fabs(q), right ? q : copysign(t, z.im)};
 }
 
 // a / b through the reciprocal of |b|^2 (|b| is of order one here).
