@@ -230,6 +230,18 @@ int sc_autotune_sweep(sc_context *ctx, int precision);
  * current and out are 2*n*n (x plane then y plane). */
 int sc_poisson_solve(int n, double h, int num_grain, int enforce_mean,
                      const double *current, double *out);
+/* The same with ComputeVectorPotential's max_error (the config's
+ * numerics.vector_potential_error, src/simulation.cu:428-434): > 0 replaces
+ * the exact kernel by the eigen-expansion the reference's separable filters
+ * sum to (ComputeVectorPotential.h:319-395,686-722), truncated at that
+ * relative Frobenius error. */
+int sc_poisson_solve_ex(int n, double h, int num_grain, int enforce_mean, double max_error,
+                        const double *current, double *out);
+/* Host only: the kernel G(|dx|, |dy|) the solve convolves with, n*n values
+ * out[dy*n + dx] (ComputeVectorPotential::initializeFilters,
+ * ComputeVectorPotential.h:296-417); *num_filters (may be null) receives
+ * the number of rank-one filters of the truncation (0 for max_error = 0). */
+int sc_poisson_kernel(int n, double h, double max_error, double *out, int *num_filters);
 
 /* Unit-level device arithmetic for closed-form checks: n midpoint steps
  * (RiccatiSolutions.h:65-94).  in = n records of (delta re, im, energy re,

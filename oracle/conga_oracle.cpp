@@ -823,6 +823,9 @@ struct Oracle {
   std::vector<double> accMean, accStep, accInertia, accPrev, accVel;
   double bbStep = 0;
   bool accInit = false;
+  // Broyden: pairs (c_i, d_i) of H = -alpha I + sum c_i d_i^T, previous x, F
+  std::vector<std::vector<double>> brC, brD;
+  std::vector<double> brLastX, brLastF;
 
   int numEnergies() const { return Ne; }
   size_t n0sq() const { return size_t(N0) * N0; }
@@ -1404,7 +1407,14 @@ static double helperBeta(int a, int b) {
   const double ang = static_cast<double>(a) / static_cast<double>(b);
   return f * std::atan(ang);
 }
-static double matrixElement(int i, int j) {
+// ComputeVectorPotential.h:246-294: the analytic matrix element, or (the
+// reference's inAnalytical = false) the value of 2 ln r at lattice offsets
+// with the origin's own value.
+static double matrixElement(int i, int j, bool analytical = true) {
+  if (!analytical) {
+    if (i == 0 && j == 0) return 0.5 * M_PI - std::log(2.0) - 3.0;
+    return 2.0 * std::log(std::hypot(static_cast<double>(i), static_cast<double>(j)));
+  }
   const int iv[2] = {-i, i}, jv[2] = {-j, j};
   double s = 0.0;
   for (int jj : jv) {
@@ -1416,11 +1426,63 @@ static double matrixElement(int i, int j) {
   }
   return s * 0.25;
 }
-// ComputeVectorPotential.h:212-244,396-415 -- full 2D filter (error = 0).
-static void makeFilter2D(int N, double h, std::vector<double> &filter) {
+// Eigen-decomposition of a symmetric n x n matrix S (row-major, destroyed)
+// by the cyclic Jacobi method (Jacobi 1846; rotations in row order, sweeps
+// until every off-diagonal element is negligible): eigenvalues in w,
+// eigenvector k in column k of V (row-major n x n).
+static void jacobiEigen(int n, std::vector<double> &S, std::vector<double> &w,
+                        std::vector<double> &V) {
+  const size_t N = size_t(n);
+  V.assign(N * N, 0.0);
+  for (size_t i = 0; i < N; ++i) V[i * N + i] = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, diag = 0.0;
+    for (size_t p = 0; p < N; ++p) {
+      diag += S[p * N + p] * S[p * N + p];
+      for (size_t q = p + 1; q < N; ++q) off += S[p * N + q] * S[p * N + q];
+    }
+    if (off <= 1e-32 * diag) break;
+    for (size_t p = 0; p + 1 < N; ++p)
+      for (size_t q = p + 1; q < N; ++q) {
+        const double apq = S[p * N + q];
+        if (apq == 0.0) continue;
+        const double app = S[p * N + p], aqq = S[q * N + q];
+        // the rotation angle that zeroes S[p][q]: tan(2 phi) = 2 apq / (aqq - app)
+        const double theta = (aqq - app) / (2.0 * apq);
+        const double t = (theta >= 0.0 ? 1.0 : -1.0) /
+                         (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+        for (size_t k = 0; k < N; ++k) { // columns p and q
+          const double skp = S[k * N + p], skq = S[k * N + q];
+          S[k * N + p] = c * skp - sn * skq;
+          S[k * N + q] = sn * skp + c * skq;
+        }
+        for (size_t k = 0; k < N; ++k) { // rows p and q
+          const double spk = S[p * N + k], sqk = S[q * N + k];
+          S[p * N + k] = c * spk - sn * sqk;
+          S[q * N + k] = sn * spk + c * sqk;
+        }
+        S[p * N + q] = 0.0;
+        S[q * N + p] = 0.0;
+        for (size_t k = 0; k < N; ++k) {
+          const double vkp = V[k * N + p], vkq = V[k * N + q];
+          V[k * N + p] = c * vkp - sn * vkq;
+          V[k * N + q] = sn * vkp + c * vkq;
+        }
+      }
+  }
+  w.resize(N);
+  for (size_t i = 0; i < N; ++i) w[i] = S[i * N + i];
+}
+// ComputeVectorPotential.h:212-244,296-417 -- the 2D filter: the full
+// kernel (maxError = 0) or C plus the eigen-expansion of Q truncated by
+// magnitude at relative Frobenius error maxError (the sum of the
+// reference's separable filters).  Returns the number of rank-one filters.
+static int makeFilter2D(int N, double h, std::vector<double> &filter,
+                        double maxError = 0.0, bool analytical = true) {
   std::vector<double> Q(size_t(N) * N);
   for (int j = 0; j < N; ++j)
-    for (int i = 0; i < N; ++i) Q[size_t(j) * N + i] = matrixElement(i, j);
+    for (int i = 0; i < N; ++i) Q[size_t(j) * N + i] = matrixElement(i, j, analytical);
   // Q(i,j) stored at [j*N + i] (column-major like the reference); symmetrize
   std::vector<double> S(size_t(N) * N);
   for (int j = 0; j < N; ++j)
@@ -1428,7 +1490,7 @@ static void makeFilter2D(int N, double h, std::vector<double> &filter) {
       S[size_t(j) * N + i] = 0.5 * (Q[size_t(j) * N + i] + Q[size_t(i) * N + j]);
   const double scale = 1.0 / (4.0 * M_PI);
   for (double &v : S) v *= scale;
-  double C = (2.0 * std::log(h) - std::log(2.0) - 3.0) * scale;
+  double C = (analytical ? (2.0 * std::log(h) - std::log(2.0) - 3.0) : 2.0 * std::log(h)) * scale;
   // arma::mean(arma::mean(Q)): mean of column means.
   double mean = 0.0;
   for (int j = 0; j < N; ++j) {
@@ -1439,7 +1501,36 @@ static void makeFilter2D(int N, double h, std::vector<double> &filter) {
   mean /= N;
   for (double &v : S) v -= mean;
   C += mean;
-  for (double &v : S) v += C;
+  int filters = 0;
+  if (maxError > 0.0) {
+    std::vector<double> w, V;
+    jacobiEigen(N, S, w, V);
+    // arma::sort_index(abs(eig), "descend"), added until the error is met
+    std::vector<int> order(static_cast<size_t>(N));
+    for (int i = 0; i < N; ++i) order[size_t(i)] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+      return std::fabs(w[size_t(a)]) > std::fabs(w[size_t(b)]);
+    });
+    double norm2 = 0.0;
+    for (double v : w) norm2 += v * v;
+    std::vector<int> kept;
+    double partial = 0.0;
+    for (int k : order) {
+      kept.push_back(k);
+      partial += w[size_t(k)] * w[size_t(k)];
+      if (std::sqrt(std::max(1.0 - partial / norm2, 0.0)) < maxError) break;
+    }
+    const bool withC = std::fabs(C) > 0.0;
+    std::fill(S.begin(), S.end(), withC ? C : 0.0);
+    for (int k : kept)
+      for (int j = 0; j < N; ++j)
+        for (int i = 0; i < N; ++i)
+          S[size_t(j) * N + i] += w[size_t(k)] * V[size_t(i) * N + size_t(k)] *
+                                  V[size_t(j) * N + size_t(k)];
+    filters = static_cast<int>(kept.size()) + (withC ? 1 : 0);
+  } else {
+    for (double &v : S) v += C;
+  }
   filter.resize(size_t(2 * N - 1) * (2 * N - 1));
   size_t p = 0;
   for (int j = -N + 1; j < N; ++j)
@@ -1447,6 +1538,7 @@ static void makeFilter2D(int N, double h, std::vector<double> &filter) {
       const int idx = N * std::abs(j) + std::abs(i);
       filter[p++] = S[idx];
     }
+  return filters;
 }
 // ComputeVectorPotential.h:519-563,575-616,647-745
 static void solveVectorPotential(int N, double h, int numGrain, bool enforce,
@@ -1473,6 +1565,7 @@ static void solveVectorPotential(int N, double h, int numGrain, bool enforce,
           double ps = 0.0;
           const double *frow = &filter[size_t(W) * (j + ofy) + ofx];
           const double *srow = &src[d * n + size_t(N) * j];
+#pragma omp simd reduction(+ : ps)
           for (int i = 0; i < N; ++i) ps += srow[i] * frow[i];
           sum += ps;
         }
@@ -2143,6 +2236,68 @@ static void picardStep(Oracle &o, const std::vector<double> &G,
   for (size_t i = 0; i < X.size(); ++i) X[i] += (G[i] - X[i]) * step;
 }
 
+// Broyden's second method, restarted limited memory.  No counterpart in the
+// reference's accelerators/: it restates scipy.optimize.broyden2 (scipy
+// 1.15, optimize/_nonlin.py: BroydenSecond._update, LowRankMatrix.matvec and
+// restart_reduce, nonlin_solve without line search) on F(x) = g(x) - x with
+// alpha = step size, the dot products summed field by field like Anderson's.
+static double fieldDot(const Oracle &o, const std::vector<double> &a,
+                       const std::vector<double> &b) {
+  const size_t n0 = o.n0sq(), nf = a.size() / n0;
+  double s = 0.0;
+  for (size_t f = 0; f < nf; ++f) {
+    double fs = 0.0;
+    for (size_t q = 0; q < n0; ++q) fs += a[f * n0 + q] * b[f * n0 + q];
+    s += fs;
+  }
+  return s;
+}
+static void broydenStep(Oracle &o, const std::vector<double> &G, std::vector<double> &X) {
+  const size_t n = X.size();
+  const size_t rank = o.cfg.rank > 0 ? size_t(o.cfg.rank) : 8;
+  const double alpha = o.cfg.step_size > 0 ? o.cfg.step_size : 1.0;
+  std::vector<double> F(n);
+  for (size_t i = 0; i < n; ++i) F[i] = G[i] - X[i];
+  // H v = -alpha v + sum_i c_i (d_i . v), terms added in the order stored
+  auto applyH = [&](const std::vector<double> &v) {
+    std::vector<double> w(n);
+    for (size_t i = 0; i < n; ++i) w[i] = -alpha * v[i];
+    for (size_t k = 0; k < o.brC.size(); ++k) {
+      const double a = fieldDot(o, o.brD[k], v);
+      for (size_t i = 0; i < n; ++i) w[i] += a * o.brC[k][i];
+    }
+    return w;
+  };
+  if (o.accIter > 0) {
+    std::vector<double> dF(n), dx(n);
+    for (size_t i = 0; i < n; ++i) {
+      dF[i] = F[i] - o.brLastF[i];
+      dx[i] = X[i] - o.brLastX[i];
+    }
+    // restart_reduce(rank - 1): drop every pair once more than rank - 1 are held
+    if (o.brC.size() > rank - 1) {
+      o.brC.clear();
+      o.brD.clear();
+    }
+    const double dfn2 = fieldDot(o, dF, dF);
+    if (dfn2 > 0.0) {
+      const std::vector<double> hdf = applyH(dF);
+      std::vector<double> c(n), d(n);
+      for (size_t i = 0; i < n; ++i) {
+        c[i] = dx[i] - hdf[i];
+        d[i] = dF[i] / dfn2;
+      }
+      o.brC.push_back(std::move(c));
+      o.brD.push_back(std::move(d));
+    }
+  }
+  o.brLastX = X;
+  o.brLastF = F;
+  const std::vector<double> hf = applyH(F);
+  for (size_t i = 0; i < n; ++i) X[i] -= hf[i];
+  o.accIter++;
+}
+
 static void acceleratorStep(Oracle &o, bool doOP, bool doA) {
   std::vector<double> X, G;
   const std::vector<double> empty;
@@ -2159,6 +2314,7 @@ static void acceleratorStep(Oracle &o, bool doOP, bool doA) {
   case OC_CONGACC: congaccStep(o, G, X); break;
   case OC_BB: bbStep(o, G, X); break;
   case OC_POLYAK: polyakStep(o, G, X); break;
+  case OC_BROYDEN: broydenStep(o, G, X); break;
   }
   if (doOP)
     split(o, doA, X, o.delta, o.A);
@@ -2172,6 +2328,8 @@ static void contextCompute(Oracle &o) {
   if (!o.firstCompute) acceleratorStep(o, true, doA);
   integrate(o, false);
   if (doA) {
+    if (o.filter2D.empty())
+      makeFilter2D(o.N0, o.h, o.filter2D, std::max(o.cfg.vector_potential_error, 0.0));
     solveVectorPotential(o.N0, o.h, o.numGrain, true, o.filter2D,
                          o.current.data(), o.Anew.data());
     o.resA = residualArrays(o.A.data(), o.Anew.data(), o.A.size(), o.cfg.norm);
@@ -2342,10 +2500,6 @@ static Oracle *build(const oc_config &cfg) {
     g_error = "invalid number of order-parameter components";
     return nullptr;
   }
-  if (cfg.vector_potential_error > 0.0 && cfg.penetration_depth > 0.0) {
-    g_error = "the oracle implements only vector_potential_error = 0";
-    return nullptr;
-  }
   // geometry scaling (geometry_utils.h:41-175) and components (:193-325)
   double width;
   std::vector<Component> comps;
@@ -2501,6 +2655,7 @@ static Oracle *build(const oc_config &cfg) {
   }
   // geometry frames: GeometryGroup::create(-systemAngle(k))
   o->frames.resize(o->Na);
+#pragma omp parallel for schedule(dynamic, 1)
   for (int k = 0; k < o->Na; ++k)
     createGroup(o->comps, -o->systemAngle[k], o->N0, o->N1, o->frames[k]);
   o->domainRef = createDomain(o->comps, o->N0);
@@ -2534,7 +2689,7 @@ static Oracle *build(const oc_config &cfg) {
   initBoundaryStorage(*o);
   o->gamma.assign(size_t(o->Ne) * 2 * o->n1sq(), 0.0);
   o->gammat.assign(size_t(o->Ne) * 2 * o->n1sq(), 0.0);
-  if (o->lambda > 0.0) makeFilter2D(o->N0, o->h, o->filter2D);
+  // the Poisson filter is built on the first solve that needs it
   return o.release();
 }
 
@@ -2860,12 +3015,55 @@ void oc_lattice(double w, double ppxi, int *n0, int *n1, double *gf) {
 double oc_bulk_gap(double T) { return bulkGap(T); }
 double oc_bulk_free_energy(double T, double d) { return bulkFreeEnergy(T, d); }
 double oc_yoshida(double T, double d, double c) { return yoshida(T, d, c); }
+void *oc_accel_create(int kind, int n0, int num_fields, int rank, double step) {
+  auto o = std::make_unique<Oracle>();
+  o->cfg.accelerator = kind;
+  o->cfg.rank = rank;
+  o->cfg.step_size = step;
+  o->N0 = n0;
+  o->Nc = num_fields;
+  return o.release();
+}
+void oc_accel_destroy(void *h) { delete static_cast<Oracle *>(h); }
+int oc_accel_step(void *h, double *x, const double *g) {
+  Oracle &o = *static_cast<Oracle *>(h);
+  const size_t n = o.n0sq() * size_t(o.Nc);
+  std::vector<double> X(x, x + n), G(g, g + n);
+  switch (o.cfg.accelerator) {
+  case OC_PICARD: picardStep(o, G, X); break;
+  case OC_ANDERSON: andersonStep(o, G, X); break;
+  case OC_CONGACC: congaccStep(o, G, X); break;
+  case OC_BB: bbStep(o, G, X); break;
+  case OC_POLYAK: polyakStep(o, G, X); break;
+  case OC_BROYDEN: broydenStep(o, G, X); break;
+  default: g_error = "unknown accelerator"; return -1;
+  }
+  std::copy(X.begin(), X.end(), x);
+  return 0;
+}
+double oc_congacc_inertia(double dn, double mn, double dot, double min_cos) {
+  return congaccInertia(dn, mn, dot, min_cos);
+}
 int oc_vector_potential(int n, double h, int ng, int enforce, const double *j,
                         double *out) {
+  return oc_vector_potential_ex(n, h, ng, enforce, 0.0, 1, j, out);
+}
+int oc_vector_potential_ex(int n, double h, int ng, int enforce, double max_error,
+                           int analytical, const double *j, double *out) {
   std::vector<double> filter;
-  makeFilter2D(n, h, filter);
+  makeFilter2D(n, h, filter, max_error, analytical != 0);
   solveVectorPotential(n, h, ng, enforce != 0, filter, j, out);
   return 0;
+}
+int oc_poisson_kernel(int n, double h, double max_error, int analytical, double *out) {
+  std::vector<double> filter;
+  const int k = makeFilter2D(n, h, filter, max_error, analytical != 0);
+  // the kernel at non-negative offsets, out[dy * n + dx]
+  const int W = 2 * n - 1;
+  for (int dy = 0; dy < n; ++dy)
+    for (int dx = 0; dx < n; ++dx)
+      out[size_t(dy) * n + dx] = filter[size_t(dy + n - 1) * W + size_t(dx + n - 1)];
+  return k;
 }
 
 } // extern "C"

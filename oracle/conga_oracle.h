@@ -31,7 +31,7 @@ enum { OC_LANDAU = 0, OC_SYMMETRIC = 1, OC_SOLENOID = 2 };
 /* Residual norms. */
 enum { OC_L1 = 0, OC_L2 = 1, OC_LINF = 2 };
 /* Accelerators. */
-enum { OC_PICARD = 0, OC_ANDERSON = 1, OC_CONGACC = 2, OC_BB = 3, OC_POLYAK = 4 };
+enum { OC_PICARD = 0, OC_ANDERSON = 1, OC_CONGACC = 2, OC_BB = 3, OC_POLYAK = 4, OC_BROYDEN = 5 };
 
 typedef struct {
   /* physics */
@@ -67,7 +67,7 @@ typedef struct {
   int norm;
   int num_fermi_momenta;
   double points_per_coherence_length;
-  double vector_potential_error; /* only 0 (the exact Green function) */
+  double vector_potential_error; /* 0: exact Green function; > 0: eigen filters */
 
   /* compute objects */
   int compute_current;
@@ -173,9 +173,28 @@ void oc_lattice(double width, double ppxi, int *n0, int *n1,
 double oc_bulk_gap(double temperature);
 double oc_bulk_free_energy(double temperature, double delta);
 double oc_yoshida(double temperature, double delta, double cutoff);
+/* A bare accelerator on vectors of num_fields * n0^2 values (fields of n0^2
+ * values each, the layout of accelerators/utils.h:58-186): oc_accel_step
+ * replaces x by the accelerator's next iterate given g = g(x).  The same
+ * step functions the iteration uses. */
+void *oc_accel_create(int kind, int n0, int num_fields, int rank, double step);
+void oc_accel_destroy(void *accel);
+int oc_accel_step(void *accel, double *x, const double *g);
+/* CongAcc's inertia bisection (CongAcc.h:300-356) for |D|, |M|, D.M. */
+double oc_congacc_inertia(double dn, double mn, double dot, double min_cos);
+
 /* ComputeVectorPotential with the exact Green function. */
 int oc_vector_potential(int n, double h, int num_grain, int enforce_mean,
                         const double *current, double *out);
+/* ComputeVectorPotential(max_error, analytical): max_error > 0 truncates the
+ * eigen-expansion of the kernel (ComputeVectorPotential.h:319-395);
+ * analytical = 0 takes the plain 2 ln r matrix elements (:281-293). */
+int oc_vector_potential_ex(int n, double h, int num_grain, int enforce_mean,
+                           double max_error, int analytical, const double *current,
+                           double *out);
+/* The kernel G(|dx|, |dy|) as out[dy * n + dx]; returns the number of
+ * rank-one filters (0 for max_error = 0). */
+int oc_poisson_kernel(int n, double h, double max_error, int analytical, double *out);
 
 #ifdef __cplusplus
 }

@@ -21,7 +21,7 @@ SYMMETRIES = {"s": 0, "dx2-y2": 1, "dxy": 2, "g": 3}
 GAUGES = {"landau": 0, "symmetric": 1, "solenoid": 2}
 NORMS = {"l1": 0, "l2": 1, "linf": 2}
 ACCELERATORS = {"picard": 0, "anderson": 1, "congacc": 2,
-                "barzilai-borwein": 3, "bb": 3, "polyak": 4}
+                "barzilai-borwein": 3, "bb": 3, "polyak": 4, "broyden": 5}
 
 
 class OcConfig(ctypes.Structure):
@@ -128,6 +128,18 @@ def lib():
         L.oc_yoshida.argtypes = [ctypes.c_double] * 3
         L.oc_vector_potential.argtypes = [ctypes.c_int, ctypes.c_double,
                                           ctypes.c_int, ctypes.c_int, dp, dp]
+        L.oc_accel_create.restype = ctypes.c_void_p
+        L.oc_accel_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_double]
+        L.oc_accel_destroy.argtypes = [ctypes.c_void_p]
+        L.oc_accel_step.argtypes = [ctypes.c_void_p, dp, dp]
+        L.oc_congacc_inertia.restype = ctypes.c_double
+        L.oc_congacc_inertia.argtypes = [ctypes.c_double] * 4
+        L.oc_vector_potential_ex.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                             ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                             dp, dp]
+        L.oc_poisson_kernel.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_int, dp]
         _lib = L
     return _lib
 
@@ -423,9 +435,43 @@ def yoshida(t, d, cutoff):
     return lib().oc_yoshida(t, d, cutoff)
 
 
-def vector_potential(n, h, num_grain, current, enforce=True):
+def vector_potential(n, h, num_grain, current, enforce=True, max_error=0.0,
+                     analytical=True):
     j = np.ascontiguousarray(current, dtype=np.float64).reshape(-1)
     out = np.empty(2 * n * n)
-    lib().oc_vector_potential(n, h, num_grain, 1 if enforce else 0, _dp(j),
-                              _dp(out))
+    lib().oc_vector_potential_ex(n, h, num_grain, 1 if enforce else 0, float(max_error),
+                                 1 if analytical else 0, _dp(j), _dp(out))
     return out.reshape(2, n, n)
+
+
+def poisson_kernel(n, h, max_error=0.0, analytical=True):
+    """(G[dy, dx], number of rank-one filters) of ComputeVectorPotential."""
+    out = np.empty(n * n)
+    k = lib().oc_poisson_kernel(n, h, float(max_error), 1 if analytical else 0, _dp(out))
+    return out.reshape(n, n), k
+
+
+class Accelerator:
+    """The oracle's accelerator step on its own: vectors of num_fields
+    fields of n0 * n0 values; step(x, g) returns the next iterate."""
+
+    def __init__(self, name, n0, num_fields, rank=-1, step=-1.0):
+        self.n = n0 * n0 * num_fields
+        self.h = lib().oc_accel_create(ACCELERATORS[name], n0, num_fields, rank, step)
+
+    def step(self, x, g):
+        x = np.array(x, dtype=np.float64).reshape(-1)
+        g = np.ascontiguousarray(g, dtype=np.float64).reshape(-1)
+        assert x.size == self.n and g.size == self.n
+        if lib().oc_accel_step(self.h, _dp(x), _dp(g)) != 0:
+            raise RuntimeError(lib().oc_last_error().decode())
+        return x
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oc_accel_destroy(self.h)
+            self.h = None
+
+
+def congacc_inertia(dn, mn, dot, min_cos=0.7):
+    return lib().oc_congacc_inertia(dn, mn, dot, min_cos)

@@ -113,6 +113,10 @@ def lib():
         L.sc_autotune_sweep.argtypes = [vp, ctypes.c_int]
         L.sc_poisson_solve.argtypes = [ctypes.c_int, ctypes.c_double,
                                        ctypes.c_int, ctypes.c_int, _DP, _DP]
+        L.sc_poisson_solve_ex.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_double, _DP, _DP]
+        L.sc_poisson_kernel.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _DP,
+                                        ctypes.POINTER(ctypes.c_int)]
         L.sc_riccati_steps.argtypes = [ctypes.c_int, _DP, _DP, ctypes.c_int]
         L.sc_num_ozaki.argtypes = [ctypes.c_double, ctypes.c_double]
         L.sc_ozaki.argtypes = [ctypes.c_int, _DP, _DP]
@@ -491,13 +495,26 @@ def describe(config, angle=None):
     return d, out.reshape(d.n1, d.n1)
 
 
-def poisson_solve(current, h, num_grain, enforce_mean=True):
-    """ComputeVectorPotential::solve with the exact Green function."""
+def poisson_solve(current, h, num_grain, enforce_mean=True, max_error=0.0):
+    """ComputeVectorPotential::solve: the exact Green function
+    (max_error = 0) or its eigen-truncated expansion (max_error > 0, the
+    config's vector_potential_error)."""
     j = np.ascontiguousarray(current, dtype=np.float64)
     n = j.shape[-1]
     out = np.empty((2, n, n))
-    _check(lib().sc_poisson_solve(n, h, num_grain, int(enforce_mean), _dp(j), _dp(out)))
+    _check(lib().sc_poisson_solve_ex(n, h, num_grain, int(enforce_mean), float(max_error),
+                                     _dp(j), _dp(out)))
     return out
+
+
+def poisson_kernel(n, h, max_error=0.0):
+    """(G, filters): the kernel G[dy, dx] the vector-potential solve
+    convolves with, and the number of rank-one filters of its truncation
+    (host only; no device needed)."""
+    out = np.empty((n, n))
+    k = ctypes.c_int()
+    _check(lib().sc_poisson_kernel(n, h, float(max_error), _dp(out), ctypes.byref(k)))
+    return out, k.value
 
 
 def riccati_steps(delta, energy, initial, step, precision="fp64"):

@@ -87,6 +87,10 @@ void hipCheck(hipError_t e, const char *what) {
 
 } // namespace
 
+namespace scamd {
+int reportError(int code, const std::string &message) { return fail(code, message); }
+} // namespace scamd
+
 extern "C" {
 
 const char *sc_last_error(void) { return g_lastError.c_str(); }
@@ -456,12 +460,31 @@ int sc_free_energy_gradient(sc_context *ctx, double *grad, double *freeEnergy) {
 
 int sc_poisson_solve(int n, double h, int numGrain, int enforce, const double *current,
                      double *out) {
+  return sc_poisson_solve_ex(n, h, numGrain, enforce, 0.0, current, out);
+}
+
+int sc_poisson_solve_ex(int n, double h, int numGrain, int enforce, double maxError,
+                        const double *current, double *out) {
   SC_CHECK_PTR(current);
   SC_CHECK_PTR(out);
   if (n < 2 || !(h > 0.0) || numGrain < 1)
     return fail(SC_ERR_ARGUMENT, "need n >= 2, h > 0 and num_grain >= 1");
+  if (!(maxError >= 0.0)) return fail(SC_ERR_ARGUMENT, "max_error must be >= 0");
   return guarded([&] {
-    scamd::poissonSolve(n, h, numGrain, enforce != 0, current, out);
+    scamd::poissonSolve(n, h, numGrain, enforce != 0, current, out, maxError);
+    return SC_OK;
+  });
+}
+
+int sc_poisson_kernel(int n, double h, double maxError, double *out, int *numFilters) {
+  SC_CHECK_PTR(out);
+  if (n < 1 || !(h > 0.0)) return fail(SC_ERR_ARGUMENT, "need n >= 1 and h > 0");
+  if (!(maxError >= 0.0)) return fail(SC_ERR_ARGUMENT, "max_error must be >= 0");
+  return guarded([&] {
+    std::vector<double> g;
+    const int k = scamd::poissonKernel(n, h, g, maxError);
+    std::copy(g.begin(), g.end(), out);
+    if (numFilters) *numFilters = k;
     return SC_OK;
   });
 }

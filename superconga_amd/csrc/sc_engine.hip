@@ -41,6 +41,9 @@ Grain::~Grain() {
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (double *p : accX_) devFree(p);
   for (double *p : accD_) devFree(p);
+  devFree(dBrLastX_);
+  devFree(dBrLastF_);
+  devFree(dBrDx_);
   devFree(dAccMean_);
   devFree(dAccPrev_);
   devFree(dAccVel_);
@@ -236,7 +239,7 @@ void Grain::upload() {
   dPartials_ = devAlloc<double>(size_t(partialsCap_) * 6);
   dWork_ = devAlloc<double>(nf);
   dWork2_ = devAlloc<double>(nf);
-  if (s.lambda > 0.0) fft_ = std::make_unique<PoissonFFT>(s.N0, s.h, stream_);
+  if (s.lambda > 0.0) fft_ = std::make_unique<PoissonFFT>(s.N0, s.h, stream_, s.cfg.vpError);
   dCounter_ = devAlloc<int>(1);
 }
 
@@ -1139,6 +1142,10 @@ void Grain::resetAccelerator() {
   SC_HIP(hipStreamSynchronize(stream_));
   for (double *p : accX_) devFree(p);
   for (double *p : accD_) devFree(p);
+  devFree(dBrLastX_);
+  devFree(dBrLastF_);
+  devFree(dBrDx_);
+  brCount_ = 0;
   // the running mean, previous step and velocity start from zero again
   devFree(dAccMean_);
   devFree(dAccPrev_);
@@ -1420,8 +1427,90 @@ void Grain::acceleratorFinish() {
     polyakKernel<<<blocksFor(n), 256, 0, stream_>>>(X, dAccVel_, G, n, st, drag);
     break;
   }
+  case ACC_BROYDEN:
+    broydenStep(X, G, n);
+    break;
   }
   SC_HIP(hipGetLastError());
+}
+
+// Broyden's second method on F(x) = g(x) - x with a limited, restarted
+// memory (no counterpart in the reference; the same iteration as
+// scipy.optimize.broyden2 with reduction_method 'restart'): the inverse
+// Jacobian is H = -alpha I + sum_i c_i d_i^T; after a step the pair
+// (dx, dF) of iterate and residual differences adds
+//   c = dx - H dF,  d = dF / |dF|^2
+// (the secant condition H_new dF = dx), the memory first emptied when it
+// already holds rank - 1 pairs; the step is x <- x - H F.  One read-back per
+// step brings every dot product (dF and F against the d_i and dF).
+void Grain::broydenStep(double *X, const double *G, long long n) {
+  const Setup &s = *s_;
+  const int rank = s.cfg.rank > 0 ? s.cfg.rank : 8;
+  if (rank > kMaxHistory)
+    throw std::runtime_error("Broyden rank above " + std::to_string(kMaxHistory));
+  const double alpha = s.cfg.stepSize > 0 ? s.cfg.stepSize : 1.0;
+  if (!accInit_) {
+    dBrLastX_ = devAlloc<double>(size_t(n));
+    dBrLastF_ = devAlloc<double>(size_t(n));
+    dBrDx_ = devAlloc<double>(size_t(n));
+    brCount_ = 0;
+    accInit_ = true;
+  }
+  double *F = dWork_, *dF = dWork2_;
+  diffKernel<<<blocksFor(n), 256, 0, stream_>>>(F, G, X, n);
+  SC_HIP(hipGetLastError());
+  std::vector<double> b; // d_i . F
+  if (accIter_ > 0) {
+    diffKernel<<<blocksFor(n), 256, 0, stream_>>>(dF, F, dBrLastF_, n);
+    diffKernel<<<blocksFor(n), 256, 0, stream_>>>(dBrDx_, X, dBrLastX_, n);
+    SC_HIP(hipGetLastError());
+    if (brCount_ > rank - 1) brCount_ = 0;
+    std::vector<double *> ds(accD_.begin(), accD_.begin() + brCount_);
+    ds.push_back(dF);
+    const auto dots = multiDotGroups({DotGroup{dF, ds, n}, DotGroup{F, ds, n}});
+    const double dfn2 = dots[0][size_t(brCount_)];
+    b.assign(dots[1].begin(), dots[1].begin() + brCount_);
+    if (dfn2 > 0.0) {
+      if (static_cast<int>(accX_.size()) <= brCount_) {
+        accX_.push_back(devAlloc<double>(size_t(n)));
+        accD_.push_back(devAlloc<double>(size_t(n)));
+      }
+      const size_t k = size_t(brCount_);
+      // c = dx + alpha dF - sum_i (d_i . dF) c_i
+      CombineTerms t{};
+      int m = 0;
+      t.v[m] = dF;
+      t.c[m++] = alpha;
+      for (size_t i = 0; i < k; ++i) {
+        t.v[m] = accX_[i];
+        t.c[m++] = -dots[0][i];
+      }
+      combineTermsKernel<<<blocksFor(n), 256, 0, stream_>>>(accX_[k], dBrDx_, 1.0, t, m, n);
+      CombineTerms u{};
+      u.v[0] = dF;
+      u.c[0] = 1.0 / dfn2;
+      combineTermsKernel<<<blocksFor(n), 256, 0, stream_>>>(accD_[k], nullptr, 0.0, u, 1, n);
+      SC_HIP(hipGetLastError());
+      b.push_back(dots[1][size_t(brCount_)] / dfn2);
+      ++brCount_;
+    }
+  }
+  SC_HIP(hipMemcpyAsync(dBrLastX_, X, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice,
+                        stream_));
+  SC_HIP(hipMemcpyAsync(dBrLastF_, F, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice,
+                        stream_));
+  // x <- x + alpha F - sum_i (d_i . F) c_i
+  CombineTerms t{};
+  int m = 0;
+  t.v[m] = F;
+  t.c[m++] = alpha;
+  for (size_t i = 0; i < b.size(); ++i) {
+    t.v[m] = accX_[i];
+    t.c[m++] = -b[i];
+  }
+  combineTermsKernel<<<blocksFor(n), 256, 0, stream_>>>(X, X, 1.0, t, m, n);
+  SC_HIP(hipGetLastError());
+  ++accIter_;
 }
 
 void Grain::beginCompute() {
@@ -1536,9 +1625,14 @@ bool Grain::run(int nBurnin, int itMin, int itMax, int precision) {
   constexpr int kRefineStall = 20;
   const bool refine = precision == PREC_MIXED_REFINE;
   int prec = refine ? PREC_MIXED : precision;
-  for (int i = 1; nBurnin > 0; ++i) {
-    burnin(prec);
-    if (i >= nBurnin) break;
+  // runBurnIn(n, n) (run_iteration.h:85-141): n iterations, or with n < 0
+  // until the boundary residual is below the criterion
+  if (nBurnin != 0) {
+    for (int i = 1;; ++i) {
+      burnin(prec);
+      const bool bconv = res_.resBoundary < s_->cfg.convergence;
+      if ((bconv && i >= nBurnin) || (nBurnin >= 0 && i >= nBurnin)) break;
+    }
   }
   if (itMin == 0 && itMax == 0) return false;
   bool conv = false;
@@ -1825,7 +1919,7 @@ void Batch::autotune(int precision) {
 }
 
 void poissonSolve(int n, double h, int numGrain, bool enforce, const double *current,
-                  double *out) {
+                  double *out, double maxError) {
   const size_t n0 = size_t(n) * n;
   double mx = 0.0, my = 0.0;
   if (enforce) {
@@ -1840,7 +1934,7 @@ void poissonSolve(int n, double h, int numGrain, bool enforce, const double *cur
   SC_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   double *dj = nullptr, *dout = nullptr;
   try {
-    PoissonFFT fft(n, h, stream);
+    PoissonFFT fft(n, h, stream, maxError);
     dj = devAlloc<double>(2 * n0 + 2);
     dout = devAlloc<double>(2 * n0);
     const double means[2] = {mx, my};

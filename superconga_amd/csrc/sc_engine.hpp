@@ -324,6 +324,11 @@ private:
   std::vector<int> accH_; // reductions queued by acceleratorQueue
   double bbStep_ = 0;
   bool accInit_ = false;
+  // Broyden: c_i in accX_, d_i in accD_ (the first brCount_ in use), the
+  // previous iterate and residual, and the iterate difference
+  int brCount_ = 0;
+  double *dBrLastX_ = nullptr, *dBrLastF_ = nullptr, *dBrDx_ = nullptr;
+  void broydenStep(double *X, const double *G, long long n);
   // bookkeeping
   bool boundaryInit_ = false, firstCompute_ = true, firstCurrent_ = true,
        firstFE_ = true;
@@ -334,9 +339,10 @@ private:
   bool convDelta_ = false, convCurrent_ = false, convFE_ = false;
 };
 
-// Stand-alone ComputeVectorPotential solve with host buffers (2*n*n each).
+// Stand-alone ComputeVectorPotential solve with host buffers (2*n*n each);
+// maxError > 0 uses the eigen-truncated kernel.
 void poissonSolve(int n, double h, int numGrain, bool enforceMean,
-                  const double *current, double *out);
+                  const double *current, double *out, double maxError = 0.0);
 
 // Many grains iterated together (a parameter sweep).  Every iteration runs
 // the per-grain stages on one stream and sweeps all grains that are still

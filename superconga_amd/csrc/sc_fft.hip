@@ -166,19 +166,38 @@ int columnsPerBlock(int M) {
   return cb < 1 ? 1 : cb;
 }
 
+// The kernel on the periodic M x M grid: G(|d|) for |d| < n in each
+// direction, zero between (g is n x n, g[dy * n + dx]).
+__global__ void kernelGridKernel(const double *g, int n, int M, double *grid) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= (long long)M * M) return;
+  const int my = int(i / M), mx = int(i - (long long)my * M);
+  const int dy = my < n ? my : (my > M - n ? M - my : -1);
+  const int dx = mx < n ? mx : (mx > M - n ? M - mx : -1);
+  grid[i] = (dy >= 0 && dx >= 0) ? g[size_t(dy) * n + dx] : 0.0;
+}
+
+size_t alignUp(size_t b) { return (b + 255) & ~size_t(255); }
+
 } // namespace
 
-PoissonFFT::PoissonFFT(int n, double h, hipStream_t stream)
-    : n_(n), m_(1), logm_(0), h_(h), stream_(stream) {
+int PoissonFFT::lengthFor(int n) {
+  int m = 1;
+  while (m < 2 * n - 1) m <<= 1;
+  return m < 2 ? 2 : m;
+}
+
+size_t PoissonFFT::scratchBytes(int n) {
+  const size_t M = size_t(lengthFor(n));
+  return alignUp(sizeof(double) * M * M) + alignUp(sizeof(double2) * M * M) +
+         alignUp(sizeof(double2) * (M / 2));
+}
+
+PoissonFFT::PoissonFFT(int n, double h, hipStream_t stream, double maxError, void *external)
+    : n_(n), m_(1), logm_(0), h_(h), stream_(stream), owns_(external == nullptr) {
   if (n < 1) throw std::invalid_argument("Poisson grid must be non-empty");
-  while (m_ < 2 * n - 1) {
-    m_ <<= 1;
-    ++logm_;
-  }
-  if (m_ < 2) {
-    m_ = 2;
-    logm_ = 1;
-  }
+  m_ = lengthFor(n);
+  while ((1 << logm_) < m_) ++logm_;
   if (m_ > kMaxLength)
     throw std::invalid_argument("Poisson FFT: grid side " + std::to_string(n) +
                                 " needs a transform longer than " +
@@ -191,48 +210,52 @@ PoissonFFT::PoissonFFT(int n, double h, hipStream_t stream)
                           static_cast<long double>(k) / static_cast<long double>(M);
     tw[k] = make_double2(static_cast<double>(cosl(a)), -static_cast<double>(sinl(a)));
   }
-  // The kernel on the periodic M x M grid: G(|d|) for |d| < n, zero between.
   std::vector<double> g;
-  poissonKernel(n, h, g);
-  std::vector<double> kgrid(size_t(M) * M, 0.0);
-  auto dist = [&](int m) { return m < n ? m : (m > M - n ? M - m : -1); };
-  for (int my = 0; my < M; ++my) {
-    const int dy = dist(my);
-    if (dy < 0) continue;
-    for (int mx = 0; mx < M; ++mx) {
-      const int dx = dist(mx);
-      if (dx >= 0) kgrid[size_t(my) * M + mx] = g[size_t(dy) * n + dx];
-    }
-  }
-  double *dk = nullptr;
+  numFilters_ = poissonKernel(n, h, g, maxError);
   try {
-    SC_FFT_HIP(hipMalloc(&tw_, sizeof(double2) * tw.size()));
-    SC_FFT_HIP(hipMalloc(&khat_, sizeof(double) * size_t(M) * M));
-    SC_FFT_HIP(hipMalloc(&work_, sizeof(double2) * size_t(M) * M));
-    SC_FFT_HIP(hipMalloc(&dk, sizeof(double) * size_t(M) * M));
+    if (owns_) {
+      SC_FFT_HIP(hipMalloc(&tw_, sizeof(double2) * tw.size()));
+      SC_FFT_HIP(hipMalloc(&khat_, sizeof(double) * size_t(M) * M));
+      SC_FFT_HIP(hipMalloc(&work_, sizeof(double2) * size_t(M) * M));
+    } else {
+      char *base = static_cast<char *>(external);
+      khat_ = reinterpret_cast<double *>(base);
+      base += alignUp(sizeof(double) * size_t(M) * M);
+      work_ = reinterpret_cast<double2 *>(base);
+      base += alignUp(sizeof(double2) * size_t(M) * M);
+      tw_ = reinterpret_cast<double2 *>(base);
+    }
+    // The n x n kernel goes into the work buffer, its periodic M x M grid
+    // into the spectrum buffer, whose forward transform (rows into the work
+    // buffer, then columns) overwrites it with the spectrum.
     SC_FFT_HIP(hipMemcpyAsync(tw_, tw.data(), sizeof(double2) * tw.size(),
                               hipMemcpyHostToDevice, stream_));
-    SC_FFT_HIP(hipMemcpyAsync(dk, kgrid.data(), sizeof(double) * kgrid.size(),
+    SC_FFT_HIP(hipMemcpyAsync(work_, g.data(), sizeof(double) * g.size(),
                               hipMemcpyHostToDevice, stream_));
+    const long long cells = static_cast<long long>(M) * M;
+    kernelGridKernel<<<static_cast<unsigned>((cells + kThreads - 1) / kThreads), kThreads, 0,
+                       stream_>>>(reinterpret_cast<const double *>(work_), n, M, khat_);
+    SC_FFT_HIP(hipGetLastError());
     const int cb = columnsPerBlock(M);
     rowsForwardKernel<<<M, kThreads, sizeof(double2) * M, stream_>>>(
-        dk, nullptr, M, M, M, nullptr, M, logm_, tw_, work_);
+        khat_, nullptr, M, M, M, nullptr, M, logm_, tw_, work_);
     SC_FFT_HIP(hipGetLastError());
     columnsKernel<<<M / cb, kThreads, sizeof(double2) * M * cb, stream_>>>(
         work_, M, 0, M, logm_, cb, tw_, nullptr, khat_);
     SC_FFT_HIP(hipGetLastError());
     SC_FFT_HIP(hipStreamSynchronize(stream_));
-    SC_FFT_HIP(hipFree(dk));
   } catch (...) {
-    if (dk) (void)hipFree(dk);
-    if (tw_) (void)hipFree(tw_);
-    if (khat_) (void)hipFree(khat_);
-    if (work_) (void)hipFree(work_);
+    if (owns_) {
+      if (tw_) (void)hipFree(tw_);
+      if (khat_) (void)hipFree(khat_);
+      if (work_) (void)hipFree(work_);
+    }
     throw;
   }
 }
 
 PoissonFFT::~PoissonFFT() {
+  if (!owns_) return;
   (void)hipFree(tw_);
   (void)hipFree(khat_);
   (void)hipFree(work_);

@@ -16,7 +16,10 @@
 //   columns       : forward FFT of each column (first N0 rows non-zero),
 //                   multiply by the kernel spectrum, inverse FFT, keep N0 rows
 //   rows inverse  : N0 rows, scaled and written out as two N0 x N0 planes
-// Work buffer: N0 x M complex doubles.  Kernel spectrum: M x M doubles.
+// Work buffer: M x M complex doubles.  Kernel spectrum: M x M doubles.
+// The buffers are allocated by the solver, or carved out of one block the
+// caller provides (scratchBytes(n) bytes, 256-byte aligned), in which case
+// the solver allocates no device memory at all.
 #ifndef SC_FFT_HPP_
 #define SC_FFT_HPP_
 
@@ -29,8 +32,15 @@ public:
   // Largest transform length: one column of M complex doubles in 64 KiB LDS.
   static constexpr int kMaxLength = 4096;
 
-  PoissonFFT(int n, double h, hipStream_t stream);
+  // maxError > 0 replaces the exact kernel by its truncated eigen-expansion
+  // (the reference's separable filters, ComputeVectorPotential.h:319-395).
+  // With `external` the buffers live there (scratchBytes(n) bytes).
+  PoissonFFT(int n, double h, hipStream_t stream, double maxError = 0.0,
+             void *external = nullptr);
   ~PoissonFFT();
+  // Transform length for grid side n, and the bytes of the buffers.
+  static int lengthFor(int n);
+  static size_t scratchBytes(int n);
   PoissonFFT(const PoissonFFT &) = delete;
   PoissonFFT &operator=(const PoissonFFT &) = delete;
 
@@ -41,13 +51,17 @@ public:
 
   int length() const { return m_; }
   int n() const { return n_; }
+  // rank-one filters of the truncated kernel (0: the exact kernel)
+  int numFilters() const { return numFilters_; }
 
 private:
   int n_, m_, logm_;
   double h_;
   hipStream_t stream_;
+  bool owns_ = true;
+  int numFilters_ = 0;
   double *khat_ = nullptr;  // M x M real spectrum of the kernel
-  double2 *work_ = nullptr; // max(N0, M) x M complex
+  double2 *work_ = nullptr; // M x M complex
   double2 *tw_ = nullptr;   // exp(-2 pi i k / M), k < M/2
 };
 

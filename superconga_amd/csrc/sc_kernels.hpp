@@ -701,6 +701,20 @@ __global__ void axpyKernel(double *x, const double *d, long long n, double s) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i < n) x[i] += d[i] * s;
 }
+// y = a * base + sum_q c_q v_q (base may alias y; no v_q may alias y)
+constexpr int kMaxCombine = 2 * kMaxHistory + 4;
+struct CombineTerms {
+  const double *v[kMaxCombine];
+  double c[kMaxCombine];
+};
+__global__ void combineTermsKernel(double *y, const double *base, double a, CombineTerms t,
+                                   int m, long long n) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double acc = base ? a * base[i] : 0.0;
+  for (int q = 0; q < m; ++q) acc += t.c[q] * t.v[q][i];
+  y[i] = acc;
+}
 // mean = (sum of the block partials, in block order) / count
 __global__ void finishMeanKernel(const double *partials, long long blocks, double count,
                                  double *mean) {

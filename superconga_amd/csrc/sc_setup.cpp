@@ -540,7 +540,139 @@ void latticeSizes(double width, double ppxi, int &n0, int &n1, double &gf) {
                     std::ceil(0.5 * (std::sqrt(2.0) - 1.0) * static_cast<double>(n0)));
 }
 
-void poissonKernel(int n, double h, std::vector<double> &g) {
+// Eigen-decomposition of a real symmetric n x n matrix (row-major in a,
+// overwritten): Householder reduction to tridiagonal form (Householder
+// 1958), the orthogonal factor formed explicitly, then the implicit QL
+// iteration with Wilkinson shifts (the EISPACK tql2 scheme) with every plane
+// rotation applied to that factor.  The factor is kept transposed, so that a
+// rotation touches two contiguous rows: on return row k of vecs is the unit
+// eigenvector of eigenvalue vals[k] (in no particular order).
+void symmetricEigen(int n, std::vector<double> &a, std::vector<double> &vals,
+                    std::vector<double> &vecs) {
+  const size_t N = size_t(n);
+  std::vector<double> d(N, 0.0), e(N, 0.0);
+  std::vector<double> house; // Householder vectors, v_k over rows k+1..n-1
+  std::vector<double> beta(N, 0.0);
+  std::vector<size_t> vOff(N, 0);
+  std::vector<double> p(N), w(N);
+  for (int k = 0; k + 2 < n; ++k) {
+    const int m = n - k - 1;
+    vOff[size_t(k)] = house.size();
+    double xx = 0.0;
+    for (int r = k + 1; r < n; ++r) xx += a[size_t(r) * N + k] * a[size_t(r) * N + k];
+    const double x0 = a[size_t(k + 1) * N + k];
+    const double norm = std::sqrt(xx);
+    d[size_t(k)] = a[size_t(k) * N + k];
+    if (norm == 0.0) {
+      e[size_t(k)] = 0.0;
+      house.insert(house.end(), size_t(m), 0.0);
+      continue;
+    }
+    const double alpha = x0 >= 0.0 ? -norm : norm;
+    e[size_t(k)] = alpha;
+    // v = x - alpha e_1, beta = 2 / |v|^2
+    house.push_back(x0 - alpha);
+    for (int r = k + 2; r < n; ++r) house.push_back(a[size_t(r) * N + k]);
+    const double *v = house.data() + vOff[size_t(k)];
+    double vv = 0.0;
+    for (int i = 0; i < m; ++i) vv += v[i] * v[i];
+    const double b = 2.0 / vv;
+    beta[size_t(k)] = b;
+    // trailing block B <- H B H: p = b B v, w = p - (b/2)(p.v) v,
+    // B <- B - v w^T - w v^T
+    for (int i = 0; i < m; ++i) {
+      const double *row = &a[size_t(k + 1 + i) * N + size_t(k + 1)];
+      double s = 0.0;
+      for (int j = 0; j < m; ++j) s += row[j] * v[j];
+      p[size_t(i)] = b * s;
+    }
+    double pv = 0.0;
+    for (int i = 0; i < m; ++i) pv += p[size_t(i)] * v[i];
+    const double kk = 0.5 * b * pv;
+    for (int i = 0; i < m; ++i) w[size_t(i)] = p[size_t(i)] - kk * v[i];
+    for (int i = 0; i < m; ++i) {
+      double *row = &a[size_t(k + 1 + i) * N + size_t(k + 1)];
+      const double vi = v[i], wi = w[size_t(i)];
+      for (int j = 0; j < m; ++j) row[j] -= vi * w[size_t(j)] + wi * v[j];
+    }
+  }
+  if (n >= 2) {
+    d[N - 2] = a[(N - 2) * N + (N - 2)];
+    e[N - 2] = a[(N - 1) * N + (N - 2)];
+  }
+  d[N - 1] = a[(N - 1) * N + (N - 1)];
+  e[N - 1] = 0.0;
+  // the transposed orthogonal factor H_{n-3} ... H_0 (each H symmetric)
+  vecs.assign(N * N, 0.0);
+  for (size_t i = 0; i < N; ++i) vecs[i * N + i] = 1.0;
+  std::vector<double> u(N);
+  for (int k = 0; k + 2 < n; ++k) {
+    const double b = beta[size_t(k)];
+    if (b == 0.0) continue;
+    const double *v = house.data() + vOff[size_t(k)];
+    const int m = n - k - 1;
+    std::fill(u.begin(), u.end(), 0.0);
+    for (int i = 0; i < m; ++i) {
+      const double *row = &vecs[size_t(k + 1 + i) * N];
+      const double vi = v[i];
+      for (size_t j = 0; j < N; ++j) u[j] += vi * row[j];
+    }
+    for (int i = 0; i < m; ++i) {
+      double *row = &vecs[size_t(k + 1 + i) * N];
+      const double f = b * v[i];
+      for (size_t j = 0; j < N; ++j) row[j] -= f * u[j];
+    }
+  }
+  // implicit QL on (d, e), e[i] coupling i and i + 1
+  const double eps = std::numeric_limits<double>::epsilon();
+  for (int l = 0; l < n; ++l) {
+    for (int iter = 0; iter < 300; ++iter) {
+      int m = l;
+      for (; m < n - 1; ++m) {
+        const double dd = std::fabs(d[size_t(m)]) + std::fabs(d[size_t(m + 1)]);
+        if (std::fabs(e[size_t(m)]) <= eps * dd) break;
+      }
+      if (m == l) break;
+      if (iter == 299) throw std::runtime_error("eigen-decomposition did not converge");
+      double g = (d[size_t(l + 1)] - d[size_t(l)]) / (2.0 * e[size_t(l)]);
+      double r = std::hypot(g, 1.0);
+      g = d[size_t(m)] - d[size_t(l)] + e[size_t(l)] / (g + (g >= 0.0 ? r : -r));
+      double s = 1.0, c = 1.0, pp = 0.0;
+      int i = m - 1;
+      for (; i >= l; --i) {
+        double f = s * e[size_t(i)];
+        const double bb = c * e[size_t(i)];
+        r = std::hypot(f, g);
+        e[size_t(i + 1)] = r;
+        if (r == 0.0) {
+          d[size_t(i + 1)] -= pp;
+          e[size_t(m)] = 0.0;
+          break;
+        }
+        s = f / r;
+        c = g / r;
+        g = d[size_t(i + 1)] - pp;
+        r = (d[size_t(i)] - g) * s + 2.0 * c * bb;
+        pp = s * r;
+        d[size_t(i + 1)] = g + pp;
+        g = c * r - bb;
+        double *ri = &vecs[size_t(i) * N], *rj = &vecs[size_t(i + 1) * N];
+        for (size_t q = 0; q < N; ++q) {
+          const double t = rj[q];
+          rj[q] = s * ri[q] + c * t;
+          ri[q] = c * ri[q] - s * t;
+        }
+      }
+      if (r == 0.0 && i >= l) continue;
+      d[size_t(l)] -= pp;
+      e[size_t(l)] = g;
+      e[size_t(m)] = 0.0;
+    }
+  }
+  vals = d;
+}
+
+int poissonKernel(int n, double h, std::vector<double> &g, double maxError) {
   // ComputeVectorPotential.h:212-294: analytic matrix elements, symmetrized,
   // scaled by 1/4pi, mean moved into the constant part, then G = Q + C.
   auto alpha = [](int a, int b) {
@@ -576,8 +708,47 @@ void poissonKernel(int n, double h, std::vector<double> &g) {
   }
   mean /= n;
   c += mean;
-  // g[dy*n + dx] = G(|dx|, |dy|); the symmetrized matrix is symmetric.
-  for (double &v : g) v = (v - mean) + c;
+  for (double &v : g) v -= mean;
+  if (maxError <= 0.0) {
+    // g[dy*n + dx] = G(|dx|, |dy|); the symmetrized matrix is symmetric.
+    for (double &v : g) v += c;
+    return 0;
+  }
+  // vector_potential_error > 0 (ComputeVectorPotential.h:319-395): Q is
+  // replaced by its eigen-expansion truncated to the eigenvalues of largest
+  // magnitude, added until the relative Frobenius error |Q - Q_K| / |Q|
+  // falls below maxError; the constant C is exact (a rank-one filter of
+  // ones).  The reference applies the expansion as K separable filter
+  // pairs; its sum is the same kernel, which the FFT convolution applies at
+  // the cost of the exact one.
+  std::vector<double> vals, vecs;
+  symmetricEigen(n, g, vals, vecs);
+  std::vector<int> order(static_cast<size_t>(n));
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return std::fabs(vals[size_t(a)]) > std::fabs(vals[size_t(b)]);
+  });
+  double total = 0.0;
+  for (double v : vals) total += v * v;
+  std::vector<int> kept;
+  double partial = 0.0;
+  for (int k : order) {
+    kept.push_back(k);
+    partial += vals[size_t(k)] * vals[size_t(k)];
+    const double err2 = 1.0 - partial / total;
+    if (std::sqrt(std::max(err2, 0.0)) < maxError) break;
+  }
+  g.assign(size_t(n) * n, std::fabs(c) > 0.0 ? c : 0.0);
+  for (int k : kept) {
+    const double lam = vals[size_t(k)];
+    const double *v = &vecs[size_t(k) * n];
+    for (int j = 0; j < n; ++j) {
+      const double f = lam * v[j];
+      double *row = &g[size_t(j) * n];
+      for (int i = 0; i < n; ++i) row[i] += f * v[i];
+    }
+  }
+  return static_cast<int>(kept.size()) + (std::fabs(c) > 0.0 ? 1 : 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -641,6 +812,7 @@ Config parseConfig(const std::string &text) {
   else if (an == "congacc") c.accel = ACC_CONGACC;
   else if (an == "barzilai-borwein" || an == "bb") c.accel = ACC_BB;
   else if (an == "polyak") c.accel = ACC_POLYAK;
+  else if (an == "broyden") c.accel = ACC_BROYDEN;
   else throw std::runtime_error("unknown accelerator '" + an + "'");
   c.stepSize = ac["step_size"].asDouble(-1);
   c.rank = ac["rank"].asInt(-1);

@@ -23,7 +23,10 @@ enum Accel {
   ACC_ANDERSON = 1,
   ACC_CONGACC = 2,
   ACC_BB = 3,
-  ACC_POLYAK = 4
+  ACC_POLYAK = 4,
+  // Broyden's second method, restarted limited memory (no counterpart in the
+  // reference's accelerators/)
+  ACC_BROYDEN = 5
 };
 
 constexpr int kMaxComponents = 4;
@@ -190,9 +193,16 @@ void ozakiPoles(int numPoles, std::vector<double> &poles,
 double bulkGapSWave(double temperature);
 void latticeSizes(double width, double ppxi, int &n0, int &n1,
                   double &grainFraction);
-// Real, even Green-function kernel of ComputeVectorPotential with
-// vector_potential_error = 0 (ComputeVectorPotential.h:212-294,396-415):
-// returns G(|dx|, |dy|) for 0 <= dx, dy < n as g[dy * n + dx].
-void poissonKernel(int n, double h, std::vector<double> &g);
+// Real, even Green-function kernel of ComputeVectorPotential
+// (ComputeVectorPotential.h:212-294,319-415): G(|dx|, |dy|) for
+// 0 <= dx, dy < n as g[dy * n + dx].  maxError = 0 gives the exact kernel
+// (returns 0); maxError > 0 the eigen-truncated one the reference's
+// separable filters sum to (returns the number of rank-one filters,
+// the constant included).
+int poissonKernel(int n, double h, std::vector<double> &g, double maxError = 0.0);
+// Eigenvalues and eigenvectors (rows of vecs) of a real symmetric n x n
+// matrix given row-major in a (overwritten).
+void symmetricEigen(int n, std::vector<double> &a, std::vector<double> &vals,
+                    std::vector<double> &vecs);
 
 } // namespace scamd

@@ -40,6 +40,10 @@ template <class T> inline void devFree(T *&p) {
   if (p) (void)hipFree(p);
   p = nullptr;
 }
+// Records the message sc_last_error() returns on this thread and returns
+// code (defined with the C ABI in sc_capi.hip).
+int reportError(int code, const std::string &message);
+
 inline unsigned blocksFor(long long n, int bs = 256) {
   return static_cast<unsigned>((n + bs - 1) / bs);
 }

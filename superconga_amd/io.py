@@ -215,16 +215,50 @@ class ConvergenceWriter:
                 "free_energy_per_area"))))
 
 
+def _check_data_format(misc):
+    fmt = misc.get("data_format", "csv")
+    if fmt not in ("csv", "h5"):
+        raise ValueError("misc.data_format must be 'csv' or 'h5', got %r" % (fmt,))
+    if fmt == "h5" and misc.get("save_path", "") and misc.get("save_frequency", -1) != 0:
+        raise ValueError("misc.data_format 'h5' (HDF5 output) is not supported by this "
+                         "build: set misc.data_format to 'csv' (the reference writes the "
+                         "same grids as CSV, io/CSVGridWriter.h)")
+
+
+def _run_iterations(step, sim, compute, n_min, n_max, criterion, writer, verbose):
+    """runIterations (run_iteration.h:85-141): iterate until converged with
+    the minimum reached, or to the maximum (< 0: none); burn-in iterations
+    converge on the boundary residual.  Every iteration writes a row."""
+    if n_min == 0 and n_max == 0:
+        return False
+    i = 1
+    while True:
+        step()
+        r = sim.results()
+        conv = bool(r["converged"]) if compute else r["residual_boundary"] < criterion
+        if verbose:
+            print("-- iteration %d residual %.3e free energy %.10f" % (
+                r["iterations"], r["residual"], r["free_energy_per_area"]))
+        if writer:
+            writer.write(r)
+        if (conv and i >= n_min) or (n_max >= 0 and i >= n_max):
+            return conv
+        i += 1
+
+
 def simulate(config, precision="fp64", verbose=False):
-    """The reference's simulation binary in process: read the config, warm
-    start from misc.load_path, iterate with burn-in/min/max limits, write
+    """The reference's simulation binary in process (src/simulation.cu,
+    runSimulation of run_simulation.h:124-187): read the config, warm start
+    from misc.load_path, burn in, iterate with the min/max limits, write
     misc.save_path.  Returns (converged, results dict)."""
     import superconga_amd as sa
     if isinstance(config, str):
         with open(config) as f:
             config = json.load(f)
     misc = config.get("misc", {})
+    _check_data_format(misc)
     nu = config["numerics"]
+    criterion = nu.get("convergence_criterion", 1e-6)
     sim = sa.Simulation(config)
     try:
         load_path = misc.get("load_path", "")
@@ -242,23 +276,23 @@ def simulate(config, precision="fp64", verbose=False):
         n_burn = nu.get("num_iterations_burnin", 0)
         n_min = nu.get("num_iterations_min", 0)
         n_max = nu.get("num_iterations_max", -1)
-        for _ in range(max(n_burn, 0)):
-            sim.burnin(precision)
+        _run_iterations(lambda: sim.burnin(precision), sim, False, n_burn, n_burn, criterion,
+                        writer, verbose)
         converged = False
-        while True:
-            sim.compute(precision)
-            r = sim.results()
-            if writer:
-                writer.write(r)
-            if verbose:
-                print("-- iteration %d residual %.3e free energy %.10f" % (
-                    r["iterations"], r["residual"], r["free_energy_per_area"]))
-            converged = bool(r["converged"])
-            it = r["iterations"] - max(n_burn, 0)
-            if (converged and it >= n_min) or (n_max >= 0 and it >= n_max):
-                break
-            if save and save_freq > 0 and it % save_freq == 0:
-                save_state(sim, config, save_path)
+        if save_freq > 0:
+            # one iteration at a time; the limits and the save schedule count
+            # the burn-in iterations too (Context::getNumIterations)
+            while True:
+                converged = _run_iterations(lambda: sim.compute(precision), sim, True, 0, 1,
+                                            criterion, writer, verbose)
+                it = sim.results()["iterations"]
+                if (converged and it >= n_min) or (n_max >= 0 and it >= n_max):
+                    break
+                if save and it % save_freq == 0:
+                    save_state(sim, config, save_path)
+        else:
+            converged = _run_iterations(lambda: sim.compute(precision), sim, True, n_min, n_max,
+                                        criterion, writer, verbose)
         results = save_state(sim, config, save_path) if save else None
         return converged, results
     finally:
@@ -306,6 +340,7 @@ def postprocess(config, precision="fp64", verbose=False):
         with open(config) as f:
             config = json.load(f)
     misc = config.get("misc", {})
+    _check_data_format(misc)
     load_path = misc.get("load_path", "")
     sim_path = os.path.join(load_path, "simulation_config.json")
     if not os.path.exists(sim_path):

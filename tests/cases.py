@@ -50,9 +50,9 @@ def regular_polygon_unit(n):
 
 
 # The accelerator known answer: tests/accelerators_test.cu:42-150.
-def accelerator_case(name):
+def accelerator_case(name, **acc):
     return base(temperature=0.5, flux=1.0, lam=10.0, angles=16, ppxi=3.0,
-                cutoff=16.0, norm="linf", criterion=1e-5, accelerator=name)
+                cutoff=16.0, norm="linf", criterion=1e-5, accelerator=name, **acc)
 
 
 # A small grain with every feature on: two components, rotated crystal

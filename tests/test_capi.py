@@ -126,9 +126,8 @@ EXAMPLES = os.path.join(GOLDEN, "examples")
 def example_config(name):
     with open(os.path.join(EXAMPLES, name + ".json")) as f:
         cfg = json.load(f)
-    # The oracle implements the exact Poisson kernel only (error 0); the
-    # product uses the exact kernel for every setting.
-    cfg["numerics"]["vector_potential_error"] = 0.0
+    # unchanged: every example asks for the eigen-truncated Poisson kernel
+    # (vector_potential_error 1e-6), which the product and the oracle build
     return cfg
 
 
@@ -173,3 +172,4 @@ def test_spectroscopy_config_errors(product):
     assert L.sc_ldos_create(sim, b"{broken", ctypes.byref(h)) != 0
     assert L.sc_ldos_compute(None, 0, 0) == 1      # SC_ERR_ARGUMENT: null context
     assert L.sc_newton_krylov(None, 1, 4, ctypes.c_double(1e-8), None) == 1
+

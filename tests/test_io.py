@@ -119,3 +119,64 @@ def test_postprocess_convergence_writer(tmp_path):
     lines = p.read_text().splitlines()
     assert lines == ["iteration,boundary_coherence_residual,ldos_residual",
                      "5,1.00000000e-03,2.00000000e-04"]
+
+
+def test_hdf5_output_is_refused_before_any_device_work(tmp_path):
+    cfg = {"physics": {}, "numerics": {},
+           "misc": {"data_format": "h5", "save_path": str(tmp_path), "save_frequency": -1}}
+    with pytest.raises(ValueError, match="h5"):
+        scio.simulate(cfg)
+    with pytest.raises(ValueError, match="data_format"):
+        scio.simulate({"physics": {}, "numerics": {}, "misc": {"data_format": "xml"}})
+
+
+class _FakeSim:
+    """Residual sequences standing in for a grain: burn-in iterations
+    converge on the boundary residual, compute iterations on `converged`."""
+
+    def __init__(self, boundary, converged):
+        self.boundary = list(boundary)
+        self.conv = list(converged)
+        self.it = 0
+        self.last = None
+
+    def burnin(self):
+        self.it += 1
+        self.last = {"iterations": self.it, "residual": 1.0, "free_energy_per_area": 0.0,
+                     "residual_boundary": self.boundary.pop(0), "converged": 0}
+
+    def compute(self):
+        self.it += 1
+        self.last = {"iterations": self.it, "residual": 1.0, "free_energy_per_area": 0.0,
+                     "residual_boundary": 0.0, "converged": self.conv.pop(0)}
+
+    def results(self):
+        return self.last
+
+
+class _Rows:
+    def __init__(self):
+        self.rows = []
+
+    def write(self, r):
+        self.rows.append(r["iterations"])
+
+
+def test_run_iterations_limits_like_the_reference():
+    # runIterations (run_iteration.h:85-141): burn-in of n runs exactly n
+    # iterations, -1 runs until the boundary residual is below the criterion,
+    # and every iteration (burn-in included) writes a row
+    sim, rows = _FakeSim([1.0, 0.5, 1e-9, 1.0], []), _Rows()
+    assert scio._run_iterations(sim.burnin, sim, False, -1, -1, 1e-6, rows, False)
+    assert rows.rows == [1, 2, 3]
+    sim, rows = _FakeSim([1e-9, 1e-9, 1e-9], []), _Rows()
+    scio._run_iterations(sim.burnin, sim, False, 2, 2, 1e-6, rows, False)
+    assert rows.rows == [1, 2]
+    assert not scio._run_iterations(sim.burnin, sim, False, 0, 0, 1e-6, rows, False)
+    # compute: converged but the minimum not reached keeps going
+    sim, rows = _FakeSim([], [1, 1, 1, 0]), _Rows()
+    assert scio._run_iterations(sim.compute, sim, True, 3, -1, 1e-6, rows, False)
+    assert rows.rows == [1, 2, 3]
+    sim, rows = _FakeSim([], [0] * 5), _Rows()
+    assert not scio._run_iterations(sim.compute, sim, True, 0, 4, 1e-6, rows, False)
+    assert rows.rows == [1, 2, 3, 4]
