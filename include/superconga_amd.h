@@ -237,6 +237,20 @@ int sc_poisson_solve(int n, double h, int num_grain, int enforce_mean,
  * relative Frobenius error. */
 int sc_poisson_solve_ex(int n, double h, int num_grain, int enforce_mean, double max_error,
                         const double *current, double *out);
+/* Largest grid side of sc_poisson_solve_noalloc. */
+#define SC_NOALLOC_MAX_GRID 512
+/* sc_poisson_solve (exact kernel) without any device allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under graph capture, tight serving loops, harnesses
+ * that forbid the allocator).  The transform buffers and the staging of
+ * current and result live in one static device array of about 34 MB per
+ * GPU, held from the load of the library until the process ends; grids
+ * above SC_NOALLOC_MAX_GRID sites a side are rejected with SC_ERR_ARGUMENT.
+ * Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+ * race.  Runs on the null stream of the current device and returns when out
+ * holds the result. */
+int sc_poisson_solve_noalloc(int n, double h, int num_grain, int enforce_mean,
+                             const double *current, double *out);
 /* Host only: the kernel G(|dx|, |dy|) the solve convolves with, n*n values
  * out[dy*n + dx] (ComputeVectorPotential::initializeFilters,
  * ComputeVectorPotential.h:296-417); *num_filters (may be null) receives

@@ -50,7 +50,8 @@ def build(verbose=False):
         ["g++"] + HOST_FLAGS + ["-c", os.path.join(SRC, "sc_setup.cpp"),
                                 "-o", os.path.join(OBJ, "sc_setup.o")],
     ]
-    for name in ("sc_engine", "sc_capi", "sc_fft", "sc_newton", "sc_ldos", "sc_adjoint"):
+    for name in ("sc_engine", "sc_capi", "sc_fft", "sc_newton", "sc_ldos", "sc_adjoint",
+                 "sc_noalloc"):
         src = os.path.join(SRC, name + ".hip")
         if not os.path.exists(src):
             continue

@@ -173,3 +173,17 @@ def test_spectroscopy_config_errors(product):
     assert L.sc_ldos_compute(None, 0, 0) == 1      # SC_ERR_ARGUMENT: null context
     assert L.sc_newton_krylov(None, 1, 4, ctypes.c_double(1e-8), None) == 1
 
+
+def test_noalloc_poisson_refuses_oversized_grid(product):
+    # refused before any device work: the static scratch covers n <= 512
+    import ctypes
+    L = product.lib()
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.sc_poisson_solve_noalloc.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                           ctypes.c_int, dp, dp]
+    j = np.zeros(2 * 600 * 600)
+    out = np.zeros(2 * 600 * 600)
+    rc = L.sc_poisson_solve_noalloc(600, 0.05, 100, 1, j.ctypes.data_as(dp),
+                                    out.ctypes.data_as(dp))
+    assert rc == 1
+    assert "600" in L.sc_last_error().decode()
