@@ -80,13 +80,20 @@ def roofline(traj_steps, sweep_ms, peak, flops_per_step=FLOPS_PER_TRAJ_STEP):
             "avg_launch_ms": round(float(np.mean(sweep_ms)), 4)}
 
 
-def load_traffic(workload, precision):
-    """HBM bytes per sweep launch from the committed rocprofv3 counter runs."""
+def load_traffic(workload, precision, shape):
+    """HBM bytes per sweep launch from the committed rocprofv3 counter runs,
+    when they were taken on the sweep variant and shape (energies per lane,
+    waves per SIMD, slots per CU) that this run timed; None otherwise."""
     path = os.path.join(ROOT, "profiles", "sweep_%s_%s_traffic.json" % (workload, precision))
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("bytes_per_launch")
+        rec = json.load(f)
+    if [int(v) for v in rec.get("shape", [])] != [int(v) for v in shape]:
+        log("traffic: the counter run of %s %s measured shape %s, this run timed %s: "
+            "traffic left null" % (workload, precision, rec.get("shape"), list(shape)))
+        return None
+    return rec.get("bytes_per_launch")
 
 
 def copy_probe(mib):
@@ -173,6 +180,44 @@ def convergence(sa, workloads, args):
             conv["speedup_newton_krylov"] = (conv["fp64"]["wall_s"]
                                              / conv["newton_krylov"]["wall_s"])
     return conv
+
+
+def octagon_convergence(sa, workloads, args):
+    """Wall time towards the 1e-8 criterion on the d-wave octagon itself,
+    each method from the same seeded start and capped: plain FP64 with the
+    config's Anderson(8), plain FP64 with CongAcc, mixed refinement
+    (Anderson), and Newton-Krylov after FP32 iterations.  A method that
+    stops at its cap reports converged false and the residual it reached."""
+    base = workloads.octagon_dwave()
+    cap = args.octagon_converge_cap
+    res = {"workload": "octagon_dwave", "criterion": 1e-8, "cap_iterations": cap}
+    runs = (("fp64_anderson", "anderson", "fp64"), ("fp64_congacc", "congacc", "fp64"),
+            ("mixed_refine_anderson", "anderson", "mixed_refine"))
+    for key, acc, prec in runs:
+        cfg = workloads.octagon_dwave()
+        cfg["accelerator"] = dict(base["accelerator"]) if acc == "anderson" else {"name": acc}
+        res[key] = converge(sa, cfg, prec, cap)
+        log("octagon converge %s: %s" % (key, res[key]))
+    if not args.no_newton:
+        sim = sa.Simulation(base)
+        sim.autotune("fp64")
+        sim.autotune("mixed")
+        t = time.perf_counter()
+        for _ in range(30):
+            sim.compute("mixed")
+        rep = sim.newton_krylov(max_newton=10, krylov_dim=30, tol=1e-11)
+        ok = sim.run(0, 0, 20, "fp64")
+        wall = time.perf_counter() - t
+        r = sim.results()
+        sim.close()
+        res["newton_krylov"] = {"converged": bool(ok), "fp32_iterations": 30,
+                                "newton_iterations": rep["newton_iterations"],
+                                "krylov_iterations": rep["krylov_iterations"],
+                                "map_evaluations": rep["map_evaluations"],
+                                "newton_residual": rep["residual"], "wall_s": round(wall, 3),
+                                "residual": r["residual"]}
+        log("octagon converge newton-krylov: %s" % res["newton_krylov"])
+    return res
 
 
 def ldos_bench(sa, workloads, args):
@@ -277,6 +322,60 @@ def rel(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
 
 
+def nproc():
+    """The CPUs this process may run on (what `nproc` prints)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "host CPU"
+
+
+def cpu_swave_disc(workloads, threads, criterion=1e-6, cap=400):
+    """The oracle on the whole s-wave disc: seconds per full iteration and
+    the wall time to the L2 criterion 1e-6 (Picard) from the seeded initial
+    state the GPU leg starts from."""
+    import superconga_amd as sa
+    from oracle import pyoracle
+    cfg = workloads.swave_disc()
+    cfg["numerics"]["convergence_criterion"] = criterion
+    sim = sa.Simulation(cfg)
+    d0 = sim.order_parameter(0)
+    sim.close()
+    o = pyoracle.Oracle(dict(cfg, num_threads=threads))
+    for c, d in enumerate(d0):
+        o.set_order_parameter(c, d)
+    t = time.perf_counter()
+    ok = o.run(0, 0, cap)
+    wall = time.perf_counter() - t
+    its = o.get_int("num_iterations")
+    return {"workload": "swave_disc", "criterion": criterion, "converged": bool(ok),
+            "iterations": its, "time_to_criterion_s": round(wall, 3),
+            "seconds_per_iteration": round(wall / max(its, 1), 4)}
+
+
+def gpu_swave_disc(sa, workloads, criterion=1e-6, cap=400):
+    """The same solve (s-wave disc, Picard, L2 criterion 1e-6, same start) on
+    the GPU in FP64, sweep shape autotuned outside the timed region."""
+    cfg = workloads.swave_disc()
+    cfg["numerics"]["convergence_criterion"] = criterion
+    r = converge(sa, cfg, "fp64", cap)
+    r["workload"] = "swave_disc"
+    r["criterion"] = criterion
+    r["seconds_per_iteration"] = round(r["wall_s"] / max(r["iterations"], 1), 6)
+    return r
+
+
 def cpu_baseline(cfg, seconds, threads):
     """The oracle (FP64 OpenMP restatement of the reference's CPU path) on
     a bounded sample: the per-angle iteration work for consecutive angles
@@ -294,21 +393,15 @@ def cpu_baseline(cfg, seconds, threads):
         steps += o.sweep_angles(k, k + 1)
         spent += time.perf_counter() - t0
         k += 1
-    cpu = ""
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    cpu = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
     return {"value": steps / spent, "unit": "traj-steps/s", "cores": threads,
             "kind": "port",
             "sample": "angles 1-%d of %d of the same grain (rotation, energy shift, "
                       "both Riccati sweeps, Green functions, angle integration; "
-                      "%d traj-steps in %.1f s), FP64, %d OpenMP threads on %s"
-                      % (k - 1, o.Na, steps, spent, threads, cpu or "host CPU")}
+                      "%d traj-steps in %.1f s), FP64, %d OpenMP threads (nproc) on %s"
+                      % (k - 1, o.Na, steps, spent, threads, cpu_model()),
+            # the angle work of one whole iteration at the sample's rate (the
+            # vector-potential solve and the accelerator not included)
+            "seconds_per_iteration_angles": round(spent / max(k - 1, 1) * o.Na, 2)}
 
 
 def main():
@@ -341,9 +434,11 @@ def main():
     ap.add_argument("--no-newton", action="store_true")
     ap.add_argument("--converge-only", action="store_true",
                     help="only the time-to-1e-8 legs (for profiling them)")
-    ap.add_argument("--cpu-threads", type=int,
-                    default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
-                    or min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-threads", type=int, default=nproc(),
+                    help="OpenMP threads of the CPU baseline (default: nproc)")
+    ap.add_argument("--octagon-converge-cap", type=int, default=300,
+                    help="iteration cap of each time-to-1e-8 method on the octagon")
+    ap.add_argument("--no-octagon-converge", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -393,7 +488,7 @@ def main():
     wall, sweep = timed_iterations(sim, "fp64", max(args.warmup - 1, 0), args.steps)
     value = d.traj_steps * args.steps / wall
     rf = roofline(d.traj_steps, sweep, PEAK_FP64_TFLOPS)
-    rf["traffic"] = load_traffic(args.workload, "fp64")
+    rf["traffic"] = load_traffic(args.workload, "fp64", shape)
     log("fp64: %.2f ms/iter, sweep %.2f ms, %.3e traj-steps/s, %.2f TFLOP/s (%.1f%%)"
         % (1e3 * wall / args.steps, np.mean(sweep), value, rf["achieved"],
            100 * rf["frac"]))
@@ -435,7 +530,7 @@ def main():
         mwall, msweep = timed_iterations(sim, "mixed", max(args.warmup - 1, 0), args.steps)
         mval = d.traj_steps * args.steps / mwall
         mrf = roofline(d.traj_steps, msweep, PEAK_FP32_TFLOPS)
-        mrf["traffic"] = load_traffic(args.workload, "mixed")
+        mrf["traffic"] = load_traffic(args.workload, "mixed", mshape)
         log("mixed: %.2f ms/iter, sweep %.2f ms, %.3e traj-steps/s; error vs fp64 %s"
             % (1e3 * mwall / args.steps, np.mean(msweep), mval, err))
         out["mixed"] = {"value": mval, "unit": "traj-steps/s",
@@ -454,9 +549,17 @@ def main():
         out["ldos"] = ldos_bench(sa, workloads, args)
         log("ldos: %s" % out["ldos"])
 
+    if not args.no_converge and not args.no_octagon_converge and \
+            args.workload == "octagon_dwave":
+        out["time_to_1e-8_octagon"] = octagon_convergence(sa, workloads, args)
+
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds, args.cpu_threads)
         log("cpu: %.3e traj-steps/s" % out["cpu_baseline"]["value"])
+        out["cpu_baseline"]["swave_disc"] = cpu_swave_disc(workloads, args.cpu_threads)
+        log("cpu swave_disc: %s" % out["cpu_baseline"]["swave_disc"])
+        out["gpu_swave_disc"] = gpu_swave_disc(sa, workloads)
+        log("gpu swave_disc: %s" % out["gpu_swave_disc"])
     print(json.dumps(out), flush=True)
 
 

@@ -31,7 +31,7 @@ extern "C" {
 /* Arithmetic of the Riccati stepping. */
 #define SC_PRECISION_FP64 0  /* everything in FP64 */
 #define SC_PRECISION_MIXED 1 /* FP32 stepping, FP64 energy sums and reductions */
-#define SC_PRECISION_MIXED_REFINE 2 /* sc_run only: MIXED to the FP32 floor, then FP64 */
+#define SC_PRECISION_MIXED_REFINE 2 /* sc_run, sc_batch_run: MIXED to the FP32 floor, then FP64 */
 
 typedef struct sc_context sc_context;
 typedef struct sc_batch sc_batch;
@@ -209,6 +209,45 @@ int sc_free_energy_gradient(sc_context *ctx, double *grad, double *free_energy);
 int sc_riccati_sweep(sc_context *ctx, int precision, int burnin);
 /* BoundaryConditionSpecular::computeBoundaryCondition (BoundaryConditionSpecular.h:361-486). */
 int sc_boundary_condition(sc_context *ctx, double *residual);
+
+/* One iteration angle by angle, the way IntegrationIteratorOzaki::compute
+ * drives the RiccatiSolver and the ComputeProperty hooks
+ * (IntegrationIteratorOzaki.h:300-420, ComputeProperty.h:27-103):
+ *   sc_integration_begin(ctx, burnin)
+ *       Context<T>::compute up to the angle loop (Context.h:391-447: the
+ *       accelerator step from the second iteration on) and
+ *       ComputeProperty::initializeIteration;
+ *   for consecutive ranges [angle_begin, angle_end) covering every angle:
+ *     sc_sweep_angles(ctx, precision, burnin, angle_begin, angle_end)
+ *       RiccatiSolver::computeCoherenceFunctions(BurnIn) of those angles
+ *       (RiccatiSolverConfined.h:513-671) with computeGreensAngle;
+ *     sc_integrate_angles(ctx, angle_begin, angle_end)
+ *       ComputeProperty::integrateGreensAngle of those angles
+ *       (ComputeOrderParameter.h:174-210, ComputeCurrent.h:142-173,
+ *       ComputeFreeEnergy.h:260-279); ranges in increasing order, none in
+ *       a burn-in;
+ *   sc_integration_end(ctx, burnin)
+ *       ComputeProperty::computePropertyFromGreens, the boundary condition,
+ *       the vector-potential solve and the residuals (the rest of
+ *       Context<T>::compute / burnIn).
+ * The result equals sc_compute / sc_burnin bit for bit.  Calls out of this
+ * order return SC_ERR_STATE-like errors (SC_ERR_DEVICE with the message). */
+int sc_integration_begin(sc_context *ctx, int burnin);
+int sc_sweep_angles(sc_context *ctx, int precision, int burnin, int angle_begin, int angle_end);
+int sc_integrate_angles(sc_context *ctx, int angle_begin, int angle_end);
+int sc_integration_end(sc_context *ctx, int burnin);
+
+/* Accelerator<T>::compute (accelerators/Accelerator.h:25-71) on
+ * caller-supplied fields: x is the iterate and g the map's result, each the
+ * concatenated field of accelerators/utils.h:58-186 -- Re and Im of every
+ * order-parameter component, then lambda^2 Ax and lambda^2 Ay when the
+ * context has a finite penetration depth -- of num_fields x N0^2 values
+ * (sc_num_fields).  x receives the accelerated iterate.  The accelerator
+ * (name, rank, step size ... of the config) and its history are the
+ * context's own, so consecutive calls continue one acceleration; the
+ * context's order parameter and vector potential are overwritten. */
+int sc_num_fields(sc_context *ctx, int *num_fields);
+int sc_accelerator_step(sc_context *ctx, double *x, const double *g);
 /* Lane layout of the sweep for one precision (no reference counterpart: the
  * reference's shape rule is getKernelDimensions, GridFunctions_GPU.h:1121-1189):
  * Ozaki energies per lane, the register budget the kernel variant is compiled
@@ -277,7 +316,7 @@ int sc_describe(const char *config_json, int angle, sc_dims *out,
 
 /* ----- Batches: many grains of a parameter sweep iterated together -----
  * No counterpart in the reference, whose sweeps are sequential processes
- * warm-started from files (examples/parameter_sweeps/*.sh).  Every iteration
+ * warm-started from files (examples/parameter_sweeps/, shell scripts).  Every iteration
  * runs the per-grain stages on one stream and sweeps all grains that are
  * still iterating in one persistent launch over a shared work queue.  The
  * handles returned by sc_batch_grain are borrowed: they work with every
@@ -290,7 +329,10 @@ int sc_batch_compute(sc_batch *batch, int precision);
 int sc_batch_burnin(sc_batch *batch, int precision);
 /* runSimulation per grain (limits of the first config when negative); each
  * grain stops when converged with the minimum reached, or at the maximum.
- * converged (count entries, may be null) receives the final status. */
+ * converged (count entries, may be null) receives the final status.  With
+ * SC_PRECISION_MIXED_REFINE each grain switches from FP32 stepping to FP64
+ * on its own (the rule of sc_run) and converges only on an FP64 iteration;
+ * an iteration sweeps the grains of each precision in one launch each. */
 int sc_batch_run(sc_batch *batch, int num_burnin, int num_min, int num_max,
                  int precision, int *converged);
 int sc_batch_set_sweep_shape(sc_batch *batch, int precision, int energies_per_lane,

@@ -106,6 +106,12 @@ def lib():
         L.sc_get_results.argtypes = [vp, ctypes.POINTER(Results)]
         L.sc_riccati_sweep.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.sc_boundary_condition.argtypes = [vp, _DP]
+        L.sc_integration_begin.argtypes = [vp, ctypes.c_int]
+        L.sc_sweep_angles.argtypes = [vp] + [ctypes.c_int] * 4
+        L.sc_integrate_angles.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.sc_integration_end.argtypes = [vp, ctypes.c_int]
+        L.sc_num_fields.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+        L.sc_accelerator_step.argtypes = [vp, _DP, _DP]
         L.sc_set_sweep_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_int]
         ip = ctypes.POINTER(ctypes.c_int)
@@ -323,6 +329,34 @@ class Simulation:
 
     def riccati_sweep(self, precision="fp64", burnin=False):
         _check(lib().sc_riccati_sweep(self._h, _precision(precision), int(burnin)))
+
+    # ----- the iteration angle by angle (IntegrationIteratorOzaki::compute) -----
+    def integration_begin(self, burnin=False):
+        _check(lib().sc_integration_begin(self._h, int(burnin)))
+
+    def sweep_angles(self, begin, end, precision="fp64", burnin=False):
+        _check(lib().sc_sweep_angles(self._h, _precision(precision), int(burnin), begin, end))
+
+    def integrate_angles(self, begin, end):
+        _check(lib().sc_integrate_angles(self._h, begin, end))
+
+    def integration_end(self, burnin=False):
+        _check(lib().sc_integration_end(self._h, int(burnin)))
+
+    def num_fields(self):
+        n = ctypes.c_int()
+        _check(lib().sc_num_fields(self._h, ctypes.byref(n)))
+        return n.value
+
+    def accelerator_step(self, x, g):
+        """Accelerator<T>::compute on caller-supplied fields: x (the
+        iterate) and g (the map's result), each (num_fields, N0, N0);
+        returns the accelerated iterate."""
+        shape = (self.num_fields(), self.N0, self.N0)
+        x = np.array(x, dtype=np.float64, order="C").reshape(shape)
+        g = np.ascontiguousarray(g, dtype=np.float64).reshape(shape)
+        _check(lib().sc_accelerator_step(self._h, _dp(x), _dp(g)))
+        return x
 
     def boundary_condition(self):
         r = ctypes.c_double()

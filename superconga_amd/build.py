@@ -16,6 +16,7 @@ SRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "libsuperconga_amd.so")
+SIMULATION = os.path.join(LIBDIR, "simulation_amd")
 ARCH = os.environ.get("SUPERCONGA_AMD_ARCH", "gfx950")
 
 
@@ -68,6 +69,15 @@ def build(verbose=False):
     _run([hipcc, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
          + ["-pthread", "-Wl,--no-undefined"])
     os.replace(tmp, LIB)
+    # the drop-in for the reference's `simulation` binary, host code on the
+    # C++ adaptors (include/superconga_amd.hpp) linked against the library
+    rocm_lib = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
+    tmp = SIMULATION + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra",
+          "-I" + os.path.join(os.path.dirname(HERE), "include"),
+          os.path.join(SRC, "simulation_amd.cpp"), "-o", tmp, "-L" + LIBDIR,
+          "-lsuperconga_amd", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link," + rocm_lib])
+    os.replace(tmp, SIMULATION)
     return LIB
 
 

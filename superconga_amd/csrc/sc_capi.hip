@@ -458,6 +458,62 @@ int sc_free_energy_gradient(sc_context *ctx, double *grad, double *freeEnergy) {
   });
 }
 
+int sc_integration_begin(sc_context *ctx, int burnin) {
+  SC_CHECK_CTX(ctx);
+  SC_CHECK_SIM(ctx);
+  return guarded([&] {
+    ctx->grain->beginIntegration(burnin != 0);
+    return SC_OK;
+  });
+}
+
+int sc_sweep_angles(sc_context *ctx, int precision, int burnin, int angleBegin, int angleEnd) {
+  SC_CHECK_CTX(ctx);
+  SC_CHECK_SIM(ctx);
+  return guarded([&] {
+    checkPrecision(precision);
+    ctx->grain->sweepAngles(precision, burnin != 0, angleBegin, angleEnd);
+    return SC_OK;
+  });
+}
+
+int sc_integrate_angles(sc_context *ctx, int angleBegin, int angleEnd) {
+  SC_CHECK_CTX(ctx);
+  SC_CHECK_SIM(ctx);
+  return guarded([&] {
+    ctx->grain->integrateAngles(angleBegin, angleEnd);
+    return SC_OK;
+  });
+}
+
+int sc_integration_end(sc_context *ctx, int burnin) {
+  SC_CHECK_CTX(ctx);
+  SC_CHECK_SIM(ctx);
+  return guarded([&] {
+    ctx->grain->endIntegration(burnin != 0);
+    return SC_OK;
+  });
+}
+
+int sc_num_fields(sc_context *ctx, int *numFields) {
+  SC_CHECK_CTX(ctx);
+  SC_CHECK_SIM(ctx);
+  SC_CHECK_PTR(numFields);
+  *numFields = ctx->grain->numFields();
+  return SC_OK;
+}
+
+int sc_accelerator_step(sc_context *ctx, double *x, const double *g) {
+  SC_CHECK_CTX(ctx);
+  SC_CHECK_SIM(ctx);
+  SC_CHECK_PTR(x);
+  SC_CHECK_PTR(g);
+  return guarded([&] {
+    ctx->grain->acceleratorStepOn(x, g);
+    return SC_OK;
+  });
+}
+
 int sc_poisson_solve(int n, double h, int numGrain, int enforce, const double *current,
                      double *out) {
   return sc_poisson_solve_ex(n, h, numGrain, enforce, 0.0, current, out);
@@ -633,7 +689,7 @@ int sc_batch_burnin(sc_batch *b, int precision) {
 int sc_batch_run(sc_batch *b, int nb, int nmin, int nmax, int precision, int *converged) {
   SC_CHECK_PTR(b);
   return guarded([&] {
-    checkPrecision(precision);
+    if (precision != SC_PRECISION_MIXED_REFINE) checkPrecision(precision);
     const scamd::Config &c = b->batch->grain(0).setup().cfg;
     std::vector<int> conv;
     b->batch->run(nb < 0 ? c.burnin : nb, nmin < 0 ? c.itMin : nmin, nmax < 0 ? c.itMax : nmax,

@@ -114,8 +114,10 @@ Grain::~Grain() {
   if (dScratch_) (void)hipFree(dScratch_);
   for (SweepPlan &pl : plan_) {
     if (pl.dItems) (void)hipFree(pl.dItems);
+    if (pl.dItemsByAngle) (void)hipFree(pl.dItemsByAngle);
     if (pl.dView) (void)hipFree(pl.dView);
   }
+  devFree(dGatherAcc_);
   devFree(dCounter_);
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
@@ -341,6 +343,22 @@ void Grain::prepareSweep(int precision) {
   pl.dItems = nullptr;
   SC_HIP(hipMalloc(&pl.dItems, std::max<size_t>(1, di.size()) * sizeof(SweepItemDev)));
   if (!di.empty()) toDevice(pl.dItems, di.data(), di.size() * sizeof(SweepItemDev));
+  if (pl.dItemsByAngle) (void)hipFree(pl.dItemsByAngle);
+  pl.dItemsByAngle = nullptr;
+  pl.angleStart.clear();
+  if (!s.keldysh) {
+    // the same items grouped by angle (longest first within an angle)
+    std::vector<SweepItemDev> byAngle(di);
+    std::stable_sort(byAngle.begin(), byAngle.end(),
+                     [](const SweepItemDev &a, const SweepItemDev &b) { return a.angle < b.angle; });
+    pl.angleStart.assign(size_t(s.Na) + 1, 0);
+    for (const SweepItemDev &it : byAngle) ++pl.angleStart[size_t(it.angle) + 1];
+    for (int k = 0; k < s.Na; ++k) pl.angleStart[size_t(k) + 1] += pl.angleStart[size_t(k)];
+    SC_HIP(hipMalloc(&pl.dItemsByAngle,
+                     std::max<size_t>(1, byAngle.size()) * sizeof(SweepItemDev)));
+    if (!byAngle.empty())
+      toDevice(pl.dItemsByAngle, byAngle.data(), byAngle.size() * sizeof(SweepItemDev));
+  }
   if (pl.dView) (void)hipFree(pl.dView);
   pl.dView = nullptr;
   SC_HIP(hipMalloc(&pl.dView, views.size() * sizeof(GrainView)));
@@ -1019,7 +1037,20 @@ void Grain::afterSweep(bool burnin) {
   if (!burnin) {
     SC_HIP(hipMemcpyAsync(dCurrentOld_, dCurrent_, 2 * n0 * sizeof(double),
                           hipMemcpyDeviceToDevice, stream_));
-    backRotate();
+    if (rangeMode_) {
+      // the angle sums were added range by range: their final scaling
+      GrainView v = view();
+      const int ns = static_cast<int>(s.domainSites.size());
+      const double jf = 2.0 * static_cast<double>(s.cfg.chargeSign) * s.temperature;
+      if (ns > 0) {
+        gatherFinishKernel<<<blocksFor(ns), 256, 0, stream_>>>(
+            v, dDomainSites_, ns, dGatherAcc_, dCoupling_, jf, dDeltaNew_, dCurrent_,
+            dEnergyDens_);
+        SC_HIP(hipGetLastError());
+      }
+    } else {
+      backRotate();
+    }
     SC_HIP(hipEventRecord(ev2_, stream_));
     // ComputeOrderParameter::computePropertyFromGreens (:212-262)
     hDelta_.assign(size_t(s.Nc), -1);
@@ -1585,6 +1616,104 @@ void Grain::compute(int precision) {
   updateSweepTime();
 }
 
+void Grain::beginIntegration(bool burnin) {
+  const Setup &s = *s_;
+  if (s.keldysh) throw std::runtime_error("a spectroscopy context only runs the LDOS iteration");
+  if (burnin) {
+    beginBurnin();
+  } else {
+    beginCompute();
+    const size_t ns = s.domainSites.size();
+    if (!dGatherAcc_) dGatherAcc_ = devAlloc<double>(std::max<size_t>(1, size_t(2 * s.Nc + 3) * ns));
+    if (ns > 0)
+      SC_HIP(hipMemsetAsync(dGatherAcc_, 0, size_t(2 * s.Nc + 3) * ns * sizeof(double), stream_));
+  }
+  nextAngle_ = 0;
+  rangeMode_ = true;
+  rangeBurnin_ = burnin;
+}
+
+void Grain::sweepAngles(int precision, bool burnin, int k0, int k1) {
+  const Setup &s = *s_;
+  if (!rangeMode_ || burnin != rangeBurnin_)
+    throw std::runtime_error("sweep of an angle range outside beginIntegration / endIntegration "
+                             "(or with the other burn-in setting)");
+  if (precision != PREC_FP64 && precision != PREC_MIXED)
+    throw std::runtime_error("unknown precision");
+  if (k0 < 0 || k1 > s.Na || k0 >= k1) throw std::runtime_error("angle range out of bounds");
+  prepareSweep(precision);
+  const SweepPlan &pl = plan_[precision];
+  const int i0 = pl.angleStart[size_t(k0)], i1 = pl.angleStart[size_t(k1)];
+  SweepArgs a;
+  a.views = reinterpret_cast<const GrainView *>(pl.dView);
+  a.items = reinterpret_cast<const SweepItemDev *>(pl.dItemsByAngle) + i0;
+  a.numItems = i1 - i0;
+  a.counter = dCounter_;
+  a.scratch = dScratch_;
+  a.slotStride = static_cast<long long>(pl.maxLen) * pl.el * 64;
+  a.burnin = burnin ? 1 : 0;
+  resetCounter<<<1, 1, 0, stream_>>>(dCounter_);
+  SC_HIP(hipEventRecord(ev0_, stream_));
+  if (a.numItems > 0)
+    launchSweepVariant(precision, pl.el, pl.waves, a, std::min(pl.numSlots, a.numItems),
+                       stream_, false);
+  SC_HIP(hipEventRecord(ev1_, stream_));
+}
+
+void Grain::integrateAngles(int k0, int k1) {
+  const Setup &s = *s_;
+  if (!rangeMode_ || rangeBurnin_)
+    throw std::runtime_error("angle integration outside beginIntegration / endIntegration "
+                             "(or in a burn-in)");
+  if (k0 != nextAngle_ || k1 <= k0 || k1 > s.Na)
+    throw std::runtime_error("angles must be integrated in order: expected a range from " +
+                             std::to_string(nextAngle_));
+  const size_t n1 = size_t(s.N1) * s.N1;
+  // the extension of the frames' Green-function planes, then their sums
+  extendFrames(dG_ + size_t(k0) * 4 * n1, k0, k1, 4, 1, static_cast<long long>(4 * n1), 4);
+  const int ns = static_cast<int>(s.domainSites.size());
+  if (ns > 0) {
+    gatherRangeKernel<<<blocksFor(ns), 256, 0, stream_>>>(view(), dDomainSites_, ns, k0, k1,
+                                                         dGatherAcc_);
+    SC_HIP(hipGetLastError());
+  }
+  nextAngle_ = k1;
+}
+
+void Grain::endIntegration(bool burnin) {
+  if (!rangeMode_ || burnin != rangeBurnin_)
+    throw std::runtime_error("endIntegration without a matching beginIntegration");
+  if (!burnin && nextAngle_ != s_->Na)
+    throw std::runtime_error("endIntegration before every angle was integrated (next angle " +
+                             std::to_string(nextAngle_) + " of " + std::to_string(s_->Na) + ")");
+  try {
+    if (burnin) endBurnin();
+    else endCompute();
+  } catch (...) {
+    rangeMode_ = false;
+    throw;
+  }
+  rangeMode_ = false;
+  updateSweepTime();
+}
+
+int Grain::numFields() const {
+  const Setup &s = *s_;
+  const bool doA = s.cfg.computeCurrent && s.lambda > 0.0;
+  return 2 * s.Nc + (doA ? 2 : 0);
+}
+
+void Grain::acceleratorStepOn(double *x, const double *g) {
+  const Setup &s = *s_;
+  if (s.keldysh) throw std::runtime_error("a spectroscopy context has no accelerator");
+  const size_t bytes = size_t(numFields()) * size_t(s.N0) * s.N0 * sizeof(double);
+  toDevice(dDelta_, x, bytes);
+  toDevice(dDeltaNew_, g, bytes);
+  clearReductions();
+  acceleratorStep();
+  fromDevice(x, dDelta_, bytes);
+}
+
 void Grain::beginBurnin() {
   if (s_->keldysh) throw std::runtime_error("a spectroscopy context only runs the LDOS iteration");
   clearReductions();
@@ -1662,7 +1791,8 @@ bool Grain::run(int nBurnin, int itMin, int itMax, int precision) {
 Batch::Batch(std::vector<std::unique_ptr<Setup>> setups) {
   if (setups.empty()) throw std::runtime_error("a batch needs at least one grain");
   SC_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-  for (hipEvent_t *e : {&ev0_, &ev1_, &evIt0_, &evIt1_, &evFork_}) SC_HIP(hipEventCreate(e));
+  for (hipEvent_t *e : {&ev0_[0], &ev0_[1], &ev1_[0], &ev1_[1], &evIt0_, &evIt1_, &evFork_})
+    SC_HIP(hipEventCreate(e));
   // three lanes: with the main stream, the four hardware queues a process has
   const size_t numLanes = std::min<size_t>(3, setups.size());
   for (size_t i = 0; i < numLanes; ++i) {
@@ -1691,7 +1821,7 @@ Batch::~Batch() {
   }
   if (dScratch_) (void)hipFree(dScratch_);
   devFree(dCounter_);
-  for (hipEvent_t e : {ev0_, ev1_, evIt0_, evIt1_, evFork_})
+  for (hipEvent_t e : {ev0_[0], ev0_[1], ev1_[0], ev1_[1], evIt0_, evIt1_, evFork_})
     if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : laneEv_) (void)hipEventDestroy(e);
   for (hipStream_t l : lanes_) (void)hipStreamDestroy(l);
@@ -1810,23 +1940,43 @@ void Batch::sweep(int precision, bool burnin, const std::vector<int> &act) {
   a.slotStride = static_cast<long long>(pl.maxLen) * pl.el * 64;
   a.burnin = burnin ? 1 : 0;
   resetCounter<<<1, 1, 0, stream_>>>(dCounter_);
-  SC_HIP(hipEventRecord(ev0_, stream_));
+  SC_HIP(hipEventRecord(ev0_[precision], stream_));
   launchSweepVariant(precision, pl.el, pl.waves, a, pl.numSlots, stream_);
-  SC_HIP(hipEventRecord(ev1_, stream_));
+  SC_HIP(hipEventRecord(ev1_[precision], stream_));
   // and the stages after it read what the sweep wrote
   forkLanes();
+}
+
+float Batch::sweepTime(int precision) const {
+  float ms = 0.f;
+  SC_HIP(hipEventElapsedTime(&ms, ev0_[precision], ev1_[precision]));
+  return ms;
 }
 
 void Batch::compute(int precision) {
   if (precision != PREC_FP64 && precision != PREC_MIXED)
     throw std::runtime_error("unknown precision");
+  computeEach(std::vector<int>(grains_.size(), precision));
+}
+
+void Batch::computeEach(const std::vector<int> &precisionOf) {
+  if (precisionOf.size() != grains_.size())
+    throw std::runtime_error("one precision per grain expected");
   const std::vector<int> act = active();
   if (act.empty()) return;
+  // the grains of each precision: one sweep launch each
+  std::vector<int> group[2];
+  for (int gi : act) {
+    const int p = precisionOf[size_t(gi)];
+    if (p != PREC_FP64 && p != PREC_MIXED) throw std::runtime_error("unknown precision");
+    group[p].push_back(gi);
+  }
   SC_HIP(hipEventRecord(evIt0_, stream_));
   for (int gi : act) grains_[size_t(gi)]->queueBeginCompute();
   for (int gi : act) grains_[size_t(gi)]->queueReadback();
   for (int gi : act) grains_[size_t(gi)]->finishBeginCompute();
-  sweep(precision, false, act);
+  for (int p : {PREC_FP64, PREC_MIXED})
+    if (!group[p].empty()) sweep(p, false, group[p]);
   // every grain's stages are queued before the first read-back waits, so
   // the lanes run them side by side
   for (int gi : act) grains_[size_t(gi)]->queueEndCompute();
@@ -1835,9 +1985,10 @@ void Batch::compute(int precision) {
   joinLanes();
   SC_HIP(hipEventRecord(evIt1_, stream_));
   SC_HIP(hipEventSynchronize(evIt1_));
+  sweepMs_ = 0.0;
+  for (int p : {PREC_FP64, PREC_MIXED})
+    if (!group[p].empty()) sweepMs_ += sweepTime(p);
   float ms = 0.f;
-  SC_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
-  sweepMs_ = ms;
   SC_HIP(hipEventElapsedTime(&ms, evIt0_, evIt1_));
   iterMs_ = ms;
   trajSteps_ = 0;
@@ -1858,15 +2009,36 @@ void Batch::burnin(int precision) {
 
 int Batch::run(int nBurnin, int itMin, int itMax, int precision, std::vector<int> &converged) {
   // runSimulation (run_simulation.h:124-187) for every grain at once.
+  if (precision != PREC_FP64 && precision != PREC_MIXED && precision != PREC_MIXED_REFINE)
+    throw std::runtime_error("unknown precision");
+  const bool refine = precision == PREC_MIXED_REFINE;
+  const size_t n = grains_.size();
+  // the precision of each grain's next iteration, and Grain::run's switch
+  // rule per grain: FP32 stepping until the residual is below kRefineSwitch
+  // or has not improved for kRefineStall iterations
+  constexpr double kRefineSwitch = 1e-5;
+  constexpr int kRefineStall = 20;
+  std::vector<int> prec(n, refine ? PREC_MIXED : precision);
+  std::vector<double> best(n, 0.0);
+  std::vector<int> stall(n, 0), mixedIts(n, 0);
+  for (auto &g : grains_) g->setMixedIterations(0);
   std::fill(done_.begin(), done_.end(), 0);
-  for (int i = 0; i < nBurnin; ++i) burnin(precision);
-  converged.assign(grains_.size(), 0);
+  for (int i = 0; i < nBurnin; ++i) burnin(refine ? PREC_MIXED : precision);
+  converged.assign(n, 0);
   if (itMin == 0 && itMax == 0) return 0;
   for (int it = 1;; ++it) {
-    compute(precision);
-    for (size_t g = 0; g < grains_.size(); ++g) {
+    computeEach(prec);
+    for (size_t g = 0; g < n; ++g) {
       if (done_[g]) continue;
-      const bool conv = grains_[g]->results().converged != 0;
+      const Results r = grains_[g]->results();
+      if (prec[g] == PREC_MIXED) grains_[g]->setMixedIterations(++mixedIts[g]);
+      const bool conv = r.converged != 0 && (!refine || prec[g] == PREC_FP64);
+      if (refine && prec[g] == PREC_MIXED) {
+        stall[g] = (it > 1 && r.residual >= best[g]) ? stall[g] + 1 : 0;
+        best[g] = (it == 1) ? r.residual : std::min(best[g], r.residual);
+        if (r.residual < kRefineSwitch || stall[g] >= kRefineStall || r.converged)
+          prec[g] = PREC_FP64;
+      }
       converged[g] = conv ? 1 : 0;
       const bool minReached = it >= itMin;
       const bool maxReached = itMax >= 0 && it >= itMax;
@@ -1875,9 +2047,9 @@ int Batch::run(int nBurnin, int itMin, int itMax, int precision, std::vector<int
     if (active().empty()) break;
   }
   std::fill(done_.begin(), done_.end(), 0);
-  int n = 0;
-  for (int c : converged) n += c;
-  return n;
+  int numConverged = 0;
+  for (int c : converged) numConverged += c;
+  return numConverged;
 }
 
 void Batch::autotune(int precision) {
@@ -1898,9 +2070,8 @@ void Batch::autotune(int precision) {
     float ms = -1.f;
     for (int rep = 0; rep < 2; ++rep) {
       sweep(precision, false, act);
-      SC_HIP(hipEventSynchronize(ev1_));
-      float t = 0.f;
-      SC_HIP(hipEventElapsedTime(&t, ev0_, ev1_));
+      SC_HIP(hipEventSynchronize(ev1_[precision]));
+      const float t = sweepTime(precision);
       if (ms < 0.f || t < ms) ms = t;
     }
     if (best < 0.f || ms < best) {

@@ -93,6 +93,26 @@ public:
   // starts the transfer of the queued reductions (flushReductions waits)
   void queueReadback();
   bool run(int burnin, int itMin, int itMax, int precision);
+  // The same iteration angle by angle, the way IntegrationIteratorOzaki::
+  // compute drives it (IntegrationIteratorOzaki.h:300-420):
+  //   beginIntegration(burnin);
+  //   for consecutive angle ranges [k0, k1) covering [0, Na):
+  //     sweepAngles(precision, burnin, k0, k1); integrateAngles(k0, k1);
+  //   endIntegration(burnin);
+  // equals compute() / burnin() bit for bit.  integrateAngles must see the
+  // ranges in increasing order (the angle sums are added in that order);
+  // a burn-in integrates nothing.
+  void beginIntegration(bool burnin);
+  void sweepAngles(int precision, bool burnin, int k0, int k1);
+  void integrateAngles(int k0, int k1);
+  void endIntegration(bool burnin);
+  // The accelerator step on caller-supplied fields: x (the iterate) and g
+  // (the map's result), each the concatenated field [Re D_0, Im D_0, ...,
+  // (lambda^2 Ax, lambda^2 Ay when the grain has an induced potential)] of
+  // numFields() N0^2 values; x receives the accelerated iterate.  The
+  // accelerator's history is the grain's own.
+  void acceleratorStepOn(double *x, const double *g);
+  int numFields() const;
   // individual stages
   // Reads the HIP-event time of the last sweep launch into the timing.
   void updateSweepTime();
@@ -104,6 +124,8 @@ public:
   Timing timing() const { return tim_; }
   // FP32-stepping iterations of the last run()
   int mixedIterations() const { return mixedIterations_; }
+  // A batch that refines grain by grain counts their FP32 iterations here.
+  void setMixedIterations(int n) { mixedIterations_ = n; }
   // Lane layout of the sweep for one precision: Ozaki energies per lane,
   // register budget (wavefronts per SIMD the kernel is compiled for) and
   // persistent wavefronts per compute unit.
@@ -289,6 +311,11 @@ private:
   std::vector<int> hDelta_, hDeltaSq_;
   int hCurrent_ = -1, hEnergy_ = -1, hCurl_ = -1, hBoundary_[2] = {-1, -1};
   bool pendingIntegrate_ = false;
+  // angle-by-angle integration: running sums per inside site, the next
+  // angle to integrate, and whether the current iteration runs that way
+  double *dGatherAcc_ = nullptr;
+  int nextAngle_ = 0;
+  bool rangeMode_ = false, rangeBurnin_ = false;
   // the free-energy gradient (sc_adjoint.hip)
   void queueExplicitFreeEnergy(std::vector<int> &hSq, int &hCurl);
   void *adjItems_ = nullptr;
@@ -306,6 +333,10 @@ private:
     int chunkAngles = 0;
     std::vector<int> chunkStart;
     void *dItems = nullptr;
+    // the same items ordered by angle (angle k's at [angleStart[k],
+    // angleStart[k + 1])), for sweeps of angle ranges
+    std::vector<int> angleStart;
+    void *dItemsByAngle = nullptr;
     void *dView = nullptr; // device copy of the grain view with this lane layout
     bool ready = false;
   };
@@ -359,9 +390,15 @@ public:
   Grain &grain(int i) { return *grains_.at(size_t(i)); }
   // One iteration of every grain that is still iterating.
   void compute(int precision);
+  // The same with a precision per grain (PREC_FP64 or PREC_MIXED, indexed
+  // by grain): one sweep launch per precision that has grains.
+  void computeEach(const std::vector<int> &precisionOf);
   void burnin(int precision);
   // runSimulation for every grain; a grain stops once converged with the
-  // minimum reached, or at the maximum.  Returns the number converged.
+  // minimum reached, or at the maximum.  With PREC_MIXED_REFINE every grain
+  // starts with FP32 stepping and switches to FP64 on its own (Grain::run's
+  // rule); it only counts as converged on an FP64 iteration.  Returns the
+  // number converged.
   int run(int nBurnin, int itMin, int itMax, int precision, std::vector<int> &converged);
   void setSweepShape(int precision, int el, int waves, int slotsPerCU);
   void sweepShape(int precision, int &el, int &waves, int &slotsPerCU) const;
@@ -381,6 +418,8 @@ private:
   };
   void prepare(int precision, const std::vector<int> &act);
   void sweep(int precision, bool burnin, const std::vector<int> &act);
+  // elapsed time of the last sweep launch of a precision
+  float sweepTime(int precision) const;
   // the main stream waits for every lane, every lane for the main stream
   void joinLanes();
   void forkLanes();
@@ -398,7 +437,9 @@ private:
   void *dScratch_ = nullptr;
   size_t scratchBytes_ = 0;
   int *dCounter_ = nullptr;
-  hipEvent_t ev0_ = nullptr, ev1_ = nullptr, evIt0_ = nullptr, evIt1_ = nullptr;
+  // start and end of the last sweep launch per precision; the iteration
+  hipEvent_t ev0_[2] = {nullptr, nullptr}, ev1_[2] = {nullptr, nullptr};
+  hipEvent_t evIt0_ = nullptr, evIt1_ = nullptr;
   double sweepMs_ = 0, iterMs_ = 0;
   long long trajSteps_ = 0;
 };

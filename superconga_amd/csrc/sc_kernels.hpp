@@ -480,6 +480,64 @@ __global__ void extendKernel(double *base, const int *target, const int *offset,
   }
 }
 
+// The contribution of frame k to the angle sums of reference site (x, y):
+// the bilinear back-rotation of the frame's four Green-function planes,
+// weighted by the basis functions and the angle weight (gatherKernel and
+// gatherRangeKernel share it so that both add the same terms in the same
+// order).
+__device__ __forceinline__ void gatherAngle(const GrainView &v, int k, int x, int y,
+                                            double (&dre)[4], double (&dim)[4], double &jx,
+                                            double &jy, double &fe) {
+  const int N0 = v.N0, N1 = v.N1, Na = v.Na, Nc = v.Nc;
+  const size_t n1 = size_t(N1) * N1;
+  const RotSample r = rotatedCell(x, y, N0, N1, double(v.bkCos[k]), double(v.bkSin[k]));
+  if (!r.valid) return;
+  // four values per site: taps (x, y), (x+1, y), (x, y+1), (x+1, y+1)
+  const double2 *Gk = reinterpret_cast<const double2 *>(v.G + size_t(k) * n1 * 4);
+  const size_t i00 = size_t(r.idx) * 2, i10 = i00 + 2, i01 = i00 + size_t(N1) * 2,
+               i11 = i01 + 2;
+  const double2 a00 = Gk[i00], b00 = Gk[i00 + 1];
+  const double2 a10 = Gk[i10], b10 = Gk[i10 + 1];
+  const double2 a01 = Gk[i01], b01 = Gk[i01 + 1];
+  const double2 a11 = Gk[i11], b11 = Gk[i11 + 1];
+  double gre, gim, gj, gf;
+  {
+#pragma clang fp contract(off)
+    const double wx0 = 1.0 - r.fx, wx1 = r.fx, wy0 = 1.0 - r.fy, wy1 = r.fy;
+    gre = wy0 * (wx0 * a00.x + wx1 * a10.x) + wy1 * (wx0 * a01.x + wx1 * a11.x);
+    gim = wy0 * (wx0 * a00.y + wx1 * a10.y) + wy1 * (wx0 * a01.y + wx1 * a11.y);
+    gj = wy0 * (wx0 * b00.x + wx1 * b10.x) + wy1 * (wx0 * b01.x + wx1 * b11.x);
+    gf = wy0 * (wx0 * b00.y + wx1 * b10.y) + wy1 * (wx0 * b01.y + wx1 * b11.y);
+  }
+  const double w = v.weight[k];
+  for (int c = 0; c < Nc && c < 4; ++c) {
+    const double b = v.basis[c * Na + k];
+    dre[c] += (gre * b) * w;
+    dim[c] += (gim * b) * w;
+  }
+  jx += (v.vFx[k] * w) * gj;
+  jy += (v.vFy[k] * w) * gj;
+  fe += w * gf;
+}
+
+// The gap equation's, the current's and the free-energy density's final
+// scaling of the angle sums of one site.
+__device__ __forceinline__ void gatherStore(const GrainView &v, int site,
+                                            const double (&dre)[4], const double (&dim)[4],
+                                            double jx, double jy, double fe,
+                                            const double *coupling, double currentFactor,
+                                            double *deltaOut, double *currentOut,
+                                            double *energyDensity) {
+  const size_t n0 = size_t(v.N0) * v.N0;
+  for (int c = 0; c < v.Nc && c < 4; ++c) {
+    deltaOut[size_t(c) * 2 * n0 + site] = dre[c] * coupling[c];
+    deltaOut[size_t(c) * 2 * n0 + n0 + site] = dim[c] * coupling[c];
+  }
+  currentOut[site] = jx * currentFactor;
+  currentOut[n0 + site] = jy * currentFactor;
+  energyDensity[site] = fe;
+}
+
 // Back-rotation of the Green functions of every frame to the reference grid
 // and the angle integration (ComputeOrderParameter.h:174-262,
 // ComputeCurrent.h:142-222, ComputeFreeEnergy.h:260-279), one thread per
@@ -492,49 +550,63 @@ __global__ void gatherKernel(GrainView v, const int *sites, int numSites,
   if (blk < 0) return;
   const int t = int(blk * blockDim.x + threadIdx.x);
   if (t >= numSites) return;
-  const int N0 = v.N0, N1 = v.N1, Na = v.Na, Nc = v.Nc;
+  const int N0 = v.N0;
   const int site = sites[t];
   const int x = site % N0, y = site / N0;
-  const size_t n0 = size_t(N0) * N0, n1 = size_t(N1) * N1;
   double dre[4] = {0, 0, 0, 0}, dim[4] = {0, 0, 0, 0};
   double jx = 0.0, jy = 0.0, fe = 0.0;
-  for (int k = 0; k < Na; ++k) {
-    const RotSample r = rotatedCell(x, y, N0, N1, double(v.bkCos[k]), double(v.bkSin[k]));
-    if (!r.valid) continue;
-    // four values per site: taps (x, y), (x+1, y), (x, y+1), (x+1, y+1)
-    const double2 *Gk = reinterpret_cast<const double2 *>(v.G + size_t(k) * n1 * 4);
-    const size_t i00 = size_t(r.idx) * 2, i10 = i00 + 2, i01 = i00 + size_t(N1) * 2,
-                 i11 = i01 + 2;
-    const double2 a00 = Gk[i00], b00 = Gk[i00 + 1];
-    const double2 a10 = Gk[i10], b10 = Gk[i10 + 1];
-    const double2 a01 = Gk[i01], b01 = Gk[i01 + 1];
-    const double2 a11 = Gk[i11], b11 = Gk[i11 + 1];
-    double gre, gim, gj, gf;
-    {
-#pragma clang fp contract(off)
-      const double wx0 = 1.0 - r.fx, wx1 = r.fx, wy0 = 1.0 - r.fy, wy1 = r.fy;
-      gre = wy0 * (wx0 * a00.x + wx1 * a10.x) + wy1 * (wx0 * a01.x + wx1 * a11.x);
-      gim = wy0 * (wx0 * a00.y + wx1 * a10.y) + wy1 * (wx0 * a01.y + wx1 * a11.y);
-      gj = wy0 * (wx0 * b00.x + wx1 * b10.x) + wy1 * (wx0 * b01.x + wx1 * b11.x);
-      gf = wy0 * (wx0 * b00.y + wx1 * b10.y) + wy1 * (wx0 * b01.y + wx1 * b11.y);
-    }
-    const double w = v.weight[k];
-    for (int c = 0; c < Nc && c < 4; ++c) {
-      const double b = v.basis[c * Na + k];
-      dre[c] += (gre * b) * w;
-      dim[c] += (gim * b) * w;
-    }
-    jx += (v.vFx[k] * w) * gj;
-    jy += (v.vFy[k] * w) * gj;
-    fe += w * gf;
-  }
+  for (int k = 0; k < v.Na; ++k) gatherAngle(v, k, x, y, dre, dim, jx, jy, fe);
+  gatherStore(v, site, dre, dim, jx, jy, fe, coupling, currentFactor, deltaOut, currentOut,
+              energyDensity);
+}
+
+// The angle sums of frames [k0, k1) added to running sums, one thread per
+// inside site: acc holds, per inside site t, Re and Im of every component's
+// sum, then jx, jy and the free-energy density (at acc[q * numSites + t]).
+// Called for consecutive ranges from k0 = 0 it adds the same terms in the
+// same order as gatherKernel.
+__global__ void gatherRangeKernel(GrainView v, const int *sites, int numSites, int k0, int k1,
+                                  double *acc) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= numSites) return;
+  const int N0 = v.N0, Nc = v.Nc;
+  const int site = sites[t];
+  const int x = site % N0, y = site / N0;
+  const size_t ns = size_t(numSites);
+  double dre[4] = {0, 0, 0, 0}, dim[4] = {0, 0, 0, 0};
   for (int c = 0; c < Nc && c < 4; ++c) {
-    deltaOut[size_t(c) * 2 * n0 + site] = dre[c] * coupling[c];
-    deltaOut[size_t(c) * 2 * n0 + n0 + site] = dim[c] * coupling[c];
+    dre[c] = acc[size_t(2 * c) * ns + t];
+    dim[c] = acc[size_t(2 * c + 1) * ns + t];
   }
-  currentOut[site] = jx * currentFactor;
-  currentOut[n0 + site] = jy * currentFactor;
-  energyDensity[site] = fe;
+  double jx = acc[size_t(2 * Nc) * ns + t], jy = acc[size_t(2 * Nc + 1) * ns + t];
+  double fe = acc[size_t(2 * Nc + 2) * ns + t];
+  for (int k = k0; k < k1; ++k) gatherAngle(v, k, x, y, dre, dim, jx, jy, fe);
+  for (int c = 0; c < Nc && c < 4; ++c) {
+    acc[size_t(2 * c) * ns + t] = dre[c];
+    acc[size_t(2 * c + 1) * ns + t] = dim[c];
+  }
+  acc[size_t(2 * Nc) * ns + t] = jx;
+  acc[size_t(2 * Nc + 1) * ns + t] = jy;
+  acc[size_t(2 * Nc + 2) * ns + t] = fe;
+}
+
+// The final scaling of the running sums of gatherRangeKernel.
+__global__ void gatherFinishKernel(GrainView v, const int *sites, int numSites,
+                                   const double *acc, const double *coupling,
+                                   double currentFactor, double *deltaOut, double *currentOut,
+                                   double *energyDensity) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= numSites) return;
+  const int Nc = v.Nc;
+  const size_t ns = size_t(numSites);
+  double dre[4] = {0, 0, 0, 0}, dim[4] = {0, 0, 0, 0};
+  for (int c = 0; c < Nc && c < 4; ++c) {
+    dre[c] = acc[size_t(2 * c) * ns + t];
+    dim[c] = acc[size_t(2 * c + 1) * ns + t];
+  }
+  gatherStore(v, sites[t], dre, dim, acc[size_t(2 * Nc) * ns + t],
+              acc[size_t(2 * Nc + 1) * ns + t], acc[size_t(2 * Nc + 2) * ns + t], coupling,
+              currentFactor, deltaOut, currentOut, energyDensity);
 }
 
 // Specular boundary condition as a precomputed stencil per entry crossing

@@ -77,6 +77,37 @@ def swave_disc(seed=1234, noise=0.01):
     }
 
 
+def dplus_is_square(seed=1234, noise=0.01, penetration_depth=3.0, s_critical_temperature=0.5):
+    """Two-component d + is square: side 30 xi0 at 20 points per xi0
+    (N0 = 606, N1 = 858), 128 angles, T = 0.3 Tc, cutoff 16 (12 Ozaki
+    poles), one flux quantum, penetration depth 3 xi0, d_{x2-y2} (Tc 1) and
+    s (Tc 0.5, a quarter turn out of phase).  The starting state puts the s
+    amplitude at 0.2 of the d amplitude (d_plus_is_initial)."""
+    return {
+        "physics": {"temperature": 0.3, "external_flux_quanta": 1.0,
+                    "penetration_depth": penetration_depth, "gauge": "symmetric",
+                    "charge_sign": -1,
+                    "order_parameter": {
+                        "dx2-y2": {"critical_temperature": 1.0,
+                                   "initial_noise_stddev": noise},
+                        "s": {"critical_temperature": s_critical_temperature,
+                              "initial_phase_shift": 0.25,
+                              "initial_noise_stddev": noise}}},
+        "geometry": [{"regular_polygon": {"num_edges": 4, "side_length": 30.0,
+                                          "rotation": 0.0, "center_x": 0.0,
+                                          "center_y": 0.0, "add": True}}],
+        "numerics": _numerics(128, 20.0, 16.0, 1e-8),
+        "accelerator": {"name": "anderson", "rank": 8, "step_size": 1.0},
+        "misc": {"seed": seed},
+    }
+
+
+def d_plus_is_initial(d_component, ratio=0.2):
+    """The d + is starting state from the d component: the s component is
+    i * ratio * d (components in the order dx2-y2, s)."""
+    return [d_component, 1j * ratio * d_component]
+
+
 def parameter_sweep(count=48, seed=2024, ppxi=20.0):
     """A parameter sweep of `count` grains drawn from a seeded generator:
     each a disc of radius U[5, 20] xi0 or a square of side U[10, 40] xi0
@@ -125,4 +156,4 @@ def postprocess_config(num_angles=720, num_energies=128, per_block=32):
 
 
 WORKLOADS = {"octagon_dwave": octagon_dwave, "vortex_disc": vortex_disc,
-             "swave_disc": swave_disc}
+             "swave_disc": swave_disc, "dplus_is_square": dplus_is_square}

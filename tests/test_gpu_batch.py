@@ -89,3 +89,31 @@ def test_batch_autotune_and_mixed(product):
     b.compute("mixed")
     for i in range(len(cfgs)):
         assert np.isfinite(b.grain(i).results()["free_energy"])
+
+
+def test_batch_mixed_refine_reaches_fp64_fixed_points(product):
+    # every grain of a mixed-refine batch steps in FP32 first, switches to
+    # FP64 on its own and stops at its FP64 fixed point: the same state, to
+    # the criterion's accuracy, as the grain run alone in FP64
+    cfgs = batch_configs()
+    for c in cfgs:
+        c["numerics"]["convergence_criterion"] = 1e-8
+        c["numerics"]["norm"] = "l2"
+    b = product.Batch(cfgs)
+    starts = [b.grain(i).order_parameter(0) for i in range(len(cfgs))]
+    conv = b.run(0, 0, 3000, "mixed_refine")
+    assert all(conv), conv
+    for i, cfg in enumerate(cfgs):
+        s = product.Simulation(cfg)
+        for c, d in enumerate(starts[i]):
+            s.set_order_parameter(c, d)
+        assert s.run(0, 0, 3000, "fp64")
+        g = b.grain(i).results()
+        r = s.results()
+        err = max(rel(b.grain(i).order_parameter(0), s.order_parameter(0)),
+                  rel(b.grain(i).current(), s.current()),
+                  abs(g["free_energy"] - r["free_energy"]) / abs(r["free_energy"]))
+        print("grain %d: %d iterations (%d FP32), alone in FP64 %d; difference %.2e"
+              % (i, g["iterations"], g["mixed_iterations"], r["iterations"], err), flush=True)
+        assert g["mixed_iterations"] > 0 and g["converged"] == 1 and g["residual"] < 1e-8
+        assert err < 1e-8

@@ -53,3 +53,62 @@ def test_simulate_writes_reference_files_and_warm_starts(product, oracle, tmp_pa
     assert rel(np.where(dom, got, 0), np.where(dom, want, 0)) < 1e-8
     s.compute("fp64")
     assert np.isfinite(s.results()["free_energy_per_area"])
+
+
+def test_simulation_binary_matches_simulate(product, tmp_path):
+    # the drop-in binary (`simulation_amd <config.json>`, the reference's
+    # src/simulation.cu called by frontend/common/cpp.py) against the
+    # in-process io.simulate on the same config: the same files, the CSV
+    # grids byte for byte, the scalar results to the last bits of their
+    # different summation orders
+    import subprocess
+    from superconga_amd import io as scio
+    from test_adaptors import simulation_binary
+    exe = simulation_binary(product)
+    runs = {}
+    for which in ("binary", "simulate"):
+        cfg = example_config("swave_disc_meissner")
+        cfg["numerics"]["num_iterations_burnin"] = 2
+        cfg["numerics"]["num_iterations_max"] = 5
+        out = tmp_path / which
+        cfg["misc"] = {"save_path": str(out), "save_frequency": 2, "load_path": "",
+                       "data_format": "csv", "verbose": False}
+        if which == "binary":
+            path = tmp_path / "config.json"
+            path.write_text(json.dumps(cfg, indent="\t"))
+            r = subprocess.run([exe, str(path)], capture_output=True, text=True, timeout=300)
+            print(r.stdout, r.stderr, flush=True)
+            assert r.returncode == 0, r.stderr
+        else:
+            scio.simulate(cfg)
+        runs[which] = out
+    names = sorted(os.listdir(runs["simulate"]))
+    assert names == sorted(os.listdir(runs["binary"]))
+    for name in names:
+        a = (runs["binary"] / name).read_text()
+        b = (runs["simulate"] / name).read_text()
+        if name.endswith(".csv"):
+            assert a == b, name
+        else:
+            ja, jb = json.loads(a), json.loads(b)
+            assert sorted(ja) == sorted(jb), name
+            for k in jb:
+                if isinstance(jb[k], (int, float)):
+                    assert abs(ja[k] - jb[k]) <= 1e-12 * max(abs(jb[k]), 1e-300), (name, k)
+                else:
+                    assert ja[k] == jb[k], (name, k)
+    conv = (runs["binary"] / "simulation_convergence.csv").read_text().splitlines()
+    assert len(conv) == 1 + 2 + 5  # header, burn-in rows, iterations
+
+
+def test_simulation_binary_refuses_hdf5_output(product, tmp_path):
+    import subprocess
+    from test_adaptors import simulation_binary
+    cfg = example_config("swave_disc_meissner")
+    cfg["misc"]["save_path"] = str(tmp_path / "out")
+    path = tmp_path / "config.json"
+    path.write_text(json.dumps(cfg))
+    r = subprocess.run([simulation_binary(product), str(path)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode != 0 and "HDF5" in r.stderr
+    assert not (tmp_path / "out").exists()

@@ -349,3 +349,30 @@ def test_octagon_full_size_invariants(product):
     assert sym < 1e-4
     assert np.isfinite(r["free_energy_per_area"]) and r["free_energy_per_area"] < 0
     assert r["last_sweep_ms"] > 0
+
+
+def test_mixed_refine_reaches_the_fp64_fixed_point(product):
+    # SC_PRECISION_MIXED_REFINE: FP32 stepping until the residual reaches the
+    # FP32 floor, then FP64 until the criterion; the converged state must be
+    # the FP64 run's fixed point to the criterion's accuracy
+    from superconga_amd import workloads
+    cfg = workloads.swave_disc()
+    cfg["numerics"]["convergence_criterion"] = 1e-8
+    ref = product.Simulation(cfg)
+    d0 = ref.order_parameter(0)[0]
+    assert ref.run(0, 0, 3000, "fp64")
+    mix = product.Simulation(cfg)
+    mix.set_order_parameter(0, d0)
+    assert mix.run(0, 0, 3000, "mixed_refine")
+    r64, rm = ref.results(), mix.results()
+    print("fp64: %d iterations; mixed refine: %d iterations, %d of them FP32"
+          % (r64["iterations"], rm["iterations"], rm["mixed_iterations"]), flush=True)
+    assert rm["mixed_iterations"] > 0
+    assert rm["converged"] == 1 and rm["residual"] < 1e-8
+    err = {
+        "delta": rel(mix.order_parameter(0), ref.order_parameter(0)),
+        "current": rel(mix.current(), ref.current()),
+        "free_energy": srel(rm["free_energy"], r64["free_energy"]),
+    }
+    print("mixed refine vs FP64 fixed point:", {k: "%.2e" % v for k, v in err.items()})
+    assert all(v < 1e-8 for v in err.values()), err
