@@ -318,6 +318,35 @@ def batch_bench(sa, workloads, args):
     }
 
 
+def batch_convergence(sa, workloads, args):
+    """The parameter sweep solved to the 1e-8 criterion as one batch: every
+    grain iterates until its own residual passes (sc_batch_run), with FP32
+    stepping switching to FP64 per grain (mixed refinement) and, from the
+    same seeded starts, in plain FP64.  Reports the wall time of each run and
+    how many grains converged within the cap."""
+    cfgs = workloads.parameter_sweep(args.batch_count)
+    res = {"workload": "parameter_sweep", "grains": len(cfgs), "criterion": 1e-8,
+           "cap_iterations": args.batch_converge_cap}
+    for prec in ("mixed_refine", "fp64"):
+        b = sa.Batch(cfgs)
+        b.autotune("fp64")
+        b.autotune("mixed")
+        t = time.perf_counter()
+        conv = b.run(0, 0, args.batch_converge_cap, prec)
+        wall = time.perf_counter() - t
+        its = [b.grain(i).results()["iterations"] for i in range(len(b))]
+        fp32 = [b.grain(i).results()["mixed_iterations"] for i in range(len(b))]
+        res[prec] = {"wall_s": round(wall, 3), "converged_grains": int(sum(conv)),
+                     "iterations_max": int(max(its)), "iterations_total": int(sum(its)),
+                     "fp32_iterations_total": int(sum(fp32))}
+        b.close()
+        log("batch converge %s: %s" % (prec, res[prec]))
+    if res["fp64"]["converged_grains"] == len(cfgs) and \
+            res["mixed_refine"]["converged_grains"] == len(cfgs):
+        res["speedup_mixed_refine"] = res["fp64"]["wall_s"] / res["mixed_refine"]["wall_s"]
+    return res
+
+
 def rel(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
 
@@ -439,6 +468,9 @@ def main():
     ap.add_argument("--octagon-converge-cap", type=int, default=300,
                     help="iteration cap of each time-to-1e-8 method on the octagon")
     ap.add_argument("--no-octagon-converge", action="store_true")
+    ap.add_argument("--batch-converge-cap", type=int, default=400,
+                    help="iteration cap of the parameter sweep's batched solve to 1e-8")
+    ap.add_argument("--no-batch-converge", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -454,7 +486,10 @@ def main():
         copy_probe(args.copy_probe_mib)
 
     if args.workload == "parameter_sweep":
-        print(json.dumps(batch_bench(sa, workloads, args)), flush=True)
+        out = batch_bench(sa, workloads, args)
+        if not args.no_converge and not args.no_batch_converge:
+            out["time_to_1e-8"] = batch_convergence(sa, workloads, args)
+        print(json.dumps(out), flush=True)
         return
     if args.ldos_only:
         print(json.dumps(ldos_bench(sa, workloads, args)), flush=True)
@@ -552,6 +587,10 @@ def main():
     if not args.no_converge and not args.no_octagon_converge and \
             args.workload == "octagon_dwave":
         out["time_to_1e-8_octagon"] = octagon_convergence(sa, workloads, args)
+
+    if not args.no_converge and not args.no_batch_converge and \
+            args.workload == "octagon_dwave":
+        out["time_to_1e-8_parameter_sweep"] = batch_convergence(sa, workloads, args)
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds, args.cpu_threads)

@@ -79,7 +79,7 @@ def swave_disc(seed=1234, noise=0.01):
 
 def dplus_is_square(seed=1234, noise=0.01, penetration_depth=3.0, s_critical_temperature=0.5):
     """Two-component d + is square: side 30 xi0 at 20 points per xi0
-    (N0 = 606, N1 = 858), 128 angles, T = 0.3 Tc, cutoff 16 (12 Ozaki
+    (N0 = 607, N1 = 859), 128 angles, T = 0.3 Tc, cutoff 16 (12 Ozaki
     poles), one flux quantum, penetration depth 3 xi0, d_{x2-y2} (Tc 1) and
     s (Tc 0.5, a quarter turn out of phase).  The starting state puts the s
     amplitude at 0.2 of the d amplitude (d_plus_is_initial)."""

@@ -94,8 +94,13 @@ def test_batch_autotune_and_mixed(product):
 def test_batch_mixed_refine_reaches_fp64_fixed_points(product):
     # every grain of a mixed-refine batch steps in FP32 first, switches to
     # FP64 on its own and stops at its FP64 fixed point: the same state, to
-    # the criterion's accuracy, as the grain run alone in FP64
-    cfgs = batch_configs()
+    # the criterion's accuracy, as the grain run alone in FP64.  The third
+    # grain of batch_configs() is left out: its s component sits at its own
+    # critical temperature (T = 0.3), a marginal mode that no accelerator
+    # brings below ~1e-5 in 1500 iterations on the oracle either; the disc
+    # of the accelerator known answer (CongAcc, 72 iterations to 1e-8 on the
+    # oracle) takes its place.
+    cfgs = batch_configs()[:2] + [cases.accelerator_case("congacc")]
     for c in cfgs:
         c["numerics"]["convergence_criterion"] = 1e-8
         c["numerics"]["norm"] = "l2"

@@ -95,10 +95,22 @@ def test_simulation_binary_matches_simulate(product, tmp_path):
             for k in jb:
                 if isinstance(jb[k], (int, float)):
                     assert abs(ja[k] - jb[k]) <= 1e-12 * max(abs(jb[k]), 1e-300), (name, k)
+                elif k == "misc":
+                    # the written config keeps each run's own save_path
+                    ma, mb = dict(ja[k]), dict(jb[k])
+                    ma.pop("save_path", None)
+                    mb.pop("save_path", None)
+                    assert ma == mb, (name, k)
                 else:
                     assert ja[k] == jb[k], (name, k)
     conv = (runs["binary"] / "simulation_convergence.csv").read_text().splitlines()
-    assert len(conv) == 1 + 2 + 5  # header, burn-in rows, iterations
+    # header, then one row per iteration: with save_frequency > 0 the
+    # reference counts the burn-in iterations towards num_iterations_max
+    # (Context::burnIn increments the iteration count, Context.h:388, and
+    # runSimulation compares getNumIterations with the maximum,
+    # run_simulation.h:144-156), so 2 burn-in rows and 3 more iterations
+    assert len(conv) == 1 + 5
+    assert [int(r.split(",")[0]) for r in conv[1:]] == [1, 2, 3, 4, 5]
 
 
 def test_simulation_binary_refuses_hdf5_output(product, tmp_path):

@@ -114,7 +114,7 @@ def test_dplus_is_square_one_iteration(product, oracle):
     cfg = workloads.dplus_is_square()
     s, o, _ = start(product, oracle, cfg,
                     initial=lambda d: workloads.d_plus_is_initial(d[0]))
-    expect_dims(s, o, 606, 858, 12, 128)
+    expect_dims(s, o, 607, 859, 12, 128)
     print("d+is square iteration 0", flush=True)
     s.compute("fp64")
     o.compute()

@@ -369,9 +369,18 @@ def test_mixed_refine_reaches_the_fp64_fixed_point(product):
           % (r64["iterations"], rm["iterations"], rm["mixed_iterations"]), flush=True)
     assert rm["mixed_iterations"] > 0
     assert rm["converged"] == 1 and rm["residual"] < 1e-8
+    # The zero-field disc carries only the weak currents that the seeded
+    # noise of its initial state leaves (far below the natural unit j0), so
+    # the current's difference is measured against the order parameter's
+    # scale (both fields are in the reference's natural units, O(1) in a
+    # superconductor) rather than against that noise.
+    jref = ref.current()
+    dscale = float(np.max(np.abs(ref.order_parameter(0))))
+    print("max |j| %.3e, max |Delta| %.3e" % (np.max(np.abs(jref)), dscale))
     err = {
         "delta": rel(mix.order_parameter(0), ref.order_parameter(0)),
-        "current": rel(mix.current(), ref.current()),
+        "current": float(np.max(np.abs(mix.current() - jref))
+                         / max(float(np.max(np.abs(jref))), dscale)),
         "free_energy": srel(rm["free_energy"], r64["free_energy"]),
     }
     print("mixed refine vs FP64 fixed point:", {k: "%.2e" % v for k, v in err.items()})
