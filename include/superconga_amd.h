@@ -178,7 +178,7 @@ typedef struct {
  * finite, and the boundary in-values of both coherence functions; G is one
  * FP64 sweep + angle integration + gap equation + boundary condition +
  * Poisson solve.  Each Newton step solves J d = -(G(u) - u) by GMRES
- * restarted every krylov_dim (1..31) directions, at most 4 cycles, with
+ * restarted every krylov_dim (1..63) directions, at most 4 cycles, with
  * finite-difference products, in the inner product weighted per block by
  * 1 / |u_block|^2,
  * followed by a backtracking line search; the forcing term follows
@@ -187,6 +187,13 @@ typedef struct {
  * residuals for it. */
 int sc_newton_krylov(sc_context *ctx, int max_newton, int krylov_dim, double tol,
                      sc_newton_report *report);
+/* sc_newton_krylov with the precision of the Jacobian-vector products:
+ * SC_PRECISION_FP64 (as sc_newton_krylov) or SC_PRECISION_MIXED (both ends
+ * of each finite difference on the map with FP32 stepping and FP64 sums, at
+ * a step of 3e-4 (1 + |u|) in the weighted norm; the Newton residuals stay
+ * FP64, so the solution is the FP64 fixed point).  krylov_dim 1..63. */
+int sc_newton_krylov_ex(sc_context *ctx, int max_newton, int krylov_dim, double tol,
+                        int jvp_precision, sc_newton_report *report);
 
 /* The free energy one iteration reports from the current state (the
  * reference's ComputeFreeEnergy.h:281-414 on the Green functions of one
@@ -256,6 +263,16 @@ int sc_accelerator_step(sc_context *ctx, double *x, const double *g);
  * (5,2) (6,2) (7,1) (7,2) (7,4) (8,1) (8,2) (8,4) (14,1) (14,2) (16,1)
  * (16,2); for spectroscopy contexts (2,4) (4,2) (4,3) (4,4) (8,1) (8,2)
  * (8,4) (16,1). */
+/* The device work after each iteration's sweep (back-rotation and angle
+ * integration, gap equation, current, free energy and residual reductions,
+ * boundary condition, Poisson solve) is captured into a hipGraph on its
+ * second run and replayed after that, while the reductions it was captured
+ * with keep their offsets (a change re-captures it).  The graph holds only
+ * kernel and copy nodes.  Results are bit for bit those of the eager
+ * launches.  On by default; enable = 0 launches every stage eagerly.  No
+ * counterpart in the reference, which launches each stage per iteration
+ * (Context.h:391-447). */
+int sc_set_graphs(sc_context *ctx, int enable);
 int sc_set_sweep_shape(sc_context *ctx, int precision, int energies_per_lane,
                        int waves_per_simd, int slots_per_cu);
 int sc_get_sweep_shape(sc_context *ctx, int precision, int *energies_per_lane,
@@ -335,6 +352,8 @@ int sc_batch_burnin(sc_batch *batch, int precision);
  * an iteration sweeps the grains of each precision in one launch each. */
 int sc_batch_run(sc_batch *batch, int num_burnin, int num_min, int num_max,
                  int precision, int *converged);
+/* sc_set_graphs for every grain of the batch. */
+int sc_batch_set_graphs(sc_batch *batch, int enable);
 int sc_batch_set_sweep_shape(sc_batch *batch, int precision, int energies_per_lane,
                              int waves_per_simd, int slots_per_cu);
 int sc_batch_get_sweep_shape(sc_batch *batch, int precision, int *energies_per_lane,

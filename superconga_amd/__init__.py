@@ -148,6 +148,10 @@ def lib():
                                   ctypes.POINTER(Dims), ctypes.POINTER(ctypes.c_byte)]
         L.sc_newton_krylov.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                        ctypes.POINTER(NewtonReport)]
+        L.sc_newton_krylov_ex.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                          ctypes.c_int, ctypes.POINTER(NewtonReport)]
+        L.sc_set_graphs.argtypes = [vp, ctypes.c_int]
+        L.sc_batch_set_graphs.argtypes = [vp, ctypes.c_int]
         L.sc_free_energy_gradient.argtypes = [vp, _DP, _DP]
         L.sc_ldos_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(vp)]
         L.sc_ldos_compute.argtypes = [vp, ctypes.c_int, ctypes.c_int]
@@ -300,15 +304,17 @@ class Simulation:
         _check(lib().sc_get_results(self._h, ctypes.byref(r)))
         return {n: getattr(r, n) for n, _ in Results._fields_}
 
-    def newton_krylov(self, max_newton=20, krylov_dim=20, tol=1e-9):
+    def newton_krylov(self, max_newton=20, krylov_dim=20, tol=1e-9, jvp_precision="fp64"):
         """Jacobian-free Newton-Krylov on the fixed point of one plain FP64
         iteration; returns the report as a dict with the weighted residual
-        history.  A following compute() reports the usual residuals."""
+        history.  jvp_precision "mixed" takes the Jacobian-vector products
+        on the map with FP32 stepping (the residuals stay FP64).  A
+        following compute() reports the usual residuals."""
         hist = (ctypes.c_double * (max_newton + 1))()
         r = NewtonReport(history=ctypes.cast(hist, ctypes.POINTER(ctypes.c_double)),
                          history_capacity=max_newton + 1)
-        _check(lib().sc_newton_krylov(self._h, max_newton, krylov_dim, ctypes.c_double(tol),
-                                      ctypes.byref(r)))
+        _check(lib().sc_newton_krylov_ex(self._h, max_newton, krylov_dim, ctypes.c_double(tol),
+                                         _precision(jvp_precision), ctypes.byref(r)))
         return {"newton_iterations": r.newton_iterations,
                 "krylov_iterations": r.krylov_iterations,
                 "map_evaluations": r.map_evaluations, "converged": bool(r.converged),
@@ -367,6 +373,11 @@ class Simulation:
                         precision="fp64"):
         _check(lib().sc_set_sweep_shape(self._h, _precision(precision), energies_per_lane,
                                         waves_per_simd, slots_per_cu))
+
+    def set_graphs(self, enable=True):
+        """Replay the stages after the sweep from a captured hipGraph (the
+        default) or launch them eagerly."""
+        _check(lib().sc_set_graphs(self._h, 1 if enable else 0))
 
     def sweep_shape(self, precision="fp64"):
         e, w, s = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
@@ -491,6 +502,11 @@ class Batch:
         _check(lib().sc_batch_set_sweep_shape(self._h, _precision(precision),
                                               energies_per_lane, waves_per_simd,
                                               slots_per_cu))
+
+    def set_graphs(self, enable=True):
+        """Captured hipGraphs (the default) or eager launches for the stages
+        after the sweep of every grain."""
+        _check(lib().sc_batch_set_graphs(self._h, 1 if enable else 0))
 
     def sweep_shape(self, precision="fp64"):
         e, w, s = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

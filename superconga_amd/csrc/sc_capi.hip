@@ -392,6 +392,13 @@ int sc_set_sweep_shape(sc_context *ctx, int precision, int el, int waves, int sl
     return SC_OK;
   });
 }
+int sc_set_graphs(sc_context *ctx, int enable) {
+  SC_CHECK_CTX(ctx);
+  return guarded([&] {
+    ctx->grain->setGraphs(enable != 0);
+    return SC_OK;
+  });
+}
 int sc_get_sweep_shape(sc_context *ctx, int precision, int *el, int *waves, int *slots) {
   SC_CHECK_CTX(ctx);
   SC_CHECK_PTR(el);
@@ -415,16 +422,25 @@ int sc_autotune_sweep(sc_context *ctx, int precision) {
 
 int sc_newton_krylov(sc_context *ctx, int maxNewton, int krylovDim, double tol,
                      sc_newton_report *report) {
+  return sc_newton_krylov_ex(ctx, maxNewton, krylovDim, tol, SC_PRECISION_FP64, report);
+}
+
+int sc_newton_krylov_ex(sc_context *ctx, int maxNewton, int krylovDim, double tol,
+                        int jvpPrecision, sc_newton_report *report) {
   SC_CHECK_CTX(ctx);
   SC_CHECK_SIM(ctx);
   return guarded([&] {
     if (maxNewton < 0) throw std::invalid_argument("max_newton must be >= 0");
-    if (krylovDim < 1 || krylovDim > 31)
-      throw std::invalid_argument("krylov_dim must be in [1, 31]");
+    if (krylovDim < 1 || krylovDim > 63)
+      throw std::invalid_argument("krylov_dim must be in [1, 63]");
+    if (jvpPrecision != SC_PRECISION_FP64 && jvpPrecision != SC_PRECISION_MIXED)
+      throw std::invalid_argument("jvp_precision must be SC_PRECISION_FP64 or SC_PRECISION_MIXED");
     if (!(tol > 0.0)) throw std::invalid_argument("tol must be positive");
     if (report && report->history_capacity > 0 && !report->history)
       throw std::invalid_argument("history is null with a positive capacity");
-    const scamd::NewtonReport r = ctx->grain->newtonKrylov(maxNewton, krylovDim, tol);
+    const scamd::NewtonReport r = ctx->grain->newtonKrylov(
+        maxNewton, krylovDim, tol,
+        jvpPrecision == SC_PRECISION_MIXED ? scamd::PREC_MIXED : scamd::PREC_FP64);
     if (report) {
       report->newton_iterations = r.newtonIterations;
       report->krylov_iterations = r.krylovIterations;
@@ -704,6 +720,13 @@ int sc_batch_set_sweep_shape(sc_batch *b, int precision, int el, int waves, int 
   return guarded([&] {
     checkPrecision(precision);
     b->batch->setSweepShape(precision, el, waves, slots);
+    return SC_OK;
+  });
+}
+int sc_batch_set_graphs(sc_batch *b, int enable) {
+  SC_CHECK_PTR(b);
+  return guarded([&] {
+    for (int i = 0; i < b->batch->size(); ++i) b->batch->grain(i).setGraphs(enable != 0);
     return SC_OK;
   });
 }

@@ -110,7 +110,10 @@ Grain::~Grain() {
   devFree(dAdjDbar_);
   devFree(dAdjExt_);
   devFree(dAdjGrad_);
+  dropEndGraph();
   fft_.reset();
+  screen_.reset();
+  devFree(dZero2_);
   if (dScratch_) (void)hipFree(dScratch_);
   for (SweepPlan &pl : plan_) {
     if (pl.dItems) (void)hipFree(pl.dItems);
@@ -242,6 +245,13 @@ void Grain::upload() {
   dWork_ = devAlloc<double>(nf);
   dWork2_ = devAlloc<double>(nf);
   if (s.lambda > 0.0) fft_ = std::make_unique<PoissonFFT>(s.N0, s.h, stream_, s.cfg.vpError);
+  if (s.lambda > 0.0 && s.cfg.computeCurrent && s.cfg.vpScreening > 0.0) {
+    // j = -rho_s A in the bulk (A unscaled), and the iterate holds lambda^2 A
+    screenK_ = s.cfg.vpScreening / (s.lambda * s.lambda);
+    screen_ = std::make_unique<PoissonFFT>(s.N0, s.h, stream_, PoissonFFT::Screened{screenK_});
+    dZero2_ = devAlloc<double>(2);
+    SC_HIP(hipMemsetAsync(dZero2_, 0, 2 * sizeof(double), stream_));
+  }
   dCounter_ = devAlloc<int>(1);
 }
 
@@ -593,6 +603,7 @@ double residualFromStats(const double *st, long long numElements, int norm) {
 // the first.
 double *Grain::reserveReduction(int kind, long long blocks, long long n, int &handle) {
   const size_t need = size_t(blocks) * (kind == 0 ? 1 : 6);
+  redReserved_ += need;
   if (readbackQueued_) flushReductions(); // the queued transfer would miss this one
   if (redUsed_ + need > redCap_) {
     flushReductions(); // keeps the results of everything queued so far
@@ -600,6 +611,7 @@ double *Grain::reserveReduction(int kind, long long blocks, long long n, int &ha
       devFree(dRed_);
       redCap_ = std::max(need, 2 * redCap_);
       dRed_ = devAlloc<double>(redCap_);
+      dropEndGraph(); // its kernels write into the old buffer
     }
   }
   Pending p;
@@ -1051,7 +1063,6 @@ void Grain::afterSweep(bool burnin) {
     } else {
       backRotate();
     }
-    SC_HIP(hipEventRecord(ev2_, stream_));
     // ComputeOrderParameter::computePropertyFromGreens (:212-262)
     hDelta_.assign(size_t(s.Nc), -1);
     for (int c = 0; c < s.Nc; ++c)
@@ -1566,8 +1577,7 @@ void Grain::finishBeginCompute() {
   extendDelta();
 }
 
-void Grain::queueEndCompute() {
-  // the rest of Context::compute after the sweeps: device work only
+void Grain::queueEndStages() {
   const Setup &s = *s_;
   const size_t n0 = size_t(s.N0) * s.N0;
   afterSweep(false);
@@ -1575,8 +1585,123 @@ void Grain::queueEndCompute() {
   if (s.cfg.computeCurrent && s.lambda > 0.0) {
     solveVectorPotential(dCurrent_, dANew_);
     hA_ = queueStats(dA_, dANew_, static_cast<long long>(2 * n0));
+    if (screen_) screenVectorPotential();
+  }
+}
+
+void Grain::dropEndGraph() {
+  if (endExec_) (void)hipGraphExecDestroy(endExec_);
+  endExec_ = nullptr;
+}
+
+void Grain::setGraphs(bool on) {
+  graphsOn_ = on;
+  if (!on) dropEndGraph();
+}
+
+void Grain::queueEndCompute() {
+  // The rest of Context::compute after the sweeps: device work only, whose
+  // launches never change between iterations (fixed buffers, fixed
+  // reduction offsets once the accelerator's read-back has reset them).
+  // The first run is eager and sizes the reductions; the second is captured
+  // into a hipGraph (no memset nodes: every buffer it clears is cleared by
+  // a kernel); later runs replay the graph and redo the host bookkeeping of
+  // the reservations it was captured with.
+  const bool keyOk = readbackQueued_ == false && redOut_.size() == endCap_.redOutBefore &&
+                     redUsed_ == endCap_.redUsedBefore && rangeMode_ == endCap_.rangeMode;
+  if (graphsOn_ && endExec_ && keyOk) {
+    SC_HIP(hipGraphLaunch(endExec_, stream_));
+    for (const Pending &p : endCap_.pend) pend_.push_back(p);
+    for (long long n : endCap_.redN) {
+      redOut_.push_back({0, 0, 0, 0, 0, 0});
+      redN_.push_back(n);
+    }
+    redUsed_ = endCap_.redUsedAfter;
+    pendingIntegrate_ = true;
+    hDelta_ = endCap_.hDelta;
+    hDeltaSq_ = endCap_.hDeltaSq;
+    hCurrent_ = endCap_.hCurrent;
+    hEnergy_ = endCap_.hEnergy;
+    hCurl_ = endCap_.hCurl;
+    hBoundary_[0] = endCap_.hBoundary[0];
+    hBoundary_[1] = endCap_.hBoundary[1];
+    hA_ = endCap_.hA;
+  } else {
+    // capture only when no reservation can flush or grow dRed_ (both would
+    // synchronise inside the capture): the sizes of the eager run fit
+    const bool capture = graphsOn_ && endSeen_ && !readbackQueued_ && !s_->keldysh &&
+                         redUsed_ + endCap_.need <= redCap_ &&
+                         rangeMode_ == endCap_.rangeMode;
+    const size_t redOutBefore = redOut_.size(), redUsedBefore = redUsed_,
+                 pendBefore = pend_.size(), reservedBefore = redReserved_;
+    hipGraph_t graph = nullptr;
+    // the bookkeeping below is rewritten: a graph of other offsets must go
+    dropEndGraph();
+    if (capture) SC_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    try {
+      queueEndStages();
+    } catch (...) {
+      if (capture) {
+        (void)hipStreamEndCapture(stream_, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+      }
+      throw;
+    }
+    endCap_.redOutBefore = redOutBefore;
+    endCap_.redUsedBefore = redUsedBefore;
+    endCap_.redUsedAfter = redUsed_;
+    endCap_.need = redReserved_ - reservedBefore;
+    endCap_.rangeMode = rangeMode_;
+    endCap_.pend.assign(pend_.begin() + static_cast<long>(pendBefore), pend_.end());
+    endCap_.redN.assign(redN_.begin() + static_cast<long>(redOutBefore), redN_.end());
+    endCap_.hDelta = hDelta_;
+    endCap_.hDeltaSq = hDeltaSq_;
+    endCap_.hCurrent = hCurrent_;
+    endCap_.hEnergy = hEnergy_;
+    endCap_.hCurl = hCurl_;
+    endCap_.hBoundary[0] = hBoundary_[0];
+    endCap_.hBoundary[1] = hBoundary_[1];
+    endCap_.hA = hA_;
+    endSeen_ = true;
+    if (capture) {
+      SC_HIP(hipStreamEndCapture(stream_, &graph));
+      const hipError_t e = hipGraphInstantiate(&endExec_, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (e != hipSuccess) {
+        endExec_ = nullptr;
+        SC_HIP(e);
+      }
+      // the captured work has not run yet
+      SC_HIP(hipGraphLaunch(endExec_, stream_));
+    }
   }
   SC_HIP(hipEventRecord(evIterStop_, stream_));
+}
+
+void Grain::screenVectorPotential() {
+  // The plain update B <- L^-1 j(B) (B = lambda^2 A, L^-1 the Poisson
+  // solve) has the linear response j = j0 - K B inside the grain (K =
+  // superfluid density / lambda^2), so its Jacobian is -L^-1 K: for every
+  // wavelength longer than 1 / sqrt(K) the gain exceeds one and the plain
+  // iteration oscillates.  The Newton step of that linear model is
+  //   dB = (I + L^-1 K)^-1 R = (-L + K)^-1 (-L) R = R - K (-L + K)^-1 R
+  // for the residual R = L^-1 j(B) - B, so the accelerator is handed
+  //   B + R - K (-L + K)^-1 R
+  // in place of L^-1 j(B).  Both have the same fixed points; the residual
+  // statistics were taken on R before this.  K is uniform over the grid (the
+  // grain's response is that of its interior), a preconditioner rather than
+  // the exact Jacobian.
+  const Setup &s = *s_;
+  const long long n2 = 2LL * s.N0 * s.N0;
+  double *R = dWork_, *S = dWork2_;
+  diffKernel<<<blocksFor(n2), 256, 0, stream_>>>(R, dANew_, dA_, n2);
+  SC_HIP(hipGetLastError());
+  screen_->solve(R, dZero2_, S);
+  CombineTerms t{};
+  t.v[0] = S;
+  t.c[0] = -screenK_;
+  combineTermsKernel<<<blocksFor(n2), 256, 0, stream_>>>(dANew_, dANew_, 1.0, t, 1, n2);
+  SC_HIP(hipGetLastError());
 }
 
 void Grain::endCompute() {

@@ -113,12 +113,23 @@ public:
   // accelerator's history is the grain's own.
   void acceleratorStepOn(double *x, const double *g);
   int numFields() const;
+  // Replay the device work after the sweep (back-rotation and angle
+  // integration, gap equation, current, boundary condition, Poisson solve
+  // and their reductions) from a hipGraph captured on its second use, while
+  // the reductions it was captured with keep the same offsets.  On by
+  // default; off runs every launch eagerly.
+  void setGraphs(bool on);
+  bool graphs() const { return graphsOn_; }
   // individual stages
   // Reads the HIP-event time of the last sweep launch into the timing.
   void updateSweepTime();
   void sweep(int precision, bool burnin);
   void boundaryCondition();
   void solveVectorPotential(const double *dCurrent, double *dOut);
+  // G~ for the vector-potential block: A_new <- A + P (A_new - A) with
+  // P = I - K (-L + K)^-1 (the Meissner response preconditioner), when the
+  // config enables it
+  void screenVectorPotential();
 
   Results results() const { return res_; }
   Timing timing() const { return tim_; }
@@ -139,7 +150,8 @@ public:
   // (state: order parameter, scaled vector potential, boundary in-values)
   // to a block-weighted residual below tol; leaves the solution as the
   // iterate and restarts the accelerators.
-  NewtonReport newtonKrylov(int maxNewton, int krylovDim, double tol);
+  NewtonReport newtonKrylov(int maxNewton, int krylovDim, double tol,
+                            int jvpPrecision = PREC_FP64);
   // The free energy one iteration reports from the current state and its
   // gradient with respect to the order parameter (vector potential and
   // boundary in-values held fixed) by a discrete adjoint: grad holds
@@ -263,6 +275,9 @@ private:
   double *dFields_ = nullptr; // site fields of every frame, [3][Na][N1^2]
   double *dPartials_ = nullptr, *dWork_ = nullptr, *dWork2_ = nullptr;
   std::unique_ptr<PoissonFFT> fft_;
+  std::unique_ptr<PoissonFFT> screen_; // (-L + K)^-1 of the preconditioner
+  double screenK_ = 0.0;               // K = superfluid density / lambda^2
+  double *dZero2_ = nullptr;           // two zeros: the screened solve's means
   // Keldysh state
   double *dEnergyRe_ = nullptr;
   double *dLdosAll_ = nullptr, *dLdosAcc_ = nullptr, *dLdosInterval_ = nullptr;
@@ -284,7 +299,7 @@ private:
   size_t nkLength() const;
   void nkGet(double *u);
   void nkSet(const double *u);
-  void nkMap(double *gu);
+  void nkMap(double *gu, int precision = PREC_FP64);
   std::vector<double *> nkVecs_;
   size_t nkLen_ = 0;
   int nkMaps_ = 0;
@@ -306,6 +321,7 @@ private:
   hipEvent_t evReadback_ = nullptr;
   double *dRed_ = nullptr;
   size_t redCap_ = 0, redUsed_ = 0;
+  size_t redReserved_ = 0; // elements reserved since construction (never reset)
   double *dMean_ = nullptr; // grain means of the current (Poisson solve)
   // handles of the reductions queued by integrate()
   std::vector<int> hDelta_, hDeltaSq_;
@@ -323,6 +339,20 @@ private:
   double *dAdjScratch_ = nullptr, *dAdjPartial_ = nullptr, *dAdjRho_ = nullptr,
          *dAdjDbar_ = nullptr, *dAdjExt_ = nullptr, *dAdjGrad_ = nullptr;
   int hA_ = -1; // queued residual of the vector potential
+  // the captured stages after the sweep (queueEndCompute)
+  void queueEndStages();
+  void dropEndGraph();
+  bool graphsOn_ = true;
+  bool endSeen_ = false;           // the stages ran eagerly once (sizes known)
+  hipGraphExec_t endExec_ = nullptr;
+  struct EndCapture {
+    size_t redOutBefore = 0, redUsedBefore = 0, redUsedAfter = 0, need = 0;
+    bool rangeMode = false;
+    std::vector<Pending> pend;      // the reservations the stages make
+    std::vector<long long> redN;    // and the element counts of their handles
+    std::vector<int> hDelta, hDeltaSq;
+    int hCurrent = -1, hEnergy = -1, hCurl = -1, hBoundary[2] = {-1, -1}, hA = -1;
+  } endCap_;
   long long partialsCap_ = 0;
   // sweep plans, one per precision
   struct SweepPlan {

@@ -177,6 +177,13 @@ __global__ void kernelGridKernel(const double *g, int n, int M, double *grid) {
   grid[i] = (dy >= 0 && dx >= 0) ? g[size_t(dy) * n + dx] : 0.0;
 }
 
+// The Green function's spectrum from the operator's: 1 / lambda, elementwise
+// (the screened operator's spectrum is positive everywhere).
+__global__ void reciprocalKernel(double *s, long long n) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) s[i] = 1.0 / s[i];
+}
+
 size_t alignUp(size_t b) { return (b + 255) & ~size_t(255); }
 
 } // namespace
@@ -196,10 +203,51 @@ size_t PoissonFFT::scratchBytes(int n) {
 PoissonFFT::PoissonFFT(int n, double h, hipStream_t stream, double maxError, void *external)
     : n_(n), m_(1), logm_(0), h_(h), stream_(stream), owns_(external == nullptr) {
   if (n < 1) throw std::invalid_argument("Poisson grid must be non-empty");
-  m_ = lengthFor(n);
+  std::vector<double> g;
+  numFilters_ = poissonKernel(n, h, g, maxError);
+  // out = -h^2 G * (j - <j>): the inverse transform's 1 / M^2 folded in
+  allocate(external);
+  scale_ = -(h_ * h_) / (static_cast<double>(m_) * static_cast<double>(m_));
+  try {
+    transformKernel(g);
+    SC_FFT_HIP(hipStreamSynchronize(stream_));
+  } catch (...) {
+    release();
+    throw;
+  }
+}
+
+PoissonFFT::PoissonFFT(int n, double h, hipStream_t stream, Screened sc)
+    : n_(n), m_(1), logm_(0), h_(h), stream_(stream), owns_(true) {
+  if (n < 2) throw std::invalid_argument("screened solver needs a grid of side >= 2");
+  if (!(sc.screening > 0.0)) throw std::invalid_argument("screening must be positive");
+  // the stencil of -L_h + screening: its transform is the operator's
+  // spectrum 4 / h^2 (sin^2(pi p / M) + sin^2(pi q / M)) + screening, whose
+  // reciprocal is the spectrum of the Green function
+  std::vector<double> g(size_t(n) * n, 0.0);
+  g[0] = 4.0 / (h * h) + sc.screening;
+  g[1] = g[size_t(n)] = -1.0 / (h * h);
+  allocate(nullptr);
+  scale_ = 1.0 / (static_cast<double>(m_) * static_cast<double>(m_));
+  try {
+    transformKernel(g);
+    const long long cells = static_cast<long long>(m_) * m_;
+    reciprocalKernel<<<static_cast<unsigned>((cells + kThreads - 1) / kThreads), kThreads, 0,
+                       stream_>>>(khat_, cells);
+    SC_FFT_HIP(hipGetLastError());
+    SC_FFT_HIP(hipStreamSynchronize(stream_));
+  } catch (...) {
+    release();
+    throw;
+  }
+}
+
+void PoissonFFT::allocate(void *external) {
+  m_ = lengthFor(n_);
+  logm_ = 0;
   while ((1 << logm_) < m_) ++logm_;
   if (m_ > kMaxLength)
-    throw std::invalid_argument("Poisson FFT: grid side " + std::to_string(n) +
+    throw std::invalid_argument("Poisson FFT: grid side " + std::to_string(n_) +
                                 " needs a transform longer than " +
                                 std::to_string(kMaxLength));
   const int M = m_;
@@ -210,8 +258,6 @@ PoissonFFT::PoissonFFT(int n, double h, hipStream_t stream, double maxError, voi
                           static_cast<long double>(k) / static_cast<long double>(M);
     tw[k] = make_double2(static_cast<double>(cosl(a)), -static_cast<double>(sinl(a)));
   }
-  std::vector<double> g;
-  numFilters_ = poissonKernel(n, h, g, maxError);
   try {
     if (owns_) {
       SC_FFT_HIP(hipMalloc(&tw_, sizeof(double2) * tw.size()));
@@ -225,41 +271,48 @@ PoissonFFT::PoissonFFT(int n, double h, hipStream_t stream, double maxError, voi
       base += alignUp(sizeof(double2) * size_t(M) * M);
       tw_ = reinterpret_cast<double2 *>(base);
     }
-    // The n x n kernel goes into the work buffer, its periodic M x M grid
-    // into the spectrum buffer, whose forward transform (rows into the work
-    // buffer, then columns) overwrites it with the spectrum.
     SC_FFT_HIP(hipMemcpyAsync(tw_, tw.data(), sizeof(double2) * tw.size(),
                               hipMemcpyHostToDevice, stream_));
-    SC_FFT_HIP(hipMemcpyAsync(work_, g.data(), sizeof(double) * g.size(),
-                              hipMemcpyHostToDevice, stream_));
-    const long long cells = static_cast<long long>(M) * M;
-    kernelGridKernel<<<static_cast<unsigned>((cells + kThreads - 1) / kThreads), kThreads, 0,
-                       stream_>>>(reinterpret_cast<const double *>(work_), n, M, khat_);
-    SC_FFT_HIP(hipGetLastError());
-    const int cb = columnsPerBlock(M);
-    rowsForwardKernel<<<M, kThreads, sizeof(double2) * M, stream_>>>(
-        khat_, nullptr, M, M, M, nullptr, M, logm_, tw_, work_);
-    SC_FFT_HIP(hipGetLastError());
-    columnsKernel<<<M / cb, kThreads, sizeof(double2) * M * cb, stream_>>>(
-        work_, M, 0, M, logm_, cb, tw_, nullptr, khat_);
-    SC_FFT_HIP(hipGetLastError());
     SC_FFT_HIP(hipStreamSynchronize(stream_));
   } catch (...) {
-    if (owns_) {
-      if (tw_) (void)hipFree(tw_);
-      if (khat_) (void)hipFree(khat_);
-      if (work_) (void)hipFree(work_);
-    }
+    release();
     throw;
   }
 }
 
-PoissonFFT::~PoissonFFT() {
-  if (!owns_) return;
-  (void)hipFree(tw_);
-  (void)hipFree(khat_);
-  (void)hipFree(work_);
+void PoissonFFT::transformKernel(const std::vector<double> &g) {
+  // The n x n kernel goes into the work buffer, its periodic M x M grid
+  // into the spectrum buffer, whose forward transform (rows into the work
+  // buffer, then columns) overwrites it with the spectrum.
+  const int M = m_, n = n_;
+  SC_FFT_HIP(hipMemcpyAsync(work_, g.data(), sizeof(double) * g.size(), hipMemcpyHostToDevice,
+                            stream_));
+  const long long cells = static_cast<long long>(M) * M;
+  kernelGridKernel<<<static_cast<unsigned>((cells + kThreads - 1) / kThreads), kThreads, 0,
+                     stream_>>>(reinterpret_cast<const double *>(work_), n, M, khat_);
+  SC_FFT_HIP(hipGetLastError());
+  const int cb = columnsPerBlock(M);
+  rowsForwardKernel<<<M, kThreads, sizeof(double2) * M, stream_>>>(
+      khat_, nullptr, M, M, M, nullptr, M, logm_, tw_, work_);
+  SC_FFT_HIP(hipGetLastError());
+  columnsKernel<<<M / cb, kThreads, sizeof(double2) * M * cb, stream_>>>(
+      work_, M, 0, M, logm_, cb, tw_, nullptr, khat_);
+  SC_FFT_HIP(hipGetLastError());
+  // the host kernel must outlive the copy
+  SC_FFT_HIP(hipStreamSynchronize(stream_));
 }
+
+void PoissonFFT::release() {
+  if (!owns_) return;
+  if (tw_) (void)hipFree(tw_);
+  if (khat_) (void)hipFree(khat_);
+  if (work_) (void)hipFree(work_);
+  tw_ = nullptr;
+  khat_ = nullptr;
+  work_ = nullptr;
+}
+
+PoissonFFT::~PoissonFFT() { release(); }
 
 void PoissonFFT::solve(const double *dj, const double *dMean, double *dout) {
   const int M = m_, n = n_;
@@ -271,9 +324,8 @@ void PoissonFFT::solve(const double *dj, const double *dMean, double *dout) {
   columnsKernel<<<M / cb, kThreads, sizeof(double2) * M * cb, stream_>>>(
       work_, n, n, M, logm_, cb, tw_, khat_, nullptr);
   SC_FFT_HIP(hipGetLastError());
-  const double scale = -(h_ * h_) / (static_cast<double>(M) * static_cast<double>(M));
   rowsInverseKernel<<<n, kThreads, sizeof(double2) * M, stream_>>>(work_, n, M, logm_, tw_,
-                                                                  scale, dout);
+                                                                  scale_, dout);
   SC_FFT_HIP(hipGetLastError());
 }
 

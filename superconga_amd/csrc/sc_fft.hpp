@@ -25,6 +25,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 namespace scamd {
 
 class PoissonFFT {
@@ -37,6 +39,16 @@ public:
   // With `external` the buffers live there (scratchBytes(n) bytes).
   PoissonFFT(int n, double h, hipStream_t stream, double maxError = 0.0,
              void *external = nullptr);
+  // The screened solver: solve() returns (-L_h + screening)^-1 (r - mean)
+  // on the n x n window, L_h the five-point lattice Laplacian of spacing h,
+  // as a zero-padded (open-boundary) convolution with the operator's lattice
+  // Green function.  Its spectrum is the reciprocal of the transform of the
+  // operator's stencil, so no real-space Green function is formed.  Used to
+  // precondition the vector-potential update (Grain::screenVectorPotential).
+  struct Screened {
+    double screening; // > 0, in units of 1 / h^2's length unit squared
+  };
+  PoissonFFT(int n, double h, hipStream_t stream, Screened sc);
   ~PoissonFFT();
   // Transform length for grid side n, and the bytes of the buffers.
   static int lengthFor(int n);
@@ -55,7 +67,14 @@ public:
   int numFilters() const { return numFilters_; }
 
 private:
+  void allocate(void *external);
+  // khat_ <- the forward transform of the even kernel g (n x n quadrant,
+  // g[dy * n + dx]) on the periodic M x M grid
+  void transformKernel(const std::vector<double> &g);
+  void release();
+
   int n_, m_, logm_;
+  double scale_ = 0.0; // of the inverse transform in solve()
   double h_;
   hipStream_t stream_;
   bool owns_ = true;

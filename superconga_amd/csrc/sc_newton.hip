@@ -10,8 +10,13 @@
 // <a, b> = sum_blocks <a_b, b_b> / |u_b|^2, so every block is measured
 // relative to its size like the reference's residuals.  Jacobian-vector
 // products are finite differences of G: J v = (G(u + e v) - G(u)) / e - v.
-// G is evaluated in FP64; the sweep and everything after it are the same
-// kernels as the plain iteration.
+// The Newton residual G(u) - u is always evaluated in FP64.  The
+// Jacobian-vector products use the FP64 map, or (mixed precision) the map
+// with FP32 stepping and FP64 sums on both ends of the difference, with a
+// step e large enough that the FP32 rounding (~1e-7 relative) stays at
+// ~1e-3 of the product: the inner solve is then an inexact Newton direction
+// whose error the FP64 residuals of the outer iteration correct.  The sweep
+// and everything after it are the same kernels as the plain iteration.
 #include "sc_engine.hpp"
 #include "sc_device.hpp"
 #include "sc_util.hpp"
@@ -33,7 +38,7 @@
 namespace scamd {
 namespace {
 
-constexpr int kMaxTerms = 32;
+constexpr int kMaxTerms = 64;
 // GMRES cycles per Newton step (each of at most krylovDim directions)
 constexpr int kRestarts = 4;
 struct Terms {
@@ -108,7 +113,7 @@ void Grain::nkSet(const double *u) {
                            stream_));
 }
 
-void Grain::nkMap(double *gu) {
+void Grain::nkMap(double *gu, int precision) {
   // One plain iteration of the current state without the accelerator and
   // without the compute objects' reductions.
   const Setup &s = *s_;
@@ -116,11 +121,14 @@ void Grain::nkMap(double *gu) {
   const bool doA = s.cfg.computeCurrent && s.lambda > 0.0;
   ensureBoundaryInit();
   extendDelta();
-  sweep(PREC_FP64, false);
+  sweep(precision, false);
   backRotate();
   int hb[2];
   queueBoundaryCondition(hb); // new in-values, in place
-  if (doA) solveVectorPotential(dCurrent_, dANew_);
+  if (doA) {
+    solveVectorPotential(dCurrent_, dANew_);
+    if (screen_) screenVectorPotential();
+  }
   // the map needs none of the residual statistics: no read-back
   discardReductions();
   const size_t nx = size_t(2 * s.Nc + (doA ? 2 : 0)) * n0;
@@ -134,7 +142,10 @@ void Grain::nkMap(double *gu) {
   ++nkMaps_;
 }
 
-NewtonReport Grain::newtonKrylov(int maxNewton, int krylovDim, double tol) {
+NewtonReport Grain::newtonKrylov(int maxNewton, int krylovDim, double tol, int jvpPrecision) {
+  if (jvpPrecision != PREC_FP64 && jvpPrecision != PREC_MIXED)
+    throw std::runtime_error("Jacobian-vector products run in fp64 or mixed precision");
+  const bool mixedJvp = jvpPrecision == PREC_MIXED;
   if (s_->keldysh) throw std::runtime_error("a spectroscopy context only runs the LDOS iteration");
   if (krylovDim < 1 || krylovDim > kMaxTerms - 1)
     throw std::runtime_error("Krylov dimension must be in [1, " + std::to_string(kMaxTerms - 1) +
@@ -142,9 +153,9 @@ NewtonReport Grain::newtonKrylov(int maxNewton, int krylovDim, double tol) {
   const std::vector<NkBlock> blocks = nkBlocks();
   const size_t L = nkLength();
   const long long nL = static_cast<long long>(L);
-  // workspace: u, G(u), F, trial, w, the step d, the linear residual r and
-  // the Krylov basis
-  const int nv = 7 + krylovDim + 1;
+  // workspace: u, G(u), F, trial, w, the step d, the linear residual r,
+  // G(u) in the products' precision, and the Krylov basis
+  const int nv = 8 + krylovDim + 1;
   if (nkVecs_.size() != size_t(nv) || nkLen_ != L) {
     for (double *p : nkVecs_) devFree(p);
     nkVecs_.clear();
@@ -152,8 +163,8 @@ NewtonReport Grain::newtonKrylov(int maxNewton, int krylovDim, double tol) {
     nkLen_ = L;
   }
   double *u = nkVecs_[0], *gu = nkVecs_[1], *f = nkVecs_[2], *trial = nkVecs_[3],
-         *w = nkVecs_[4], *dstep = nkVecs_[5], *rlin = nkVecs_[6];
-  std::vector<double *> V(nkVecs_.begin() + 7, nkVecs_.end());
+         *w = nkVecs_[4], *dstep = nkVecs_[5], *rlin = nkVecs_[6], *guJ = nkVecs_[7];
+  std::vector<double *> V(nkVecs_.begin() + 8, nkVecs_.end());
   nkMaps_ = 0;
   NewtonReport rep;
 
@@ -210,7 +221,14 @@ NewtonReport Grain::newtonKrylov(int maxNewton, int krylovDim, double tol) {
   for (int it = 0; it < maxNewton && fn >= tol; ++it) {
     // restarted GMRES(m) on J d = -F from d = 0
     const double unorm = wnorm(u);
-    const double e = 1e-7 * (1.0 + unorm);
+    const double e = (mixedJvp ? 3e-4 : 1e-7) * (1.0 + unorm);
+    // the base point of the differences, in the products' precision
+    const double *base = gu;
+    if (mixedJvp) {
+      nkSet(u);
+      nkMap(guJ, PREC_MIXED);
+      base = guJ;
+    }
     combine(dstep, nullptr, 0.0, {}, {});
     combine(rlin, nullptr, 0.0, {f}, {-1.0});
     double rn = fn;
@@ -227,8 +245,8 @@ NewtonReport Grain::newtonKrylov(int maxNewton, int krylovDim, double tol) {
       for (; k < m; ++k) {
         combine(trial, u, 1.0, {V[size_t(k)]}, {e});
         nkSet(trial);
-        nkMap(w);
-        combine(w, w, 1.0 / e, {gu, V[size_t(k)]}, {-1.0 / e, -1.0});
+        nkMap(w, jvpPrecision);
+        combine(w, w, 1.0 / e, {base, V[size_t(k)]}, {-1.0 / e, -1.0});
         // classical Gram-Schmidt with the norm of w read back with the
         // projections: |w - sum d_i v_i|^2 = |w|^2 - sum d_i^2 for the
         // orthonormal basis.  A second pass only when the first removed

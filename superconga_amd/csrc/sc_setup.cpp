@@ -817,6 +817,9 @@ Config parseConfig(const std::string &text) {
   c.stepSize = ac["step_size"].asDouble(-1);
   c.rank = ac["rank"].asInt(-1);
   c.blockScaling = ac["block_scaling"].asBool(false);
+  c.vpScreening = ac["vector_potential_screening"].asDouble(0.0);
+  if (c.vpScreening < 0.0)
+    throw std::runtime_error("accelerator.vector_potential_screening must be >= 0");
   c.drag = ac["drag"].asDouble(-1);
   c.stepFactor = ac["step_size_factor"].asDouble(-1);
   c.cosMin = ac["cos_similarity_min"].asDouble(-1);

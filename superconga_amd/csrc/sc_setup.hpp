@@ -79,6 +79,9 @@ struct Config {
   // Anderson extension: weight each field block (a component of Delta, the
   // vector potential) by 1 / |x_block|^2 in the least squares
   bool blockScaling = false;
+  // Extension: precondition the vector-potential update by the Meissner
+  // response of a superfluid density of this value (0: off)
+  double vpScreening = 0.0;
   // misc
   std::string loadPath;
   unsigned long long seed = 0;

@@ -87,8 +87,59 @@ def test_newton_krylov_without_vector_potential(product):
 
 def test_newton_krylov_rejects_bad_arguments(product):
     s = product.Simulation(cases.accelerator_case("picard"))
-    for kw in ({"krylov_dim": 0}, {"krylov_dim": 32}, {"tol": 0.0}, {"max_newton": -1}):
+    for kw in ({"krylov_dim": 0}, {"krylov_dim": 64}, {"tol": 0.0}, {"max_newton": -1},
+               {"jvp_precision": "mixed_refine"}):
         args = dict(max_newton=2, krylov_dim=4, tol=1e-8)
         args.update(kw)
         with pytest.raises(product.SupercongaError):
             s.newton_krylov(**args)
+
+
+def test_newton_krylov_with_fp32_products_finds_the_fp64_fixed_point(product):
+    # Jacobian-vector products on the map with FP32 stepping, residuals in
+    # FP64: an inexact Newton direction whose error the FP64 residuals
+    # correct, so the answer is the FP64 fixed point
+    cfg = cases.accelerator_case("picard")
+    ref = converged_plain(product, cfg)
+    s = product.Simulation(cfg)
+    rep = s.newton_krylov(max_newton=30, krylov_dim=20, tol=1e-10, jvp_precision="mixed")
+    print("newton-krylov, FP32 products:", {k: v for k, v in rep.items() if k != "history"},
+          "history", ["%.2e" % h for h in rep["history"]])
+    assert rep["converged"], rep
+    s.compute("fp64")
+    r = s.results()
+    for k in ("residual_order_parameter", "residual_boundary", "residual_vector_potential"):
+        assert r[k] < 1e-8, (k, r)
+    assert rel(s.order_parameter(1), ref.order_parameter(1)) < 1e-6
+    assert rel(s.vector_potential(1), ref.vector_potential(1)) < 1e-5
+    assert abs(r["free_energy_per_area"] - ref.results()["free_energy_per_area"]) < 1e-8 * abs(
+        r["free_energy_per_area"])
+
+
+@pytest.mark.parametrize("acc", ["congacc", "anderson", "picard"])
+def test_screened_vector_potential_update_keeps_the_fixed_point(product, acc):
+    # accelerator.vector_potential_screening hands the accelerator
+    # A + P (G_A - A) instead of G_A: the same fixed points, so every
+    # accelerator converges to the plain run's state
+    def solve(rho):
+        c = cases.accelerator_case(acc)
+        c["numerics"]["convergence_criterion"] = 1e-10
+        c["numerics"]["norm"] = "l2"
+        c["accelerator"]["vector_potential_screening"] = rho
+        sim = product.Simulation(c)
+        ok = sim.run(0, 0, 3000, "fp64")
+        return ok, sim
+    ok0, plain = solve(0.0)
+    ok1, scr = solve(1.0)
+    r0, r1 = plain.results(), scr.results()
+    print("%s: plain %d iterations, screened %d iterations" % (acc, r0["iterations"],
+                                                              r1["iterations"]))
+    assert ok1
+    if ok0:
+        assert rel(scr.order_parameter(0), plain.order_parameter(0)) < 1e-7
+        assert rel(scr.vector_potential(0), plain.vector_potential(0)) < 1e-6
+        assert abs(r1["free_energy_per_area"] - r0["free_energy_per_area"]) < 1e-9 * abs(
+            r0["free_energy_per_area"])
+    with open(os.path.join(GOLDEN, "known_answers.json")) as f:
+        ka = json.load(f)["accelerator_free_energy_per_area"]
+    assert abs(r1["free_energy_per_area"] - ka["value"]) < ka["tolerance_rel"] * abs(ka["value"])
