@@ -82,18 +82,20 @@ def roofline(traj_steps, sweep_ms, peak, flops_per_step=FLOPS_PER_TRAJ_STEP):
 
 def load_traffic(workload, precision, shape):
     """HBM bytes per sweep launch from the committed rocprofv3 counter runs,
-    when they were taken on the sweep variant and shape (energies per lane,
-    waves per SIMD, slots per CU) that this run timed; None otherwise."""
+    when one of them was taken on the sweep variant and shape (energies per
+    lane, waves per SIMD, slots per CU) that this run timed; None otherwise."""
     path = os.path.join(ROOT, "profiles", "sweep_%s_%s_traffic.json" % (workload, precision))
     if not os.path.exists(path):
         return None
     with open(path) as f:
         rec = json.load(f)
-    if [int(v) for v in rec.get("shape", [])] != [int(v) for v in shape]:
-        log("traffic: the counter run of %s %s measured shape %s, this run timed %s: "
-            "traffic left null" % (workload, precision, rec.get("shape"), list(shape)))
-        return None
-    return rec.get("bytes_per_launch")
+    want = [int(v) for v in shape]
+    for r in rec.get("records", [rec]):
+        if [int(v) for v in r.get("shape", [])] == want:
+            return r.get("bytes_per_launch")
+    log("traffic: no counter run of %s %s at shape %s: traffic left null"
+        % (workload, precision, want))
+    return None
 
 
 def copy_probe(mib):
@@ -132,10 +134,11 @@ def converge(sa, cfg, precision, cap):
     return {"converged": bool(ok), "iterations": r["iterations"],
             "fp32_iterations": r["mixed_iterations"], "wall_s": round(wall, 4),
             "residual": r["residual"], "free_energy_per_area": r["free_energy_per_area"],
+            "residual_vector_potential": r["residual_vector_potential"],
             "sweep_shapes": [list(shape64), list(shape32)]}
 
 
-def converge_newton(sa, cfg, cap, warm_residual=1e-3, warm_max=30):
+def converge_newton(sa, cfg, cap, warm_residual=1e-3, warm_max=30, jvp="fp64"):
     """Wall time to the config's criterion with Newton-Krylov: FP32-stepping
     iterations until the residual is below warm_residual (at most warm_max),
     Jacobian-free Newton-Krylov on the FP64 map, then FP64 iterations until
@@ -149,7 +152,7 @@ def converge_newton(sa, cfg, cap, warm_residual=1e-3, warm_max=30):
         sim.compute("mixed")
         if sim.results()["residual"] < warm_residual:
             break
-    rep = sim.newton_krylov(max_newton=20, krylov_dim=30, tol=1e-11)
+    rep = sim.newton_krylov(max_newton=20, krylov_dim=30, tol=1e-11, jvp_precision=jvp)
     ok = sim.run(0, 0, cap, "fp64")
     wall = time.perf_counter() - t
     r = sim.results()
@@ -179,44 +182,59 @@ def convergence(sa, workloads, args):
         if conv["fp64"]["converged"] and conv["newton_krylov"]["converged"]:
             conv["speedup_newton_krylov"] = (conv["fp64"]["wall_s"]
                                              / conv["newton_krylov"]["wall_s"])
+        conv["newton_krylov_mixed"] = converge_newton(sa, ccfg, args.converge_cap, jvp="mixed")
+        log("converge %s newton-krylov (FP32 products): %s"
+            % (args.converge_workload, conv["newton_krylov_mixed"]))
+        if conv["fp64"]["converged"] and conv["newton_krylov_mixed"]["converged"]:
+            conv["speedup_newton_krylov_mixed"] = (conv["fp64"]["wall_s"]
+                                                   / conv["newton_krylov_mixed"]["wall_s"])
     return conv
 
 
 def octagon_convergence(sa, workloads, args):
     """Wall time towards the 1e-8 criterion on the d-wave octagon itself,
     each method from the same seeded start and capped: plain FP64 with the
-    config's Anderson(8), plain FP64 with CongAcc, mixed refinement
-    (Anderson), and Newton-Krylov after FP32 iterations.  A method that
-    stops at its cap reports converged false and the residual it reached."""
+    config's Anderson(8) and with CongAcc (the reference's default
+    accelerator), mixed refinement with CongAcc, and Newton-Krylov with FP32
+    Jacobian-vector products after FP32 CongAcc iterations.  A method that
+    stops at its cap reports converged false and the residual it reached,
+    with the residual of each field (the vector potential's is the one that
+    stays highest)."""
     base = workloads.octagon_dwave()
     cap = args.octagon_converge_cap
     res = {"workload": "octagon_dwave", "criterion": 1e-8, "cap_iterations": cap}
     runs = (("fp64_anderson", "anderson", "fp64"), ("fp64_congacc", "congacc", "fp64"),
-            ("mixed_refine_anderson", "anderson", "mixed_refine"))
+            ("mixed_refine_congacc", "congacc", "mixed_refine"))
     for key, acc, prec in runs:
         cfg = workloads.octagon_dwave()
         cfg["accelerator"] = dict(base["accelerator"]) if acc == "anderson" else {"name": acc}
         res[key] = converge(sa, cfg, prec, cap)
         log("octagon converge %s: %s" % (key, res[key]))
     if not args.no_newton:
-        sim = sa.Simulation(base)
+        cfg = workloads.octagon_dwave()
+        cfg["accelerator"] = {"name": "congacc"}
+        sim = sa.Simulation(cfg)
         sim.autotune("fp64")
         sim.autotune("mixed")
         t = time.perf_counter()
-        for _ in range(30):
+        for _ in range(cap):
             sim.compute("mixed")
-        rep = sim.newton_krylov(max_newton=10, krylov_dim=30, tol=1e-11)
-        ok = sim.run(0, 0, 20, "fp64")
+        warm = sim.results()["residual"]
+        rep = sim.newton_krylov(max_newton=8, krylov_dim=40, tol=1e-11, jvp_precision="mixed")
+        ok = sim.run(0, 0, 5, "fp64")
         wall = time.perf_counter() - t
         r = sim.results()
         sim.close()
-        res["newton_krylov"] = {"converged": bool(ok), "fp32_iterations": 30,
-                                "newton_iterations": rep["newton_iterations"],
-                                "krylov_iterations": rep["krylov_iterations"],
-                                "map_evaluations": rep["map_evaluations"],
-                                "newton_residual": rep["residual"], "wall_s": round(wall, 3),
-                                "residual": r["residual"]}
-        log("octagon converge newton-krylov: %s" % res["newton_krylov"])
+        res["newton_krylov_mixed"] = {
+            "converged": bool(ok), "fp32_iterations": cap, "residual_after_fp32": warm,
+            "newton_iterations": rep["newton_iterations"],
+            "krylov_iterations": rep["krylov_iterations"],
+            "map_evaluations": rep["map_evaluations"], "newton_residual": rep["residual"],
+            "wall_s": round(wall, 3), "residual": r["residual"],
+            "residuals": {k: r[k] for k in ("residual_order_parameter", "residual_current",
+                                             "residual_vector_potential", "residual_boundary",
+                                             "residual_free_energy")}}
+        log("octagon converge newton-krylov (FP32 products): %s" % res["newton_krylov_mixed"])
     return res
 
 
