@@ -362,6 +362,33 @@ def batch_convergence(sa, workloads, args):
     if res["fp64"]["converged_grains"] == len(cfgs) and \
             res["mixed_refine"]["converged_grains"] == len(cfgs):
         res["speedup_mixed_refine"] = res["fp64"]["wall_s"] / res["mixed_refine"]["wall_s"]
+    if not args.no_newton:
+        # batched Newton-Krylov: FP32 iterations of the whole batch, every
+        # grain's Newton-Krylov solve with FP32 products in shared sweep
+        # launches, then FP64 iterations until the reference's test passes
+        b = sa.Batch(cfgs)
+        b.autotune("fp64")
+        b.autotune("mixed")
+        t = time.perf_counter()
+        for _ in range(args.batch_newton_warm):
+            b.compute("mixed")
+        reps = b.newton_krylov(max_newton=12, krylov_dim=20, tol=1e-11, jvp_precision="mixed")
+        conv = b.run(0, 0, 30, "fp64")
+        wall = time.perf_counter() - t
+        its = [b.grain(i).results()["iterations"] for i in range(len(b))]
+        res["newton_krylov_mixed"] = {
+            "wall_s": round(wall, 3), "converged_grains": int(sum(conv)),
+            "fp32_iterations": args.batch_newton_warm,
+            "newton_converged_grains": int(sum(r["converged"] for r in reps)),
+            "newton_iterations_max": int(max(r["newton_iterations"] for r in reps)),
+            "map_evaluations_max": int(max(r["map_evaluations"] for r in reps)),
+            "fp64_iterations_after_max": int(max(its))}
+        b.close()
+        log("batch converge newton-krylov (FP32 products): %s" % res["newton_krylov_mixed"])
+        if res["newton_krylov_mixed"]["converged_grains"] == len(cfgs) and \
+                res["fp64"]["converged_grains"] == len(cfgs):
+            res["speedup_newton_krylov_mixed"] = (res["fp64"]["wall_s"]
+                                                  / res["newton_krylov_mixed"]["wall_s"])
     return res
 
 
@@ -489,6 +516,8 @@ def main():
     ap.add_argument("--batch-converge-cap", type=int, default=400,
                     help="iteration cap of the parameter sweep's batched solve to 1e-8")
     ap.add_argument("--no-batch-converge", action="store_true")
+    ap.add_argument("--batch-newton-warm", type=int, default=20,
+                    help="FP32 batch iterations before the batched Newton-Krylov solve")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "

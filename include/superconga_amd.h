@@ -352,6 +352,15 @@ int sc_batch_burnin(sc_batch *batch, int precision);
  * an iteration sweeps the grains of each precision in one launch each. */
 int sc_batch_run(sc_batch *batch, int num_burnin, int num_min, int num_max,
                  int precision, int *converged);
+/* sc_newton_krylov_ex on every grain of the batch at once: each grain runs
+ * its own Newton-Krylov solve (Krylov basis, line search, stopping), and
+ * every round evaluates the maps of all grains still solving with one sweep
+ * launch per precision (no counterpart in the reference, whose parameter
+ * sweeps run one process per point, examples/parameter_sweeps/*.sh).
+ * reports holds one sc_newton_report per grain (or is null).  On return
+ * every grain's state holds its last accepted u. */
+int sc_batch_newton_krylov(sc_batch *batch, int max_newton, int krylov_dim, double tol,
+                           int jvp_precision, sc_newton_report *reports);
 /* sc_set_graphs for every grain of the batch. */
 int sc_batch_set_graphs(sc_batch *batch, int enable);
 int sc_batch_set_sweep_shape(sc_batch *batch, int precision, int energies_per_lane,

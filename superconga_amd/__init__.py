@@ -151,6 +151,8 @@ def lib():
         L.sc_newton_krylov_ex.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                           ctypes.c_int, ctypes.POINTER(NewtonReport)]
         L.sc_set_graphs.argtypes = [vp, ctypes.c_int]
+        L.sc_batch_newton_krylov.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                             ctypes.c_int, ctypes.POINTER(NewtonReport)]
         L.sc_batch_set_graphs.argtypes = [vp, ctypes.c_int]
         L.sc_free_energy_gradient.argtypes = [vp, _DP, _DP]
         L.sc_ldos_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(vp)]
@@ -507,6 +509,26 @@ class Batch:
         """Captured hipGraphs (the default) or eager launches for the stages
         after the sweep of every grain."""
         _check(lib().sc_batch_set_graphs(self._h, 1 if enable else 0))
+
+    def newton_krylov(self, max_newton=20, krylov_dim=20, tol=1e-9, jvp_precision="fp64"):
+        """Newton-Krylov on every grain at once (Simulation.newton_krylov per
+        grain, the maps of all solving grains in one sweep launch per
+        precision); one report dict per grain."""
+        n = len(self._grains)
+        hists = [(ctypes.c_double * (max_newton + 1))() for _ in range(n)]
+        reps = (NewtonReport * n)()
+        for i in range(n):
+            reps[i].history = ctypes.cast(hists[i], ctypes.POINTER(ctypes.c_double))
+            reps[i].history_capacity = max_newton + 1
+        _check(lib().sc_batch_newton_krylov(self._h, max_newton, krylov_dim,
+                                            ctypes.c_double(tol), _precision(jvp_precision),
+                                            reps))
+        return [{"newton_iterations": r.newton_iterations,
+                 "krylov_iterations": r.krylov_iterations,
+                 "map_evaluations": r.map_evaluations, "converged": bool(r.converged),
+                 "residual": r.residual,
+                 "history": [hists[i][q] for q in range(r.history_length)]}
+                for i, r in enumerate(reps)]
 
     def sweep_shape(self, precision="fp64"):
         e, w, s = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

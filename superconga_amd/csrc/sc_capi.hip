@@ -715,6 +715,40 @@ int sc_batch_run(sc_batch *b, int nb, int nmin, int nmax, int precision, int *co
     return SC_OK;
   });
 }
+int sc_batch_newton_krylov(sc_batch *b, int maxNewton, int krylovDim, double tol,
+                           int jvpPrecision, sc_newton_report *reports) {
+  SC_CHECK_PTR(b);
+  return guarded([&] {
+    if (maxNewton < 0) throw std::invalid_argument("max_newton must be >= 0");
+    if (krylovDim < 1 || krylovDim > 63)
+      throw std::invalid_argument("krylov_dim must be in [1, 63]");
+    if (jvpPrecision != SC_PRECISION_FP64 && jvpPrecision != SC_PRECISION_MIXED)
+      throw std::invalid_argument("jvp_precision must be SC_PRECISION_FP64 or SC_PRECISION_MIXED");
+    if (!(tol > 0.0)) throw std::invalid_argument("tol must be positive");
+    const int n = b->batch->size();
+    if (reports)
+      for (int i = 0; i < n; ++i)
+        if (reports[i].history_capacity > 0 && !reports[i].history)
+          throw std::invalid_argument("history is null with a positive capacity");
+    const std::vector<scamd::NewtonReport> rs = b->batch->newtonKrylov(
+        maxNewton, krylovDim, tol,
+        jvpPrecision == SC_PRECISION_MIXED ? scamd::PREC_MIXED : scamd::PREC_FP64);
+    if (reports)
+      for (int i = 0; i < n; ++i) {
+        const scamd::NewtonReport &r = rs[size_t(i)];
+        sc_newton_report &o = reports[i];
+        o.newton_iterations = r.newtonIterations;
+        o.krylov_iterations = r.krylovIterations;
+        o.map_evaluations = r.mapEvaluations;
+        o.converged = r.converged ? 1 : 0;
+        o.residual = r.residual;
+        const int h = std::min(static_cast<int>(r.history.size()), std::max(o.history_capacity, 0));
+        for (int q = 0; q < h; ++q) o.history[q] = r.history[static_cast<size_t>(q)];
+        o.history_length = h;
+      }
+    return SC_OK;
+  });
+}
 int sc_batch_set_sweep_shape(sc_batch *b, int precision, int el, int waves, int slots) {
   SC_CHECK_PTR(b);
   return guarded([&] {

@@ -2002,16 +2002,21 @@ void Batch::prepare(int precision, const std::vector<int> &act) {
   // views of every grain with this lane layout (index = grain number)
   std::vector<GrainView> views;
   for (auto &g : grains_) views.push_back(g->sweepView(pl.el));
+  // every grain's items at this layout, built once per layout
+  if (pl.cacheEl != pl.el) {
+    pl.itemsOf.assign(grains_.size(), {});
+    pl.maxLenOf.assign(grains_.size(), 0);
+    for (size_t gi = 0; gi < grains_.size(); ++gi)
+      pl.itemsOf[gi] = sweepItems(grains_[gi]->setup(), views[gi].cols, &pl.maxLenOf[gi]);
+    pl.cacheEl = pl.el;
+  }
   // the items of the active grains, longest first
   std::vector<SweepItemDev> items;
   std::vector<int> lens;
   pl.maxLen = 0;
   for (int gi : act) {
-    int maxLen = 0;
-    const std::vector<SweepItem> its =
-        sweepItems(grains_[size_t(gi)]->setup(), views[size_t(gi)].cols, &maxLen);
-    pl.maxLen = std::max(pl.maxLen, maxLen);
-    for (const SweepItem &it : its) {
+    pl.maxLen = std::max(pl.maxLen, pl.maxLenOf[size_t(gi)]);
+    for (const SweepItem &it : pl.itemsOf[size_t(gi)]) {
       items.push_back({gi, it.angle, it.x0, it.ncols, it.ylo, it.yhi});
       lens.push_back(it.yhi - it.ylo + 1);
     }
@@ -2023,11 +2028,13 @@ void Batch::prepare(int precision, const std::vector<int> &act) {
   std::vector<SweepItemDev> sorted(items.size());
   for (size_t i = 0; i < order.size(); ++i) sorted[i] = items[order[i]];
   pl.numItems = static_cast<int>(sorted.size());
-  if (pl.dItems) (void)hipFree(pl.dItems);
-  if (pl.dViews) (void)hipFree(pl.dViews);
-  pl.dItems = pl.dViews = nullptr;
-  SC_HIP(hipMalloc(&pl.dItems, std::max<size_t>(1, sorted.size()) * sizeof(SweepItemDev)));
-  SC_HIP(hipMalloc(&pl.dViews, views.size() * sizeof(GrainView)));
+  if (sorted.size() > pl.itemsCap || !pl.dItems) {
+    if (pl.dItems) (void)hipFree(pl.dItems);
+    pl.dItems = nullptr;
+    pl.itemsCap = std::max<size_t>(1, sorted.size());
+    SC_HIP(hipMalloc(&pl.dItems, pl.itemsCap * sizeof(SweepItemDev)));
+  }
+  if (!pl.dViews) SC_HIP(hipMalloc(&pl.dViews, views.size() * sizeof(GrainView)));
   if (!sorted.empty())
     SC_HIP(hipMemcpyAsync(pl.dItems, sorted.data(), sorted.size() * sizeof(SweepItemDev),
                           hipMemcpyHostToDevice, stream_));

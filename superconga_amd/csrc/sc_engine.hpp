@@ -299,7 +299,12 @@ private:
   size_t nkLength() const;
   void nkGet(double *u);
   void nkSet(const double *u);
-  void nkMap(double *gu, int precision = PREC_FP64);
+  // one evaluation of the map G of the state: nkMapBegin, the sweep at the
+  // chosen precision (the grain's own or a batch's), nkMapEnd(G)
+  void nkMapBegin();
+  void nkMapEnd(double *gu);
+  friend class NewtonKrylov;
+  friend class Batch;
   std::vector<double *> nkVecs_;
   size_t nkLen_ = 0;
   int nkMaps_ = 0;
@@ -409,6 +414,56 @@ void poissonSolve(int n, double h, int numGrain, bool enforceMean,
 // the per-grain stages on one stream and sweeps all grains that are still
 // iterating in one persistent launch whose work queue holds the items of all
 // of them, longest first, so that small grains fill the gaps of large ones.
+// One grain's Jacobian-free Newton-Krylov solve (sc_newton.hip), advanced
+// one map evaluation at a time: next() writes the point to evaluate into the
+// grain's state, starts its map (nkMapBegin) and names the sweep precision;
+// the caller sweeps and calls grain.nkMapEnd(out()); advance() continues the
+// algorithm.  Grain::newtonKrylov drives one; Batch::newtonKrylov drives one
+// per grain with one sweep launch per precision for all of them.
+class NewtonKrylov {
+public:
+  NewtonKrylov(Grain &g, int maxNewton, int krylovDim, double tol, int jvpPrecision);
+  bool next(int &precision);
+  double *out() const { return out_; }
+  void advance();
+  bool done() const { return phase_ == DONE; }
+  NewtonReport finish();
+
+private:
+  enum Phase { INIT, BASE, KRYLOV, LINE, DONE };
+  std::vector<double> wdots(const double *a, const std::vector<double *> &bs);
+  double wnorm(double *a);
+  void combine(double *y, const double *base, double a, const std::vector<const double *> &vs,
+               const std::vector<double> &cs);
+  void want(const double *x, double *out, int precision);
+  void startNewton();
+  void startCycle();
+  void startProduct();
+  void productDone();
+  void endCycle();
+  void startLineSearch();
+  void lineDone();
+
+  Grain &g_;
+  int maxNewton_, m_;
+  double tol_;
+  int jvp_;
+  std::vector<Grain::NkBlock> blocks_;
+  std::vector<double> wt_;
+  long long nL_ = 0;
+  double *u_, *gu_, *f_, *trial_, *w_, *dstep_, *rlin_, *guJ_;
+  std::vector<double *> V_;
+  Phase phase_ = INIT;
+  const double *x_ = nullptr;
+  double *out_ = nullptr;
+  int prec_ = PREC_FP64;
+  NewtonReport rep_;
+  double fn_ = 0, eta_ = 0.1, e_ = 0, rn_ = 0, beta_ = 0, lambda_ = 1;
+  int cycle_ = 0, k_ = 0, ls_ = 0;
+  bool breakdown_ = false;
+  std::vector<double> H_, R_, gv_, cs_, sn_;
+};
+
 class Batch {
 public:
   explicit Batch(std::vector<std::unique_ptr<Setup>> setups);
@@ -437,6 +492,11 @@ public:
   double lastIterationMs() const { return iterMs_; }
   long long lastTrajSteps() const { return trajSteps_; }
   std::vector<int> active() const;
+  // Newton-Krylov on every grain at once (Grain::newtonKrylov per grain),
+  // the map evaluations of all solving grains in one sweep launch per
+  // precision; one report per grain.
+  std::vector<NewtonReport> newtonKrylov(int maxNewton, int krylovDim, double tol,
+                                         int jvpPrecision);
 
 private:
   struct Plan {
@@ -444,7 +504,12 @@ private:
     std::vector<int> key; // the active grains the items were built for
     int numItems = 0, numSlots = 0, maxLen = 0;
     void *dItems = nullptr, *dViews = nullptr;
+    size_t itemsCap = 0; // items dItems can hold
     bool ready = false;
+    // each grain's items at this layout (built once per layout)
+    int cacheEl = -1;
+    std::vector<std::vector<SweepItem>> itemsOf;
+    std::vector<int> maxLenOf;
   };
   void prepare(int precision, const std::vector<int> &act);
   void sweep(int precision, bool burnin, const std::vector<int> &act);

@@ -25,6 +25,7 @@
 #include <cmath>
 #include <stdexcept>
 #include <string>
+#include <memory>
 #include <vector>
 
 #define SC_NK_HIP(call)                                                             \
@@ -113,15 +114,19 @@ void Grain::nkSet(const double *u) {
                            stream_));
 }
 
-void Grain::nkMap(double *gu, int precision) {
+void Grain::nkMapBegin() {
   // One plain iteration of the current state without the accelerator and
-  // without the compute objects' reductions.
+  // without the compute objects' reductions: the order parameter's
+  // extension and the site fields, then (the caller) the sweep, then
+  // nkMapEnd.
+  ensureBoundaryInit();
+  extendDelta();
+}
+
+void Grain::nkMapEnd(double *gu) {
   const Setup &s = *s_;
   const size_t n0 = size_t(s.N0) * s.N0;
   const bool doA = s.cfg.computeCurrent && s.lambda > 0.0;
-  ensureBoundaryInit();
-  extendDelta();
-  sweep(precision, false);
   backRotate();
   int hb[2];
   queueBoundaryCondition(hb); // new in-values, in place
@@ -142,219 +147,345 @@ void Grain::nkMap(double *gu, int precision) {
   ++nkMaps_;
 }
 
-NewtonReport Grain::newtonKrylov(int maxNewton, int krylovDim, double tol, int jvpPrecision) {
+// ---------------------------------------------------------------------------
+// The solve, one map evaluation at a time.  Every point of the algorithm
+// that needs G(x) ends a phase: next() names the point (and writes it into
+// the grain's state), the caller evaluates the map, advance() continues.  A
+// single grain drives it with nkMap; a batch evaluates the maps of all its
+// solving grains with one sweep launch per precision.
+// ---------------------------------------------------------------------------
+NewtonKrylov::NewtonKrylov(Grain &g, int maxNewton, int krylovDim, double tol,
+                           int jvpPrecision)
+    : g_(g), maxNewton_(maxNewton), m_(krylovDim), tol_(tol), jvp_(jvpPrecision) {
   if (jvpPrecision != PREC_FP64 && jvpPrecision != PREC_MIXED)
     throw std::runtime_error("Jacobian-vector products run in fp64 or mixed precision");
-  const bool mixedJvp = jvpPrecision == PREC_MIXED;
-  if (s_->keldysh) throw std::runtime_error("a spectroscopy context only runs the LDOS iteration");
+  if (g.s_->keldysh) throw std::runtime_error("a spectroscopy context only runs the LDOS iteration");
   if (krylovDim < 1 || krylovDim > kMaxTerms - 1)
     throw std::runtime_error("Krylov dimension must be in [1, " + std::to_string(kMaxTerms - 1) +
                              "]");
-  const std::vector<NkBlock> blocks = nkBlocks();
-  const size_t L = nkLength();
-  const long long nL = static_cast<long long>(L);
+  if (maxNewton < 0) throw std::runtime_error("max_newton must be >= 0");
+  blocks_ = g.nkBlocks();
+  const size_t L = g.nkLength();
+  nL_ = static_cast<long long>(L);
   // workspace: u, G(u), F, trial, w, the step d, the linear residual r,
   // G(u) in the products' precision, and the Krylov basis
   const int nv = 8 + krylovDim + 1;
-  if (nkVecs_.size() != size_t(nv) || nkLen_ != L) {
-    for (double *p : nkVecs_) devFree(p);
-    nkVecs_.clear();
-    for (int i = 0; i < nv; ++i) nkVecs_.push_back(devAlloc<double>(L));
-    nkLen_ = L;
+  if (g.nkVecs_.size() != size_t(nv) || g.nkLen_ != L) {
+    for (double *p : g.nkVecs_) devFree(p);
+    g.nkVecs_.clear();
+    for (int i = 0; i < nv; ++i) g.nkVecs_.push_back(devAlloc<double>(L));
+    g.nkLen_ = L;
   }
-  double *u = nkVecs_[0], *gu = nkVecs_[1], *f = nkVecs_[2], *trial = nkVecs_[3],
-         *w = nkVecs_[4], *dstep = nkVecs_[5], *rlin = nkVecs_[6], *guJ = nkVecs_[7];
-  std::vector<double *> V(nkVecs_.begin() + 8, nkVecs_.end());
-  nkMaps_ = 0;
-  NewtonReport rep;
-
+  u_ = g.nkVecs_[0];
+  gu_ = g.nkVecs_[1];
+  f_ = g.nkVecs_[2];
+  trial_ = g.nkVecs_[3];
+  w_ = g.nkVecs_[4];
+  dstep_ = g.nkVecs_[5];
+  rlin_ = g.nkVecs_[6];
+  guJ_ = g.nkVecs_[7];
+  V_.assign(g.nkVecs_.begin() + 8, g.nkVecs_.end());
+  g.nkMaps_ = 0;
   // block weights 1 / |u_b|^2 of the starting state (blocks at zero: 1 / size)
-  ensureBoundaryInit();
-  nkGet(u);
-  std::vector<double> wt(blocks.size(), 1.0);
-  for (size_t b = 0; b < blocks.size(); ++b) {
-    const double uu = multiDot(u + blocks[b].offset, {u + blocks[b].offset},
-                               static_cast<long long>(blocks[b].length))[0];
-    wt[b] = uu > 0.0 ? 1.0 / uu : 1.0 / static_cast<double>(blocks[b].length);
+  g.ensureBoundaryInit();
+  g.nkGet(u_);
+  wt_.assign(blocks_.size(), 1.0);
+  for (size_t b = 0; b < blocks_.size(); ++b) {
+    const double uu = g.multiDot(u_ + blocks_[b].offset, {u_ + blocks_[b].offset},
+                                 static_cast<long long>(blocks_[b].length))[0];
+    wt_[b] = uu > 0.0 ? 1.0 / uu : 1.0 / static_cast<double>(blocks_[b].length);
   }
-  // weighted dots of a with every vector of bs
-  // (the dots of every block with one read-back)
-  auto wdots = [&](const double *a, const std::vector<double *> &bs) {
-    std::vector<DotGroup> groups;
-    for (size_t b = 0; b < blocks.size(); ++b) {
-      std::vector<double *> part(bs.size());
-      for (size_t i = 0; i < bs.size(); ++i) part[i] = bs[i] + blocks[b].offset;
-      groups.push_back({a + blocks[b].offset, part, static_cast<long long>(blocks[b].length)});
-    }
-    const std::vector<std::vector<double>> d = multiDotGroups(groups);
-    std::vector<double> out(bs.size(), 0.0);
-    for (size_t b = 0; b < blocks.size(); ++b)
-      for (size_t i = 0; i < bs.size(); ++i) out[i] += wt[b] * d[b][i];
-    return out;
-  };
-  auto wnorm = [&](double *a) { return std::sqrt(wdots(a, {a})[0]); };
-  auto combine = [&](double *y, const double *base, double a,
-                     const std::vector<const double *> &vs, const std::vector<double> &cs) {
-    if (vs.size() > size_t(kMaxTerms)) throw std::runtime_error("too many terms");
-    Terms t{};
-    for (size_t q = 0; q < vs.size(); ++q) {
-      t.v[q] = vs[q];
-      t.c[q] = cs[q];
-    }
-    combineKernel<<<blocks256(nL), 256, 0, stream_>>>(y, base, a, t, static_cast<int>(vs.size()),
-                                                      nL);
-    SC_NK_HIP(hipGetLastError());
-  };
-  // F(x) into fout given x: writes G(x) into gout
-  auto residual = [&](const double *x, double *gout, double *fout) {
-    nkSet(x);
-    nkMap(gout);
-    combine(fout, gout, 1.0, {x}, {-1.0});
-    return wnorm(fout);
-  };
+  phase_ = INIT;
+  want(u_, gu_, PREC_FP64);
+}
 
-  const int m = krylovDim;
-  double fn = residual(u, gu, f);
-  double fnPrev = fn;
-  rep.history.push_back(fn);
-  double eta = 0.1;
-  for (int it = 0; it < maxNewton && fn >= tol; ++it) {
-    // restarted GMRES(m) on J d = -F from d = 0
-    const double unorm = wnorm(u);
-    const double e = (mixedJvp ? 3e-4 : 1e-7) * (1.0 + unorm);
-    // the base point of the differences, in the products' precision
-    const double *base = gu;
-    if (mixedJvp) {
-      nkSet(u);
-      nkMap(guJ, PREC_MIXED);
-      base = guJ;
-    }
-    combine(dstep, nullptr, 0.0, {}, {});
-    combine(rlin, nullptr, 0.0, {f}, {-1.0});
-    double rn = fn;
-    for (int cycle = 0; cycle < kRestarts; ++cycle) {
-      const double beta = rn;
-      combine(V[0], nullptr, 0.0, {rlin}, {1.0 / beta});
-      // H: Hessenberg matrix as built (for the restart residual), R: with
-      // the Givens rotations applied
-      std::vector<double> H(size_t(m + 1) * m, 0.0), R(size_t(m + 1) * m, 0.0),
-          g(size_t(m + 1), 0.0), cs(size_t(m), 0.0), sn(size_t(m), 0.0);
-      g[0] = beta;
-      int k = 0;
-      bool breakdown = false;
-      for (; k < m; ++k) {
-        combine(trial, u, 1.0, {V[size_t(k)]}, {e});
-        nkSet(trial);
-        nkMap(w, jvpPrecision);
-        combine(w, w, 1.0 / e, {base, V[size_t(k)]}, {-1.0 / e, -1.0});
-        // classical Gram-Schmidt with the norm of w read back with the
-        // projections: |w - sum d_i v_i|^2 = |w|^2 - sum d_i^2 for the
-        // orthonormal basis.  A second pass only when the first removed
-        // more than half of |w|^2 (Kahan-Parlett: then the difference has
-        // lost digits and the projection must be repeated).
-        std::vector<double *> basis(V.begin(), V.begin() + k + 1);
-        std::vector<const double *> cbasis(basis.begin(), basis.end());
-        std::vector<double *> withW(basis);
-        withW.push_back(w);
-        std::vector<double> h(size_t(k) + 1, 0.0);
-        double hn = 0.0;
-        for (int pass = 0; pass < 2; ++pass) {
-          const std::vector<double> d = wdots(w, withW);
-          const double ww = d[size_t(k) + 1];
-          double removed = 0.0;
-          std::vector<double> neg(size_t(k) + 1);
-          for (size_t i = 0; i <= size_t(k); ++i) {
-            h[i] += d[i];
-            neg[i] = -d[i];
-            removed += d[i] * d[i];
-          }
-          combine(w, w, 1.0, cbasis, neg);
-          const double rem = ww - removed;
-          hn = std::sqrt(std::max(rem, 0.0));
-          if (rem > 0.5 * ww) break;
-        }
-        for (int i = 0; i <= k; ++i) H[size_t(i) * m + k] = R[size_t(i) * m + k] = h[size_t(i)];
-        H[size_t(k + 1) * m + k] = R[size_t(k + 1) * m + k] = hn;
-        for (int i = 0; i < k; ++i) {
-          const double a = R[size_t(i) * m + k], b = R[size_t(i + 1) * m + k];
-          R[size_t(i) * m + k] = cs[size_t(i)] * a + sn[size_t(i)] * b;
-          R[size_t(i + 1) * m + k] = -sn[size_t(i)] * a + cs[size_t(i)] * b;
-        }
-        const double a = R[size_t(k) * m + k], b = R[size_t(k + 1) * m + k];
-        const double r = std::hypot(a, b);
-        cs[size_t(k)] = r > 0.0 ? a / r : 1.0;
-        sn[size_t(k)] = r > 0.0 ? b / r : 0.0;
-        R[size_t(k) * m + k] = r;
-        R[size_t(k + 1) * m + k] = 0.0;
-        g[size_t(k + 1)] = -sn[size_t(k)] * g[size_t(k)];
-        g[size_t(k)] = cs[size_t(k)] * g[size_t(k)];
-        breakdown = !(hn > 0.0);
-        if (!breakdown) combine(V[size_t(k + 1)], nullptr, 0.0, {w}, {1.0 / hn});
-        if (std::fabs(g[size_t(k + 1)]) <= eta * fn || breakdown) {
-          ++k;
-          break;
-        }
-      }
-      // back substitution: R y = g on the first k rows
-      std::vector<double> y(size_t(k), 0.0);
-      for (int i = k - 1; i >= 0; --i) {
-        double acc = g[size_t(i)];
-        for (int q = i + 1; q < k; ++q) acc -= R[size_t(i) * m + q] * y[size_t(q)];
-        y[size_t(i)] = R[size_t(i) * m + i] != 0.0 ? acc / R[size_t(i) * m + i] : 0.0;
-      }
-      rep.krylovIterations += k;
-      std::vector<const double *> vk(V.begin(), V.begin() + k);
-      combine(dstep, dstep, 1.0, vk, y);
-      rn = std::fabs(g[size_t(k)]);
-      if (rn <= eta * fn || breakdown || cycle + 1 == kRestarts) break;
-      // the linear residual -F - J d = V_(k+1) (beta e1 - H y)
-      std::vector<double> c(size_t(k) + 1, 0.0);
-      std::vector<const double *> vk1(V.begin(), V.begin() + k + 1);
-      for (int i = 0; i <= k; ++i) {
-        double acc = i == 0 ? beta : 0.0;
-        for (int q = 0; q < k; ++q) acc -= H[size_t(i) * m + q] * y[size_t(q)];
-        c[size_t(i)] = acc;
-      }
-      combine(rlin, nullptr, 0.0, vk1, c);
-      rn = wnorm(rlin);
-    }
-    // backtracking line search on |F| along d
-    double lambda = 1.0, fnew = 0.0;
-    bool accepted = false;
-    for (int ls = 0; ls < 6; ++ls) {
-      combine(trial, u, 1.0, {dstep}, {lambda});
-      fnew = residual(trial, w, rlin);
-      if (fnew < (1.0 - 1e-4 * lambda) * fn) {
-        accepted = true;
-        break;
-      }
-      lambda *= 0.5;
-    }
-    if (!accepted) break; // no descent along the Newton direction
-    std::swap(u, trial);
-    std::swap(gu, w);
-    std::swap(f, rlin);
-    fnPrev = fn;
-    fn = fnew;
-    rep.history.push_back(fn);
-    ++rep.newtonIterations;
-    // Eisenstat-Walker forcing term (choice 2)
-    eta = std::clamp(0.9 * (fn / fnPrev) * (fn / fnPrev), 1e-4, 0.3);
-    nkVecs_[0] = u;
-    nkVecs_[1] = gu;
-    nkVecs_[2] = f;
-    nkVecs_[3] = trial;
-    nkVecs_[4] = w;
-    nkVecs_[6] = rlin;
+std::vector<double> NewtonKrylov::wdots(const double *a, const std::vector<double *> &bs) {
+  // weighted dots of a with every vector of bs (the dots of every block with
+  // one read-back)
+  std::vector<Grain::DotGroup> groups;
+  for (size_t b = 0; b < blocks_.size(); ++b) {
+    std::vector<double *> part(bs.size());
+    for (size_t i = 0; i < bs.size(); ++i) part[i] = bs[i] + blocks_[b].offset;
+    groups.push_back({a + blocks_[b].offset, part, static_cast<long long>(blocks_[b].length)});
   }
-  nkSet(u);
-  SC_NK_HIP(hipStreamSynchronize(stream_));
-  rep.residual = fn;
-  rep.converged = fn < tol;
-  rep.mapEvaluations = nkMaps_;
+  const std::vector<std::vector<double>> d = g_.multiDotGroups(groups);
+  std::vector<double> out(bs.size(), 0.0);
+  for (size_t b = 0; b < blocks_.size(); ++b)
+    for (size_t i = 0; i < bs.size(); ++i) out[i] += wt_[b] * d[b][i];
+  return out;
+}
+
+double NewtonKrylov::wnorm(double *a) { return std::sqrt(wdots(a, {a})[0]); }
+
+void NewtonKrylov::combine(double *y, const double *base, double a,
+                           const std::vector<const double *> &vs, const std::vector<double> &cs) {
+  if (vs.size() > size_t(kMaxTerms)) throw std::runtime_error("too many terms");
+  Terms t{};
+  for (size_t q = 0; q < vs.size(); ++q) {
+    t.v[q] = vs[q];
+    t.c[q] = cs[q];
+  }
+  combineKernel<<<blocks256(nL_), 256, 0, g_.stream_>>>(y, base, a, t,
+                                                         static_cast<int>(vs.size()), nL_);
+  SC_NK_HIP(hipGetLastError());
+}
+
+void NewtonKrylov::want(const double *x, double *out, int precision) {
+  x_ = x;
+  out_ = out;
+  prec_ = precision;
+}
+
+bool NewtonKrylov::next(int &precision) {
+  if (phase_ == DONE) return false;
+  g_.nkSet(x_);
+  g_.nkMapBegin();
+  precision = prec_;
+  return true;
+}
+
+void NewtonKrylov::startNewton() {
+  // restarted GMRES(m) on J d = -F from d = 0
+  const double unorm = wnorm(u_);
+  e_ = (jvp_ == PREC_MIXED ? 3e-4 : 1e-7) * (1.0 + unorm);
+  combine(dstep_, nullptr, 0.0, {}, {});
+  combine(rlin_, nullptr, 0.0, {f_}, {-1.0});
+  rn_ = fn_;
+  cycle_ = 0;
+  if (jvp_ == PREC_MIXED) {
+    // the base point of the differences, in the products' precision
+    phase_ = BASE;
+    want(u_, guJ_, PREC_MIXED);
+  } else {
+    startCycle();
+  }
+}
+
+void NewtonKrylov::startCycle() {
+  beta_ = rn_;
+  combine(V_[0], nullptr, 0.0, {rlin_}, {1.0 / beta_});
+  // H: Hessenberg matrix as built (for the restart residual), R: with the
+  // Givens rotations applied
+  const size_t m = size_t(m_);
+  H_.assign((m + 1) * m, 0.0);
+  R_.assign((m + 1) * m, 0.0);
+  gv_.assign(m + 1, 0.0);
+  cs_.assign(m, 0.0);
+  sn_.assign(m, 0.0);
+  gv_[0] = beta_;
+  k_ = 0;
+  breakdown_ = false;
+  startProduct();
+}
+
+void NewtonKrylov::startProduct() {
+  combine(trial_, u_, 1.0, {V_[size_t(k_)]}, {e_});
+  phase_ = KRYLOV;
+  want(trial_, w_, jvp_);
+}
+
+void NewtonKrylov::productDone() {
+  const int m = m_, k = k_;
+  const double *base = jvp_ == PREC_MIXED ? guJ_ : gu_;
+  combine(w_, w_, 1.0 / e_, {base, V_[size_t(k)]}, {-1.0 / e_, -1.0});
+  // classical Gram-Schmidt with the norm of w read back with the
+  // projections: |w - sum d_i v_i|^2 = |w|^2 - sum d_i^2 for the orthonormal
+  // basis.  A second pass only when the first removed more than half of
+  // |w|^2 (Kahan-Parlett: then the difference has lost digits and the
+  // projection must be repeated).
+  std::vector<double *> basis(V_.begin(), V_.begin() + k + 1);
+  std::vector<const double *> cbasis(basis.begin(), basis.end());
+  std::vector<double *> withW(basis);
+  withW.push_back(w_);
+  std::vector<double> h(size_t(k) + 1, 0.0);
+  double hn = 0.0;
+  for (int pass = 0; pass < 2; ++pass) {
+    const std::vector<double> d = wdots(w_, withW);
+    const double ww = d[size_t(k) + 1];
+    double removed = 0.0;
+    std::vector<double> neg(size_t(k) + 1);
+    for (size_t i = 0; i <= size_t(k); ++i) {
+      h[i] += d[i];
+      neg[i] = -d[i];
+      removed += d[i] * d[i];
+    }
+    combine(w_, w_, 1.0, cbasis, neg);
+    const double rem = ww - removed;
+    hn = std::sqrt(std::max(rem, 0.0));
+    if (rem > 0.5 * ww) break;
+  }
+  for (int i = 0; i <= k; ++i) H_[size_t(i) * m + k] = R_[size_t(i) * m + k] = h[size_t(i)];
+  H_[size_t(k + 1) * m + k] = R_[size_t(k + 1) * m + k] = hn;
+  for (int i = 0; i < k; ++i) {
+    const double a = R_[size_t(i) * m + k], b = R_[size_t(i + 1) * m + k];
+    R_[size_t(i) * m + k] = cs_[size_t(i)] * a + sn_[size_t(i)] * b;
+    R_[size_t(i + 1) * m + k] = -sn_[size_t(i)] * a + cs_[size_t(i)] * b;
+  }
+  const double a = R_[size_t(k) * m + k], b = R_[size_t(k + 1) * m + k];
+  const double r = std::hypot(a, b);
+  cs_[size_t(k)] = r > 0.0 ? a / r : 1.0;
+  sn_[size_t(k)] = r > 0.0 ? b / r : 0.0;
+  R_[size_t(k) * m + k] = r;
+  R_[size_t(k + 1) * m + k] = 0.0;
+  gv_[size_t(k + 1)] = -sn_[size_t(k)] * gv_[size_t(k)];
+  gv_[size_t(k)] = cs_[size_t(k)] * gv_[size_t(k)];
+  breakdown_ = !(hn > 0.0);
+  if (!breakdown_) combine(V_[size_t(k + 1)], nullptr, 0.0, {w_}, {1.0 / hn});
+  ++k_;
+  if (std::fabs(gv_[size_t(k + 1)]) <= eta_ * fn_ || breakdown_ || k_ == m) endCycle();
+  else startProduct();
+}
+
+void NewtonKrylov::endCycle() {
+  const int m = m_, k = k_;
+  // back substitution: R y = g on the first k rows
+  std::vector<double> y(size_t(k), 0.0);
+  for (int i = k - 1; i >= 0; --i) {
+    double acc = gv_[size_t(i)];
+    for (int q = i + 1; q < k; ++q) acc -= R_[size_t(i) * m + q] * y[size_t(q)];
+    y[size_t(i)] = R_[size_t(i) * m + i] != 0.0 ? acc / R_[size_t(i) * m + i] : 0.0;
+  }
+  rep_.krylovIterations += k;
+  std::vector<const double *> vk(V_.begin(), V_.begin() + k);
+  combine(dstep_, dstep_, 1.0, vk, y);
+  rn_ = std::fabs(gv_[size_t(k)]);
+  if (rn_ <= eta_ * fn_ || breakdown_ || cycle_ + 1 == kRestarts) {
+    startLineSearch();
+    return;
+  }
+  // the linear residual -F - J d = V_(k+1) (beta e1 - H y)
+  std::vector<double> c(size_t(k) + 1, 0.0);
+  std::vector<const double *> vk1(V_.begin(), V_.begin() + k + 1);
+  for (int i = 0; i <= k; ++i) {
+    double acc = i == 0 ? beta_ : 0.0;
+    for (int q = 0; q < k; ++q) acc -= H_[size_t(i) * m + q] * y[size_t(q)];
+    c[size_t(i)] = acc;
+  }
+  combine(rlin_, nullptr, 0.0, vk1, c);
+  rn_ = wnorm(rlin_);
+  ++cycle_;
+  startCycle();
+}
+
+void NewtonKrylov::startLineSearch() {
+  // backtracking line search on |F| along d
+  lambda_ = 1.0;
+  ls_ = 0;
+  combine(trial_, u_, 1.0, {dstep_}, {lambda_});
+  phase_ = LINE;
+  want(trial_, w_, PREC_FP64);
+}
+
+void NewtonKrylov::lineDone() {
+  combine(rlin_, w_, 1.0, {trial_}, {-1.0});
+  const double fnew = wnorm(rlin_);
+  if (fnew < (1.0 - 1e-4 * lambda_) * fn_) {
+    std::swap(u_, trial_);
+    std::swap(gu_, w_);
+    std::swap(f_, rlin_);
+    const double fnPrev = fn_;
+    fn_ = fnew;
+    rep_.history.push_back(fn_);
+    ++rep_.newtonIterations;
+    // Eisenstat-Walker forcing term (choice 2)
+    eta_ = std::clamp(0.9 * (fn_ / fnPrev) * (fn_ / fnPrev), 1e-4, 0.3);
+    if (fn_ < tol_ || rep_.newtonIterations >= maxNewton_) phase_ = DONE;
+    else startNewton();
+    return;
+  }
+  lambda_ *= 0.5;
+  if (++ls_ == 6) {
+    phase_ = DONE; // no descent along the Newton direction
+    return;
+  }
+  combine(trial_, u_, 1.0, {dstep_}, {lambda_});
+  want(trial_, w_, PREC_FP64);
+}
+
+void NewtonKrylov::advance() {
+  switch (phase_) {
+  case INIT:
+    combine(f_, gu_, 1.0, {u_}, {-1.0});
+    fn_ = wnorm(f_);
+    rep_.history.push_back(fn_);
+    if (fn_ < tol_ || maxNewton_ == 0) phase_ = DONE;
+    else startNewton();
+    break;
+  case BASE:
+    startCycle();
+    break;
+  case KRYLOV:
+    productDone();
+    break;
+  case LINE:
+    lineDone();
+    break;
+  case DONE:
+    break;
+  }
+}
+
+NewtonReport NewtonKrylov::finish() {
+  // the workspace pointers were permuted by the accepted steps: hand the
+  // pool back, then leave the last accepted u in the state
+  g_.nkVecs_[0] = u_;
+  g_.nkVecs_[1] = gu_;
+  g_.nkVecs_[2] = f_;
+  g_.nkVecs_[3] = trial_;
+  g_.nkVecs_[4] = w_;
+  g_.nkVecs_[6] = rlin_;
+  g_.nkSet(u_);
+  SC_NK_HIP(hipStreamSynchronize(g_.stream_));
+  rep_.residual = fn_;
+  rep_.converged = fn_ < tol_;
+  rep_.mapEvaluations = g_.nkMaps_;
   // The iterate now lives in the state; start the accelerators afresh.
-  resetAccelerator();
-  return rep;
+  g_.resetAccelerator();
+  return rep_;
+}
+
+std::vector<NewtonReport> Batch::newtonKrylov(int maxNewton, int krylovDim, double tol,
+                                              int jvpPrecision) {
+  // Every grain runs its own solve (its own Krylov bases, Hessenberg
+  // matrices, line search and stopping), in lockstep: each round every
+  // solve that still runs names the state it needs G of, one sweep launch
+  // per precision covers all of them, each grain finishes its map on its
+  // lane, and the solves advance.  Grains whose solve has ended leave the
+  // launches, so the last rounds sweep only the slow grains.
+  const size_t n = grains_.size();
+  std::vector<std::unique_ptr<NewtonKrylov>> nk(n);
+  for (size_t g = 0; g < n; ++g)
+    nk[g] = std::make_unique<NewtonKrylov>(*grains_[g], maxNewton, krylovDim, tol, jvpPrecision);
+  for (;;) {
+    std::vector<int> group[2], all;
+    for (size_t g = 0; g < n; ++g) {
+      int precision = PREC_FP64;
+      if (!nk[g]->next(precision)) continue;
+      group[precision].push_back(static_cast<int>(g));
+      all.push_back(static_cast<int>(g));
+    }
+    if (all.empty()) break;
+    for (int p : {PREC_FP64, PREC_MIXED})
+      if (!group[p].empty()) sweep(p, false, group[p]);
+    for (int g : all) grains_[size_t(g)]->nkMapEnd(nk[size_t(g)]->out());
+    joinLanes();
+    SC_NK_HIP(hipStreamSynchronize(stream_));
+    for (int g : all) nk[size_t(g)]->advance();
+  }
+  std::vector<NewtonReport> reps;
+  for (size_t g = 0; g < n; ++g) reps.push_back(nk[g]->finish());
+  return reps;
+}
+
+NewtonReport Grain::newtonKrylov(int maxNewton, int krylovDim, double tol, int jvpPrecision) {
+  NewtonKrylov nk(*this, maxNewton, krylovDim, tol, jvpPrecision);
+  int precision = PREC_FP64;
+  while (nk.next(precision)) {
+    sweep(precision, false);
+    nkMapEnd(nk.out());
+    nk.advance();
+  }
+  return nk.finish();
 }
 
 } // namespace scamd

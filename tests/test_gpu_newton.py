@@ -143,3 +143,36 @@ def test_screened_vector_potential_update_keeps_the_fixed_point(product, acc):
     with open(os.path.join(GOLDEN, "known_answers.json")) as f:
         ka = json.load(f)["accelerator_free_energy_per_area"]
     assert abs(r1["free_energy_per_area"] - ka["value"]) < ka["tolerance_rel"] * abs(ka["value"])
+
+
+@pytest.mark.parametrize("jvp", ["fp64", "mixed"])
+def test_batched_newton_krylov_equals_each_grain_alone(product, jvp):
+    # sc_batch_newton_krylov: every grain's own solve, the maps of all of
+    # them in shared sweep launches; each grain must end where its solve
+    # alone ends (the batch sweep and the single-grain sweep agree to the
+    # rounding of their energy sums)
+    cfgs = [cases.accelerator_case("picard"),
+            cases.base(temperature=0.3, angles=16, ppxi=3.0,
+                       order_parameter={"dx2-y2": {"critical_temperature": 1.0}},
+                       rotation=0.125, components=[cases.regular_polygon_unit(6)]),
+            cases.base(temperature=0.4, flux=0.5, lam=6.0, angles=16, ppxi=3.0, width=8.0)]
+    b = product.Batch(cfgs)
+    starts = [b.grain(i).order_parameter(0) for i in range(len(cfgs))]
+    reps = b.newton_krylov(max_newton=30, krylov_dim=20, tol=1e-10, jvp_precision=jvp)
+    for i, cfg in enumerate(cfgs):
+        s = product.Simulation(cfg)
+        for c, d in enumerate(starts[i]):
+            s.set_order_parameter(c, d)
+        rep = s.newton_krylov(max_newton=30, krylov_dim=20, tol=1e-10, jvp_precision=jvp)
+        print("grain %d: batch %s | alone %s" % (
+            i, {k: v for k, v in reps[i].items() if k != "history"},
+            {k: v for k, v in rep.items() if k != "history"}))
+        assert reps[i]["converged"] and rep["converged"]
+        assert rel(b.grain(i).order_parameter(0), s.order_parameter(0)) < 1e-8
+        # a plain iteration from the batch's answer leaves it in place
+        g = b.grain(i)
+        g.compute("fp64")
+        r = g.results()
+        for k in ("residual_order_parameter", "residual_boundary"):
+            assert r[k] < 1e-8, (i, k, r)
+        s.close()
