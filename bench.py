@@ -363,32 +363,39 @@ def batch_convergence(sa, workloads, args):
             res["mixed_refine"]["converged_grains"] == len(cfgs):
         res["speedup_mixed_refine"] = res["fp64"]["wall_s"] / res["mixed_refine"]["wall_s"]
     if not args.no_newton:
-        # batched Newton-Krylov: FP32 iterations of the whole batch, every
-        # grain's Newton-Krylov solve with FP32 products in shared sweep
+        # plain iterations (FP32 stepping, then FP64 per grain) until most
+        # grains have converged, then every grain's Newton-Krylov solve with
+        # FP32 products on the grains still iterating, in shared sweep
         # launches, then FP64 iterations until the reference's test passes
         b = sa.Batch(cfgs)
         b.autotune("fp64")
         b.autotune("mixed")
         t = time.perf_counter()
-        for _ in range(args.batch_newton_warm):
-            b.compute("mixed")
-        reps = b.newton_krylov(max_newton=12, krylov_dim=20, tol=1e-11, jvp_precision="mixed")
+        conv0 = b.run(0, 0, args.batch_newton_warm, "mixed_refine")
+        left = [i for i, c in enumerate(conv0) if not c]
+        reps = []
+        if left:
+            reps = b.newton_krylov(max_newton=12, krylov_dim=20, tol=1e-10,
+                                   jvp_precision="mixed", grains=left)
         conv = b.run(0, 0, 30, "fp64")
         wall = time.perf_counter() - t
-        its = [b.grain(i).results()["iterations"] for i in range(len(b))]
+        its = [b.grain(i).results()["iterations"] for i in left]
         res["newton_krylov_mixed"] = {
             "wall_s": round(wall, 3), "converged_grains": int(sum(conv)),
-            "fp32_iterations": args.batch_newton_warm,
-            "newton_converged_grains": int(sum(r["converged"] for r in reps)),
-            "newton_iterations_max": int(max(r["newton_iterations"] for r in reps)),
-            "map_evaluations_max": int(max(r["map_evaluations"] for r in reps)),
-            "fp64_iterations_after_max": int(max(its))}
+            "plain_iterations": args.batch_newton_warm,
+            "converged_by_plain_iterations": int(sum(conv0)),
+            "newton_grains": len(left),
+            "newton_converged_grains": int(sum(reps[i]["converged"] for i in left)),
+            "newton_iterations_max": int(max([reps[i]["newton_iterations"] for i in left] or [0])),
+            "map_evaluations_max": int(max([reps[i]["map_evaluations"] for i in left] or [0])),
+            "fp64_iterations_after_max": int(max(its or [0]))}
         b.close()
         log("batch converge newton-krylov (FP32 products): %s" % res["newton_krylov_mixed"])
-        if res["newton_krylov_mixed"]["converged_grains"] == len(cfgs) and \
-                res["fp64"]["converged_grains"] == len(cfgs):
-            res["speedup_newton_krylov_mixed"] = (res["fp64"]["wall_s"]
-                                                  / res["newton_krylov_mixed"]["wall_s"])
+        if res["newton_krylov_mixed"]["converged_grains"] == len(cfgs):
+            base = res["fp64"]
+            key = "speedup_newton_krylov_mixed" if base["converged_grains"] == len(cfgs) \
+                else "speedup_newton_krylov_mixed_lower_bound"
+            res[key] = base["wall_s"] / res["newton_krylov_mixed"]["wall_s"]
     return res
 
 
@@ -513,11 +520,11 @@ def main():
     ap.add_argument("--octagon-converge-cap", type=int, default=300,
                     help="iteration cap of each time-to-1e-8 method on the octagon")
     ap.add_argument("--no-octagon-converge", action="store_true")
-    ap.add_argument("--batch-converge-cap", type=int, default=400,
+    ap.add_argument("--batch-converge-cap", type=int, default=1000,
                     help="iteration cap of the parameter sweep's batched solve to 1e-8")
     ap.add_argument("--no-batch-converge", action="store_true")
-    ap.add_argument("--batch-newton-warm", type=int, default=20,
-                    help="FP32 batch iterations before the batched Newton-Krylov solve")
+    ap.add_argument("--batch-newton-warm", type=int, default=150,
+                    help="plain batch iterations before the batched Newton-Krylov solve")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "

@@ -357,10 +357,12 @@ int sc_batch_run(sc_batch *batch, int num_burnin, int num_min, int num_max,
  * every round evaluates the maps of all grains still solving with one sweep
  * launch per precision (no counterpart in the reference, whose parameter
  * sweeps run one process per point, examples/parameter_sweeps/*.sh).
+ * grains (one flag per grain, or null for all) selects the grains to
+ * solve; the others are left as they are and their reports zeroed.
  * reports holds one sc_newton_report per grain (or is null).  On return
- * every grain's state holds its last accepted u. */
+ * every solved grain's state holds its last accepted u. */
 int sc_batch_newton_krylov(sc_batch *batch, int max_newton, int krylov_dim, double tol,
-                           int jvp_precision, sc_newton_report *reports);
+                           int jvp_precision, const int *grains, sc_newton_report *reports);
 /* sc_set_graphs for every grain of the batch. */
 int sc_batch_set_graphs(sc_batch *batch, int enable);
 int sc_batch_set_sweep_shape(sc_batch *batch, int precision, int energies_per_lane,

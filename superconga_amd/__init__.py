@@ -152,7 +152,8 @@ def lib():
                                           ctypes.c_int, ctypes.POINTER(NewtonReport)]
         L.sc_set_graphs.argtypes = [vp, ctypes.c_int]
         L.sc_batch_newton_krylov.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
-                                             ctypes.c_int, ctypes.POINTER(NewtonReport)]
+                                             ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                             ctypes.POINTER(NewtonReport)]
         L.sc_batch_set_graphs.argtypes = [vp, ctypes.c_int]
         L.sc_free_energy_gradient.argtypes = [vp, _DP, _DP]
         L.sc_ldos_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(vp)]
@@ -510,11 +511,17 @@ class Batch:
         after the sweep of every grain."""
         _check(lib().sc_batch_set_graphs(self._h, 1 if enable else 0))
 
-    def newton_krylov(self, max_newton=20, krylov_dim=20, tol=1e-9, jvp_precision="fp64"):
-        """Newton-Krylov on every grain at once (Simulation.newton_krylov per
+    def newton_krylov(self, max_newton=20, krylov_dim=20, tol=1e-9, jvp_precision="fp64",
+                      grains=None):
+        """Newton-Krylov on many grains at once (Simulation.newton_krylov per
         grain, the maps of all solving grains in one sweep launch per
-        precision); one report dict per grain."""
+        precision); `grains` (indices, default all) selects the grains to
+        solve.  One report dict per grain (zeros for grains not solved)."""
         n = len(self._grains)
+        mask = None
+        if grains is not None:
+            sel = set(int(i) for i in grains)
+            mask = (ctypes.c_int * n)(*[1 if i in sel else 0 for i in range(n)])
         hists = [(ctypes.c_double * (max_newton + 1))() for _ in range(n)]
         reps = (NewtonReport * n)()
         for i in range(n):
@@ -522,7 +529,7 @@ class Batch:
             reps[i].history_capacity = max_newton + 1
         _check(lib().sc_batch_newton_krylov(self._h, max_newton, krylov_dim,
                                             ctypes.c_double(tol), _precision(jvp_precision),
-                                            reps))
+                                            mask, reps))
         return [{"newton_iterations": r.newton_iterations,
                  "krylov_iterations": r.krylov_iterations,
                  "map_evaluations": r.map_evaluations, "converged": bool(r.converged),

@@ -716,7 +716,7 @@ int sc_batch_run(sc_batch *b, int nb, int nmin, int nmax, int precision, int *co
   });
 }
 int sc_batch_newton_krylov(sc_batch *b, int maxNewton, int krylovDim, double tol,
-                           int jvpPrecision, sc_newton_report *reports) {
+                           int jvpPrecision, const int *grains, sc_newton_report *reports) {
   SC_CHECK_PTR(b);
   return guarded([&] {
     if (maxNewton < 0) throw std::invalid_argument("max_newton must be >= 0");
@@ -730,9 +730,11 @@ int sc_batch_newton_krylov(sc_batch *b, int maxNewton, int krylovDim, double tol
       for (int i = 0; i < n; ++i)
         if (reports[i].history_capacity > 0 && !reports[i].history)
           throw std::invalid_argument("history is null with a positive capacity");
+    std::vector<int> which;
+    if (grains) which.assign(grains, grains + n);
     const std::vector<scamd::NewtonReport> rs = b->batch->newtonKrylov(
         maxNewton, krylovDim, tol,
-        jvpPrecision == SC_PRECISION_MIXED ? scamd::PREC_MIXED : scamd::PREC_FP64);
+        jvpPrecision == SC_PRECISION_MIXED ? scamd::PREC_MIXED : scamd::PREC_FP64, which);
     if (reports)
       for (int i = 0; i < n; ++i) {
         const scamd::NewtonReport &r = rs[size_t(i)];

@@ -492,11 +492,13 @@ public:
   double lastIterationMs() const { return iterMs_; }
   long long lastTrajSteps() const { return trajSteps_; }
   std::vector<int> active() const;
-  // Newton-Krylov on every grain at once (Grain::newtonKrylov per grain),
+  // Newton-Krylov on many grains at once (Grain::newtonKrylov per grain),
   // the map evaluations of all solving grains in one sweep launch per
   // precision; one report per grain.
+  // `which` (one flag per grain, empty: all) selects the grains to solve;
+  // the others keep their state and report nothing.
   std::vector<NewtonReport> newtonKrylov(int maxNewton, int krylovDim, double tol,
-                                         int jvpPrecision);
+                                         int jvpPrecision, const std::vector<int> &which = {});
 
 private:
   struct Plan {

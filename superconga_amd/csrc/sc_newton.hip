@@ -445,7 +445,7 @@ NewtonReport NewtonKrylov::finish() {
 }
 
 std::vector<NewtonReport> Batch::newtonKrylov(int maxNewton, int krylovDim, double tol,
-                                              int jvpPrecision) {
+                                              int jvpPrecision, const std::vector<int> &which) {
   // Every grain runs its own solve (its own Krylov bases, Hessenberg
   // matrices, line search and stopping), in lockstep: each round every
   // solve that still runs names the state it needs G of, one sweep launch
@@ -453,14 +453,18 @@ std::vector<NewtonReport> Batch::newtonKrylov(int maxNewton, int krylovDim, doub
   // lane, and the solves advance.  Grains whose solve has ended leave the
   // launches, so the last rounds sweep only the slow grains.
   const size_t n = grains_.size();
+  if (!which.empty() && which.size() != n)
+    throw std::runtime_error("one selection flag per grain expected");
   std::vector<std::unique_ptr<NewtonKrylov>> nk(n);
   for (size_t g = 0; g < n; ++g)
-    nk[g] = std::make_unique<NewtonKrylov>(*grains_[g], maxNewton, krylovDim, tol, jvpPrecision);
+    if (which.empty() || which[g])
+      nk[g] = std::make_unique<NewtonKrylov>(*grains_[g], maxNewton, krylovDim, tol,
+                                             jvpPrecision);
   for (;;) {
     std::vector<int> group[2], all;
     for (size_t g = 0; g < n; ++g) {
       int precision = PREC_FP64;
-      if (!nk[g]->next(precision)) continue;
+      if (!nk[g] || !nk[g]->next(precision)) continue;
       group[precision].push_back(static_cast<int>(g));
       all.push_back(static_cast<int>(g));
     }
@@ -472,8 +476,9 @@ std::vector<NewtonReport> Batch::newtonKrylov(int maxNewton, int krylovDim, doub
     SC_NK_HIP(hipStreamSynchronize(stream_));
     for (int g : all) nk[size_t(g)]->advance();
   }
-  std::vector<NewtonReport> reps;
-  for (size_t g = 0; g < n; ++g) reps.push_back(nk[g]->finish());
+  std::vector<NewtonReport> reps(n);
+  for (size_t g = 0; g < n; ++g)
+    if (nk[g]) reps[g] = nk[g]->finish();
   return reps;
 }
 

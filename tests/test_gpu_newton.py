@@ -176,3 +176,19 @@ def test_batched_newton_krylov_equals_each_grain_alone(product, jvp):
         for k in ("residual_order_parameter", "residual_boundary"):
             assert r[k] < 1e-8, (i, k, r)
         s.close()
+
+
+def test_batched_newton_krylov_on_selected_grains(product):
+    # only the selected grains are solved; the others keep their state
+    cfgs = [cases.accelerator_case("picard"), cases.accelerator_case("congacc"),
+            cases.base(temperature=0.4, flux=0.5, lam=6.0, angles=16, ppxi=3.0, width=8.0)]
+    b = product.Batch(cfgs)
+    before = b.grain(1).order_parameter(0)
+    reps = b.newton_krylov(max_newton=30, krylov_dim=20, tol=1e-10, grains=[0, 2])
+    assert reps[0]["converged"] and reps[2]["converged"]
+    assert reps[1]["newton_iterations"] == 0 and reps[1]["map_evaluations"] == 0
+    assert np.array_equal(b.grain(1).order_parameter(0), before)
+    for i in (0, 2):
+        g = b.grain(i)
+        g.compute("fp64")
+        assert g.results()["residual_order_parameter"] < 1e-8
