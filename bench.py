@@ -345,7 +345,7 @@ def batch_convergence(sa, workloads, args):
     cfgs = workloads.parameter_sweep(args.batch_count)
     res = {"workload": "parameter_sweep", "grains": len(cfgs), "criterion": 1e-8,
            "cap_iterations": args.batch_converge_cap}
-    for prec in ("mixed_refine", "fp64"):
+    for prec in (("mixed_refine", "fp64") if args.workload == "parameter_sweep" else ("fp64",)):
         b = sa.Batch(cfgs)
         b.autotune("fp64")
         b.autotune("mixed")
@@ -359,7 +359,7 @@ def batch_convergence(sa, workloads, args):
                      "fp32_iterations_total": int(sum(fp32))}
         b.close()
         log("batch converge %s: %s" % (prec, res[prec]))
-    if res["fp64"]["converged_grains"] == len(cfgs) and \
+    if "mixed_refine" in res and res["fp64"]["converged_grains"] == len(cfgs) and \
             res["mixed_refine"]["converged_grains"] == len(cfgs):
         res["speedup_mixed_refine"] = res["fp64"]["wall_s"] / res["mixed_refine"]["wall_s"]
     if not args.no_newton:
@@ -520,7 +520,7 @@ def main():
     ap.add_argument("--octagon-converge-cap", type=int, default=300,
                     help="iteration cap of each time-to-1e-8 method on the octagon")
     ap.add_argument("--no-octagon-converge", action="store_true")
-    ap.add_argument("--batch-converge-cap", type=int, default=1000,
+    ap.add_argument("--batch-converge-cap", type=int, default=600,
                     help="iteration cap of the parameter sweep's batched solve to 1e-8")
     ap.add_argument("--no-batch-converge", action="store_true")
     ap.add_argument("--batch-newton-warm", type=int, default=150,

@@ -273,6 +273,11 @@ int sc_accelerator_step(sc_context *ctx, double *x, const double *g);
  * counterpart in the reference, which launches each stage per iteration
  * (Context.h:391-447). */
 int sc_set_graphs(sc_context *ctx, int enable);
+/* Switch the screened vector-potential update (config extension
+ * accelerator.vector_potential_screening) on with the given superfluid
+ * density (> 0) or off (0) between iterations; the accelerator restarts.
+ * The fixed points are unchanged.  No counterpart in the reference. */
+int sc_set_screening(sc_context *ctx, double superfluid_density);
 int sc_set_sweep_shape(sc_context *ctx, int precision, int energies_per_lane,
                        int waves_per_simd, int slots_per_cu);
 int sc_get_sweep_shape(sc_context *ctx, int precision, int *energies_per_lane,

@@ -151,6 +151,7 @@ def lib():
         L.sc_newton_krylov_ex.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                           ctypes.c_int, ctypes.POINTER(NewtonReport)]
         L.sc_set_graphs.argtypes = [vp, ctypes.c_int]
+        L.sc_set_screening.argtypes = [vp, ctypes.c_double]
         L.sc_batch_newton_krylov.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                              ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                              ctypes.POINTER(NewtonReport)]
@@ -381,6 +382,11 @@ class Simulation:
         """Replay the stages after the sweep from a captured hipGraph (the
         default) or launch them eagerly."""
         _check(lib().sc_set_graphs(self._h, 1 if enable else 0))
+
+    def set_screening(self, superfluid_density):
+        """The screened vector-potential update on (superfluid density > 0)
+        or off (0); the accelerator restarts."""
+        _check(lib().sc_set_screening(self._h, ctypes.c_double(superfluid_density)))
 
     def sweep_shape(self, precision="fp64"):
         e, w, s = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

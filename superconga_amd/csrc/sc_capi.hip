@@ -392,6 +392,14 @@ int sc_set_sweep_shape(sc_context *ctx, int precision, int el, int waves, int sl
     return SC_OK;
   });
 }
+int sc_set_screening(sc_context *ctx, double superfluidDensity) {
+  SC_CHECK_CTX(ctx);
+  SC_CHECK_SIM(ctx);
+  return guarded([&] {
+    ctx->grain->setScreening(superfluidDensity);
+    return SC_OK;
+  });
+}
 int sc_set_graphs(sc_context *ctx, int enable) {
   SC_CHECK_CTX(ctx);
   return guarded([&] {

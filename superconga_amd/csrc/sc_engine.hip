@@ -1599,6 +1599,26 @@ void Grain::setGraphs(bool on) {
   if (!on) dropEndGraph();
 }
 
+void Grain::setScreening(double rho) {
+  const Setup &s = *s_;
+  if (rho < 0.0) throw std::runtime_error("superfluid density must be >= 0");
+  if (rho > 0.0 && !(s.lambda > 0.0 && s.cfg.computeCurrent))
+    throw std::runtime_error("screening needs an induced vector potential (penetration depth > 0)");
+  SC_HIP(hipStreamSynchronize(stream_));
+  dropEndGraph(); // the captured stages include (or omit) the screening
+  screen_.reset();
+  screenK_ = 0.0;
+  if (rho > 0.0) {
+    screenK_ = rho / (s.lambda * s.lambda);
+    screen_ = std::make_unique<PoissonFFT>(s.N0, s.h, stream_, PoissonFFT::Screened{screenK_});
+    if (!dZero2_) {
+      dZero2_ = devAlloc<double>(2);
+      SC_HIP(hipMemsetAsync(dZero2_, 0, 2 * sizeof(double), stream_));
+    }
+  }
+  resetAccelerator();
+}
+
 void Grain::queueEndCompute() {
   // The rest of Context::compute after the sweeps: device work only, whose
   // launches never change between iterations (fixed buffers, fixed

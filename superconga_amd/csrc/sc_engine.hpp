@@ -119,6 +119,10 @@ public:
   // the reductions it was captured with keep the same offsets.  On by
   // default; off runs every launch eagerly.
   void setGraphs(bool on);
+  // The screened vector-potential update (accelerator.vector_potential_
+  // screening) switched on with superfluid density rho > 0, or off (0);
+  // restarts the accelerator's history.
+  void setScreening(double rho);
   bool graphs() const { return graphsOn_; }
   // individual stages
   // Reads the HIP-event time of the last sweep launch into the timing.

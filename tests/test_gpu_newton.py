@@ -192,3 +192,25 @@ def test_batched_newton_krylov_on_selected_grains(product):
         g = b.grain(i)
         g.compute("fp64")
         assert g.results()["residual_order_parameter"] < 1e-8
+
+
+def test_screening_switched_between_iterations(product):
+    # sc_set_screening: on mid-run and off again, the run still ends at the
+    # plain run's fixed point
+    cfg = cases.accelerator_case("picard")
+    cfg["numerics"]["convergence_criterion"] = 1e-10
+    cfg["numerics"]["norm"] = "l2"
+    plain = product.Simulation(cfg)
+    assert plain.run(0, 0, 3000, "fp64")
+    s = product.Simulation(cfg)
+    for _ in range(5):
+        s.compute("fp64")
+    s.set_screening(0.9)
+    for _ in range(20):
+        s.compute("fp64")
+    s.set_screening(0.0)
+    assert s.run(0, 0, 3000, "fp64")
+    assert rel(s.order_parameter(0), plain.order_parameter(0)) < 1e-7
+    assert rel(s.vector_potential(0), plain.vector_potential(0)) < 1e-6
+    with pytest.raises(product.SupercongaError):
+        s.set_screening(-1.0)
