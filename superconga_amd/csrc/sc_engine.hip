@@ -803,7 +803,7 @@ void Grain::discardReductions() {
 // ---------------------------------------------------------------------------
 // the iteration
 // ---------------------------------------------------------------------------
-void Grain::extendDelta() {
+void Grain::extendDelta(bool fields) {
   const Setup &s = *s_;
   const size_t n0 = size_t(s.N0) * s.N0;
   SC_HIP(hipMemcpyAsync(dExtDelta_, dDelta_, size_t(2 * s.Nc) * n0 * sizeof(double),
@@ -816,8 +816,10 @@ void Grain::extendDelta() {
     SC_HIP(hipGetLastError());
   }
   // the site fields every sweep of this iteration reads (Keldysh grains
-  // compute them per angle chunk)
-  if (!s.keldysh) computeFields(0, s.Na, static_cast<long long>(s.Na) * s.N1 * s.N1, dFields_);
+  // compute them per angle chunk; a batch computes those of all its grains
+  // in one launch)
+  if (!s.keldysh && fields)
+    computeFields(0, s.Na, static_cast<long long>(s.Na) * s.N1 * s.N1, dFields_);
 }
 
 void Grain::computeFields(int k0, int frames, long long plane, double *F) {
@@ -839,6 +841,8 @@ GrainView Grain::view() const {
   v.fields = s_->keldysh ? dLdosFields_ : dFields_;
   v.fieldPlane = (s_->keldysh ? ldosFieldFrames_ : s_->Na) * n1;
   v.fieldBase = 0;
+  v.fieldTiles = dFieldTiles_;
+  v.numFieldTiles = s_->fieldTileStart[size_t(s_->Na)];
   return v;
 }
 
@@ -1569,12 +1573,12 @@ void Grain::queueBeginCompute() {
   if (!firstCompute_) acceleratorQueue();
 }
 
-void Grain::finishBeginCompute() {
+void Grain::finishBeginCompute(bool fields) {
   // the accelerator's update once its reductions are back, then the
   // extended order parameter and the site fields
   if (!firstCompute_) acceleratorFinish();
   ensureBoundaryInit();
-  extendDelta();
+  extendDelta(fields);
 }
 
 void Grain::queueEndStages() {
@@ -1859,11 +1863,11 @@ void Grain::acceleratorStepOn(double *x, const double *g) {
   fromDevice(x, dDelta_, bytes);
 }
 
-void Grain::beginBurnin() {
+void Grain::beginBurnin(bool fields) {
   if (s_->keldysh) throw std::runtime_error("a spectroscopy context only runs the LDOS iteration");
   clearReductions();
   ensureBoundaryInit();
-  extendDelta();
+  extendDelta(fields);
 }
 
 void Grain::endBurnin() {
@@ -1963,6 +1967,7 @@ Batch::~Batch() {
   for (Plan &pl : plan_) {
     if (pl.dItems) (void)hipFree(pl.dItems);
     if (pl.dViews) (void)hipFree(pl.dViews);
+    if (pl.dFieldJobs) (void)hipFree(pl.dFieldJobs);
   }
   if (dScratch_) (void)hipFree(dScratch_);
   devFree(dCounter_);
@@ -2047,6 +2052,23 @@ void Batch::prepare(int precision, const std::vector<int> &act) {
                    [&](size_t a, size_t b) { return lens[a] > lens[b]; });
   std::vector<SweepItemDev> sorted(items.size());
   for (size_t i = 0; i < order.size(); ++i) sorted[i] = items[order[i]];
+  // the site-field jobs of the active grains (fieldsBatchKernel)
+  std::vector<int> jobs(2 * act.size() + 1, 0);
+  int tiles = 0;
+  for (size_t j = 0; j < act.size(); ++j) {
+    jobs[j] = act[j];
+    jobs[act.size() + j] = tiles;
+    tiles += grains_[size_t(act[j])]->setup().fieldTileStart[size_t(
+        grains_[size_t(act[j])]->setup().Na)];
+  }
+  jobs[2 * act.size()] = tiles;
+  if (pl.dFieldJobs) (void)hipFree(pl.dFieldJobs);
+  pl.dFieldJobs = nullptr;
+  SC_HIP(hipMalloc(&pl.dFieldJobs, jobs.size() * sizeof(int)));
+  SC_HIP(hipMemcpyAsync(pl.dFieldJobs, jobs.data(), jobs.size() * sizeof(int),
+                        hipMemcpyHostToDevice, stream_));
+  pl.numFieldJobs = static_cast<int>(act.size());
+  pl.numFieldTiles = tiles;
   pl.numItems = static_cast<int>(sorted.size());
   if (sorted.size() > pl.itemsCap || !pl.dItems) {
     if (pl.dItems) (void)hipFree(pl.dItems);
@@ -2078,11 +2100,19 @@ void Batch::prepare(int precision, const std::vector<int> &act) {
   pl.ready = true;
 }
 
-void Batch::sweep(int precision, bool burnin, const std::vector<int> &act) {
+void Batch::sweep(int precision, bool burnin, const std::vector<int> &act, bool fields) {
   // the sweep reads what the grains' stages queued on their lanes
   joinLanes();
   prepare(precision, act);
   const Plan &pl = plan_[precision];
+  if (fields && pl.numFieldTiles > 0) {
+    // the site fields of every active grain in one launch (each grain's
+    // extension of the order parameter is done on its lane)
+    fieldsBatchKernel<<<xcdGrid(pl.numFieldTiles), 256, 0, stream_>>>(
+        reinterpret_cast<const GrainView *>(pl.dViews), pl.dFieldJobs,
+        pl.dFieldJobs + pl.numFieldJobs, pl.numFieldJobs, pl.numFieldTiles);
+    SC_HIP(hipGetLastError());
+  }
   SweepArgs a;
   a.views = reinterpret_cast<const GrainView *>(pl.dViews);
   a.items = reinterpret_cast<const SweepItemDev *>(pl.dItems);
@@ -2126,9 +2156,9 @@ void Batch::computeEach(const std::vector<int> &precisionOf) {
   SC_HIP(hipEventRecord(evIt0_, stream_));
   for (int gi : act) grains_[size_t(gi)]->queueBeginCompute();
   for (int gi : act) grains_[size_t(gi)]->queueReadback();
-  for (int gi : act) grains_[size_t(gi)]->finishBeginCompute();
+  for (int gi : act) grains_[size_t(gi)]->finishBeginCompute(false);
   for (int p : {PREC_FP64, PREC_MIXED})
-    if (!group[p].empty()) sweep(p, false, group[p]);
+    if (!group[p].empty()) sweep(p, false, group[p], true);
   // every grain's stages are queued before the first read-back waits, so
   // the lanes run them side by side
   for (int gi : act) grains_[size_t(gi)]->queueEndCompute();
@@ -2150,8 +2180,8 @@ void Batch::computeEach(const std::vector<int> &precisionOf) {
 void Batch::burnin(int precision) {
   const std::vector<int> act = active();
   if (act.empty()) return;
-  for (int gi : act) grains_[size_t(gi)]->beginBurnin();
-  sweep(precision, true, act);
+  for (int gi : act) grains_[size_t(gi)]->beginBurnin(false);
+  sweep(precision, true, act, true);
   for (int gi : act) grains_[size_t(gi)]->queueEndBurnin();
   for (int gi : act) grains_[size_t(gi)]->queueReadback();
   for (int gi : act) grains_[size_t(gi)]->finishEndBurnin();

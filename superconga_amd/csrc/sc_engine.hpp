@@ -82,11 +82,13 @@ public:
   // so that a batch can queue every grain before it waits on any.
   void beginCompute();
   void queueBeginCompute();
-  void finishBeginCompute();
+  // fields = false leaves the site fields to the caller (a batch computes
+  // those of all its grains in one launch)
+  void finishBeginCompute(bool fields = true);
   void endCompute();
   void queueEndCompute();
   void finishEndCompute();
-  void beginBurnin();
+  void beginBurnin(bool fields = true);
   void endBurnin();
   void queueEndBurnin();
   void finishEndBurnin();
@@ -212,7 +214,7 @@ private:
   void ldosSweepAll(int precision, bool burnin);
   void toDevice(void *dst, const void *src, size_t bytes);
   void fromDevice(void *dst, const void *src, size_t bytes);
-  void extendDelta();
+  void extendDelta(bool fields = true);
   void ensureBoundaryInit();
   void afterSweep(bool burnin);
   void backRotate();
@@ -305,7 +307,7 @@ private:
   void nkSet(const double *u);
   // one evaluation of the map G of the state: nkMapBegin, the sweep at the
   // chosen precision (the grain's own or a batch's), nkMapEnd(G)
-  void nkMapBegin();
+  void nkMapBegin(bool fields = true);
   void nkMapEnd(double *gu);
   friend class NewtonKrylov;
   friend class Batch;
@@ -427,7 +429,7 @@ void poissonSolve(int n, double h, int numGrain, bool enforceMean,
 class NewtonKrylov {
 public:
   NewtonKrylov(Grain &g, int maxNewton, int krylovDim, double tol, int jvpPrecision);
-  bool next(int &precision);
+  bool next(int &precision, bool fields = true);
   double *out() const { return out_; }
   void advance();
   bool done() const { return phase_ == DONE; }
@@ -516,9 +518,14 @@ private:
     int cacheEl = -1;
     std::vector<std::vector<SweepItem>> itemsOf;
     std::vector<int> maxLenOf;
+    // the site-field jobs of the active grains: grain index and first tile
+    // (numJobs + 1 prefix entries), for fieldsBatchKernel
+    int *dFieldJobs = nullptr;
+    int numFieldJobs = 0, numFieldTiles = 0;
   };
   void prepare(int precision, const std::vector<int> &act);
-  void sweep(int precision, bool burnin, const std::vector<int> &act);
+  // fields: compute the site fields of the active grains (one launch) first
+  void sweep(int precision, bool burnin, const std::vector<int> &act, bool fields = false);
   // elapsed time of the last sweep launch of a precision
   float sweepTime(int precision) const;
   // the main stream waits for every lane, every lane for the main stream

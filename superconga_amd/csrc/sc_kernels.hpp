@@ -65,11 +65,8 @@ __global__ void resetCounter(int *c) { *c = 0; }
 // sweep may read there), 16 columns x 16 rows per pass, so that the writes
 // of a row of the tile are contiguous.  F is planar [3][frames][N1^2] with
 // plane stride `plane`; frame k is at (k - k0) * N1^2.
-__global__ void fieldsKernel(GrainView v, const int *tiles, int numTiles, int k0,
-                             long long plane, double *F) {
-  const long long tile = xcdBlock(numTiles);
-  if (tile < 0) return;
-  const int *tl = tiles + 4 * size_t(tile);
+__device__ __forceinline__ void fieldsTile(const GrainView &v, const int *tl, int k0,
+                                           long long plane, double *F) {
   const int k = tl[0], x = tl[1] + int(threadIdx.x & 15), y0 = tl[2], y1 = tl[3];
   const int N1 = v.N1;
   if (x >= N1) return;
@@ -82,6 +79,31 @@ __global__ void fieldsKernel(GrainView v, const int *tiles, int numTiles, int k0
     F[plane + i] = im;
     F[2 * plane + i] = es;
   }
+}
+__global__ void fieldsKernel(GrainView v, const int *tiles, int numTiles, int k0,
+                             long long plane, double *F) {
+  const long long tile = xcdBlock(numTiles);
+  if (tile < 0) return;
+  fieldsTile(v, tiles + 4 * size_t(tile), k0, plane, F);
+}
+// The site fields of several grains in one launch (a batch): grain j of the
+// job list (view grainOf[j]) owns the blocks [tileStart[j], tileStart[j+1])
+// of the numTiles in total, one tile each, every frame of the grain.
+__global__ void fieldsBatchKernel(const GrainView *views, const int *grainOf,
+                                  const int *tileStart, int numJobs, int numTiles) {
+  const long long b = xcdBlock(numTiles);
+  if (b < 0) return;
+  int lo = 0, hi = numJobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tileStart[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const GrainView &v = views[grainOf[lo]];
+  // the view's site-field buffer is the grain's own, written here and read
+  // by the sweep that follows
+  fieldsTile(v, v.fieldTiles + 4 * (b - tileStart[lo]), 0, v.fieldPlane,
+             const_cast<double *>(v.fields));
 }
 
 // One step for one energy: Matsubara energies (i omega) or, with KEL,

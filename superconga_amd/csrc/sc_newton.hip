@@ -114,13 +114,13 @@ void Grain::nkSet(const double *u) {
                            stream_));
 }
 
-void Grain::nkMapBegin() {
+void Grain::nkMapBegin(bool fields) {
   // One plain iteration of the current state without the accelerator and
   // without the compute objects' reductions: the order parameter's
   // extension and the site fields, then (the caller) the sweep, then
   // nkMapEnd.
   ensureBoundaryInit();
-  extendDelta();
+  extendDelta(fields);
 }
 
 void Grain::nkMapEnd(double *gu) {
@@ -236,10 +236,10 @@ void NewtonKrylov::want(const double *x, double *out, int precision) {
   prec_ = precision;
 }
 
-bool NewtonKrylov::next(int &precision) {
+bool NewtonKrylov::next(int &precision, bool fields) {
   if (phase_ == DONE) return false;
   g_.nkSet(x_);
-  g_.nkMapBegin();
+  g_.nkMapBegin(fields);
   precision = prec_;
   return true;
 }
@@ -464,13 +464,13 @@ std::vector<NewtonReport> Batch::newtonKrylov(int maxNewton, int krylovDim, doub
     std::vector<int> group[2], all;
     for (size_t g = 0; g < n; ++g) {
       int precision = PREC_FP64;
-      if (!nk[g] || !nk[g]->next(precision)) continue;
+      if (!nk[g] || !nk[g]->next(precision, false)) continue;
       group[precision].push_back(static_cast<int>(g));
       all.push_back(static_cast<int>(g));
     }
     if (all.empty()) break;
     for (int p : {PREC_FP64, PREC_MIXED})
-      if (!group[p].empty()) sweep(p, false, group[p]);
+      if (!group[p].empty()) sweep(p, false, group[p], true);
     for (int g : all) grains_[size_t(g)]->nkMapEnd(nk[size_t(g)]->out());
     joinLanes();
     SC_NK_HIP(hipStreamSynchronize(stream_));

@@ -36,6 +36,10 @@ struct GrainView {
   const double *fields;
   long long fieldPlane;
   int fieldBase;
+  // the tiles of swept sites the site fields cover, 4 ints each (frame,
+  // first column, first row, last row), all frames
+  const int *fieldTiles;
+  int numFieldTiles;
   int cols;   // sweep lane layout: columns per item (64 / groups)
   int groups; // energy groups per column (ceil(Ne / EL)), one lane each
 };

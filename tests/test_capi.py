@@ -172,6 +172,13 @@ def test_spectroscopy_config_errors(product):
     assert L.sc_ldos_create(sim, b"{broken", ctypes.byref(h)) != 0
     assert L.sc_ldos_compute(None, 0, 0) == 1      # SC_ERR_ARGUMENT: null context
     assert L.sc_newton_krylov(None, 1, 4, ctypes.c_double(1e-8), None) == 1
+    # the entry points of this build without reference counterpart refuse a
+    # null handle before any device work
+    assert L.sc_newton_krylov_ex(None, 1, 4, ctypes.c_double(1e-8), 1, None) == 1
+    assert L.sc_batch_newton_krylov(None, 1, 4, ctypes.c_double(1e-8), 1, None, None) == 1
+    assert L.sc_set_graphs(None, 1) == 1
+    assert L.sc_batch_set_graphs(None, 1) == 1
+    assert L.sc_set_screening(None, ctypes.c_double(0.5)) == 1
 
 
 def test_noalloc_poisson_refuses_oversized_grid(product):
