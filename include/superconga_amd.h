@@ -204,9 +204,8 @@ int sc_newton_krylov_ex(sc_context *ctx, int max_newton, int krylov_dim, double 
  * in-values are held fixed.  grad holds num_components x 2 x N0^2 values:
  * dF/dRe Delta_c, then dF/dIm Delta_c, per component in order.  The state is
  * left as it was.  No counterpart in the reference: it serves gradient
- * studies (torch.autograd in the Python frontend).  Disabled in this build
- * (returns an error): its reverse-sweep kernels have not passed their GPU
- * test. */
+ * studies (torch.autograd in the Python frontend).  Its FP64 atomic
+ * scatters make two calls agree to rounding, not bit for bit. */
 int sc_free_energy_gradient(sc_context *ctx, double *grad, double *free_energy);
 
 /* Individual stages. */

@@ -487,9 +487,13 @@ __device__ void adjointItem(const AdjointArgs &a, const SweepItemDev &it, int it
     writeDbar(it.ylo, dbA, true);
   }
 
-  // this item's share of the free-energy sum, in lane order
+  // this item's share of the free-energy sum.  The butterfly leaves the
+  // same bits in every lane (each level adds the same two operands, only
+  // swapped), so every lane stores it: a store behind a lane-0 branch here
+  // and the lane-0 fetch of the next item were threaded by the compiler
+  // into one path that let lanes 1-63 re-enter the item loop without lane 0.
   for (int m = 1; m < 64; m <<= 1) omega += __shfl_xor(omega, m);
-  if (lane == 0) a.partial[itemIndex] = omega;
+  a.partial[itemIndex] = omega;
 }
 
 template <int EL>
@@ -497,9 +501,11 @@ __global__ __launch_bounds__(64, 2) void adjointKernel(AdjointArgs a) {
   Cx<double> *scrT = a.scratch + size_t(blockIdx.x) * 2 * a.slotStride;
   Cx<double> *scrG = scrT + a.slotStride;
   for (;;) {
-    int item = 0;
-    if (threadIdx.x == 0) item = atomicAdd(a.counter, 1);
-    item = __builtin_amdgcn_readfirstlane(__shfl(item, 0));
+    // every lane takes part in the fetch (lane 0 adds one, the others
+    // zero), so no branch on the lane index surrounds it; lane 0's old
+    // value is the wavefront's item
+    const int item =
+        __builtin_amdgcn_readfirstlane(atomicAdd(a.counter, threadIdx.x == 0 ? 1 : 0));
     if (item >= a.numItems) break;
     adjointItem<EL>(a, a.items[item], item, scrT, scrG);
   }

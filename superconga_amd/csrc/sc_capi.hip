@@ -464,18 +464,11 @@ int sc_newton_krylov_ex(sc_context *ctx, int maxNewton, int krylovDim, double to
   });
 }
 
-constexpr bool kGradientEnabled = false;
-
 int sc_free_energy_gradient(sc_context *ctx, double *grad, double *freeEnergy) {
   SC_CHECK_CTX(ctx);
   SC_CHECK_SIM(ctx);
   SC_CHECK_PTR(grad);
   return guarded([&] {
-    // The reverse-sweep kernels (Grain::freeEnergyGradient) have not passed
-    // their GPU test: the entry point reports that instead of running them.
-    if (!kGradientEnabled)
-      throw std::runtime_error("sc_free_energy_gradient is disabled in this build: its "
-                               "reverse-sweep kernels have not passed their GPU test");
     const double f = ctx->grain->freeEnergyGradient(grad);
     if (freeEnergy) *freeEnergy = f;
     return SC_OK;

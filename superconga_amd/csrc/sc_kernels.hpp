@@ -440,11 +440,14 @@ template <typename T, int EL, int WAVES, bool KEL>
 __global__ __launch_bounds__(64, WAVES) void sweepKernel(SweepArgs a) {
   Cx<T> *scr = reinterpret_cast<Cx<T> *>(a.scratch) + size_t(blockIdx.x) * a.slotStride;
   for (;;) {
-    int item = 0;
-    if (threadIdx.x == 0) item = atomicAdd(a.counter, 1);
-    // uniform across the wavefront: the item and its grain view are read
-    // with scalar loads
-    item = __builtin_amdgcn_readfirstlane(__shfl(item, 0));
+    // every lane takes part in the fetch (lane 0 adds one, the others
+    // zero): no branch on the lane index surrounds it, so the compiler
+    // cannot merge it with a lane-0 branch of the item into a loop that
+    // some lanes re-enter without lane 0.  Lane 0's old value is the item,
+    // uniform across the wavefront (the item and its grain view are read
+    // with scalar loads).
+    const int item =
+        __builtin_amdgcn_readfirstlane(atomicAdd(a.counter, threadIdx.x == 0 ? 1 : 0));
     if (item >= a.numItems) break;
     sweepItem<T, EL, KEL>(a, a.items[item], scr);
   }

@@ -11,13 +11,7 @@ import pytest
 
 import cases
 
-pytestmark = [
-    pytest.mark.gpu,
-    pytest.mark.skip(reason="sc_free_energy_gradient is disabled: the first run of this file "
-                            "on the GPU did not finish its first test within 180 s and the cause "
-                            "is not found (the kernels terminate and stay in bounds in a CPU "
-                            "emulation of the wavefront)"),
-]
+pytestmark = pytest.mark.gpu
 
 TOL = 1e-10
 
@@ -122,12 +116,14 @@ def test_autograd_backward_is_the_adjoint(product):
     assert abs(float(f) - f0) <= 1e-14 * abs(f0)
     expect = 2.0 * gp + 2.0 * planes
     assert float(np.max(np.abs(x.grad.numpy() - expect))) <= 1e-14 * float(np.max(np.abs(expect)))
-    # a complex tensor through view_as_real
+    # a complex tensor through view_as_real.  Each evaluation is a new run
+    # of the adjoint, whose FP64 atomic scatters add in a different order
+    # every time: evaluations agree to rounding, not bit for bit.
     z = torch.tensor(d, dtype=torch.complex128, requires_grad=True)
     fz = free_energy(s, torch.view_as_real(z).movedim(-1, 1))
     fz.backward()
-    assert np.allclose(z.grad.numpy().real, g.real, rtol=0, atol=1e-15 * np.max(np.abs(g)))
-    assert np.allclose(z.grad.numpy().imag, g.imag, rtol=0, atol=1e-15 * np.max(np.abs(g)))
+    assert np.allclose(z.grad.numpy().real, g.real, rtol=0, atol=1e-14 * np.max(np.abs(g)))
+    assert np.allclose(z.grad.numpy().imag, g.imag, rtol=0, atol=1e-14 * np.max(np.abs(g)))
     s.close()
 
 
@@ -170,4 +166,58 @@ def test_gradient_central_differences_d_plus_is(product):
     print("d+is square: F %.10f, |grad| max %.3e, FD error %.2e of max" % (
         f0, np.max(np.abs(g)), err))
     assert err < 1e-6
+    s.close()
+
+
+def test_gradient_central_differences_full_size_d_plus_is(product):
+    # the two-component d + is square at its full size (side 30 xi0 at 20
+    # points per xi0, 128 angles, one flux quantum, penetration depth 3 xi0)
+    # two iterations in: central differences of F at 64 seeded inside sites,
+    # both components, real and imaginary parts, at steps 1e-6 and 1e-5
+    import time
+    from superconga_amd import workloads
+    cfg = workloads.dplus_is_square()
+    s = product.Simulation(cfg)
+    d0 = s.order_parameter(0)
+    for c, x in enumerate(workloads.d_plus_is_initial(d0[0])):
+        s.set_order_parameter(c, x)
+    for _ in range(2):
+        s.compute("fp64")
+    d = s.order_parameter(0)
+    t = time.perf_counter()
+    f0, g = s.free_energy_gradient()
+    t = time.perf_counter() - t
+    f1, _ = s.free_energy_gradient()
+    dom = s.domain() > 0
+    rng = np.random.default_rng(5)
+    pts = np.argwhere(dom)[rng.choice(int(dom.sum()), 64, replace=False)]
+    gmax = float(np.max(np.abs(g)))
+    err = {}
+    for step in (1e-6, 1e-5):
+        worst = 0.0
+        for (y, x) in pts:
+            for c in range(2):
+                for unit in (1.0, 1j):
+                    dd = d[c].copy()
+                    dd[y, x] += step * unit
+                    s.set_order_parameter(c, dd)
+                    fp, _ = s.free_energy_gradient()
+                    dd[y, x] -= 2 * step * unit
+                    s.set_order_parameter(c, dd)
+                    fm, _ = s.free_energy_gradient()
+                    s.set_order_parameter(c, d[c])
+                    fd = (fp - fm) / (2 * step)
+                    an = g[c][y, x].real if unit == 1.0 else g[c][y, x].imag
+                    worst = max(worst, abs(fd - an))
+        err[step] = worst / gmax
+    print("full-size d+is square: N0 %d, F %.10f (again: %+.1e), |grad| max %.3e, one gradient "
+          "%.1f ms, FD error of max: %.2e at step 1e-6, %.2e at step 1e-5" % (
+              s.N0, f0, f1 - f0, gmax, 1e3 * t, err[1e-6], err[1e-5]))
+    # a step changes F (about -17, a sum over 3.6e5 sites of terms far
+    # larger than the change) by about 1e-10 of itself: the rounding of F
+    # divided by the step is about 1e-5 of the largest entry (of order
+    # h^2 |Delta|) at step 1e-6 and shrinks as 1/step, which a wrong
+    # gradient would not
+    assert err[1e-6] < 2e-4
+    assert err[1e-5] < 2e-5
     s.close()
