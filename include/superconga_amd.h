@@ -208,6 +208,17 @@ int sc_newton_krylov_ex(sc_context *ctx, int max_newton, int krylov_dim, double 
  * scatters make two calls agree to rounding, not bit for bit. */
 int sc_free_energy_gradient(sc_context *ctx, double *grad, double *free_energy);
 
+/* sc_free_energy_gradient with, when grad_a is not NULL, also the gradient
+ * with respect to the induced vector potential: grad_a holds 2 x N0^2
+ * values, dF/dA_x then dF/dA_y, per unit of A as sc_set_vector_potential
+ * takes it.  A enters F through the Doppler shift of every Riccati step
+ * (the reverse sweep's adjoint of the energy shift, transposed through the
+ * bilinear taps of A) and through the magnetic term; without a penetration
+ * depth grad_a is zero.  No counterpart in the reference, like
+ * sc_free_energy_gradient. */
+int sc_free_energy_gradient_ex(sc_context *ctx, double *grad, double *grad_a,
+                               double *free_energy);
+
 /* Individual stages. */
 /* RiccatiSolver::computeCoherenceFunctions for every angle (RiccatiSolverConfined.h:513-671)
  * fused with ComputeProperty::computeGreensAngle; burnin = 1 only updates the

@@ -1762,9 +1762,11 @@ static void integrate(Oracle &o, bool burnin) {
 // a = s conj(delta) / 4 and se = s energy, so by implicit differentiation
 // dy = -(F_y0 dy0 + F_a da + F_sd d(s delta)) / F_y with
 // F_y = 2 a (y + y0) + se - 1, F_y0 = 2 a (y + y0) + se + 1, F_a = (y + y0)^2.
-// Returns y0bar and adds the adjoint of delta to dbar.
+// F_se = y + y0, so the energy's adjoint is conj(q s (y + y0)) ybar.
+// Returns y0bar and adds the adjoints of delta and of the energy to dbar and
+// ebar.
 static cplx riccatiStepAdjoint(cplx delta, cplx energy, cplx y0, cplx y, double h,
-                               cplx ybar, cplx &dbar) {
+                               cplx ybar, cplx &dbar, cplx &ebar) {
   const cplx s(0.0, h);
   const cplx a = 0.25 * cmul(s, std::conj(delta));
   const cplx se = cmul(s, energy);
@@ -1775,6 +1777,7 @@ static cplx riccatiStepAdjoint(cplx delta, cplx energy, cplx y0, cplx y, double 
   const cplx sdbar = cmul(std::conj(q), ybar);
   // a = (s / 4) conj(delta): deltabar += conj(abar) s / 4; s delta: conj(s) sdbar
   dbar += cmul(std::conj(abar), 0.25 * s) + cmul(std::conj(s), sdbar);
+  ebar += cmul(std::conj(cmul(q, cmul(s, w))), ybar);
   return cmul(std::conj(cmul(q, aw2 + se + 1.0)), ybar);
 }
 
@@ -1841,11 +1844,12 @@ static void extendValuesAdjoint(const signed char *labels, int N, double *bar) {
 // Reverse sweep of one column (x) and energy (e) of frame k for one
 // coherence function: grid holds the forward values at every site of the
 // column, rho the adjoint seed of the free-energy density per frame site.
-// Adds the adjoint of the frame's order parameter to dbar (planar complex).
+// Adds the adjoint of the frame's order parameter to dbar (planar complex)
+// and that of its Doppler shift (a real shift of every energy) to esbar.
 template <bool GAMMA>
 static void reverseColumn(const Oracle &o, int k, int x, int e, const std::vector<double> &drot,
                           const std::vector<double> &es, const std::vector<double> &rho,
-                          std::vector<double> &dbar) {
+                          std::vector<double> &dbar, std::vector<double> &esbar) {
   const int N1 = o.N1, Ne = o.Ne;
   const size_t n1 = o.n1sq();
   const FrameGeometry &g = o.frames[k];
@@ -1896,18 +1900,23 @@ static void reverseColumn(const Oracle &o, int k, int x, int e, const std::vecto
         dprev = dlOf(pidx);
         eprev = e0 + es[pidx];
       }
-      cplx db(0.0, 0.0);
+      cplx db(0.0, 0.0), eb(0.0, 0.0);
       carry = riccatiStepAdjoint((dprev + dlOf(idx)) * 0.5, (eprev + en) * 0.5, y0, y, step, ybar,
-                                 db);
+                                 db, eb);
       addDbar(idx, 0.5 * db);
-      if (i > 0) addDbar(pidx, 0.5 * db);
+      esbar[idx] += 0.5 * eb.real();
+      if (i > 0) {
+        addDbar(pidx, 0.5 * db);
+        esbar[pidx] += 0.5 * eb.real();
+      }
     } else if (lab == ENTRY) {
       const size_t sp = sparseIdx(idxIn, N1, k, x, entryCount[size_t(i)], Ne, e);
       const cplx bc(bin[sp], bin[size_t(nin * Ne) + sp]);
       const double d = std::fabs(g.dist[idx]);
-      cplx db(0.0, 0.0);
-      (void)riccatiStepAdjoint(dlOf(idx), en, bc, y, d * step, ybar, db);
+      cplx db(0.0, 0.0), eb(0.0, 0.0);
+      (void)riccatiStepAdjoint(dlOf(idx), en, bc, y, d * step, ybar, db, eb);
       addDbar(idx, db);
+      esbar[idx] += eb.real();
       carry = cplx(0.0, 0.0);
     } else {
       carry = cplx(0.0, 0.0);
@@ -1916,17 +1925,22 @@ static void reverseColumn(const Oracle &o, int k, int x, int e, const std::vecto
 }
 
 // Omega and dOmega/dRe Delta_c, dOmega/dIm Delta_c (grad: component c at
-// [c * 2 N0^2, re plane then im plane]).  Leaves the state as it was (the
-// out-values the sweeps write are restored).
-static double freeEnergyGradient(Oracle &o, std::vector<double> &grad) {
+// [c * 2 N0^2, re plane then im plane]), and with gradA (2 N0^2: x plane,
+// y plane) dOmega/dA of the induced vector potential in the units the
+// config's setters take (the state stores lambda^2 A).  Leaves the state as
+// it was (the out-values the sweeps write are restored).
+static double freeEnergyGradient(Oracle &o, std::vector<double> &grad,
+                                 std::vector<double> *gradA = nullptr) {
   if (!o.boundaryInit) setBoundaryInitialCondition(o);
   const std::vector<double> keepG = o.bOutG, keepGt = o.bOutGt;
   const size_t n0 = o.n0sq(), n1 = o.n1sq();
   const int N1 = o.N1, Ne = o.Ne;
   const double area = o.h * o.h;
+  const bool withA = gradA != nullptr && o.lambda > 0.0;
   grad.assign(size_t(o.Nc) * 2 * n0, 0.0);
+  if (gradA) gradA->assign(2 * n0, 0.0);
   std::vector<double> drot, es, gD, gJ, gF, fe(n0, 0.0), rho0(n0), rho(n1), dbar(2 * n1),
-      sumbar(2 * n0);
+      sumbar(2 * n0), esbar(n1), arbar(n1);
   for (int k = 0; k < o.Na; ++k) {
     rotatedOrderParameter(o, k, drot);
     energyShift(o, k, es);
@@ -1946,6 +1960,7 @@ static double freeEnergyGradient(Oracle &o, std::vector<double> &grad) {
     extendValuesAdjoint(lab, N1, rho.data());
     // the Green sums' own dependence on D, then the two recurrences
     std::fill(dbar.begin(), dbar.end(), 0.0);
+    std::fill(esbar.begin(), esbar.end(), 0.0);
     for (size_t i = 0; i < n1; ++i) {
       if (rho[i] == 0.0) continue;
       double su = 0.0, sw = 0.0;
@@ -1960,9 +1975,22 @@ static double freeEnergyGradient(Oracle &o, std::vector<double> &grad) {
 #pragma omp parallel for schedule(dynamic, 1)
     for (int x = o.xmin[k]; x <= o.xmax[k]; ++x)
       for (int e = 0; e < Ne; ++e) {
-        reverseColumn<true>(o, k, x, e, drot, es, rho, dbar);
-        reverseColumn<false>(o, k, x, e, drot, es, rho, dbar);
+        reverseColumn<true>(o, k, x, e, drot, es, rho, dbar, esbar);
+        reverseColumn<false>(o, k, x, e, drot, es, rho, dbar, esbar);
       }
+    if (withA) {
+      // es = q (v . (A_rot + A_ext)), A_rot the stored lambda^2 A rotated
+      // into the frame and divided by lambda^2 (energyShift): the adjoint
+      // of A is the rotation's transpose of q v_f esbar
+      const double pa = polarAngle(k, o.Na);
+      const double vf[2] = {std::cos(pa), std::sin(pa)};
+      const int q = sgn(o.cfg.charge_sign);
+      for (int f = 0; f < 2; ++f) {
+        for (size_t i = 0; i < n1; ++i) arbar[i] = static_cast<double>(q) * vf[f] * esbar[i];
+        insertRotatedAdjoint(arbar.data(), N1, N1, gradA->data() + size_t(f) * n0, o.N0, o.N0,
+                             static_cast<float>(-o.systemAngle[k]));
+      }
+    }
     // the rotation and extension of the total order parameter
     std::fill(sumbar.begin(), sumbar.end(), 0.0);
     const int kk = wrapToRangeInt(-k, 0, o.Na);
@@ -2000,6 +2028,26 @@ static double freeEnergyGradient(Oracle &o, std::vector<double> &grad) {
   }
   double mag = 0.0;
   if (o.lambda > 0.0) mag = 0.5 / (o.lambda * o.lambda) * sumCurlSquare(o);
+  if (withA) {
+    // (area / 2 lambda^2) sum b^2 with b the curl of the stored lambda^2 A:
+    // per unit of A, area sum_t b_t db_t/dA (sumCurlSquare's stencil)
+    const int N = o.N0;
+    const double inv = 1.0 / o.h;
+    const double *Ax = o.A.data(), *Ay = o.A.data() + n0;
+    double *gx = gradA->data(), *gy = gradA->data() + n0;
+    for (int y = 0; y < N; ++y)
+      for (int x = 0; x < N; ++x) {
+        const int xn = std::min(x + 1, N - 1), xp = std::max(x - 1, 0);
+        const int yn = std::min(y + 1, N - 1), yp = std::max(y - 1, 0);
+        const double cx = inv / double(xn - xp), cy = inv / double(yn - yp);
+        const double b = (Ay[size_t(y) * N + xn] - Ay[size_t(y) * N + xp]) * cx -
+                         (Ax[size_t(yn) * N + x] - Ax[size_t(yp) * N + x]) * cy;
+        gy[size_t(y) * N + xn] += area * b * cx;
+        gy[size_t(y) * N + xp] -= area * b * cx;
+        gx[size_t(yn) * N + x] -= area * b * cy;
+        gx[size_t(yp) * N + x] += area * b * cy;
+      }
+  }
   o.bOutG = keepG;
   o.bOutGt = keepGt;
   return (nonMag + mag) * area;
@@ -2961,6 +3009,21 @@ int oc_free_energy_gradient(void *hh, double *grad, double *free_energy) {
     std::vector<double> g;
     const double f = freeEnergyGradient(o, g);
     std::copy(g.begin(), g.end(), grad);
+    if (free_energy) *free_energy = f;
+    return 0;
+  } catch (const std::exception &e) {
+    g_error = e.what();
+    return -1;
+  }
+}
+
+int oc_free_energy_gradient_a(void *hh, double *grad, double *gradA, double *free_energy) {
+  try {
+    Oracle &o = *static_cast<Oracle *>(hh);
+    std::vector<double> g, ga;
+    const double f = freeEnergyGradient(o, g, &ga);
+    std::copy(g.begin(), g.end(), grad);
+    std::copy(ga.begin(), ga.end(), gradA);
     if (free_energy) *free_energy = f;
     return 0;
   } catch (const std::exception &e) {

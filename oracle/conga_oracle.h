@@ -158,6 +158,10 @@ long long oc_sweep_angles(void *handle, int angle_begin, int angle_end);
  * in-values held fixed): grad holds num_components x 2 x N0^2 values,
  * dF/dRe Delta_c then dF/dIm Delta_c.  No counterpart in the reference. */
 int oc_free_energy_gradient(void *handle, double *grad, double *free_energy);
+/* The same with dF/dA of the induced vector potential (2 N0^2: x plane, then
+ * y plane; zero without a penetration depth), A in the units the setters
+ * take. */
+int oc_free_energy_gradient_a(void *handle, double *grad, double *grad_a, double *free_energy);
 
 /* ----- unit-level restatements (closed-form checks) ----- */
 void oc_riccati_step(const double *delta, const double *energy,

@@ -109,6 +109,7 @@ def lib():
         L.oc_ldos_compute.argtypes = [vp, ctypes.c_int]
         L.oc_ldos_run.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.oc_free_energy_gradient.argtypes = [vp, dp, dp]
+        L.oc_free_energy_gradient_a.argtypes = [vp, dp, dp, dp]
         L.oc_sweep_angles.restype = ctypes.c_longlong
         L.oc_sweep_angles.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.oc_riccati_step.argtypes = [dp, dp, dp, ctypes.c_double, dp]
@@ -332,16 +333,25 @@ class Oracle:
     def sweep_angles(self, a0, a1):
         return lib().oc_sweep_angles(self.h, a0, a1)
 
-    def free_energy_gradient(self):
+    def free_energy_gradient(self, vector_potential=False):
         """(F, grad): the free energy one iteration reports from the current
         state and dF/dRe Delta_c + i dF/dIm Delta_c as an (Nc, N0, N0)
-        complex array (vector potential and boundary in-values fixed)."""
+        complex array (vector potential and boundary in-values fixed).  With
+        vector_potential, (F, grad, gradA): gradA the (2, N0, N0) dF/dA_x,
+        dF/dA_y of the induced vector potential as set_vector_potential
+        takes it."""
         n = self.N0
         g = np.zeros(self.Nc * 2 * n * n)
         f = np.zeros(1)
-        if lib().oc_free_energy_gradient(self.h, _dp(g), _dp(f)) != 0:
+        if vector_potential:
+            ga = np.zeros(2 * n * n)
+            if lib().oc_free_energy_gradient_a(self.h, _dp(g), _dp(ga), _dp(f)) != 0:
+                raise RuntimeError(lib().oc_last_error().decode())
+        elif lib().oc_free_energy_gradient(self.h, _dp(g), _dp(f)) != 0:
             raise RuntimeError(lib().oc_last_error().decode())
         g = g.reshape(self.Nc, 2, n, n)
+        if vector_potential:
+            return float(f[0]), g[:, 0] + 1j * g[:, 1], ga.reshape(2, n, n)
         return float(f[0]), g[:, 0] + 1j * g[:, 1]
 
     # Keldysh spectroscopy (src/postprocess.cu)

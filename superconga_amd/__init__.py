@@ -157,6 +157,7 @@ def lib():
                                              ctypes.POINTER(NewtonReport)]
         L.sc_batch_set_graphs.argtypes = [vp, ctypes.c_int]
         L.sc_free_energy_gradient.argtypes = [vp, _DP, _DP]
+        L.sc_free_energy_gradient_ex.argtypes = [vp, _DP, _DP, _DP]
         L.sc_ldos_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(vp)]
         L.sc_ldos_compute.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.sc_ldos_run.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -325,16 +326,25 @@ class Simulation:
                 "residual": r.residual,
                 "history": [hist[i] for i in range(r.history_length)]}
 
-    def free_energy_gradient(self):
+    def free_energy_gradient(self, vector_potential=False):
         """(F, grad): the free energy one iteration reports from the current
         state and its gradient dF/dRe Delta_c + i dF/dIm Delta_c, an
         (Nc, N0, N0) complex array, by the discrete adjoint of the iteration
         (vector potential and boundary in-values held fixed; the state is
-        left as it was)."""
-        g = np.empty(self.Nc * 2 * self.N0 * self.N0)
+        left as it was).  With vector_potential, (F, grad, grad_a): grad_a
+        the (2, N0, N0) dF/dA_x, dF/dA_y of the induced vector potential per
+        unit of A as set_vector_potential takes it."""
+        n = self.N0
+        g = np.empty(self.Nc * 2 * n * n)
         f = np.zeros(1)
-        _check(lib().sc_free_energy_gradient(self._h, _dp(g), _dp(f)))
-        g = g.reshape(self.Nc, 2, self.N0, self.N0)
+        if vector_potential:
+            ga = np.empty(2 * n * n)
+            _check(lib().sc_free_energy_gradient_ex(self._h, _dp(g), _dp(ga), _dp(f)))
+        else:
+            _check(lib().sc_free_energy_gradient(self._h, _dp(g), _dp(f)))
+        g = g.reshape(self.Nc, 2, n, n)
+        if vector_potential:
+            return float(f[0]), g[:, 0] + 1j * g[:, 1], ga.reshape(2, n, n)
         return float(f[0]), g[:, 0] + 1j * g[:, 1]
 
     def riccati_sweep(self, precision="fp64", burnin=False):

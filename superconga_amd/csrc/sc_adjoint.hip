@@ -22,6 +22,13 @@
 //      gamma downwards; each step's adjoint adds to dF/dD at its sites;
 //   3. the transpose of the rotation and extension of the order parameter
 //      and the basis functions, plus the explicit |Delta|^2 terms.
+// With a penetration depth the induced vector potential A enters too: the
+// Doppler shift es_k(s) = q v_k . (A(rotated s) + A_ext) is a real shift of
+// every energy of every step, so the reverse sweep also returns dF/des per
+// frame site, whose transpose through the bilinear taps of A gives dF/dA;
+// the magnetic term (1 / 2 lambda^2) h^2 sum (curl lambda^2 A)^2 adds its
+// own.  dF/dA is per unit of A as the setters take it (the state stores
+// lambda^2 A).
 // Complex adjoints follow zbar = dF/dRe z + i dF/dIm z; for y = f(x) with f
 // holomorphic, xbar = conj(f'(x)) ybar.
 #include "sc_engine.hpp"
@@ -43,16 +50,28 @@ constexpr int kAdjSlotsPerCU = 4; // persistent wavefronts per compute unit
 
 __device__ __forceinline__ Cx<double> conjc(Cx<double> a) { return {a.re, -a.im}; }
 
+// The adjoints of one frame site's fields: dF/dD (re, im) and dF/des.
+struct SiteBar {
+  double re, im, es;
+};
+__device__ __forceinline__ void addBar(SiteBar &b, Cx<double> d, double e) {
+  b.re += d.re;
+  b.im += d.im;
+  b.es += e;
+}
+
 // Adjoint of stepEnergy(p, ser, y0) = y.  y is the root of
 //   F(y) = a (y + y0)^2 + (se - 1) y + (se + 1) y0 + sD = 0,
 // a = i hs conj(D) / 4, sD = i hs D, se = ser + i sei (the StepSite of D
-// over the step length hs), so by implicit differentiation
-//   dy = q (F_y0 dy0 + (y + y0)^2 da + d(sD)),  q = -1 / F_y,
-//   F_y = 2 a (y + y0) + se - 1,  F_y0 = 2 a (y + y0) + se + 1.
-// Returns y0bar; adds dF/dD of the step's D argument to dbar.
+// and the shift es over the step length hs, sei = hs es), so by implicit
+// differentiation
+//   dy = q (F_y0 dy0 + (y + y0)^2 da + d(sD) + i (y + y0) dsei),
+//   q = -1 / F_y,  F_y = 2 a (y + y0) + se - 1,  F_y0 = 2 a (y + y0) + se + 1.
+// Returns y0bar; adds dF/dD of the step's D argument to dbar and dF/des of
+// its shift to ebar.
 __device__ __forceinline__ Cx<double> stepAdjoint(const StepSite<double> &p, double ser,
                                                   Cx<double> y0, Cx<double> y, Cx<double> ybar,
-                                                  double hs, Cx<double> &dbar) {
+                                                  double hs, Cx<double> &dbar, double &ebar) {
   const Cx<double> w = y + y0;
   const Cx<double> aw = p.a * w;
   const Cx<double> fy = {2.0 * aw.re + ser - 1.0, 2.0 * aw.im + p.sei};
@@ -64,6 +83,9 @@ __device__ __forceinline__ Cx<double> stepAdjoint(const StepSite<double> &p, dou
   // a = (i hs / 4) conj(D): Dbar += conj(abar) i hs / 4;  sD = i hs D: Dbar += -i hs sdbar
   dbar.re += 0.25 * hs * abar.im + hs * sdbar.im;
   dbar.im += 0.25 * hs * abar.re - hs * sdbar.re;
+  // a real parameter's adjoint is Re(conj(dy/dparam) ybar)
+  const Cx<double> qiw = q * Cx<double>{-w.im, w.re};
+  ebar += hs * (qiw.re * ybar.re + qiw.im * ybar.im);
   return conjc(q * fy0) * ybar;
 }
 
@@ -123,23 +145,42 @@ __global__ void extendAdjointKernel(double *bar, const int *target, const int *o
   }
 }
 
-// 3a. Transpose of the site fields' order parameter
-// D_k(s) = sum_c b_c(k) bilinear(extDelta_c)(rotated s): one thread per frame
-// site, dF/dD_k(s) scattered to the four taps of every component.
+// 3a. Transpose of the site fields: of the order parameter
+// D_k(s) = sum_c b_c(k) bilinear(extDelta_c)(rotated s), dF/dD_k(s)
+// scattered to the four taps of every component; with aBar, of the Doppler
+// shift es_k(s) = q v_k . bilinear(A)(rotated s) (siteFields), dF/des_k(s)
+// scattered to the four taps of A_x and A_y.  One thread per frame site.
 __global__ void fieldsAdjointKernel(GrainView v, const double *dbar, long long framePlane,
-                                    double *extBar) {
+                                    double *extBar, double *aBar) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= framePlane) return;
   const double br = dbar[i], bi = dbar[framePlane + i];
-  if (br == 0.0 && bi == 0.0) return;
+  const double be = aBar ? dbar[2 * framePlane + i] : 0.0;
+  if (br == 0.0 && bi == 0.0 && be == 0.0) return;
   const int N0 = v.N0, N1 = v.N1;
   const long long n1 = (long long)N1 * N1;
   const int k = int(i / n1);
   const int site = int(i - (long long)k * n1);
   const int x = site % N1, y = site / N1;
+  const size_t n0 = size_t(N0) * N0;
+  if (be != 0.0) {
+    const RotSample q = rotatedCell(x, y, N1, N0, double(v.aCos[k]), double(v.aSin[k]));
+    if (q.valid) {
+      const double w00 = (1.0 - q.fy) * (1.0 - q.fx), w10 = (1.0 - q.fy) * q.fx;
+      const double w01 = q.fy * (1.0 - q.fx), w11 = q.fy * q.fx;
+      for (int f = 0; f < 2; ++f) {
+        const double c = be * double(v.charge) * (f == 0 ? v.vFx[k] : v.vFy[k]);
+        double *pa = aBar + size_t(f) * n0 + q.idx;
+        atomicAdd(pa, c * w00);
+        atomicAdd(pa + 1, c * w10);
+        atomicAdd(pa + N0, c * w01);
+        atomicAdd(pa + N0 + 1, c * w11);
+      }
+    }
+  }
+  if (br == 0.0 && bi == 0.0) return;
   const RotSample r = rotatedCell(x, y, N1, N0, double(v.opCos[k]), double(v.opSin[k]));
   if (!r.valid) return;
-  const size_t n0 = size_t(N0) * N0;
   const double w00 = (1.0 - r.fy) * (1.0 - r.fx), w10 = (1.0 - r.fy) * r.fx;
   const double w01 = r.fy * (1.0 - r.fx), w11 = r.fy * r.fx;
   for (int c = 0; c < v.Nc; ++c) {
@@ -156,6 +197,28 @@ __global__ void fieldsAdjointKernel(GrainView v, const double *dbar, long long f
     atomicAdd(pi + N0, vi * w01);
     atomicAdd(pi + N0 + 1, vi * w11);
   }
+}
+
+// 3c. The magnetic term's gradient: with b_t the curl of the stored
+// lambda^2 A at site t (curlSquareKernel's stencil), d/dA of
+// (area / 2 lambda^2) sum_t b_t^2 is area sum_t b_t db_t/d(lambda^2 A) per
+// unit of A.  One thread per site, its four stencil entries by atomics.
+__global__ void curlAdjointKernel(const double *A, int N, double invH, double area,
+                                  double *aBar) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N * N) return;
+  const int x = t % N, y = t / N;
+  const size_t n = size_t(N) * N;
+  const int xn = min(x + 1, N - 1), xp = max(x - 1, 0);
+  const int yn = min(y + 1, N - 1), yp = max(y - 1, 0);
+  const double cx = invH / double(xn - xp), cy = invH / double(yn - yp);
+  const double b = (A[n + size_t(y) * N + xn] - A[n + size_t(y) * N + xp]) * cx -
+                   (A[size_t(yn) * N + x] - A[size_t(yp) * N + x]) * cy;
+  const double ab = area * b;
+  atomicAdd(aBar + n + size_t(y) * N + xn, ab * cx);
+  atomicAdd(aBar + n + size_t(y) * N + xp, -ab * cx);
+  atomicAdd(aBar + size_t(yn) * N + x, -ab * cy);
+  atomicAdd(aBar + size_t(yp) * N + x, ab * cy);
 }
 
 // The explicit terms: grad = extBar + 2 h^2 factor_c Delta_c at inside sites.
@@ -181,7 +244,7 @@ struct AdjointArgs {
   Cx<double> *scratch;    // per slot: gamma~ then gamma, [maxLen][EL][64] each
   long long slotStride;   // complex elements of one of the two buffers
   const double *rho;      // [Na][N1^2]
-  double *dbar;           // [2][Na][N1^2]
+  double *dbar;           // [3][Na][N1^2]: dF/dD re, im, dF/des
   long long framePlane;   // Na * N1^2
   double *partial;        // per item: its share of the free-energy sum
 };
@@ -218,19 +281,22 @@ __device__ void adjointItem(const AdjointArgs &a, const SweepItemDev &it, int it
   const int len = it.yhi - it.ylo + 1;
   const size_t col = size_t(k) * N1 + x;
   auto slot = [&](int t, int j) { return (size_t(t) * EL + j) * 64 + lane; };
-  // dF/dD of a finished site: the column's groups summed, written (or added)
-  // by the group-0 lane
-  auto writeDbar = [&](int y, Cx<double> part, bool add) {
+  // dF/dD and dF/des of a finished site: the column's groups summed,
+  // written (or added) by the group-0 lane
+  auto writeDbar = [&](int y, SiteBar part, bool add) {
     const double re = groupSum(part.re, gp, cols, c);
     const double im = groupSum(part.im, gp, cols, c);
+    const double es = groupSum(part.es, gp, cols, c);
     if (g == 0 && colActive) {
       double *p = dbK + size_t(y) * N1;
       if (add) {
         p[0] += re;
         p[a.framePlane] += im;
+        p[2 * a.framePlane] += es;
       } else {
         p[0] = re;
         p[a.framePlane] = im;
+        p[2 * a.framePlane] = es;
       }
     }
   };
@@ -293,7 +359,8 @@ __device__ void adjointItem(const AdjointArgs &a, const SweepItemDev &it, int it
     Cx<double> coh[EL], mu[EL], gtB[EL];
 #pragma unroll
     for (int j = 0; j < EL; ++j) coh[j] = mu[j] = gtB[j] = {0.0, 0.0};
-    Cx<double> dPrev = {0.0, 0.0}, dtB = {0.0, 0.0}, dbB = {0.0, 0.0};
+    Cx<double> dPrev = {0.0, 0.0}, dtB = {0.0, 0.0};
+    SiteBar dbB = {0.0, 0.0, 0.0};
     double esPrev = 0.0;
     bool havePrev = false;
     int8_t labB = 0;
@@ -308,7 +375,7 @@ __device__ void adjointItem(const AdjointArgs &a, const SweepItemDev &it, int it
       Cx<double> gt[EL];
 #pragma unroll
       for (int j = 0; j < EL; ++j) gt[j] = active ? scrT[slot(t, j)] : Cx<double>{0.0, 0.0};
-      Cx<double> dbCur = {0.0, 0.0};
+      SiteBar dbCur = {0.0, 0.0, 0.0};
       if (active) {
         // gamma into this site (the sweep's pass 2)
         if (lc == 1 || lc == 3) {
@@ -343,7 +410,7 @@ __device__ void adjointItem(const AdjointArgs &a, const SweepItemDev &it, int it
           sw += res[j] * (coh[j].re + gt[j].re);
         }
         omega += rho * (dre * su + dim * sw);
-        dbCur = {rho * su, rho * sw};
+        dbCur = {rho * su, rho * sw, 0.0};
         // adjoint of gamma~ here: what the step into the site below carries
         // back, plus dF/dgt = rho r (D.im + i D.re)
         Cx<double> carry[EL];
@@ -353,13 +420,14 @@ __device__ void adjointItem(const AdjointArgs &a, const SweepItemDev &it, int it
           const StepSite<double> ps = stepSite(
               Cx<double>{(Dt.re + dtB.re) * 0.5, (Dt.im + dtB.im) * 0.5}, (es + esPrev) * 0.5, h);
           Cx<double> db = {0.0, 0.0};
+          double eb = 0.0;
 #pragma unroll
           for (int j = 0; j < EL; ++j)
-            if (eOk[j]) carry[j] = stepAdjoint(ps, ser[j], gt[j], gtB[j], mu[j], h, db);
+            if (eOk[j]) carry[j] = stepAdjoint(ps, ser[j], gt[j], gtB[j], mu[j], h, db, eb);
           // the step's D is -conj(D): dF/dD = -conj(dF/dD~), half to each site
           const Cx<double> dd = {-0.5 * db.re, 0.5 * db.im};
-          dbCur = dbCur + dd;
-          dbB = dbB + dd;
+          addBar(dbCur, dd, 0.5 * eb);
+          addBar(dbB, dd, 0.5 * eb);
         }
 #pragma unroll
         for (int j = 0; j < EL; ++j)
@@ -371,14 +439,15 @@ __device__ void adjointItem(const AdjointArgs &a, const SweepItemDev &it, int it
           const double dr = v.distInGt[ci];
           const StepSite<double> ps = stepSite(Dt, es, dr * h);
           Cx<double> db = {0.0, 0.0};
+          double eb = 0.0;
 #pragma unroll
           for (int j = 0; j < EL; ++j) {
             if (!eOk[j]) continue;
             const size_t si = size_t(ci) * Ne + g * EL + j;
             const Cx<double> b0 = {v.bInGt[si], v.bInGt[size_t(v.nInGt) * Ne + si]};
-            (void)stepAdjoint(ps, ser[j] * dr, b0, gt[j], mu[j], dr * h, db);
+            (void)stepAdjoint(ps, ser[j] * dr, b0, gt[j], mu[j], dr * h, db, eb);
           }
-          dbCur = dbCur + Cx<double>{-db.re, db.im};
+          addBar(dbCur, Cx<double>{-db.re, db.im}, eb);
         }
       }
       if (lc == 3) ++seenT;
@@ -399,10 +468,11 @@ __device__ void adjointItem(const AdjointArgs &a, const SweepItemDev &it, int it
       const StepSite<double> ps =
           stepSite(Cx<double>{dtB.re * 0.5, dtB.im * 0.5}, esPrev * 0.5, h);
       Cx<double> db = {0.0, 0.0};
+      double eb = 0.0;
 #pragma unroll
       for (int j = 0; j < EL; ++j)
-        if (eOk[j]) (void)stepAdjoint(ps, ser[j] * 0.5, Cx<double>{0.0, 0.0}, gtB[j], mu[j], h, db);
-      dbB = dbB + Cx<double>{-0.5 * db.re, 0.5 * db.im};
+        if (eOk[j]) (void)stepAdjoint(ps, ser[j] * 0.5, Cx<double>{0.0, 0.0}, gtB[j], mu[j], h, db, eb);
+      addBar(dbB, Cx<double>{-0.5 * db.re, 0.5 * db.im}, 0.5 * eb);
     }
     writeDbar(it.yhi, dbB, false);
   }
@@ -414,7 +484,8 @@ __device__ void adjointItem(const AdjointArgs &a, const SweepItemDev &it, int it
     Cx<double> lam[EL], gA[EL];
 #pragma unroll
     for (int j = 0; j < EL; ++j) lam[j] = gA[j] = {0.0, 0.0};
-    Cx<double> dA = {0.0, 0.0}, dbA = {0.0, 0.0};
+    Cx<double> dA = {0.0, 0.0};
+    SiteBar dbA = {0.0, 0.0, 0.0};
     double esA = 0.0;
     int8_t labA = 0;
     for (int t = 0; t < len; ++t) {
@@ -427,7 +498,7 @@ __device__ void adjointItem(const AdjointArgs &a, const SweepItemDev &it, int it
       Cx<double> gc[EL];
 #pragma unroll
       for (int j = 0; j < EL; ++j) gc[j] = active ? scrG[slot(t, j)] : Cx<double>{0.0, 0.0};
-      Cx<double> dbCur = {0.0, 0.0};
+      SiteBar dbCur = {0.0, 0.0, 0.0};
       if (active) {
         Cx<double> carry[EL];
 #pragma unroll
@@ -437,12 +508,13 @@ __device__ void adjointItem(const AdjointArgs &a, const SweepItemDev &it, int it
           const StepSite<double> ps = stepSite(
               Cx<double>{(D.re + dA.re) * 0.5, (D.im + dA.im) * 0.5}, (es + esA) * 0.5, h);
           Cx<double> db = {0.0, 0.0};
+          double eb = 0.0;
 #pragma unroll
           for (int j = 0; j < EL; ++j)
-            if (eOk[j]) carry[j] = stepAdjoint(ps, ser[j], gc[j], gA[j], lam[j], h, db);
+            if (eOk[j]) carry[j] = stepAdjoint(ps, ser[j], gc[j], gA[j], lam[j], h, db, eb);
           const Cx<double> dd = {0.5 * db.re, 0.5 * db.im};
-          dbCur = dbCur + dd;
-          dbA = dbA + dd;
+          addBar(dbCur, dd, 0.5 * eb);
+          addBar(dbA, dd, 0.5 * eb);
         }
         // dF/dg = rho r (D.im - i D.re)
 #pragma unroll
@@ -455,14 +527,15 @@ __device__ void adjointItem(const AdjointArgs &a, const SweepItemDev &it, int it
           const double dr = v.distInG[ci];
           const StepSite<double> ps = stepSite(D, es, dr * h);
           Cx<double> db = {0.0, 0.0};
+          double eb = 0.0;
 #pragma unroll
           for (int j = 0; j < EL; ++j) {
             if (!eOk[j]) continue;
             const size_t si = size_t(ci) * Ne + g * EL + j;
             const Cx<double> b0 = {v.bInG[si], v.bInG[size_t(v.nInG) * Ne + si]};
-            (void)stepAdjoint(ps, ser[j] * dr, b0, gc[j], lam[j], dr * h, db);
+            (void)stepAdjoint(ps, ser[j] * dr, b0, gc[j], lam[j], dr * h, db, eb);
           }
-          dbCur = dbCur + db;
+          addBar(dbCur, db, eb);
         }
       }
       if (lc == 2) ++seenG;
@@ -479,10 +552,11 @@ __device__ void adjointItem(const AdjointArgs &a, const SweepItemDev &it, int it
     if (active && (labA == 1 || labA == 3)) {
       const StepSite<double> ps = stepSite(Cx<double>{dA.re * 0.5, dA.im * 0.5}, esA * 0.5, h);
       Cx<double> db = {0.0, 0.0};
+      double eb = 0.0;
 #pragma unroll
       for (int j = 0; j < EL; ++j)
-        if (eOk[j]) (void)stepAdjoint(ps, ser[j] * 0.5, Cx<double>{0.0, 0.0}, gA[j], lam[j], h, db);
-      dbA = dbA + Cx<double>{0.5 * db.re, 0.5 * db.im};
+        if (eOk[j]) (void)stepAdjoint(ps, ser[j] * 0.5, Cx<double>{0.0, 0.0}, gA[j], lam[j], h, db, eb);
+      addBar(dbA, Cx<double>{0.5 * db.re, 0.5 * db.im}, 0.5 * eb);
     }
     writeDbar(it.ylo, dbA, true);
   }
@@ -515,7 +589,7 @@ __global__ void adjointResetCounter(int *c) { *c = 0; }
 
 } // namespace
 
-double Grain::freeEnergyGradient(double *grad) {
+double Grain::freeEnergyGradient(double *grad, double *gradA) {
   const Setup &s = *s_;
   if (s.keldysh) throw std::runtime_error("a spectroscopy context has no free-energy gradient");
   if (s.Nc > 4) throw std::runtime_error("at most 4 order-parameter components");
@@ -542,12 +616,16 @@ double Grain::freeEnergyGradient(double *grad) {
     dAdjScratch_ = devAlloc<double>(size_t(adjSlots_) * 4 * stride);
     dAdjPartial_ = devAlloc<double>(size_t(std::max(adjNumItems_, 1)));
     dAdjRho_ = devAlloc<double>(size_t(framePlane));
-    dAdjDbar_ = devAlloc<double>(2 * size_t(framePlane));
+    dAdjDbar_ = devAlloc<double>(3 * size_t(framePlane));
     dAdjExt_ = devAlloc<double>(size_t(2 * s.Nc) * n0);
     dAdjGrad_ = devAlloc<double>(size_t(2 * s.Nc) * n0);
+    dAdjGradA_ = devAlloc<double>(2 * n0);
   }
+  // A enters through the Doppler shift only with a penetration depth
+  const bool withA = gradA != nullptr && s.lambda > 0.0;
   SC_HIP(hipMemsetAsync(dAdjRho_, 0, size_t(framePlane) * sizeof(double), stream_));
-  SC_HIP(hipMemsetAsync(dAdjDbar_, 0, 2 * size_t(framePlane) * sizeof(double), stream_));
+  SC_HIP(hipMemsetAsync(dAdjDbar_, 0, 3 * size_t(framePlane) * sizeof(double), stream_));
+  if (withA) SC_HIP(hipMemsetAsync(dAdjGradA_, 0, 2 * n0 * sizeof(double), stream_));
   SC_HIP(hipMemsetAsync(dAdjExt_, 0, size_t(2 * s.Nc) * n0 * sizeof(double), stream_));
   const GrainView v = view();
   const double area = s.h * s.h;
@@ -582,9 +660,14 @@ double Grain::freeEnergyGradient(double *grad) {
     SC_HIP(hipGetLastError());
   }
   // 3. the rotation, basis and extension of the order parameter
-  fieldsAdjointKernel<<<blocksFor(framePlane), 256, 0, stream_>>>(v, dAdjDbar_, framePlane,
-                                                                  dAdjExt_);
+  fieldsAdjointKernel<<<blocksFor(framePlane), 256, 0, stream_>>>(
+      v, dAdjDbar_, framePlane, dAdjExt_, withA ? dAdjGradA_ : nullptr);
   SC_HIP(hipGetLastError());
+  if (withA) {
+    curlAdjointKernel<<<blocksFor(n0), 256, 0, stream_>>>(dA_, s.N0, 1.0 / s.h, s.h * s.h,
+                                                          dAdjGradA_);
+    SC_HIP(hipGetLastError());
+  }
   const int ne = static_cast<int>(s.extRef.target.size());
   if (ne > 0) {
     extendAdjointKernel<<<blocksFor(ne), 256, 0, stream_>>>(dAdjExt_, dExtRefT_, dExtRefO_,
@@ -607,6 +690,10 @@ double Grain::freeEnergyGradient(double *grad) {
   int hCurl = -1;
   queueExplicitFreeEnergy(hSq, hCurl);
   fromDevice(grad, dAdjGrad_, size_t(ng) * sizeof(double));
+  if (withA)
+    fromDevice(gradA, dAdjGradA_, 2 * n0 * sizeof(double));
+  else if (gradA)
+    std::fill(gradA, gradA + 2 * n0, 0.0);
   flushReductions();
   double nonMag = 0.0;
   for (int c = 0; c < s.Nc; ++c)

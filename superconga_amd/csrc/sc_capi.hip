@@ -465,11 +465,16 @@ int sc_newton_krylov_ex(sc_context *ctx, int maxNewton, int krylovDim, double to
 }
 
 int sc_free_energy_gradient(sc_context *ctx, double *grad, double *freeEnergy) {
+  return sc_free_energy_gradient_ex(ctx, grad, nullptr, freeEnergy);
+}
+
+int sc_free_energy_gradient_ex(sc_context *ctx, double *grad, double *gradA,
+                               double *freeEnergy) {
   SC_CHECK_CTX(ctx);
   SC_CHECK_SIM(ctx);
   SC_CHECK_PTR(grad);
   return guarded([&] {
-    const double f = ctx->grain->freeEnergyGradient(grad);
+    const double f = ctx->grain->freeEnergyGradient(grad, gradA);
     if (freeEnergy) *freeEnergy = f;
     return SC_OK;
   });

@@ -110,6 +110,7 @@ Grain::~Grain() {
   devFree(dAdjDbar_);
   devFree(dAdjExt_);
   devFree(dAdjGrad_);
+  devFree(dAdjGradA_);
   dropEndGraph();
   fft_.reset();
   screen_.reset();

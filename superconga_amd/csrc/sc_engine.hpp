@@ -161,9 +161,11 @@ public:
   // The free energy one iteration reports from the current state and its
   // gradient with respect to the order parameter (vector potential and
   // boundary in-values held fixed) by a discrete adjoint: grad holds
-  // Nc x 2 x N0^2 values, dF/dRe Delta_c then dF/dIm Delta_c.  The state
-  // is left as it was.
-  double freeEnergyGradient(double *grad);
+  // Nc x 2 x N0^2 values, dF/dRe Delta_c then dF/dIm Delta_c.  With gradA
+  // (2 x N0^2: x then y) also dF/dA of the induced vector potential, per
+  // unit of A as setVectorPotential takes it (zero without a penetration
+  // depth).  The state is left as it was.
+  double freeEnergyGradient(double *grad, double *gradA = nullptr);
   void resetAccelerator();
 
   // Keldysh spectroscopy (src/postprocess.cu), for a grain built by
@@ -348,7 +350,8 @@ private:
   void *adjItems_ = nullptr;
   int adjNumItems_ = 0, adjMaxLen_ = 0, adjSlots_ = 0;
   double *dAdjScratch_ = nullptr, *dAdjPartial_ = nullptr, *dAdjRho_ = nullptr,
-         *dAdjDbar_ = nullptr, *dAdjExt_ = nullptr, *dAdjGrad_ = nullptr;
+         *dAdjDbar_ = nullptr, *dAdjExt_ = nullptr, *dAdjGrad_ = nullptr,
+         *dAdjGradA_ = nullptr;
   int hA_ = -1; // queued residual of the vector potential
   // the captured stages after the sweep (queueEndCompute)
   void queueEndStages();

@@ -1,10 +1,10 @@
-"""The free-energy gradient on the GPU (sc_free_energy_gradient, the
+"""The free-energy gradient on the GPU (sc_free_energy_gradient_ex, the
 discrete adjoint of one iteration) against the oracle's adjoint, which
 tests/test_oracle_gradient.py pins by central differences: every grid point
-of dF/dRe Delta and dF/dIm Delta within 1e-10 of the largest entry, F within
-1e-12 relative, in FP64.  The same gradient through torch.autograd, and a
-central-difference check of the product itself on a two-component d + is
-square.
+of dF/dRe Delta, dF/dIm Delta and dF/dA within 1e-10 of the largest entry, F
+within 1e-12 relative, in FP64.  The same gradients through torch.autograd,
+and central-difference checks of the product itself on a two-component
+d + is square, in Delta and in A.
 """
 import numpy as np
 import pytest
@@ -69,12 +69,19 @@ def test_gradient_matches_oracle(product, oracle, name):
     cfg = CASES[name]()
     d, a = state(product, cfg)
     s, o = fresh_pair(product, oracle, cfg, d, a)
-    f, g = s.free_energy_gradient()
-    fo, go = o.free_energy_gradient()
+    f, g, ga = s.free_energy_gradient(vector_potential=True)
+    fo, go, gao = o.free_energy_gradient(vector_potential=True)
     err = float(np.max(np.abs(g - go)) / np.max(np.abs(go)))
-    print("%s: F %.12f (oracle %.12f), gradient error %.2e of max" % (name, f, fo, err))
+    if lam(cfg) > 0:
+        err_a = float(np.max(np.abs(ga - gao)) / np.max(np.abs(gao)))
+    else:
+        assert not np.any(ga) and not np.any(gao)
+        err_a = 0.0
+    print("%s: F %.12f (oracle %.12f), gradient error %.2e of max, in A %.2e of max" % (
+        name, f, fo, err, err_a))
     assert abs(f - fo) <= 1e-12 * abs(fo)
     assert err < TOL
+    assert err_a < TOL
     # the evaluation leaves the state as it was: the next iteration matches
     # the oracle's
     s.compute("fp64")
@@ -220,4 +227,69 @@ def test_gradient_central_differences_full_size_d_plus_is(product):
     # gradient would not
     assert err[1e-6] < 2e-4
     assert err[1e-5] < 2e-5
+    s.close()
+
+
+def test_autograd_backward_in_the_vector_potential(product):
+    torch = pytest.importorskip("torch")
+    from superconga_amd.autograd import free_energy
+    cfg = CASES["vortex_field"]()
+    d, a = state(product, cfg)
+    s = product.Simulation(cfg)
+    planes = np.stack([d.real, d.imag], axis=1)
+    x = torch.tensor(planes, dtype=torch.float64, requires_grad=True)
+    at = torch.tensor(a, dtype=torch.float64, requires_grad=True)
+    f = free_energy(s, x, at)
+    (3.0 * f).backward()
+    f0, g, ga = s.free_energy_gradient(vector_potential=True)
+    assert abs(float(f.detach()) - f0) <= 1e-14 * abs(f0)
+    assert np.allclose(at.grad.numpy(), 3.0 * ga, rtol=0, atol=1e-14 * np.max(np.abs(3.0 * ga)))
+    assert np.allclose(x.grad.numpy(), 3.0 * np.stack([g.real, g.imag], axis=1), rtol=0,
+                       atol=1e-14 * 3.0 * np.max(np.abs(g)))
+    s.close()
+
+
+def test_vector_potential_gradient_central_differences_full_size_d_plus_is(product):
+    # the full-size d + is square two iterations in: central differences of
+    # F in A_x and A_y at 64 seeded sites (inside and near the edge) and
+    # three far outside, where only the magnetic term acts, at steps 1e-6
+    # and 1e-3
+    from superconga_amd import workloads
+    cfg = workloads.dplus_is_square()
+    s = product.Simulation(cfg)
+    d0 = s.order_parameter(0)
+    for c, x in enumerate(workloads.d_plus_is_initial(d0[0])):
+        s.set_order_parameter(c, x)
+    for _ in range(2):
+        s.compute("fp64")
+    a = s.vector_potential(0)
+    s.set_vector_potential(a[0], a[1])
+    f0, _, ga = s.free_energy_gradient(vector_potential=True)
+    dom = s.domain() > 0
+    rng = np.random.default_rng(9)
+    pts = [tuple(p) for p in np.argwhere(dom)[rng.choice(int(dom.sum()), 60, replace=False)]]
+    n = s.N0
+    pts += [(1, 1), (n // 2, 2), (n - 3, n // 3), (n // 2, n - 2)]
+    gmax = float(np.max(np.abs(ga)))
+    err = {}
+    for step, sub in ((1e-6, pts), (1e-3, pts[:12] + pts[-4:])):
+        worst = 0.0
+        for (y, x) in sub:
+            for f in range(2):
+                aa = a.copy()
+                aa[f, y, x] += step
+                s.set_vector_potential(aa[0], aa[1])
+                fp, _ = s.free_energy_gradient()
+                aa[f, y, x] -= 2 * step
+                s.set_vector_potential(aa[0], aa[1])
+                fm, _ = s.free_energy_gradient()
+                worst = max(worst, abs((fp - fm) / (2 * step) - ga[f, y, x]))
+        s.set_vector_potential(a[0], a[1])
+        err[step] = worst / gmax
+    print("full-size d+is square: |dF/dA| max %.3e, FD error of max: %.2e at step 1e-6, "
+          "%.2e at step 1e-3" % (gmax, err[1e-6], err[1e-3]))
+    # the rounding of F over the step (5e-5 of the largest entry at 1e-6)
+    # falls as 1/step; no floor above 4e-8
+    assert err[1e-6] < 3e-4
+    assert err[1e-3] < 5e-7
     s.close()

@@ -4,9 +4,10 @@ the oracle's own free energy, step 1e-6, at 64 seeded grid points per case in
 FP64 (re and im parts of every component), and the free energy itself equal
 to what one iteration of the oracle reports from the same state.
 
-The function differentiated is F(Delta) of ComputeFreeEnergy.h:281-414 on
-the Green functions of one sweep, with the vector potential and the boundary
-in-values held fixed.  Finite differences of a forward sum over ~1e5 sites
+The function differentiated is F(Delta, A) of ComputeFreeEnergy.h:281-414
+on the Green functions of one sweep, with the boundary in-values held fixed;
+A enters through the Doppler shift of the Riccati steps and the magnetic
+term.  Finite differences of a forward sum over ~1e5 sites
 resolve a derivative to ~1e-14 / step, so the bound is 2e-6 of the largest
 gradient entry at step 1e-6 (measured 1e-7 to 3e-7) and 1e-7 at step 1e-5
 (measured 1e-8): a dropped term, a wrong sign or a transposed tap moves
@@ -91,6 +92,54 @@ def test_gradient_matches_central_differences(name):
     # a larger step resolves it further (rounding of the differences ~ 1/step)
     err5, _, _ = fd_errors(o, pts[:8], 1e-5)
     assert err5 < 1e-7
+
+
+def fd_errors_vector_potential(o, lam, pts, step):
+    """Central differences of F in A_x and A_y (the units set_vector_potential
+    takes; the oracle stores lambda^2 A) against the adjoint's dF/dA."""
+    f0, _, ga = o.free_energy_gradient(vector_potential=True)
+    a = o.vector_potential() / lam ** 2
+    worst = 0.0
+    for (y, x) in pts:
+        for f in range(2):
+            aa = a.copy()
+            aa[f, y, x] += step
+            o.set_vector_potential(aa[0], aa[1])
+            fp, _ = o.free_energy_gradient()
+            aa[f, y, x] -= 2 * step
+            o.set_vector_potential(aa[0], aa[1])
+            fm, _ = o.free_energy_gradient()
+            worst = max(worst, abs((fp - fm) / (2 * step) - ga[f, y, x]))
+    o.set_vector_potential(a[0], a[1])
+    return worst / np.max(np.abs(ga)), ga
+
+
+@pytest.mark.parametrize("name", ["d_plus_s_hole", "vortex_field"])
+def test_vector_potential_gradient_matches_central_differences(name):
+    # A enters F through the Doppler shift of every Riccati step (the
+    # energy's real part) and through the magnetic term (1 / 2 lambda^2)
+    # sum (curl lambda^2 A)^2 h^2, which alone acts far outside the grain.
+    # dF/dA is smaller than dF/dDelta, so the rounding of the differences is
+    # larger: about 8e-6 of the largest entry at step 1e-6, falling as
+    # 1/step to 2e-8 at step 1e-3 (no floor: a dropped term or a wrong
+    # factor would leave one of order 1).
+    o = state(name)
+    lam = CASES[name]()["physics"]["penetration_depth"]
+    pts = sample_points(o, 24, seed=7)
+    n = o.N0
+    far = [(1, 1), (n // 2, 2), (n - 3, n // 3)]
+    err6, ga = fd_errors_vector_potential(o, lam, pts + far, 1e-6)
+    err3, _ = fd_errors_vector_potential(o, lam, pts[:5] + far, 1e-3)
+    print("%s: |dF/dA| max %.3e, FD error %.2e of max at step 1e-6, %.2e at step 1e-3" % (
+        name, np.max(np.abs(ga)), err6, err3))
+    assert err6 < 3e-5
+    assert err3 < 1e-7
+    # both routes carry weight: the far sites only through the curl term
+    assert min(abs(ga[:, y, x]).max() for (y, x) in far) > 0.0
+    # without an induced potential the gradient in A is zero
+    o2 = state("dwave_hexagon")
+    _, _, ga2 = o2.free_energy_gradient(vector_potential=True)
+    assert not np.any(ga2)
 
 
 def test_gradient_free_energy_is_the_iterations():
