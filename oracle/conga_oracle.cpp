@@ -1846,10 +1846,13 @@ static void extendValuesAdjoint(const signed char *labels, int N, double *bar) {
 // column, rho the adjoint seed of the free-energy density per frame site.
 // Adds the adjoint of the frame's order parameter to dbar (planar complex)
 // and that of its Doppler shift (a real shift of every energy) to esbar.
+// With loc, the local adjoint of the coherence function at every site and
+// energy (loc[e n1 + site]) replaces the free-energy density's.
 template <bool GAMMA>
 static void reverseColumn(const Oracle &o, int k, int x, int e, const std::vector<double> &drot,
                           const std::vector<double> &es, const std::vector<double> &rho,
-                          std::vector<double> &dbar, std::vector<double> &esbar) {
+                          std::vector<double> &dbar, std::vector<double> &esbar,
+                          const std::vector<cplx> *loc = nullptr) {
   const int N1 = o.N1, Ne = o.Ne;
   const size_t n1 = o.n1sq();
   const FrameGeometry &g = o.frames[k];
@@ -1884,9 +1887,10 @@ static void reverseColumn(const Oracle &o, int k, int x, int e, const std::vecto
     const cplx D(drot[idx], drot[n1 + idx]);
     // dOmega/dgamma at the site through the free-energy density
     // gF = sum_e r_e [D.re (gt.im - g.im) + D.im (g.re + gt.re)]
-    const double rr = rho[idx] * o.residue[e];
-    const cplx local = GAMMA ? cplx(rr * D.imag(), -rr * D.real())
-                             : cplx(rr * D.imag(), rr * D.real());
+    const double rr = loc ? 0.0 : rho[idx] * o.residue[e];
+    const cplx local = loc ? (*loc)[size_t(e) * n1 + idx]
+                           : (GAMMA ? cplx(rr * D.imag(), -rr * D.real())
+                                    : cplx(rr * D.imag(), rr * D.real()));
     const cplx ybar = carry + local;
     const signed char lab = g.label[idx];
     const cplx y(grid[idx], grid[n1 + idx]);
@@ -2051,6 +2055,195 @@ static double freeEnergyGradient(Oracle &o, std::vector<double> &grad,
   o.bOutG = keepG;
   o.bOutGt = keepGt;
   return (nonMag + mag) * area;
+}
+
+// The self-consistency residual R = 1/2 sum_inside sum_c |Delta'_c - Delta_c|^2
+// + 1/2 sum |A' - A|^2 of one plain iteration u -> G(u) (the gap equation of
+// ComputeOrderParameter.h:174-262, the current of ComputeCurrent.h:142-222
+// and its vector potential, ComputeVectorPotential.h:647-745), the boundary
+// in-values held fixed, and its gradient (J - I)^T (G(u) - u) in Delta
+// (grad as in freeEnergyGradient) and in A (gradA, per unit of A as the
+// setters take it; zero without a penetration depth).  Reverse mode of the
+// same stages as freeEnergyGradient with other seeds: the transposes of the
+// Poisson solve, of the current's and the gap equation's back-rotation and
+// extension, the local derivatives of the Green functions (LocalGreens.h:
+// 32-70) at every site and energy, then the same reverse sweeps.
+static double residualGradient(Oracle &o, std::vector<double> &grad, std::vector<double> &gradA) {
+  if (!o.boundaryInit) setBoundaryInitialCondition(o);
+  const std::vector<double> keepG = o.bOutG, keepGt = o.bOutGt;
+  const size_t n0 = o.n0sq(), n1 = o.n1sq();
+  const int N1 = o.N1, Ne = o.Ne, Nc = o.Nc;
+  const bool doA = o.cfg.compute_current && o.lambda > 0.0;
+  const double l2 = doA ? o.lambda * o.lambda : 1.0;
+  grad.assign(size_t(Nc) * 2 * n0, 0.0);
+  gradA.assign(2 * n0, 0.0);
+  std::vector<double> drot, es, gD, gJ, gF, tmp(2 * n0);
+  // forward: G(u) from one sweep of every angle
+  std::vector<std::vector<double>> dnew(size_t(Nc), std::vector<double>(2 * n0, 0.0));
+  std::vector<double> jacc(4 * n0, 0.0), j(2 * n0, 0.0), anew(2 * n0, 0.0);
+  const double jfac = static_cast<double>(2 * sgn(o.cfg.charge_sign)) * o.T;
+  for (int k = 0; k < o.Na; ++k) {
+    rotatedOrderParameter(o, k, drot);
+    energyShift(o, k, es);
+    riccatiAngle(o, k, drot, es, true);
+    greensAngle(o, drot, gD, gJ, gF);
+    const signed char *lab = o.frames[k].label.data();
+    const float phi = static_cast<float>(o.systemAngle[k]);
+    const double wk = angleWeight(k, o.Na) / static_cast<double>(o.Na);
+    const double pa = polarAngle(k, o.Na);
+    extendValues(lab, N1, gD.data());
+    extendValues(lab, N1, gD.data() + n1);
+    std::fill(tmp.begin(), tmp.end(), 0.0);
+    insertRotated(tmp.data(), o.N0, o.N0, gD.data(), N1, N1, phi, false);
+    insertRotated(tmp.data() + n0, o.N0, o.N0, gD.data() + n1, N1, N1, phi, false);
+    for (int c = 0; c < Nc; ++c) {
+      const double b = basisMomentum(o, c, k);
+      for (size_t i = 0; i < 2 * n0; ++i) dnew[size_t(c)][i] += tmp[i] * b * wk;
+    }
+    if (doA) {
+      extendValues(lab, N1, gJ.data());
+      for (int f = 0; f < 2; ++f) {
+        const double fac = (f == 0 ? std::cos(pa) : std::sin(pa)) * wk;
+        std::vector<double> t2(gJ.begin(), gJ.begin() + n1);
+        for (double &v : t2) v *= fac;
+        insertRotated(jacc.data() + size_t(f) * n0, o.N0, o.N0, t2.data(), N1, N1, phi, true);
+      }
+    }
+  }
+  std::vector<std::vector<double>> rD(size_t(Nc), std::vector<double>(2 * n0, 0.0));
+  double R = 0.0;
+  for (int c = 0; c < Nc; ++c)
+    for (size_t i = 0; i < n0; ++i) {
+      if (o.domainRef[i] == 0) continue;
+      for (int q = 0; q < 2; ++q) {
+        const double v = dnew[size_t(c)][q * n0 + i] * o.coupling[c] - o.delta[c][q * n0 + i];
+        rD[size_t(c)][q * n0 + i] = v;
+        R += 0.5 * v * v;
+      }
+    }
+  std::vector<double> rA(2 * n0, 0.0), jbar(2 * n0, 0.0);
+  if (doA) {
+    for (int f = 0; f < 2; ++f)
+      for (size_t i = 0; i < n0; ++i)
+        j[f * n0 + i] = o.domainRef[i] != 0 ? jacc[f * n0 + i] * jfac : 0.0;
+    if (o.filter2D.empty())
+      makeFilter2D(o.N0, o.h, o.filter2D, std::max(o.cfg.vector_potential_error, 0.0));
+    solveVectorPotential(o.N0, o.h, o.numGrain, true, o.filter2D, j.data(), anew.data());
+    for (size_t i = 0; i < 2 * n0; ++i) {
+      rA[i] = (anew[i] - o.A[i]) / l2; // per unit of A (the state stores lambda^2 A)
+      R += 0.5 * rA[i] * rA[i];
+    }
+    // transpose of A' = -h^2 K * (j - mean_grain j) (solveVectorPotential's
+    // correlation: out(x) = sum_p src(p) K(p - x)), seeded with rA / lambda^2
+    const int N = o.N0, W = 2 * N - 1;
+    for (int d = 0; d < 2; ++d) {
+      std::vector<double> sbar(n0, 0.0);
+      for (int py = 0; py < N; ++py)
+        for (int px = 0; px < N; ++px) {
+          double sum = 0.0;
+          for (int y = 0; y < N; ++y)
+            for (int x = 0; x < N; ++x)
+              sum += rA[d * n0 + size_t(y) * N + x] / l2 *
+                     o.filter2D[size_t(W) * (py + N - 1 - y) + (px + N - 1 - x)];
+          sbar[size_t(py) * N + px] = -(o.h * o.h) * sum;
+        }
+      double tot = 0.0;
+      for (size_t i = 0; i < n0; ++i) tot += sbar[i];
+      const double mean = tot / static_cast<double>(o.numGrain);
+      for (size_t i = 0; i < n0; ++i)
+        jbar[d * n0 + i] = o.domainRef[i] != 0 ? (sbar[i] - mean) * jfac : 0.0;
+    }
+  }
+  // reverse, angle by angle
+  std::vector<double> seedD(2 * n0), rbarD(2 * n1), rbarJ(n1), dbar(2 * n1), esbar(n1),
+      arbar(n1), sumbar(2 * n0), rho(n1, 0.0);
+  std::vector<cplx> locG(size_t(Ne) * n1), locGt(size_t(Ne) * n1);
+  for (int k = 0; k < o.Na; ++k) {
+    rotatedOrderParameter(o, k, drot);
+    energyShift(o, k, es);
+    riccatiAngle(o, k, drot, es, true);
+    const signed char *lab = o.frames[k].label.data();
+    const float phi = static_cast<float>(o.systemAngle[k]);
+    const double wk = angleWeight(k, o.Na) / static_cast<double>(o.Na);
+    const double pa = polarAngle(k, o.Na);
+    // seeds of the frame's Green sums: transposes of the back-rotation and
+    // the extension
+    std::fill(seedD.begin(), seedD.end(), 0.0);
+    for (int c = 0; c < Nc; ++c) {
+      const double b = basisMomentum(o, c, k) * o.coupling[c] * wk;
+      for (size_t i = 0; i < 2 * n0; ++i) seedD[i] += b * rD[size_t(c)][i];
+    }
+    std::fill(rbarD.begin(), rbarD.end(), 0.0);
+    insertRotatedAdjoint(seedD.data(), o.N0, o.N0, rbarD.data(), N1, N1, phi);
+    insertRotatedAdjoint(seedD.data() + n0, o.N0, o.N0, rbarD.data() + n1, N1, N1, phi);
+    extendValuesAdjoint(lab, N1, rbarD.data());
+    extendValuesAdjoint(lab, N1, rbarD.data() + n1);
+    std::fill(rbarJ.begin(), rbarJ.end(), 0.0);
+    if (doA) {
+      std::vector<double> sj(n0);
+      for (size_t i = 0; i < n0; ++i)
+        sj[i] = wk * (std::cos(pa) * jbar[i] + std::sin(pa) * jbar[n0 + i]);
+      insertRotatedAdjoint(sj.data(), o.N0, o.N0, rbarJ.data(), N1, N1, phi);
+      extendValuesAdjoint(lab, N1, rbarJ.data());
+    }
+    // local adjoints of gamma and gamma~: with p = g gt, q2 = 1 / (1 + p)^2,
+    // gD = -2i g / (1 + p) + conj(2i gt / (1 + p)) and Re of
+    // gJ = -i (1 - p) / (1 + p) (dgJ/dp = 2i q2)
+    for (size_t i = 0; i < n1; ++i)
+      for (int e = 0; e < Ne; ++e) {
+        const size_t gi = size_t(e) * 2 * n1 + i;
+        const cplx g(o.gamma[gi], o.gamma[gi + n1]), gt(o.gammat[gi], o.gammat[gi + n1]);
+        const cplx Db(rbarD[i], rbarD[n1 + i]), Jb(rbarJ[i], 0.0);
+        const cplx q2 = cdiv(cplx(1.0, 0.0), cmul(1.0 + cmul(g, gt), 1.0 + cmul(g, gt)));
+        const cplx i2(0.0, 2.0);
+        const cplx du_dg = -cmul(i2, q2), du_dgt = cmul(i2, cmul(cmul(g, g), q2));
+        const cplx dv_dg = -cmul(i2, cmul(cmul(gt, gt), q2)), dv_dgt = cmul(i2, q2);
+        const cplx dj_dg = cmul(i2, cmul(gt, q2)), dj_dgt = cmul(i2, cmul(g, q2));
+        const double r = o.residue[e];
+        locG[size_t(e) * n1 + i] = r * (cmul(std::conj(du_dg), Db) + std::conj(cmul(dv_dg, Db)) +
+                                        cmul(std::conj(dj_dg), Jb));
+        locGt[size_t(e) * n1 + i] =
+            r * (cmul(std::conj(du_dgt), Db) + std::conj(cmul(dv_dgt, Db)) +
+                 cmul(std::conj(dj_dgt), Jb));
+      }
+    std::fill(dbar.begin(), dbar.end(), 0.0);
+    std::fill(esbar.begin(), esbar.end(), 0.0);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int x = o.xmin[k]; x <= o.xmax[k]; ++x)
+      for (int e = 0; e < Ne; ++e) {
+        reverseColumn<true>(o, k, x, e, drot, es, rho, dbar, esbar, &locG);
+        reverseColumn<false>(o, k, x, e, drot, es, rho, dbar, esbar, &locGt);
+      }
+    // the order parameter's rotation and extension, as in freeEnergyGradient
+    std::fill(sumbar.begin(), sumbar.end(), 0.0);
+    const int kk = wrapToRangeInt(-k, 0, o.Na);
+    const float phiD = static_cast<float>(o.systemAngle[kk]);
+    insertRotatedAdjoint(dbar.data(), N1, N1, sumbar.data(), o.N0, o.N0, phiD);
+    insertRotatedAdjoint(dbar.data() + n1, N1, N1, sumbar.data() + n0, o.N0, o.N0, phiD);
+    extendValuesAdjoint(o.domainRef.data(), o.N0, sumbar.data());
+    extendValuesAdjoint(o.domainRef.data(), o.N0, sumbar.data() + n0);
+    for (int c = 0; c < Nc; ++c) {
+      const double b = basisMomentum(o, c, k);
+      double *gc = grad.data() + size_t(c) * 2 * n0;
+      for (size_t i = 0; i < 2 * n0; ++i) gc[i] += b * sumbar[i];
+    }
+    if (doA) {
+      const double vf[2] = {std::cos(pa), std::sin(pa)};
+      const int q = sgn(o.cfg.charge_sign);
+      for (int f = 0; f < 2; ++f) {
+        for (size_t i = 0; i < n1; ++i) arbar[i] = static_cast<double>(q) * vf[f] * esbar[i];
+        insertRotatedAdjoint(arbar.data(), N1, N1, gradA.data() + size_t(f) * n0, o.N0, o.N0,
+                             static_cast<float>(-o.systemAngle[k]));
+      }
+    }
+  }
+  // the identity part of (J - I)^T r
+  for (int c = 0; c < Nc; ++c)
+    for (size_t i = 0; i < 2 * n0; ++i) grad[size_t(c) * 2 * n0 + i] -= rD[size_t(c)][i];
+  for (size_t i = 0; i < 2 * n0; ++i) gradA[i] -= rA[i];
+  o.bOutG = keepG;
+  o.bOutGt = keepGt;
+  return R;
 }
 
 // ---------------------------------------------------------------------------
@@ -3010,6 +3203,21 @@ int oc_free_energy_gradient(void *hh, double *grad, double *free_energy) {
     const double f = freeEnergyGradient(o, g);
     std::copy(g.begin(), g.end(), grad);
     if (free_energy) *free_energy = f;
+    return 0;
+  } catch (const std::exception &e) {
+    g_error = e.what();
+    return -1;
+  }
+}
+
+int oc_residual_gradient(void *hh, double *grad, double *gradA, double *residual) {
+  try {
+    Oracle &o = *static_cast<Oracle *>(hh);
+    std::vector<double> g, ga;
+    const double r = residualGradient(o, g, ga);
+    std::copy(g.begin(), g.end(), grad);
+    std::copy(ga.begin(), ga.end(), gradA);
+    if (residual) *residual = r;
     return 0;
   } catch (const std::exception &e) {
     g_error = e.what();

@@ -162,6 +162,11 @@ int oc_free_energy_gradient(void *handle, double *grad, double *free_energy);
  * y plane; zero without a penetration depth), A in the units the setters
  * take. */
 int oc_free_energy_gradient_a(void *handle, double *grad, double *grad_a, double *free_energy);
+/* The self-consistency residual R = 1/2 sum over inside sites of
+ * |Delta' - Delta|^2 (every component) + 1/2 sum |A' - A|^2 of one plain
+ * iteration with the boundary in-values held fixed, and its gradient in
+ * Delta (as above) and in A. */
+int oc_residual_gradient(void *handle, double *grad, double *grad_a, double *residual);
 
 /* ----- unit-level restatements (closed-form checks) ----- */
 void oc_riccati_step(const double *delta, const double *energy,

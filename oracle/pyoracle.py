@@ -110,6 +110,7 @@ def lib():
         L.oc_ldos_run.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.oc_free_energy_gradient.argtypes = [vp, dp, dp]
         L.oc_free_energy_gradient_a.argtypes = [vp, dp, dp, dp]
+        L.oc_residual_gradient.argtypes = [vp, dp, dp, dp]
         L.oc_sweep_angles.restype = ctypes.c_longlong
         L.oc_sweep_angles.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.oc_riccati_step.argtypes = [dp, dp, dp, ctypes.c_double, dp]
@@ -332,6 +333,21 @@ class Oracle:
 
     def sweep_angles(self, a0, a1):
         return lib().oc_sweep_angles(self.h, a0, a1)
+
+    def residual_gradient(self):
+        """(R, grad, grad_a): the self-consistency residual
+        R = 1/2 sum_inside |Delta' - Delta|^2 + 1/2 sum |A' - A|^2 of one
+        plain iteration (boundary in-values fixed) and its gradient, grad
+        as free_energy_gradient's, grad_a (2, N0, N0) per unit of A as
+        set_vector_potential takes it."""
+        n = self.N0
+        g = np.zeros(self.Nc * 2 * n * n)
+        ga = np.zeros(2 * n * n)
+        r = np.zeros(1)
+        if lib().oc_residual_gradient(self.h, _dp(g), _dp(ga), _dp(r)) != 0:
+            raise RuntimeError(lib().oc_last_error().decode())
+        g = g.reshape(self.Nc, 2, n, n)
+        return float(r[0]), g[:, 0] + 1j * g[:, 1], ga.reshape(2, n, n)
 
     def free_energy_gradient(self, vector_potential=False):
         """(F, grad): the free energy one iteration reports from the current
