@@ -219,6 +219,19 @@ int sc_free_energy_gradient(sc_context *ctx, double *grad, double *free_energy);
 int sc_free_energy_gradient_ex(sc_context *ctx, double *grad, double *grad_a,
                                double *free_energy);
 
+/* The self-consistency residual of one plain FP64 iteration u -> G(u) (the
+ * gap equation, the current and its vector potential without accelerator;
+ * the boundary in-values held fixed),
+ *   R = 1/2 sum over inside sites of |Delta'_c - Delta_c|^2 (all components)
+ *       + 1/2 sum over the grid of |A' - A|^2 (A as sc_set_vector_potential
+ *       takes it),
+ * and its gradient (J - I)^T (G(u) - u): grad as sc_free_energy_gradient's,
+ * grad_a (may be NULL; 2 x N0^2, zero without a penetration depth) in A.
+ * The same reverse sweep as the free-energy gradient, seeded through the
+ * Green functions' derivatives from the gap equation and the current.  The
+ * state is left as it was.  No counterpart in the reference. */
+int sc_residual_gradient(sc_context *ctx, double *grad, double *grad_a, double *residual);
+
 /* Individual stages. */
 /* RiccatiSolver::computeCoherenceFunctions for every angle (RiccatiSolverConfined.h:513-671)
  * fused with ComputeProperty::computeGreensAngle; burnin = 1 only updates the
@@ -371,7 +384,8 @@ int sc_batch_run(sc_batch *batch, int num_burnin, int num_min, int num_max,
  * its own Newton-Krylov solve (Krylov basis, line search, stopping), and
  * every round evaluates the maps of all grains still solving with one sweep
  * launch per precision (no counterpart in the reference, whose parameter
- * sweeps run one process per point, examples/parameter_sweeps/*.sh).
+ * sweeps run one process per point from the shell scripts in
+ * examples/parameter_sweeps/).
  * grains (one flag per grain, or null for all) selects the grains to
  * solve; the others are left as they are and their reports zeroed.
  * reports holds one sc_newton_report per grain (or is null).  On return

@@ -158,6 +158,7 @@ def lib():
         L.sc_batch_set_graphs.argtypes = [vp, ctypes.c_int]
         L.sc_free_energy_gradient.argtypes = [vp, _DP, _DP]
         L.sc_free_energy_gradient_ex.argtypes = [vp, _DP, _DP, _DP]
+        L.sc_residual_gradient.argtypes = [vp, _DP, _DP, _DP]
         L.sc_ldos_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(vp)]
         L.sc_ldos_compute.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.sc_ldos_run.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -346,6 +347,21 @@ class Simulation:
         if vector_potential:
             return float(f[0]), g[:, 0] + 1j * g[:, 1], ga.reshape(2, n, n)
         return float(f[0]), g[:, 0] + 1j * g[:, 1]
+
+    def residual_gradient(self):
+        """(R, grad, grad_a): the self-consistency residual
+        R = 1/2 sum_inside |Delta' - Delta|^2 + 1/2 sum |A' - A|^2 of one
+        plain FP64 iteration u -> G(u) (boundary in-values held fixed) and
+        its gradient (J - I)^T (G(u) - u): grad an (Nc, N0, N0) complex
+        array (dR/dRe + i dR/dIm), grad_a (2, N0, N0) in A as
+        set_vector_potential takes it.  The state is left as it was."""
+        n = self.N0
+        g = np.empty(self.Nc * 2 * n * n)
+        ga = np.empty(2 * n * n)
+        r = np.zeros(1)
+        _check(lib().sc_residual_gradient(self._h, _dp(g), _dp(ga), _dp(r)))
+        g = g.reshape(self.Nc, 2, n, n)
+        return float(r[0]), g[:, 0] + 1j * g[:, 1], ga.reshape(2, n, n)
 
     def riccati_sweep(self, precision="fp64", burnin=False):
         _check(lib().sc_riccati_sweep(self._h, _precision(precision), int(burnin)))

@@ -244,13 +244,40 @@ struct AdjointArgs {
   Cx<double> *scratch;    // per slot: gamma~ then gamma, [maxLen][EL][64] each
   long long slotStride;   // complex elements of one of the two buffers
   const double *rho;      // [Na][N1^2]
+  const double *seed;     // residual mode: [3][Na][N1^2] adjoints of the Green
+                          // sums per frame site (gap equation re, im; current)
   double *dbar;           // [3][Na][N1^2]: dF/dD re, im, dF/des
   long long framePlane;   // Na * N1^2
   double *partial;        // per item: its share of the free-energy sum
 };
 
+// Local adjoints of gamma and gamma~ at one site and energy through the
+// Green sums of the gap equation and the current (the sweep's Green terms,
+// LocalGreens.h:32-70): with p = g gt and Q = 2i / (1 + p)^2,
+//   gD = -2i g / (1 + p) + conj(2i gt / (1 + p)),  Re gJ, gJ = -i (1 - p) / (1 + p),
+//   dgD/dg = -Q and, conjugated, -Q gt^2;  dgD/dgt = Q g^2 and, conjugated, Q;
+//   dgJ/dg = Q gt,  dgJ/dgt = Q g,
+// seeded with db (the gap sum's adjoint, complex) and jb (the current's,
+// real), times the residue r.
+__device__ __forceinline__ void greensAdjoint(Cx<double> g, Cx<double> gt, Cx<double> db, double jb,
+                                              double r, Cx<double> &gbar, Cx<double> &gtbar) {
+  const Cx<double> z = {1.0 + (g.re * gt.re - g.im * gt.im), g.re * gt.im + g.im * gt.re};
+  const Cx<double> z2 = z * z;
+  const double rn = rcpT(z2.re * z2.re + z2.im * z2.im);
+  // Q = 2i conj(z2) / |z2|^2
+  const Cx<double> Q = {2.0 * z2.im * rn, 2.0 * z2.re * rn};
+  const Cx<double> cQ = conjc(Q);
+  const Cx<double> Qgt = Q * gt, Qg = Q * g;
+  const Cx<double> jbc = {jb, 0.0};
+  gbar = Cx<double>{-r, 0.0} * (cQ * db + conjc(Qgt * gt * db)) + Cx<double>{r, 0.0} * (conjc(Qgt) * jbc);
+  gtbar = Cx<double>{r, 0.0} * (conjc(Qg * g) * db + conjc(Q * db) + conjc(Qg) * jbc);
+}
+
 // 2. The reverse sweep of one item (the sweep's items and lane layout).
-template <int EL>
+// RES: the residual mode, seeded from a.seed through greensAdjoint instead
+// of the free-energy density; pass 2 leaves the local adjoint of gamma in
+// the gamma~ scratch (read there already) for pass 3.
+template <int EL, bool RES>
 __device__ void adjointItem(const AdjointArgs &a, const SweepItemDev &it, int itemIndex,
                             Cx<double> *scrT, Cx<double> *scrG) {
   const GrainView &v = a.view;
@@ -268,6 +295,7 @@ __device__ void adjointItem(const AdjointArgs &a, const SweepItemDev &it, int it
   const int8_t *lab = v.labels + size_t(k) * n1 + x;
   const double *F = v.fields + size_t(k - v.fieldBase) * n1 + x;
   const double *rhoK = a.rho + size_t(k) * n1 + x;
+  const double *seedK = RES ? a.seed + size_t(k) * n1 + x : nullptr;
   double *dbK = a.dbar + size_t(k) * n1 + x;
   double ser[EL], res[EL];
   bool eOk[EL];
@@ -371,7 +399,7 @@ __device__ void adjointItem(const AdjointArgs &a, const SweepItemDev &it, int it
       const double dre = F[o], dim = F[v.fieldPlane + o], es = F[2 * v.fieldPlane + o];
       const Cx<double> D = {dre, dim}, Dt = {-dre, dim};
       const int8_t lc = lab[o];
-      const double rho = rhoK[o];
+      const double rho = RES ? 0.0 : rhoK[o];
       Cx<double> gt[EL];
 #pragma unroll
       for (int j = 0; j < EL; ++j) gt[j] = active ? scrT[slot(t, j)] : Cx<double>{0.0, 0.0};
@@ -401,16 +429,18 @@ __device__ void adjointItem(const AdjointArgs &a, const SweepItemDev &it, int it
         }
 #pragma unroll
         for (int j = 0; j < EL; ++j) scrG[slot(t, j)] = coh[j];
-        // the free-energy density and its own dependence on D
-        double su = 0.0, sw = 0.0;
+        if constexpr (!RES) {
+          // the free-energy density and its own dependence on D
+          double su = 0.0, sw = 0.0;
 #pragma unroll
-        for (int j = 0; j < EL; ++j) {
-          if (!eOk[j]) continue;
-          su += res[j] * (gt[j].im - coh[j].im);
-          sw += res[j] * (coh[j].re + gt[j].re);
+          for (int j = 0; j < EL; ++j) {
+            if (!eOk[j]) continue;
+            su += res[j] * (gt[j].im - coh[j].im);
+            sw += res[j] * (coh[j].re + gt[j].re);
+          }
+          omega += rho * (dre * su + dim * sw);
+          dbCur = {rho * su, rho * sw, 0.0};
         }
-        omega += rho * (dre * su + dim * sw);
-        dbCur = {rho * su, rho * sw, 0.0};
         // adjoint of gamma~ here: what the step into the site below carries
         // back, plus dF/dgt = rho r (D.im + i D.re)
         Cx<double> carry[EL];
@@ -429,10 +459,22 @@ __device__ void adjointItem(const AdjointArgs &a, const SweepItemDev &it, int it
           addBar(dbCur, dd, 0.5 * eb);
           addBar(dbB, dd, 0.5 * eb);
         }
+        if constexpr (RES) {
+          const Cx<double> sd = {seedK[o], seedK[a.framePlane + o]};
+          const double sj = seedK[2 * a.framePlane + o];
 #pragma unroll
-        for (int j = 0; j < EL; ++j)
-          mu[j] = eOk[j] ? carry[j] + Cx<double>{rho * res[j] * dim, rho * res[j] * dre}
-                         : Cx<double>{0.0, 0.0};
+          for (int j = 0; j < EL; ++j) {
+            Cx<double> lg = {0.0, 0.0}, lt = {0.0, 0.0};
+            if (eOk[j]) greensAdjoint(coh[j], gt[j], sd, sj, res[j], lg, lt);
+            mu[j] = eOk[j] ? carry[j] + lt : Cx<double>{0.0, 0.0};
+            scrT[slot(t, j)] = lg;
+          }
+        } else {
+#pragma unroll
+          for (int j = 0; j < EL; ++j)
+            mu[j] = eOk[j] ? carry[j] + Cx<double>{rho * res[j] * dim, rho * res[j] * dre}
+                           : Cx<double>{0.0, 0.0};
+        }
         // gamma~ entered here from its in-value (the entries in reverse order)
         if (lc == 3) {
           const int ci = baseInT + (numEntryT - 1 - seenT);
@@ -494,7 +536,7 @@ __device__ void adjointItem(const AdjointArgs &a, const SweepItemDev &it, int it
       const double dre = F[o], dim = F[v.fieldPlane + o], es = F[2 * v.fieldPlane + o];
       const Cx<double> D = {dre, dim};
       const int8_t lc = lab[o];
-      const double rho = rhoK[o];
+      const double rho = RES ? 0.0 : rhoK[o];
       Cx<double> gc[EL];
 #pragma unroll
       for (int j = 0; j < EL; ++j) gc[j] = active ? scrG[slot(t, j)] : Cx<double>{0.0, 0.0};
@@ -516,11 +558,18 @@ __device__ void adjointItem(const AdjointArgs &a, const SweepItemDev &it, int it
           addBar(dbCur, dd, 0.5 * eb);
           addBar(dbA, dd, 0.5 * eb);
         }
-        // dF/dg = rho r (D.im - i D.re)
+        if constexpr (RES) {
+          // the local adjoint of gamma that pass 2 left in the gamma~ scratch
 #pragma unroll
-        for (int j = 0; j < EL; ++j)
-          lam[j] = eOk[j] ? carry[j] + Cx<double>{rho * res[j] * dim, -rho * res[j] * dre}
-                          : Cx<double>{0.0, 0.0};
+          for (int j = 0; j < EL; ++j)
+            lam[j] = eOk[j] ? carry[j] + scrT[slot(t, j)] : Cx<double>{0.0, 0.0};
+        } else {
+          // dF/dg = rho r (D.im - i D.re)
+#pragma unroll
+          for (int j = 0; j < EL; ++j)
+            lam[j] = eOk[j] ? carry[j] + Cx<double>{rho * res[j] * dim, -rho * res[j] * dre}
+                            : Cx<double>{0.0, 0.0};
+        }
         // gamma entered here from its in-value (the entries in reverse order)
         if (lc == 2) {
           const int ci = baseInG + (numEntryG - 1 - seenG);
@@ -570,7 +619,7 @@ __device__ void adjointItem(const AdjointArgs &a, const SweepItemDev &it, int it
   a.partial[itemIndex] = omega;
 }
 
-template <int EL>
+template <int EL, bool RES>
 __global__ __launch_bounds__(64, 2) void adjointKernel(AdjointArgs a) {
   Cx<double> *scrT = a.scratch + size_t(blockIdx.x) * 2 * a.slotStride;
   Cx<double> *scrG = scrT + a.slotStride;
@@ -581,25 +630,121 @@ __global__ __launch_bounds__(64, 2) void adjointKernel(AdjointArgs a) {
     const int item =
         __builtin_amdgcn_readfirstlane(atomicAdd(a.counter, threadIdx.x == 0 ? 1 : 0));
     if (item >= a.numItems) break;
-    adjointItem<EL>(a, a.items[item], item, scrT, scrG);
+    adjointItem<EL, RES>(a, a.items[item], item, scrT, scrG);
   }
 }
 
 __global__ void adjointResetCounter(int *c) { *c = 0; }
 
+// Residual mode, 1. The residual r = G(u) - u per unit of the setters'
+// variables: the order parameter at inside sites (G's is zero outside), the
+// vector potential (stored as lambda^2 A) everywhere; and r_A / lambda^2,
+// the seed of the Poisson solve's output.
+__global__ void residualSeedKernel(const double *delta, const double *deltaNew,
+                                   const uint8_t *domain, const double *A, const double *ANew,
+                                   long long n0, int Nc, double invL2, double *rD, double *rA,
+                                   double *poisSeed) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n0) return;
+  const bool in = domain[i] != 0;
+  for (int q = 0; q < 2 * Nc; ++q)
+    rD[q * n0 + i] = in ? deltaNew[q * n0 + i] - delta[q * n0 + i] : 0.0;
+  if (rA) {
+    for (int q = 0; q < 2; ++q) {
+      const double r = (ANew[q * n0 + i] - A[q * n0 + i]) * invL2;
+      rA[q * n0 + i] = r;
+      poisSeed[q * n0 + i] = r * invL2;
+    }
+  }
+}
+
+// 2. Transpose of the grain-mean subtraction and of the current's mask and
+// factor: jbar = (sbar - sum(sbar) / Ng) jf at inside sites.
+__global__ void currentBarKernel(const double *sbar, const double *mean, const uint8_t *domain,
+                                 long long n0, double jf, double *jbar) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= 2 * n0) return;
+  const long long site = i % n0;
+  jbar[i] = domain[site] ? (sbar[i] - mean[i / n0]) * jf : 0.0;
+}
+
+// The sum of each of the two planes of x (block b: plane b) times scale.
+__global__ void planeMeanKernel(const double *x, long long n, double scale, double *out) {
+  const double *p = x + size_t(blockIdx.x) * n;
+  double s = 0.0;
+  for (long long i = threadIdx.x; i < n; i += blockDim.x) s += p[i];
+  __shared__ double sh[256];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int m = blockDim.x / 2; m > 0; m >>= 1) {
+    if (threadIdx.x < m) sh[threadIdx.x] += sh[threadIdx.x + m];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = sh[0] * scale;
+}
+
+// 3. Transpose of the back-rotation and angle integration (gatherAngle and
+// gatherStore): every inside site scatters, per frame, its gap-equation
+// adjoint w_k sum_c coupling_c b_c(k) rD_c and its current adjoint
+// w_k v_k . jbar to the four taps of the frame's Green planes
+// seed[0..2] (gap re, gap im, current).
+__global__ void gatherAdjointKernel(GrainView v, const int *sites, int numSites,
+                                    const double *coupling, const double *rD,
+                                    const double *jbar, long long framePlane, double *seed) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= numSites) return;
+  const int N0 = v.N0, N1 = v.N1, Na = v.Na, Nc = v.Nc;
+  const size_t n0 = size_t(N0) * N0, n1 = size_t(N1) * N1;
+  const int site = sites[t];
+  const int x = site % N0, y = site / N0;
+  double dr[4] = {0, 0, 0, 0}, di[4] = {0, 0, 0, 0};
+  for (int c = 0; c < Nc && c < 4; ++c) {
+    dr[c] = coupling[c] * rD[size_t(c) * 2 * n0 + site];
+    di[c] = coupling[c] * rD[size_t(c) * 2 * n0 + n0 + site];
+  }
+  const double jx = jbar ? jbar[site] : 0.0, jy = jbar ? jbar[n0 + site] : 0.0;
+  for (int k = 0; k < Na; ++k) {
+    const RotSample r = rotatedCell(x, y, N0, N1, double(v.bkCos[k]), double(v.bkSin[k]));
+    if (!r.valid) continue;
+    const double w = v.weight[k];
+    double sr = 0.0, si = 0.0;
+    for (int c = 0; c < Nc && c < 4; ++c) {
+      const double b = v.basis[c * Na + k];
+      sr += dr[c] * b;
+      si += di[c] * b;
+    }
+    const double vals[3] = {sr * w, si * w, (v.vFx[k] * jx + v.vFy[k] * jy) * w};
+    const double w00 = (1.0 - r.fy) * (1.0 - r.fx), w10 = (1.0 - r.fy) * r.fx;
+    const double w01 = r.fy * (1.0 - r.fx), w11 = r.fy * r.fx;
+    for (int q = 0; q < 3; ++q) {
+      if (vals[q] == 0.0) continue;
+      double *p = seed + q * framePlane + size_t(k) * n1 + r.idx;
+      atomicAdd(p, vals[q] * w00);
+      atomicAdd(p + 1, vals[q] * w10);
+      atomicAdd(p + N1, vals[q] * w01);
+      atomicAdd(p + N1 + 1, vals[q] * w11);
+    }
+  }
+}
+
+// 4. The identity part of (J - I)^T r.
+__global__ void residualFinishKernel(const double *extBar, const double *rD, long long n,
+                                     double *grad, double *gradA, const double *rA, long long nA) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) grad[i] = extBar[i] - rD[i];
+  if (gradA && i < nA) gradA[i] -= rA[i];
+}
+
 } // namespace
 
-double Grain::freeEnergyGradient(double *grad, double *gradA) {
+// The reverse sweep's items (the sweep's own, at kAdjEL energies per lane),
+// its slots' scratch and the adjoint planes, once per grain.
+void Grain::prepareAdjoint() {
   const Setup &s = *s_;
-  if (s.keldysh) throw std::runtime_error("a spectroscopy context has no free-energy gradient");
-  if (s.Nc > 4) throw std::runtime_error("at most 4 order-parameter components");
   const size_t n0 = size_t(s.N0) * s.N0, n1 = size_t(s.N1) * s.N1;
   const long long framePlane = static_cast<long long>(s.Na) * static_cast<long long>(n1);
-  // the order parameter's extension and the site fields of every frame
-  ensureBoundaryInit();
-  extendDelta();
-  // the reverse sweep's items (the sweep's own, at kAdjEL energies per lane)
-  if (!adjItems_) {
+  if (adjItems_) return;
+  {
     const GrainView gv = sweepView(kAdjEL);
     std::vector<SweepItem> items = sweepItems(s, gv.cols, &adjMaxLen_);
     std::vector<SweepItemDev> di(items.size());
@@ -621,6 +766,18 @@ double Grain::freeEnergyGradient(double *grad, double *gradA) {
     dAdjGrad_ = devAlloc<double>(size_t(2 * s.Nc) * n0);
     dAdjGradA_ = devAlloc<double>(2 * n0);
   }
+}
+
+double Grain::freeEnergyGradient(double *grad, double *gradA) {
+  const Setup &s = *s_;
+  if (s.keldysh) throw std::runtime_error("a spectroscopy context has no free-energy gradient");
+  if (s.Nc > 4) throw std::runtime_error("at most 4 order-parameter components");
+  const size_t n0 = size_t(s.N0) * s.N0, n1 = size_t(s.N1) * s.N1;
+  const long long framePlane = static_cast<long long>(s.Na) * static_cast<long long>(n1);
+  // the order parameter's extension and the site fields of every frame
+  ensureBoundaryInit();
+  extendDelta();
+  prepareAdjoint();
   // A enters through the Doppler shift only with a penetration depth
   const bool withA = gradA != nullptr && s.lambda > 0.0;
   SC_HIP(hipMemsetAsync(dAdjRho_, 0, size_t(framePlane) * sizeof(double), stream_));
@@ -651,12 +808,13 @@ double Grain::freeEnergyGradient(double *grad, double *gradA) {
   a.scratch = reinterpret_cast<Cx<double> *>(dAdjScratch_);
   a.slotStride = static_cast<long long>(adjMaxLen_) * kAdjEL * 64;
   a.rho = dAdjRho_;
+  a.seed = nullptr;
   a.dbar = dAdjDbar_;
   a.framePlane = framePlane;
   a.partial = dAdjPartial_;
   adjointResetCounter<<<1, 1, 0, stream_>>>(dCounter_);
   if (adjNumItems_ > 0) {
-    adjointKernel<kAdjEL><<<adjSlots_, 64, 0, stream_>>>(a);
+    adjointKernel<kAdjEL, false><<<adjSlots_, 64, 0, stream_>>>(a);
     SC_HIP(hipGetLastError());
   }
   // 3. the rotation, basis and extension of the order parameter
@@ -703,6 +861,153 @@ double Grain::freeEnergyGradient(double *grad, double *gradA) {
   const double sweepPart = reductionSum(hSweep);
   clearReductions();
   return sweepPart + (nonMag + mag) * area;
+}
+
+double Grain::residualGradient(double *grad, double *gradA) {
+  const Setup &s = *s_;
+  if (s.keldysh) throw std::runtime_error("a spectroscopy context has no residual gradient");
+  if (s.Nc > 4) throw std::runtime_error("at most 4 order-parameter components");
+  const size_t n0 = size_t(s.N0) * s.N0, n1 = size_t(s.N1) * s.N1;
+  const long long framePlane = static_cast<long long>(s.Na) * static_cast<long long>(n1);
+  const bool doA = s.cfg.computeCurrent && s.lambda > 0.0;
+  const bool withA = gradA != nullptr && doA;
+  const size_t nd = size_t(2 * s.Nc) * n0;
+  prepareAdjoint();
+  // the state the forward map overwrites: G's order parameter and vector
+  // potential, the current, the energy density and the boundary out-values
+  const size_t nNew = nd + 2 * n0;
+  const size_t nOutG = size_t(2 * s.nOutG * s.Ne), nOutGt = size_t(2 * s.nOutGt * s.Ne);
+  const size_t nSave = nNew + 2 * n0 + n0 + nOutG + nOutGt;
+  if (!dResD_) {
+    dResD_ = devAlloc<double>(nd);
+    dResA_ = devAlloc<double>(2 * n0);
+    dResPois_ = devAlloc<double>(2 * n0);
+    dResSbar_ = devAlloc<double>(2 * n0);
+    dResJbar_ = devAlloc<double>(2 * n0);
+    dResZero_ = devAlloc<double>(2);
+    dResMean_ = devAlloc<double>(2);
+    dResSeed_ = devAlloc<double>(3 * size_t(framePlane));
+    dResSave_ = devAlloc<double>(nSave);
+    SC_HIP(hipMemsetAsync(dResZero_, 0, 2 * sizeof(double), stream_));
+  }
+  struct Piece {
+    double *p;
+    size_t n;
+  };
+  const Piece pieces[] = {{dDeltaNew_, nNew}, {dCurrent_, 2 * n0}, {dEnergyDens_, n0},
+                          {dBOutG_, nOutG}, {dBOutGt_, nOutGt}};
+  {
+    size_t off = 0;
+    for (const Piece &pc : pieces) {
+      if (pc.n > 0)
+        SC_HIP(hipMemcpyAsync(dResSave_ + off, pc.p, pc.n * sizeof(double),
+                              hipMemcpyDeviceToDevice, stream_));
+      off += pc.n;
+    }
+  }
+  // 1. G(u): one plain FP64 iteration without the accelerator and without
+  // the boundary condition (the in-values stay as they are)
+  ensureBoundaryInit();
+  extendDelta(true);
+  sweep(PREC_FP64, false);
+  backRotate();
+  if (doA) solveVectorPotential(dCurrent_, dANew_);
+  clearReductions();
+  const double invL2 = doA ? 1.0 / (s.lambda * s.lambda) : 1.0;
+  residualSeedKernel<<<blocksFor(static_cast<long long>(n0)), 256, 0, stream_>>>(
+      dDelta_, dDeltaNew_, dDomain_, dA_, dANew_, static_cast<long long>(n0), s.Nc, invL2, dResD_,
+      doA ? dResA_ : nullptr, dResPois_);
+  SC_HIP(hipGetLastError());
+  const GrainView v = view();
+  // 2. the current's adjoint: A' = -h^2 K * (j - mean_grain j) with the
+  // kernel symmetric under reflection, so its transpose is the same solve
+  // with zero means, then the transpose of the mean subtraction
+  if (doA) {
+    fft_->solve(dResPois_, dResZero_, dResSbar_);
+    planeMeanKernel<<<2, 256, 0, stream_>>>(dResSbar_, static_cast<long long>(n0),
+                                            1.0 / static_cast<double>(s.numGrain), dResMean_);
+    SC_HIP(hipGetLastError());
+    const double jf = 2.0 * static_cast<double>(s.cfg.chargeSign) * s.temperature;
+    currentBarKernel<<<blocksFor(2 * static_cast<long long>(n0)), 256, 0, stream_>>>(
+        dResSbar_, dResMean_, dDomain_, static_cast<long long>(n0), jf, dResJbar_);
+    SC_HIP(hipGetLastError());
+  }
+  const int hD = queueSum(dResD_, nullptr, nullptr, static_cast<long long>(nd), 2);
+  const int hA = doA ? queueSum(dResA_, nullptr, nullptr, 2 * static_cast<long long>(n0), 2) : -1;
+  // 3. the seeds of every frame's Green sums: transposes of the
+  // back-rotation and of the frames' extension
+  SC_HIP(hipMemsetAsync(dResSeed_, 0, 3 * size_t(framePlane) * sizeof(double), stream_));
+  const int ns = static_cast<int>(s.domainSites.size());
+  if (ns > 0) {
+    gatherAdjointKernel<<<blocksFor(ns), 256, 0, stream_>>>(
+        v, dDomainSites_, ns, dCoupling_, dResD_, doA ? dResJbar_ : nullptr, framePlane,
+        dResSeed_);
+    SC_HIP(hipGetLastError());
+  }
+  const int nr = static_cast<int>(s.extRot.target.size());
+  if (nr > 0) {
+    extendAdjointKernel<<<blocksFor(nr), 256, 0, stream_>>>(dResSeed_, dExtRotT_, dExtRotO_,
+                                                            dExtRotS_, nr, 3, framePlane);
+    SC_HIP(hipGetLastError());
+  }
+  // 4. the reverse sweep in residual mode
+  SC_HIP(hipMemsetAsync(dAdjDbar_, 0, 3 * size_t(framePlane) * sizeof(double), stream_));
+  SC_HIP(hipMemsetAsync(dAdjExt_, 0, nd * sizeof(double), stream_));
+  if (withA) SC_HIP(hipMemsetAsync(dAdjGradA_, 0, 2 * n0 * sizeof(double), stream_));
+  AdjointArgs a;
+  a.view = sweepView(kAdjEL);
+  a.items = reinterpret_cast<const SweepItemDev *>(adjItems_);
+  a.numItems = adjNumItems_;
+  a.counter = dCounter_;
+  a.scratch = reinterpret_cast<Cx<double> *>(dAdjScratch_);
+  a.slotStride = static_cast<long long>(adjMaxLen_) * kAdjEL * 64;
+  a.rho = dAdjRho_;
+  a.seed = dResSeed_;
+  a.dbar = dAdjDbar_;
+  a.framePlane = framePlane;
+  a.partial = dAdjPartial_;
+  adjointResetCounter<<<1, 1, 0, stream_>>>(dCounter_);
+  if (adjNumItems_ > 0) {
+    adjointKernel<kAdjEL, true><<<adjSlots_, 64, 0, stream_>>>(a);
+    SC_HIP(hipGetLastError());
+  }
+  // 5. the transposes of the site fields and of the order parameter's
+  // extension, then the identity part
+  fieldsAdjointKernel<<<blocksFor(framePlane), 256, 0, stream_>>>(
+      v, dAdjDbar_, framePlane, dAdjExt_, withA ? dAdjGradA_ : nullptr);
+  SC_HIP(hipGetLastError());
+  const int ne = static_cast<int>(s.extRef.target.size());
+  if (ne > 0) {
+    extendAdjointKernel<<<blocksFor(ne), 256, 0, stream_>>>(dAdjExt_, dExtRefT_, dExtRefO_,
+                                                            dExtRefS_, ne, 2 * s.Nc,
+                                                            static_cast<long long>(n0));
+    SC_HIP(hipGetLastError());
+  }
+  residualFinishKernel<<<blocksFor(static_cast<long long>(nd)), 256, 0, stream_>>>(
+      dAdjExt_, dResD_, static_cast<long long>(nd), dAdjGrad_, withA ? dAdjGradA_ : nullptr,
+      dResA_, 2 * static_cast<long long>(n0));
+  SC_HIP(hipGetLastError());
+  fromDevice(grad, dAdjGrad_, nd * sizeof(double));
+  if (withA)
+    fromDevice(gradA, dAdjGradA_, 2 * n0 * sizeof(double));
+  else if (gradA)
+    std::fill(gradA, gradA + 2 * n0, 0.0);
+  flushReductions();
+  double R = 0.5 * reductionSum(hD);
+  if (hA >= 0) R += 0.5 * reductionSum(hA);
+  clearReductions();
+  // the state as it was
+  {
+    size_t off = 0;
+    for (const Piece &pc : pieces) {
+      if (pc.n > 0)
+        SC_HIP(hipMemcpyAsync(pc.p, dResSave_ + off, pc.n * sizeof(double),
+                              hipMemcpyDeviceToDevice, stream_));
+      off += pc.n;
+    }
+  }
+  SC_HIP(hipStreamSynchronize(stream_));
+  return R;
 }
 
 } // namespace scamd

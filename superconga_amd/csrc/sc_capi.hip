@@ -468,6 +468,17 @@ int sc_free_energy_gradient(sc_context *ctx, double *grad, double *freeEnergy) {
   return sc_free_energy_gradient_ex(ctx, grad, nullptr, freeEnergy);
 }
 
+int sc_residual_gradient(sc_context *ctx, double *grad, double *gradA, double *residual) {
+  SC_CHECK_CTX(ctx);
+  SC_CHECK_SIM(ctx);
+  SC_CHECK_PTR(grad);
+  return guarded([&] {
+    const double r = ctx->grain->residualGradient(grad, gradA);
+    if (residual) *residual = r;
+    return SC_OK;
+  });
+}
+
 int sc_free_energy_gradient_ex(sc_context *ctx, double *grad, double *gradA,
                                double *freeEnergy) {
   SC_CHECK_CTX(ctx);

@@ -111,6 +111,9 @@ Grain::~Grain() {
   devFree(dAdjExt_);
   devFree(dAdjGrad_);
   devFree(dAdjGradA_);
+  for (double *p : {dResD_, dResA_, dResPois_, dResSbar_, dResJbar_, dResZero_, dResMean_,
+                    dResSeed_, dResSave_})
+    devFree(p);
   dropEndGraph();
   fft_.reset();
   screen_.reset();

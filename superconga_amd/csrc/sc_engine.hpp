@@ -166,6 +166,11 @@ public:
   // unit of A as setVectorPotential takes it (zero without a penetration
   // depth).  The state is left as it was.
   double freeEnergyGradient(double *grad, double *gradA = nullptr);
+  // The self-consistency residual R = 1/2 sum_inside |Delta' - Delta|^2 +
+  // 1/2 sum |A' - A|^2 of one plain FP64 iteration u -> G(u) (boundary
+  // in-values held fixed) and its gradient (J - I)^T (G(u) - u), grad and
+  // gradA laid out as freeEnergyGradient's.  The state is left as it was.
+  double residualGradient(double *grad, double *gradA = nullptr);
   void resetAccelerator();
 
   // Keldysh spectroscopy (src/postprocess.cu), for a grain built by
@@ -352,6 +357,13 @@ private:
   double *dAdjScratch_ = nullptr, *dAdjPartial_ = nullptr, *dAdjRho_ = nullptr,
          *dAdjDbar_ = nullptr, *dAdjExt_ = nullptr, *dAdjGrad_ = nullptr,
          *dAdjGradA_ = nullptr;
+  // the residual gradient's buffers: rD, rA, the Poisson seed and its
+  // transpose, jbar, two zero means, two means, the frames' Green-sum seeds,
+  // the saved state
+  double *dResD_ = nullptr, *dResA_ = nullptr, *dResPois_ = nullptr, *dResSbar_ = nullptr,
+         *dResJbar_ = nullptr, *dResZero_ = nullptr, *dResMean_ = nullptr, *dResSeed_ = nullptr,
+         *dResSave_ = nullptr;
+  void prepareAdjoint();
   int hA_ = -1; // queued residual of the vector potential
   // the captured stages after the sweep (queueEndCompute)
   void queueEndStages();

@@ -293,3 +293,96 @@ def test_vector_potential_gradient_central_differences_full_size_d_plus_is(produ
     assert err[1e-6] < 3e-4
     assert err[1e-3] < 5e-7
     s.close()
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_residual_gradient_matches_oracle(product, oracle, name):
+    # sc_residual_gradient against the oracle's restatement (pinned by
+    # central differences in tests/test_oracle_gradient.py): R and every grid
+    # point of the gradient in Delta and A; the evaluation leaves the state
+    # as it was (the next iteration still matches the oracle's)
+    cfg = CASES[name]()
+    d, a = state(product, cfg)
+    s, o = fresh_pair(product, oracle, cfg, d, a)
+    r, g, ga = s.residual_gradient()
+    ro, go, gao = o.residual_gradient()
+    err = float(np.max(np.abs(g - go)) / np.max(np.abs(go)))
+    err_a = float(np.max(np.abs(ga - gao)) / np.max(np.abs(gao))) if lam(cfg) > 0 else 0.0
+    print("%s: R %.12e (oracle %.12e), gradient error %.2e of max, in A %.2e of max" % (
+        name, r, ro, err, err_a))
+    assert abs(r - ro) <= 1e-10 * ro
+    assert err < TOL
+    assert err_a < TOL
+    if lam(cfg) <= 0:
+        assert not np.any(ga)
+    s.compute("fp64")
+    o.compute()
+    dd = s.order_parameter(1)
+    assert float(np.max(np.abs(dd - o.delta(True))) / np.max(np.abs(dd))) < TOL
+    assert abs(s.results()["free_energy"] - o.get("free_energy")) <= 1e-10 * abs(
+        o.get("free_energy"))
+    s.close()
+
+
+def test_residual_gradient_central_differences_full_size_d_plus_is(product):
+    # the full-size d + is square two iterations in: central differences of
+    # R in Delta (24 seeded inside sites, both components, re and im) and in
+    # A (16 sites, two of them far outside) at steps 1e-6 and 1e-4
+    import time
+    from superconga_amd import workloads
+    cfg = workloads.dplus_is_square()
+    s = product.Simulation(cfg)
+    d0 = s.order_parameter(0)
+    for c, x in enumerate(workloads.d_plus_is_initial(d0[0])):
+        s.set_order_parameter(c, x)
+    for _ in range(2):
+        s.compute("fp64")
+    d = s.order_parameter(0)
+    a = s.vector_potential(0)
+    s.set_vector_potential(a[0], a[1])
+    t = time.perf_counter()
+    r0, g, ga = s.residual_gradient()
+    t = time.perf_counter() - t
+    dom = s.domain() > 0
+    rng = np.random.default_rng(13)
+    pts = [tuple(p) for p in np.argwhere(dom)[rng.choice(int(dom.sum()), 24, replace=False)]]
+    n = s.N0
+    pts_a = pts[:14] + [(1, 1), (n // 2, 2)]
+    gmax, gamax = float(np.max(np.abs(g))), float(np.max(np.abs(ga)))
+    out = {}
+    for step in (1e-6, 1e-4):
+        worst = 0.0
+        for (y, x) in pts:
+            for c in range(2):
+                for unit in (1.0, 1j):
+                    dd = d[c].copy()
+                    dd[y, x] += step * unit
+                    s.set_order_parameter(c, dd)
+                    fp, _, _ = s.residual_gradient()
+                    dd[y, x] -= 2 * step * unit
+                    s.set_order_parameter(c, dd)
+                    fm, _, _ = s.residual_gradient()
+                    s.set_order_parameter(c, d[c])
+                    an = g[c][y, x].real if unit == 1.0 else g[c][y, x].imag
+                    worst = max(worst, abs((fp - fm) / (2 * step) - an))
+        worst_a = 0.0
+        for (y, x) in pts_a:
+            for f in range(2):
+                aa = a.copy()
+                aa[f, y, x] += step
+                s.set_vector_potential(aa[0], aa[1])
+                fp, _, _ = s.residual_gradient()
+                aa[f, y, x] -= 2 * step
+                s.set_vector_potential(aa[0], aa[1])
+                fm, _, _ = s.residual_gradient()
+                worst_a = max(worst_a, abs((fp - fm) / (2 * step) - ga[f, y, x]))
+        s.set_vector_potential(a[0], a[1])
+        out[step] = (worst / gmax, worst_a / gamax)
+    print("full-size d+is square: R %.6e, one residual gradient %.1f ms, |grad| max %.3e, in A "
+          "%.3e; FD error of max (Delta, A): %.2e, %.2e at step 1e-6; %.2e, %.2e at step 1e-4" % (
+              r0, 1e3 * t, gmax, gamax, out[1e-6][0], out[1e-6][1], out[1e-4][0], out[1e-4][1]))
+    # measured 4e-5 / 2e-5 at step 1e-6 (rounding of R, 2.6e4 here, over the
+    # step) and 5e-7 / 2e-7 at step 1e-4
+    assert out[1e-6][0] < 2e-4 and out[1e-6][1] < 2e-4
+    assert out[1e-4][0] < 2e-6 and out[1e-4][1] < 2e-6
+    s.close()
