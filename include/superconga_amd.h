@@ -32,6 +32,7 @@ extern "C" {
 #define SC_PRECISION_FP64 0  /* everything in FP64 */
 #define SC_PRECISION_MIXED 1 /* FP32 stepping, FP64 energy sums and reductions */
 #define SC_PRECISION_MIXED_REFINE 2 /* sc_run, sc_batch_run: MIXED to the FP32 floor, then FP64 */
+#define SC_JVP_TANGENT 3 /* sc_newton_krylov_ex, sc_batch_newton_krylov: exact products */
 
 typedef struct sc_context sc_context;
 typedef struct sc_batch sc_batch;
@@ -188,10 +189,13 @@ typedef struct {
 int sc_newton_krylov(sc_context *ctx, int max_newton, int krylov_dim, double tol,
                      sc_newton_report *report);
 /* sc_newton_krylov with the precision of the Jacobian-vector products:
- * SC_PRECISION_FP64 (as sc_newton_krylov) or SC_PRECISION_MIXED (both ends
+ * SC_PRECISION_FP64 (as sc_newton_krylov), SC_PRECISION_MIXED (both ends
  * of each finite difference on the map with FP32 stepping and FP64 sums, at
  * a step of 3e-4 (1 + |u|) in the weighted norm; the Newton residuals stay
- * FP64, so the solution is the FP64 fixed point).  krylov_dim 1..63. */
+ * FP64, so the solution is the FP64 fixed point) or SC_JVP_TANGENT (exact
+ * FP64 products J v by the tangent-linear sweep, as sc_jacobian_vector
+ * forms them, with the boundary condition's and the screened update's
+ * tangents).  krylov_dim 1..63. */
 int sc_newton_krylov_ex(sc_context *ctx, int max_newton, int krylov_dim, double tol,
                         int jvp_precision, sc_newton_report *report);
 
@@ -231,6 +235,19 @@ int sc_free_energy_gradient_ex(sc_context *ctx, double *grad, double *grad_a,
  * Green functions' derivatives from the gap equation and the current.  The
  * state is left as it was.  No counterpart in the reference. */
 int sc_residual_gradient(sc_context *ctx, double *grad, double *grad_a, double *residual);
+
+/* The Jacobian-vector product J v of one plain FP64 iteration u -> G(u)
+ * (the map of sc_residual_gradient: gap equation, current and vector
+ * potential, boundary in-values held fixed) at the current state, exactly,
+ * by the tangent-linear sweep: both Riccati recurrences with their
+ * tangents, the Green functions' tangents, then the linear stages (the
+ * extensions and site fields, the back-rotation and angle integration, the
+ * Poisson solve) on the tangents.  v and jv hold num_components x 2 x N0^2
+ * values (dRe Delta_c, dIm Delta_c) then 2 x N0^2 (dA_x, dA_y, in the units
+ * sc_set_vector_potential takes; ignored, and zero in jv, without a
+ * penetration depth).  The state is left as it was.  No counterpart in the
+ * reference. */
+int sc_jacobian_vector(sc_context *ctx, const double *v, double *jv);
 
 /* Individual stages. */
 /* RiccatiSolver::computeCoherenceFunctions for every angle (RiccatiSolverConfined.h:513-671)

@@ -3210,6 +3210,74 @@ int oc_free_energy_gradient(void *hh, double *grad, double *free_energy) {
   }
 }
 
+int oc_plain_map(void *hh, double *deltaNew, double *aNew) {
+  // G(u) of one plain iteration with the boundary in-values held fixed:
+  // the gap equation's order parameter (zero outside) and the vector
+  // potential in the setters' units; the out-values restored
+  try {
+    Oracle &o = *static_cast<Oracle *>(hh);
+    if (!o.boundaryInit) setBoundaryInitialCondition(o);
+    const std::vector<double> keepG = o.bOutG, keepGt = o.bOutGt;
+    const size_t n0 = o.n0sq();
+    const bool doA = o.cfg.compute_current && o.lambda > 0.0;
+    std::vector<double> drot, es, gD, gJ, gF, tmp(2 * n0), jacc(2 * n0, 0.0), j(2 * n0, 0.0),
+        anew(2 * n0, 0.0);
+    std::fill(deltaNew, deltaNew + size_t(o.Nc) * 2 * n0, 0.0);
+    for (int k = 0; k < o.Na; ++k) {
+      rotatedOrderParameter(o, k, drot);
+      energyShift(o, k, es);
+      riccatiAngle(o, k, drot, es, true);
+      greensAngle(o, drot, gD, gJ, gF);
+      const signed char *lab = o.frames[k].label.data();
+      const float phi = static_cast<float>(o.systemAngle[k]);
+      const double wk = angleWeight(k, o.Na) / static_cast<double>(o.Na);
+      const double pa = polarAngle(k, o.Na);
+      extendValues(lab, o.N1, gD.data());
+      extendValues(lab, o.N1, gD.data() + o.n1sq());
+      std::fill(tmp.begin(), tmp.end(), 0.0);
+      insertRotated(tmp.data(), o.N0, o.N0, gD.data(), o.N1, o.N1, phi, false);
+      insertRotated(tmp.data() + n0, o.N0, o.N0, gD.data() + o.n1sq(), o.N1, o.N1, phi, false);
+      for (int c = 0; c < o.Nc; ++c) {
+        const double b = basisMomentum(o, c, k) * wk;
+        for (size_t i = 0; i < 2 * n0; ++i) deltaNew[size_t(c) * 2 * n0 + i] += tmp[i] * b;
+      }
+      if (doA) {
+        extendValues(lab, o.N1, gJ.data());
+        for (int f = 0; f < 2; ++f) {
+          std::vector<double> t2(gJ.begin(), gJ.begin() + o.n1sq());
+          const double fac = (f == 0 ? std::cos(pa) : std::sin(pa)) * wk;
+          for (double &v : t2) v *= fac;
+          insertRotated(jacc.data() + size_t(f) * n0, o.N0, o.N0, t2.data(), o.N1, o.N1, phi,
+                        true);
+        }
+      }
+    }
+    for (int c = 0; c < o.Nc; ++c)
+      for (size_t i = 0; i < 2 * n0; ++i) {
+        double &v = deltaNew[size_t(c) * 2 * n0 + i];
+        v = o.domainRef[i % n0] != 0 ? v * o.coupling[c] : 0.0;
+      }
+    std::fill(aNew, aNew + 2 * n0, 0.0);
+    if (doA) {
+      const double jfac = static_cast<double>(2 * sgn(o.cfg.charge_sign)) * o.T;
+      for (int f = 0; f < 2; ++f)
+        for (size_t i = 0; i < n0; ++i)
+          j[f * n0 + i] = o.domainRef[i] != 0 ? jacc[f * n0 + i] * jfac : 0.0;
+      if (o.filter2D.empty())
+        makeFilter2D(o.N0, o.h, o.filter2D, std::max(o.cfg.vector_potential_error, 0.0));
+      solveVectorPotential(o.N0, o.h, o.numGrain, true, o.filter2D, j.data(), anew.data());
+      const double l2 = o.lambda * o.lambda;
+      for (size_t i = 0; i < 2 * n0; ++i) aNew[i] = anew[i] / l2;
+    }
+    o.bOutG = keepG;
+    o.bOutGt = keepGt;
+    return 0;
+  } catch (const std::exception &e) {
+    g_error = e.what();
+    return -1;
+  }
+}
+
 int oc_residual_gradient(void *hh, double *grad, double *gradA, double *residual) {
   try {
     Oracle &o = *static_cast<Oracle *>(hh);

@@ -167,6 +167,11 @@ int oc_free_energy_gradient_a(void *handle, double *grad, double *grad_a, double
  * iteration with the boundary in-values held fixed, and its gradient in
  * Delta (as above) and in A. */
 int oc_residual_gradient(void *handle, double *grad, double *grad_a, double *residual);
+/* G(u) of one plain iteration with the boundary in-values held fixed: the
+ * gap equation's order parameter (Nc x 2 x N0^2, zero outside the grain) and
+ * the vector potential (2 x N0^2, in the setters' units; zero without a
+ * penetration depth). */
+int oc_plain_map(void *handle, double *delta_new, double *a_new);
 
 /* ----- unit-level restatements (closed-form checks) ----- */
 void oc_riccati_step(const double *delta, const double *energy,

@@ -111,6 +111,7 @@ def lib():
         L.oc_free_energy_gradient.argtypes = [vp, dp, dp]
         L.oc_free_energy_gradient_a.argtypes = [vp, dp, dp, dp]
         L.oc_residual_gradient.argtypes = [vp, dp, dp, dp]
+        L.oc_plain_map.argtypes = [vp, dp, dp]
         L.oc_sweep_angles.restype = ctypes.c_longlong
         L.oc_sweep_angles.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.oc_riccati_step.argtypes = [dp, dp, dp, ctypes.c_double, dp]
@@ -333,6 +334,18 @@ class Oracle:
 
     def sweep_angles(self, a0, a1):
         return lib().oc_sweep_angles(self.h, a0, a1)
+
+    def plain_map(self):
+        """G(u) of one plain iteration with the boundary in-values held
+        fixed: ((Nc, N0, N0) complex order parameter, (2, N0, N0) vector
+        potential in the setters' units)."""
+        n = self.N0
+        d = np.zeros(self.Nc * 2 * n * n)
+        a = np.zeros(2 * n * n)
+        if lib().oc_plain_map(self.h, _dp(d), _dp(a)) != 0:
+            raise RuntimeError(lib().oc_last_error().decode())
+        d = d.reshape(self.Nc, 2, n, n)
+        return d[:, 0] + 1j * d[:, 1], a.reshape(2, n, n)
 
     def residual_gradient(self):
         """(R, grad, grad_a): the self-consistency residual

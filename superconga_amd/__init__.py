@@ -159,6 +159,7 @@ def lib():
         L.sc_free_energy_gradient.argtypes = [vp, _DP, _DP]
         L.sc_free_energy_gradient_ex.argtypes = [vp, _DP, _DP, _DP]
         L.sc_residual_gradient.argtypes = [vp, _DP, _DP, _DP]
+        L.sc_jacobian_vector.argtypes = [vp, _DP, _DP]
         L.sc_ldos_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(vp)]
         L.sc_ldos_compute.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.sc_ldos_run.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -177,6 +178,18 @@ def _check(code):
 
 def _dp(a):
     return a.ctypes.data_as(_DP)
+
+
+# Jacobian-vector products of Newton-Krylov: finite differences on the FP64
+# or the FP32-stepping map, or exact by the tangent-linear sweep
+_JVP = {"fp64": FP64, "mixed": MIXED, "mixed_refine": MIXED_REFINE, "tangent": 3}
+
+
+def _jvp(p):
+    try:
+        return _JVP[p]
+    except KeyError:
+        raise ValueError("jvp_precision must be 'fp64', 'mixed' or 'tangent'") from None
 
 
 def _precision(p):
@@ -314,13 +327,14 @@ class Simulation:
         """Jacobian-free Newton-Krylov on the fixed point of one plain FP64
         iteration; returns the report as a dict with the weighted residual
         history.  jvp_precision "mixed" takes the Jacobian-vector products
-        on the map with FP32 stepping (the residuals stay FP64).  A
+        on the map with FP32 stepping (the residuals stay FP64), "tangent"
+        exactly by the tangent-linear sweep.  A
         following compute() reports the usual residuals."""
         hist = (ctypes.c_double * (max_newton + 1))()
         r = NewtonReport(history=ctypes.cast(hist, ctypes.POINTER(ctypes.c_double)),
                          history_capacity=max_newton + 1)
         _check(lib().sc_newton_krylov_ex(self._h, max_newton, krylov_dim, ctypes.c_double(tol),
-                                         _precision(jvp_precision), ctypes.byref(r)))
+                                         _jvp(jvp_precision), ctypes.byref(r)))
         return {"newton_iterations": r.newton_iterations,
                 "krylov_iterations": r.krylov_iterations,
                 "map_evaluations": r.map_evaluations, "converged": bool(r.converged),
@@ -362,6 +376,25 @@ class Simulation:
         _check(lib().sc_residual_gradient(self._h, _dp(g), _dp(ga), _dp(r)))
         g = g.reshape(self.Nc, 2, n, n)
         return float(r[0]), g[:, 0] + 1j * g[:, 1], ga.reshape(2, n, n)
+
+    def jacobian_vector(self, v_delta, v_a=None):
+        """J v of one plain FP64 iteration u -> G(u) at the current state
+        (boundary in-values held fixed), exactly by the tangent-linear
+        sweep: v_delta an (Nc, N0, N0) complex array, v_a (2, N0, N0) in the
+        units set_vector_potential takes (or None); returns
+        (jv_delta, jv_a) alike."""
+        n = self.N0
+        v = np.zeros(self.Nc * 2 * n * n + 2 * n * n)
+        vd = np.asarray(v_delta, dtype=np.complex128).reshape(self.Nc, n, n)
+        for c in range(self.Nc):
+            v[(2 * c) * n * n:(2 * c + 1) * n * n] = vd[c].real.ravel()
+            v[(2 * c + 1) * n * n:(2 * c + 2) * n * n] = vd[c].imag.ravel()
+        if v_a is not None:
+            v[self.Nc * 2 * n * n:] = np.asarray(v_a, dtype=np.float64).ravel()
+        jv = np.empty_like(v)
+        _check(lib().sc_jacobian_vector(self._h, _dp(v), _dp(jv)))
+        jd = jv[:self.Nc * 2 * n * n].reshape(self.Nc, 2, n, n)
+        return jd[:, 0] + 1j * jd[:, 1], jv[self.Nc * 2 * n * n:].reshape(2, n, n)
 
     def riccati_sweep(self, precision="fp64", burnin=False):
         _check(lib().sc_riccati_sweep(self._h, _precision(precision), int(burnin)))
@@ -560,7 +593,7 @@ class Batch:
             reps[i].history = ctypes.cast(hists[i], ctypes.POINTER(ctypes.c_double))
             reps[i].history_capacity = max_newton + 1
         _check(lib().sc_batch_newton_krylov(self._h, max_newton, krylov_dim,
-                                            ctypes.c_double(tol), _precision(jvp_precision),
+                                            ctypes.c_double(tol), _jvp(jvp_precision),
                                             mask, reps))
         return [{"newton_iterations": r.newton_iterations,
                  "krylov_iterations": r.krylov_iterations,

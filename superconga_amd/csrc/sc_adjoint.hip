@@ -45,7 +45,7 @@ namespace scamd {
 
 namespace {
 
-constexpr int kAdjEL = 4;        // Ozaki energies per lane of the reverse sweep
+constexpr int kAdjEL = kAdjointEL; // Ozaki energies per lane of the reverse sweep
 constexpr int kAdjSlotsPerCU = 4; // persistent wavefronts per compute unit
 
 __device__ __forceinline__ Cx<double> conjc(Cx<double> a) { return {a.re, -a.im}; }
@@ -636,6 +636,275 @@ __global__ __launch_bounds__(64, 2) void adjointKernel(AdjointArgs a) {
 
 __global__ void adjointResetCounter(int *c) { *c = 0; }
 
+// ---------------------------------------------------------------------------
+// The tangent-linear sweep.  The tangent of one midpoint step y = stepEnergy
+// (p, ser, y0): from F(y) = a (y + y0)^2 + (se - 1) y + (se + 1) y0 + sD = 0,
+//   dy = q (F_y0 dy0 + (y + y0)^2 da + d(sD) + i (y + y0) dsei),  q = -1 / F_y,
+// where (da, d(sD), dsei) is the StepSite pt of the tangent fields over the
+// same step length (all three are linear in D and es).
+__device__ __forceinline__ Cx<double> stepTangent(const StepSite<double> &p,
+                                                  const StepSite<double> &pt, double ser,
+                                                  Cx<double> y0, Cx<double> y, Cx<double> dy0) {
+  const Cx<double> w = y + y0;
+  const Cx<double> aw = p.a * w;
+  const Cx<double> fy = {2.0 * aw.re + ser - 1.0, 2.0 * aw.im + p.sei};
+  const Cx<double> fy0 = {2.0 * aw.re + ser + 1.0, 2.0 * aw.im + p.sei};
+  const double r = rcpT(fy.re * fy.re + fy.im * fy.im);
+  const Cx<double> q = {-fy.re * r, fy.im * r};
+  const Cx<double> dF = fy0 * dy0 + (w * w) * pt.a + pt.sD + Cx<double>{-w.im * pt.sei, w.re * pt.sei};
+  return q * dF;
+}
+
+struct TangentArgs {
+  GrainView view;
+  const double *tfields;   // tangent site fields, laid out like view.fields
+  const SweepItemDev *items;
+  int numItems;
+  int *counter;
+  Cx<double> *scratch;     // per slot: gamma~ then its tangent, [maxLen][EL][64] each
+  long long slotStride;
+  const double *tInG, *tInGt; // tangents of the in-values (null: zero)
+  double *tOutG, *tOutGt;     // tangents of the out-values
+  double *tG;                 // tangents of the Green sums [Na][N1^2][4]
+};
+
+// One item: pass 1 gamma~ and its tangent from the top, kept; pass 2 gamma
+// and its tangent from the bottom, with the tangents of the Green sums
+// (greensAdjoint's derivatives applied forward) at every inside site.
+template <int EL>
+__device__ void tangentItem(const TangentArgs &a, const SweepItemDev &it, Cx<double> *scrT,
+                            Cx<double> *scrD) {
+  const GrainView &v = a.view;
+  const int lane = threadIdx.x;
+  const int cols = v.cols, gp = v.groups;
+  const int g = lane / cols;
+  const int c = lane - g * cols;
+  const int N1 = v.N1, Ne = v.Ne;
+  const size_t n1 = size_t(N1) * N1;
+  const double h = v.h;
+  const int k = it.angle;
+  const bool colActive = c < it.ncols;
+  const int x = it.x0 + (colActive ? c : it.ncols - 1);
+  const bool active = colActive && g < gp;
+  const int8_t *lab = v.labels + size_t(k) * n1 + x;
+  const size_t fo = size_t(k - v.fieldBase) * n1 + x;
+  const double *F = v.fields + fo;
+  const double *T = a.tfields + fo;
+  const long long fp = v.fieldPlane;
+  double ser[EL], res[EL];
+  bool eOk[EL];
+#pragma unroll
+  for (int j = 0; j < EL; ++j) {
+    const int e = g * EL + j;
+    eOk[j] = active && e < Ne;
+    ser[j] = eOk[j] ? -h * v.energy[e] : 0.0;
+    res[j] = eOk[j] ? v.residue[e] : 0.0;
+  }
+  const int len = it.yhi - it.ylo + 1;
+  const size_t col = size_t(k) * N1 + x;
+  auto slot = [&](int t, int j) { return (size_t(t) * EL + j) * 64 + lane; };
+
+  // ---------------- pass 1: gamma~ and its tangent from the top ----------------
+  {
+    const int baseIn = v.idxInGt[col], baseOut = v.idxOutGt[col];
+    int cntIn = 0, cntOut = 0;
+    Cx<double> coh[EL], dc[EL];
+#pragma unroll
+    for (int j = 0; j < EL; ++j) coh[j] = dc[j] = {0.0, 0.0};
+    Cx<double> dPrev = {0.0, 0.0}, tPrev = {0.0, 0.0};
+    double esPrev = 0.0, tesPrev = 0.0;
+    bool havePrev = false;
+    for (int t = 0; t < len; ++t) {
+      const size_t o = size_t(it.yhi - t) * N1;
+      const Cx<double> D = {-F[o], F[fp + o]}, tD = {-T[o], T[fp + o]};
+      const double es = F[2 * fp + o], tes = T[2 * fp + o];
+      const int8_t lc = lab[o];
+      if (active) {
+        if (lc == 1 || lc == 2) {
+          const StepSite<double> ps = stepSite(
+              Cx<double>{(dPrev.re + D.re) * 0.5, (dPrev.im + D.im) * 0.5}, (esPrev + es) * 0.5, h);
+          const StepSite<double> pt = stepSite(
+              Cx<double>{(tPrev.re + tD.re) * 0.5, (tPrev.im + tD.im) * 0.5}, (tesPrev + tes) * 0.5,
+              h);
+          const double half = havePrev ? 1.0 : 0.5;
+#pragma unroll
+          for (int j = 0; j < EL; ++j) {
+            const Cx<double> y = stepEnergy(ps, ser[j] * half, coh[j]);
+            dc[j] = stepTangent(ps, pt, ser[j] * half, coh[j], y, dc[j]);
+            coh[j] = y;
+          }
+          if (lc == 2) {
+            const int ci = baseOut + cntOut;
+            const double dr = v.distOutGt[ci];
+            const StepSite<double> pe = stepSite(D, es, dr * h), pte = stepSite(tD, tes, dr * h);
+#pragma unroll
+            for (int j = 0; j < EL; ++j) {
+              if (!eOk[j]) continue;
+              const Cx<double> bv = stepEnergy(pe, ser[j] * dr, coh[j]);
+              const Cx<double> tb = stepTangent(pe, pte, ser[j] * dr, coh[j], bv, dc[j]);
+              const size_t si = size_t(ci) * Ne + g * EL + j;
+              a.tOutGt[si] = tb.re;
+              a.tOutGt[size_t(v.nOutGt) * Ne + si] = tb.im;
+            }
+          }
+        } else if (lc == 3) {
+          const int ci = baseIn + cntIn;
+          const double dr = v.distInGt[ci];
+          const StepSite<double> pe = stepSite(D, es, dr * h), pte = stepSite(tD, tes, dr * h);
+#pragma unroll
+          for (int j = 0; j < EL; ++j) {
+            const size_t si = size_t(ci) * Ne + g * EL + j;
+            const Cx<double> b0 = eOk[j] ? Cx<double>{v.bInGt[si], v.bInGt[size_t(v.nInGt) * Ne + si]}
+                                         : Cx<double>{0.0, 0.0};
+            const Cx<double> tb0 =
+                eOk[j] && a.tInGt ? Cx<double>{a.tInGt[si], a.tInGt[size_t(v.nInGt) * Ne + si]}
+                                  : Cx<double>{0.0, 0.0};
+            const Cx<double> y = stepEnergy(pe, ser[j] * dr, b0);
+            dc[j] = stepTangent(pe, pte, ser[j] * dr, b0, y, tb0);
+            coh[j] = y;
+          }
+        } else {
+#pragma unroll
+          for (int j = 0; j < EL; ++j) coh[j] = dc[j] = {0.0, 0.0};
+        }
+#pragma unroll
+        for (int j = 0; j < EL; ++j) {
+          scrT[slot(t, j)] = coh[j];
+          scrD[slot(t, j)] = dc[j];
+        }
+      }
+      if (lc == 2) ++cntOut;
+      if (lc == 3) ++cntIn;
+      dPrev = D;
+      tPrev = tD;
+      esPrev = es;
+      tesPrev = tes;
+      havePrev = true;
+    }
+  }
+
+  // ------ pass 2: gamma and its tangent from the bottom, the Green tangents ------
+  {
+    const int baseIn = v.idxInG[col], baseOut = v.idxOutG[col];
+    int cntIn = 0, cntOut = 0;
+    Cx<double> coh[EL], dc[EL];
+#pragma unroll
+    for (int j = 0; j < EL; ++j) coh[j] = dc[j] = {0.0, 0.0};
+    Cx<double> dPrev = {0.0, 0.0}, tPrev = {0.0, 0.0};
+    double esPrev = 0.0, tesPrev = 0.0;
+    bool havePrev = false;
+    for (int u = 0; u < len; ++u) {
+      const int t = len - 1 - u;
+      const int y = it.ylo + u;
+      const size_t o = size_t(y) * N1;
+      const Cx<double> D = {F[o], F[fp + o]}, tD = {T[o], T[fp + o]};
+      const double es = F[2 * fp + o], tes = T[2 * fp + o];
+      const int8_t lc = lab[o];
+      double sgr = 0.0, sgi = 0.0, sgj = 0.0;
+      if (active) {
+        if (lc == 1 || lc == 3) {
+          const StepSite<double> ps = stepSite(
+              Cx<double>{(dPrev.re + D.re) * 0.5, (dPrev.im + D.im) * 0.5}, (esPrev + es) * 0.5, h);
+          const StepSite<double> pt = stepSite(
+              Cx<double>{(tPrev.re + tD.re) * 0.5, (tPrev.im + tD.im) * 0.5}, (tesPrev + tes) * 0.5,
+              h);
+          const double half = havePrev ? 1.0 : 0.5;
+#pragma unroll
+          for (int j = 0; j < EL; ++j) {
+            const Cx<double> yv = stepEnergy(ps, ser[j] * half, coh[j]);
+            dc[j] = stepTangent(ps, pt, ser[j] * half, coh[j], yv, dc[j]);
+            coh[j] = yv;
+          }
+          if (lc == 3) {
+            const int ci = baseOut + cntOut;
+            const double dr = v.distOutG[ci];
+            const StepSite<double> pe = stepSite(D, es, dr * h), pte = stepSite(tD, tes, dr * h);
+#pragma unroll
+            for (int j = 0; j < EL; ++j) {
+              if (!eOk[j]) continue;
+              const Cx<double> bv = stepEnergy(pe, ser[j] * dr, coh[j]);
+              const Cx<double> tb = stepTangent(pe, pte, ser[j] * dr, coh[j], bv, dc[j]);
+              const size_t si = size_t(ci) * Ne + g * EL + j;
+              a.tOutG[si] = tb.re;
+              a.tOutG[size_t(v.nOutG) * Ne + si] = tb.im;
+            }
+          }
+        } else if (lc == 2) {
+          const int ci = baseIn + cntIn;
+          const double dr = v.distInG[ci];
+          const StepSite<double> pe = stepSite(D, es, dr * h), pte = stepSite(tD, tes, dr * h);
+#pragma unroll
+          for (int j = 0; j < EL; ++j) {
+            const size_t si = size_t(ci) * Ne + g * EL + j;
+            const Cx<double> b0 = eOk[j] ? Cx<double>{v.bInG[si], v.bInG[size_t(v.nInG) * Ne + si]}
+                                         : Cx<double>{0.0, 0.0};
+            const Cx<double> tb0 =
+                eOk[j] && a.tInG ? Cx<double>{a.tInG[si], a.tInG[size_t(v.nInG) * Ne + si]}
+                                 : Cx<double>{0.0, 0.0};
+            const Cx<double> yv = stepEnergy(pe, ser[j] * dr, b0);
+            dc[j] = stepTangent(pe, pte, ser[j] * dr, b0, yv, tb0);
+            coh[j] = yv;
+          }
+        } else {
+#pragma unroll
+          for (int j = 0; j < EL; ++j) coh[j] = dc[j] = {0.0, 0.0};
+        }
+        if (lc > 0) {
+          // tangents of the Green sums: with Q = 2i / (1 + g gt)^2,
+          // dgD = -Q dg + Q g^2 dgt + conj(-Q gt^2 dg + Q dgt),
+          // dgJ = Re(Q (gt dg + g dgt))
+#pragma unroll
+          for (int j = 0; j < EL; ++j) {
+            if (!eOk[j]) continue;
+            const Cx<double> gt = scrT[slot(t, j)], tgt = scrD[slot(t, j)];
+            const Cx<double> ga = coh[j], tga = dc[j];
+            const Cx<double> z = Cx<double>{1.0, 0.0} + ga * gt;
+            const Cx<double> z2 = z * z;
+            const double rn = rcpT(z2.re * z2.re + z2.im * z2.im);
+            const Cx<double> Q = {2.0 * z2.im * rn, 2.0 * z2.re * rn};
+            const Cx<double> du = Q * (ga * ga * tgt - tga);
+            const Cx<double> dv = Q * (tgt - gt * gt * tga);
+            const Cx<double> dj = Q * (gt * tga + ga * tgt);
+            sgr += res[j] * (du.re + dv.re);
+            sgi += res[j] * (du.im - dv.im);
+            sgj += res[j] * dj.re;
+          }
+        }
+      }
+      sgr = groupSum(sgr, gp, cols, c);
+      sgi = groupSum(sgi, gp, cols, c);
+      sgj = groupSum(sgj, gp, cols, c);
+      if (g == 0 && colActive && lc > 0) {
+        double *Gk = a.tG + (size_t(k) * n1 + o + x) * 4;
+        Gk[0] = sgr;
+        Gk[1] = sgi;
+        Gk[2] = sgj;
+        Gk[3] = 0.0;
+      }
+      if (lc == 3) ++cntOut;
+      if (lc == 2) ++cntIn;
+      dPrev = D;
+      tPrev = tD;
+      esPrev = es;
+      tesPrev = tes;
+      havePrev = true;
+    }
+  }
+}
+
+template <int EL>
+__global__ __launch_bounds__(64, 2) void tangentKernel(TangentArgs a) {
+  Cx<double> *scrT = a.scratch + size_t(blockIdx.x) * 2 * a.slotStride;
+  Cx<double> *scrD = scrT + a.slotStride;
+  for (;;) {
+    // the branch-free fetch of adjointKernel
+    const int item =
+        __builtin_amdgcn_readfirstlane(atomicAdd(a.counter, threadIdx.x == 0 ? 1 : 0));
+    if (item >= a.numItems) break;
+    tangentItem<EL>(a, a.items[item], scrT, scrD);
+  }
+}
+
 // Residual mode, 1. The residual r = G(u) - u per unit of the setters'
 // variables: the order parameter at inside sites (G's is zero outside), the
 // vector potential (stored as lambda^2 A) everywhere; and r_A / lambda^2,
@@ -736,6 +1005,31 @@ __global__ void residualFinishKernel(const double *extBar, const double *rD, lon
 }
 
 } // namespace
+
+void tangentSweep(const GrainView &view, const double *tfields, const void *items, int numItems,
+                  int *counter, double *scratch, long long slotStride, const double *tInG,
+                  const double *tInGt, double *tOutG, double *tOutGt, double *tG, int slots,
+                  hipStream_t stream) {
+  TangentArgs a;
+  a.view = view;
+  a.tfields = tfields;
+  a.items = reinterpret_cast<const SweepItemDev *>(items);
+  a.numItems = numItems;
+  a.counter = counter;
+  a.scratch = reinterpret_cast<Cx<double> *>(scratch);
+  a.slotStride = slotStride;
+  a.tInG = tInG;
+  a.tInGt = tInGt;
+  a.tOutG = tOutG;
+  a.tOutGt = tOutGt;
+  a.tG = tG;
+  adjointResetCounter<<<1, 1, 0, stream>>>(counter);
+  SC_HIP(hipGetLastError());
+  if (numItems > 0) {
+    tangentKernel<kAdjEL><<<slots, 64, 0, stream>>>(a);
+    SC_HIP(hipGetLastError());
+  }
+}
 
 // The reverse sweep's items (the sweep's own, at kAdjEL energies per lane),
 // its slots' scratch and the adjoint planes, once per grain.

@@ -441,14 +441,18 @@ int sc_newton_krylov_ex(sc_context *ctx, int maxNewton, int krylovDim, double to
     if (maxNewton < 0) throw std::invalid_argument("max_newton must be >= 0");
     if (krylovDim < 1 || krylovDim > 63)
       throw std::invalid_argument("krylov_dim must be in [1, 63]");
-    if (jvpPrecision != SC_PRECISION_FP64 && jvpPrecision != SC_PRECISION_MIXED)
-      throw std::invalid_argument("jvp_precision must be SC_PRECISION_FP64 or SC_PRECISION_MIXED");
+    if (jvpPrecision != SC_PRECISION_FP64 && jvpPrecision != SC_PRECISION_MIXED &&
+        jvpPrecision != SC_JVP_TANGENT)
+      throw std::invalid_argument(
+          "jvp_precision must be SC_PRECISION_FP64, SC_PRECISION_MIXED or SC_JVP_TANGENT");
     if (!(tol > 0.0)) throw std::invalid_argument("tol must be positive");
     if (report && report->history_capacity > 0 && !report->history)
       throw std::invalid_argument("history is null with a positive capacity");
     const scamd::NewtonReport r = ctx->grain->newtonKrylov(
         maxNewton, krylovDim, tol,
-        jvpPrecision == SC_PRECISION_MIXED ? scamd::PREC_MIXED : scamd::PREC_FP64);
+        jvpPrecision == SC_PRECISION_MIXED
+            ? scamd::PREC_MIXED
+            : (jvpPrecision == SC_JVP_TANGENT ? scamd::JVP_TANGENT : scamd::PREC_FP64));
     if (report) {
       report->newton_iterations = r.newtonIterations;
       report->krylov_iterations = r.krylovIterations;
@@ -466,6 +470,17 @@ int sc_newton_krylov_ex(sc_context *ctx, int maxNewton, int krylovDim, double to
 
 int sc_free_energy_gradient(sc_context *ctx, double *grad, double *freeEnergy) {
   return sc_free_energy_gradient_ex(ctx, grad, nullptr, freeEnergy);
+}
+
+int sc_jacobian_vector(sc_context *ctx, const double *v, double *jv) {
+  SC_CHECK_CTX(ctx);
+  SC_CHECK_SIM(ctx);
+  SC_CHECK_PTR(v);
+  SC_CHECK_PTR(jv);
+  return guarded([&] {
+    ctx->grain->jacobianVectorHost(v, jv);
+    return SC_OK;
+  });
 }
 
 int sc_residual_gradient(sc_context *ctx, double *grad, double *gradA, double *residual) {
@@ -739,8 +754,10 @@ int sc_batch_newton_krylov(sc_batch *b, int maxNewton, int krylovDim, double tol
     if (maxNewton < 0) throw std::invalid_argument("max_newton must be >= 0");
     if (krylovDim < 1 || krylovDim > 63)
       throw std::invalid_argument("krylov_dim must be in [1, 63]");
-    if (jvpPrecision != SC_PRECISION_FP64 && jvpPrecision != SC_PRECISION_MIXED)
-      throw std::invalid_argument("jvp_precision must be SC_PRECISION_FP64 or SC_PRECISION_MIXED");
+    if (jvpPrecision != SC_PRECISION_FP64 && jvpPrecision != SC_PRECISION_MIXED &&
+        jvpPrecision != SC_JVP_TANGENT)
+      throw std::invalid_argument(
+          "jvp_precision must be SC_PRECISION_FP64, SC_PRECISION_MIXED or SC_JVP_TANGENT");
     if (!(tol > 0.0)) throw std::invalid_argument("tol must be positive");
     const int n = b->batch->size();
     if (reports)
@@ -751,7 +768,10 @@ int sc_batch_newton_krylov(sc_batch *b, int maxNewton, int krylovDim, double tol
     if (grains) which.assign(grains, grains + n);
     const std::vector<scamd::NewtonReport> rs = b->batch->newtonKrylov(
         maxNewton, krylovDim, tol,
-        jvpPrecision == SC_PRECISION_MIXED ? scamd::PREC_MIXED : scamd::PREC_FP64, which);
+        jvpPrecision == SC_PRECISION_MIXED
+            ? scamd::PREC_MIXED
+            : (jvpPrecision == SC_JVP_TANGENT ? scamd::JVP_TANGENT : scamd::PREC_FP64),
+        which);
     if (reports)
       for (int i = 0; i < n; ++i) {
         const scamd::NewtonReport &r = rs[size_t(i)];

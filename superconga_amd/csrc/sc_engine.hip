@@ -112,7 +112,8 @@ Grain::~Grain() {
   devFree(dAdjGrad_);
   devFree(dAdjGradA_);
   for (double *p : {dResD_, dResA_, dResPois_, dResSbar_, dResJbar_, dResZero_, dResMean_,
-                    dResSeed_, dResSave_})
+                    dResSeed_, dResSave_, dTanExt_, dTanFields_, dTanG_, dTanJ_, dTanE_,
+                    dTanOutG_, dTanOutGt_, dTanV_, dTanJV_})
     devFree(p);
   dropEndGraph();
   fft_.reset();
@@ -1735,6 +1736,142 @@ void Grain::screenVectorPotential() {
 void Grain::endCompute() {
   queueEndCompute();
   finishEndCompute();
+}
+
+// ---------------------------------------------------------------------------
+// Exact Jacobian-vector products of the plain iteration (tangent-linear
+// mode).  Every stage after the sweep is linear (the extensions, the site
+// fields without the external gauge field, the back-rotation and angle
+// integration, the Poisson solve with its mean, the screened update, the
+// boundary condition), so each runs on the tangents as it runs on the state;
+// the sweep's tangent is tangentSweep.
+// ---------------------------------------------------------------------------
+void Grain::jacobianVector(const double *dv, double *djv, bool inValues, bool screened) {
+  const Setup &s = *s_;
+  if (s.keldysh) throw std::runtime_error("a spectroscopy context has no Jacobian products");
+  if (s.Nc > 4) throw std::runtime_error("at most 4 order-parameter components");
+  const size_t n0 = size_t(s.N0) * s.N0, n1 = size_t(s.N1) * s.N1;
+  const long long framePlane = static_cast<long long>(s.Na) * static_cast<long long>(n1);
+  const bool doA = s.cfg.computeCurrent && s.lambda > 0.0;
+  const size_t nd = size_t(2 * s.Nc) * n0, nx = nd + (doA ? 2 * n0 : 0);
+  const size_t nG = size_t(2 * s.nInG * s.Ne);
+  prepareAdjoint();
+  if (!dTanExt_) {
+    dTanExt_ = devAlloc<double>(nd);
+    dTanFields_ = devAlloc<double>(3 * size_t(framePlane));
+    dTanG_ = devAlloc<double>(4 * size_t(framePlane));
+    dTanJ_ = devAlloc<double>(2 * n0);
+    dTanE_ = devAlloc<double>(n0);
+    dTanOutG_ = devAlloc<double>(std::max<size_t>(1, size_t(2 * s.nOutG * s.Ne)));
+    dTanOutGt_ = devAlloc<double>(std::max<size_t>(1, size_t(2 * s.nOutGt * s.Ne)));
+  }
+  // the state's fields, then the tangents of the extended order parameter
+  // and of the site fields (the induced vector potential's share of the
+  // Doppler shift only: the external gauge field is fixed)
+  ensureBoundaryInit();
+  extendDelta(true);
+  SC_HIP(hipMemcpyAsync(dTanExt_, dv, nd * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  const int ne = static_cast<int>(s.extRef.target.size());
+  if (ne > 0) {
+    extendKernel<<<blocksFor(ne), 256, 0, stream_>>>(
+        dTanExt_, dExtRefT_, dExtRefO_, dExtRefS_, ne, 2 * s.Nc, static_cast<long long>(n0),
+        static_cast<long long>(n0), 0, 1, 0);
+    SC_HIP(hipGetLastError());
+  }
+  GrainView tv = view();
+  tv.extDelta = dTanExt_;
+  tv.A = doA ? dv + nd : dA_;
+  tv.fluxDensity = 0.0;
+  tv.useInduced = doA ? 1 : 0;
+  tv.useField = doA ? 1 : 0;
+  SC_HIP(hipMemsetAsync(dTanFields_, 0, 3 * size_t(framePlane) * sizeof(double), stream_));
+  const int nt = s.fieldTileStart[size_t(s.Na)];
+  if (nt > 0) {
+    fieldsKernel<<<xcdGrid(nt), 256, 0, stream_>>>(tv, dFieldTiles_, nt, 0, framePlane,
+                                                    dTanFields_);
+    SC_HIP(hipGetLastError());
+  }
+  // the sweep's tangent
+  SC_HIP(hipMemsetAsync(dTanG_, 0, 4 * size_t(framePlane) * sizeof(double), stream_));
+  tangentSweep(sweepView(kAdjointEL), dTanFields_, adjItems_, adjNumItems_, dCounter_,
+               dAdjScratch_, static_cast<long long>(adjMaxLen_) * kAdjointEL * 64,
+               inValues ? dv + nx : nullptr, inValues ? dv + nx + nG : nullptr, dTanOutG_,
+               dTanOutGt_, dTanG_, adjSlots_, stream_);
+  // the back-rotation, angle integration and gap equation of the tangents
+  const int nr = static_cast<int>(s.extRot.target.size());
+  if (nr > 0) {
+    extendKernel<<<blocksFor(nr), 256, 0, stream_>>>(
+        dTanG_, dExtRotT_, dExtRotO_, dExtRotS_, nr, 4, 1, static_cast<long long>(n1),
+        static_cast<long long>(4 * n1), 4, 0);
+    SC_HIP(hipGetLastError());
+  }
+  // (the gather writes inside sites only; the Poisson solve reads them all)
+  SC_HIP(hipMemsetAsync(djv, 0, nx * sizeof(double), stream_));
+  SC_HIP(hipMemsetAsync(dTanJ_, 0, 2 * n0 * sizeof(double), stream_));
+  GrainView gv = view();
+  gv.G = dTanG_;
+  const int ns = static_cast<int>(s.domainSites.size());
+  const double jf = 2.0 * static_cast<double>(s.cfg.chargeSign) * s.temperature;
+  if (ns > 0) {
+    gatherKernel<<<xcdGrid(blocksFor(ns)), 256, 0, stream_>>>(gv, dDomainSites_, ns, dCoupling_,
+                                                              jf, djv, dTanJ_, dTanE_);
+    SC_HIP(hipGetLastError());
+  }
+  if (doA) {
+    solveVectorPotential(dTanJ_, djv + nd);
+    if (screened && screen_) {
+      // the screened update's tangent: dA' - K (-L + K)^-1 (dA' - dA)
+      const long long n2 = 2LL * static_cast<long long>(n0);
+      double *R = dWork_, *S = dWork2_;
+      diffKernel<<<blocksFor(n2), 256, 0, stream_>>>(R, djv + nd, dv + nd, n2);
+      SC_HIP(hipGetLastError());
+      screen_->solve(R, dZero2_, S);
+      CombineTerms t{};
+      t.v[0] = S;
+      t.c[0] = -screenK_;
+      combineTermsKernel<<<blocksFor(n2), 256, 0, stream_>>>(djv + nd, djv + nd, 1.0, t, 1, n2);
+      SC_HIP(hipGetLastError());
+    }
+  }
+  // the boundary condition's tangent: the same interpolation of the
+  // out-values' tangents
+  if (inValues) {
+    for (int dir = 0; dir < 2; ++dir) {
+      const long long nIn = dir == 0 ? s.nInG : s.nInGt;
+      const long long nOut = dir == 0 ? s.nOutG : s.nOutGt;
+      const long long total = nIn * s.Ne;
+      if (total == 0) continue;
+      const long long blocks = blocksFor(total);
+      int h = -1;
+      double *dst = reserveReduction(1, blocks, 2 * total, h);
+      boundaryKernel<<<blocks, 256, 0, stream_>>>(
+          djv + nx + (dir == 0 ? 0 : nG), dir == 0 ? dTanOutG_ : dTanOutGt_, nIn, nOut, s.Ne,
+          dir == 0 ? dBcOffG_ : dBcOffGt_, dir == 0 ? dBcSrcG_ : dBcSrcGt_,
+          dir == 0 ? dBcWG_ : dBcWGt_, dir == 0 ? dBcWsG_ : dBcWsGt_, dst);
+      SC_HIP(hipGetLastError());
+    }
+  }
+  discardReductions();
+}
+
+void Grain::jacobianVectorHost(const double *v, double *jv) {
+  const Setup &s = *s_;
+  const size_t n0 = size_t(s.N0) * s.N0;
+  const bool doA = s.cfg.computeCurrent && s.lambda > 0.0;
+  const size_t nd = size_t(2 * s.Nc) * n0, nx = nd + (doA ? 2 * n0 : 0);
+  if (!dTanV_) {
+    dTanV_ = devAlloc<double>(nd + 2 * n0);
+    dTanJV_ = devAlloc<double>(nd + 2 * n0);
+  }
+  // the state holds lambda^2 A
+  const double l2 = doA ? s.lambda * s.lambda : 1.0;
+  std::vector<double> buf(v, v + nx);
+  for (size_t i = nd; i < nx; ++i) buf[i] *= l2;
+  toDevice(dTanV_, buf.data(), nx * sizeof(double));
+  jacobianVector(dTanV_, dTanJV_, false, false);
+  fromDevice(jv, dTanJV_, nx * sizeof(double));
+  for (size_t i = nd; i < nx; ++i) jv[i] /= l2;
+  for (size_t i = nx; i < nd + 2 * n0; ++i) jv[i] = 0.0;
 }
 
 void Grain::finishEndCompute() {

@@ -15,6 +15,23 @@
 namespace scamd {
 
 enum Precision { PREC_FP64 = 0, PREC_MIXED = 1, PREC_MIXED_REFINE = 2 };
+// Newton-Krylov's exact Jacobian-vector products (the tangent-linear sweep)
+constexpr int JVP_TANGENT = 3;
+
+// Energies per lane of the reverse and tangent-linear sweeps (sc_adjoint.hip).
+constexpr int kAdjointEL = 4;
+
+// The tangent-linear sweep (sc_adjoint.hip): on the items of the reverse
+// sweep (SweepItemDev, kAdjointEL energies per lane, view = sweepView), both
+// Riccati recurrences with their tangents for the tangent site fields
+// tfields (laid out like view.fields) and the tangents of the in-values
+// tInG, tInGt (null: zero); writes the tangents of the out-values at every
+// exit crossing and, per inside frame site, the tangents of the Green sums
+// into tG ([frame][site][4]: gap re, gap im, current, 0, the sweep's layout).
+void tangentSweep(const GrainView &view, const double *tfields, const void *items, int numItems,
+                  int *counter, double *scratch, long long slotStride, const double *tInG,
+                  const double *tInGt, double *tOutG, double *tOutGt, double *tG, int slots,
+                  hipStream_t stream);
 
 struct Results {
   double residual = 0, resBoundary = 0, resDelta = 0, resCurrent = 0,
@@ -171,6 +188,17 @@ public:
   // in-values held fixed) and its gradient (J - I)^T (G(u) - u), grad and
   // gradA laid out as freeEnergyGradient's.  The state is left as it was.
   double residualGradient(double *grad, double *gradA = nullptr);
+  // The Jacobian-vector product J v of the Newton-Krylov map (one plain FP64
+  // iteration: gap equation, vector potential, with inValues also the new
+  // in-values of the boundary condition; with screened the screened
+  // vector-potential update as nkMapEnd applies it) at the current state, by
+  // the tangent-linear sweep.  dv and djv are device vectors in the map's
+  // layout ([Delta planes, lambda^2 A planes when the vector potential is
+  // solved, with inValues the gamma then gamma~ in-values]).
+  void jacobianVector(const double *dv, double *djv, bool inValues, bool screened);
+  // The same on host vectors [Delta planes (Nc x 2 x N0^2), A planes
+  // (2 x N0^2, in the setters' units)], in-values held fixed, unscreened.
+  void jacobianVectorHost(const double *v, double *jv);
   void resetAccelerator();
 
   // Keldysh spectroscopy (src/postprocess.cu), for a grain built by
@@ -364,6 +392,10 @@ private:
          *dResJbar_ = nullptr, *dResZero_ = nullptr, *dResMean_ = nullptr, *dResSeed_ = nullptr,
          *dResSave_ = nullptr;
   void prepareAdjoint();
+  // the tangent-linear products' buffers
+  double *dTanExt_ = nullptr, *dTanFields_ = nullptr, *dTanG_ = nullptr, *dTanJ_ = nullptr,
+         *dTanE_ = nullptr, *dTanOutG_ = nullptr, *dTanOutGt_ = nullptr, *dTanV_ = nullptr,
+         *dTanJV_ = nullptr;
   int hA_ = -1; // queued residual of the vector potential
   // the captured stages after the sweep (queueEndCompute)
   void queueEndStages();

@@ -157,8 +157,9 @@ void Grain::nkMapEnd(double *gu) {
 NewtonKrylov::NewtonKrylov(Grain &g, int maxNewton, int krylovDim, double tol,
                            int jvpPrecision)
     : g_(g), maxNewton_(maxNewton), m_(krylovDim), tol_(tol), jvp_(jvpPrecision) {
-  if (jvpPrecision != PREC_FP64 && jvpPrecision != PREC_MIXED)
-    throw std::runtime_error("Jacobian-vector products run in fp64 or mixed precision");
+  if (jvpPrecision != PREC_FP64 && jvpPrecision != PREC_MIXED && jvpPrecision != JVP_TANGENT)
+    throw std::runtime_error("Jacobian-vector products run in fp64 or mixed precision, or "
+                             "exactly by the tangent-linear sweep");
   if (g.s_->keldysh) throw std::runtime_error("a spectroscopy context only runs the LDOS iteration");
   if (krylovDim < 1 || krylovDim > kMaxTerms - 1)
     throw std::runtime_error("Krylov dimension must be in [1, " + std::to_string(kMaxTerms - 1) +
@@ -237,6 +238,13 @@ void NewtonKrylov::want(const double *x, double *out, int precision) {
 }
 
 bool NewtonKrylov::next(int &precision, bool fields) {
+  // exact products need no map: each is the tangent-linear sweep at the
+  // Newton point, until the solve asks for G again
+  while (phase_ == KRYLOV && jvp_ == JVP_TANGENT) {
+    g_.nkSet(u_);
+    g_.jacobianVector(V_[size_t(k_)], w_, true, true);
+    productDone();
+  }
   if (phase_ == DONE) return false;
   g_.nkSet(x_);
   g_.nkMapBegin(fields);
@@ -279,15 +287,21 @@ void NewtonKrylov::startCycle() {
 }
 
 void NewtonKrylov::startProduct() {
-  combine(trial_, u_, 1.0, {V_[size_t(k_)]}, {e_});
   phase_ = KRYLOV;
+  if (jvp_ == JVP_TANGENT) return; // next() forms J v itself
+  combine(trial_, u_, 1.0, {V_[size_t(k_)]}, {e_});
   want(trial_, w_, jvp_);
 }
 
 void NewtonKrylov::productDone() {
   const int m = m_, k = k_;
-  const double *base = jvp_ == PREC_MIXED ? guJ_ : gu_;
-  combine(w_, w_, 1.0 / e_, {base, V_[size_t(k)]}, {-1.0 / e_, -1.0});
+  if (jvp_ == JVP_TANGENT) {
+    // w holds J v: (J - I) v
+    combine(w_, w_, 1.0, {V_[size_t(k)]}, {-1.0});
+  } else {
+    const double *base = jvp_ == PREC_MIXED ? guJ_ : gu_;
+    combine(w_, w_, 1.0 / e_, {base, V_[size_t(k)]}, {-1.0 / e_, -1.0});
+  }
   // classical Gram-Schmidt with the norm of w read back with the
   // projections: |w - sum d_i v_i|^2 = |w|^2 - sum d_i^2 for the orthonormal
   // basis.  A second pass only when the first removed more than half of

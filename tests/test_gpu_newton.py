@@ -145,7 +145,7 @@ def test_screened_vector_potential_update_keeps_the_fixed_point(product, acc):
     assert abs(r1["free_energy_per_area"] - ka["value"]) < ka["tolerance_rel"] * abs(ka["value"])
 
 
-@pytest.mark.parametrize("jvp", ["fp64", "mixed"])
+@pytest.mark.parametrize("jvp", ["fp64", "mixed", "tangent"])
 def test_batched_newton_krylov_equals_each_grain_alone(product, jvp):
     # sc_batch_newton_krylov: every grain's own solve, the maps of all of
     # them in shared sweep launches; each grain must end where its solve
@@ -214,3 +214,35 @@ def test_screening_switched_between_iterations(product):
     assert rel(s.vector_potential(0), plain.vector_potential(0)) < 1e-6
     with pytest.raises(product.SupercongaError):
         s.set_screening(-1.0)
+
+
+def test_newton_krylov_with_tangent_products(product):
+    # exact products: the same fixed point as the finite differences, with
+    # map evaluations only for the Newton points and the line search (each
+    # product is a tangent-linear sweep, not a map evaluation)
+    for cfg in (cases.accelerator_case("picard"),
+                cases.base(temperature=0.4, flux=0.5, lam=6.0, angles=16, ppxi=3.0, width=8.0)):
+        ref = converged_plain(product, cfg)
+        fd = product.Simulation(cfg)
+        rep_fd = fd.newton_krylov(max_newton=20, krylov_dim=20, tol=1e-10)
+        s = product.Simulation(cfg)
+        rep = s.newton_krylov(max_newton=20, krylov_dim=20, tol=1e-10, jvp_precision="tangent")
+        print("tangent products:", {k: v for k, v in rep.items() if k != "history"},
+              "| differences:", {k: v for k, v in rep_fd.items() if k != "history"})
+        assert rep["converged"], rep
+        assert rep["map_evaluations"] < rep["newton_iterations"] + 8
+        assert rep_fd["map_evaluations"] > rep_fd["krylov_iterations"]
+        s.compute("fp64")
+        r = s.results()
+        for k in ("residual_order_parameter", "residual_boundary"):
+            assert r[k] < 1e-8, (k, r)
+        # the global phase of the order parameter is a symmetry of the map
+        # (a circle of fixed points): compare after removing it
+        a, b = s.order_parameter(1), ref.order_parameter(1)
+        phase = np.angle(np.sum(np.conj(b) * a))
+        print("global phase offset from the plain run: %.2e rad" % phase)
+        assert rel(a * np.exp(-1j * phase), b) < 1e-6
+        assert abs(r["free_energy"] - ref.results()["free_energy"]) < 1e-8 * abs(
+            r["free_energy"])
+        s.close()
+        fd.close()
