@@ -181,7 +181,8 @@ def convergence(sa, workloads, args, name=None):
     """Wall time to a 1e-8 residual from the same initial state: plain FP64
     iterations with the config's accelerator and with CongAcc (the
     reference's default), mixed refinement, Newton-Krylov with FP64 and with
-    FP32 Jacobian-vector products.  Speedups are against the faster plain
+    FP32 finite-difference Jacobian-vector products and with exact ones (the
+    tangent-linear sweep).  Speedups are against the faster plain
     FP64 run that converged; when neither did, against the capped time of
     the faster one, as a lower bound."""
     name = name or args.converge_workload
@@ -204,7 +205,8 @@ def convergence(sa, workloads, args, name=None):
     log("converge %s mixed_refine: %s" % (name, conv["mixed_refine"]))
     methods = ["mixed_refine"]
     if not args.no_newton:
-        for key, jvp in (("newton_krylov", "fp64"), ("newton_krylov_mixed", "mixed")):
+        for key, jvp in (("newton_krylov", "fp64"), ("newton_krylov_mixed", "mixed"),
+                         ("newton_krylov_tangent", "tangent")):
             conv[key] = converge_newton(sa, ccfg, args.converge_cap, jvp=jvp,
                                         workloads=workloads, name=name)
             log("converge %s %s: %s" % (name, key, conv[key]))
