@@ -120,7 +120,7 @@ def test_autograd_backward_is_the_adjoint(product):
     loss.backward()
     f0, g = s.free_energy_gradient()
     gp = np.stack([g.real, g.imag], axis=1)
-    assert abs(float(f) - f0) <= 1e-14 * abs(f0)
+    assert abs(float(f.detach()) - f0) <= 1e-14 * abs(f0)
     expect = 2.0 * gp + 2.0 * planes
     assert float(np.max(np.abs(x.grad.numpy() - expect))) <= 1e-14 * float(np.max(np.abs(expect)))
     # a complex tensor through view_as_real.  Each evaluation is a new run
