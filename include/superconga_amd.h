@@ -249,6 +249,16 @@ int sc_residual_gradient(sc_context *ctx, double *grad, double *grad_a, double *
  * reference. */
 int sc_jacobian_vector(sc_context *ctx, const double *v, double *jv);
 
+/* How the reverse sweeps of sc_free_energy_gradient(_ex) and
+ * sc_residual_gradient keep the coherence functions they run back over:
+ * k > 0 keeps a checkpoint every k sites and integrates each segment again
+ * when the reverse passes reach it (twice the forward steps, scratch of
+ * ceil(L / k) checkpoints and one segment instead of L sites for the
+ * longest trajectory L); -1 (the default) takes k = ceil(sqrt(L)); 0 keeps
+ * every site of every trajectory.  The results are the same.  No
+ * counterpart in the reference. */
+int sc_set_adjoint_checkpoints(sc_context *ctx, int k);
+
 /* Individual stages. */
 /* RiccatiSolver::computeCoherenceFunctions for every angle (RiccatiSolverConfined.h:513-671)
  * fused with ComputeProperty::computeGreensAngle; burnin = 1 only updates the

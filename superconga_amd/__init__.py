@@ -160,6 +160,7 @@ def lib():
         L.sc_free_energy_gradient_ex.argtypes = [vp, _DP, _DP, _DP]
         L.sc_residual_gradient.argtypes = [vp, _DP, _DP, _DP]
         L.sc_jacobian_vector.argtypes = [vp, _DP, _DP]
+        L.sc_set_adjoint_checkpoints.argtypes = [vp, ctypes.c_int]
         L.sc_ldos_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(vp)]
         L.sc_ldos_compute.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.sc_ldos_run.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -376,6 +377,13 @@ class Simulation:
         _check(lib().sc_residual_gradient(self._h, _dp(g), _dp(ga), _dp(r)))
         g = g.reshape(self.Nc, 2, n, n)
         return float(r[0]), g[:, 0] + 1j * g[:, 1], ga.reshape(2, n, n)
+
+    def set_adjoint_checkpoints(self, k):
+        """The reverse sweeps' storage: k > 0 a checkpoint every k sites
+        (each segment integrated again), -1 (the default) picks
+        k = ceil(sqrt(longest trajectory)), 0 keeps every site.  The
+        gradients are the same."""
+        _check(lib().sc_set_adjoint_checkpoints(self._h, int(k)))
 
     def jacobian_vector(self, v_delta, v_a=None):
         """J v of one plain FP64 iteration u -> G(u) at the current state

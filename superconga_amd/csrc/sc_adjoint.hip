@@ -249,6 +249,8 @@ struct AdjointArgs {
   double *dbar;           // [3][Na][N1^2]: dF/dD re, im, dF/des
   long long framePlane;   // Na * N1^2
   double *partial;        // per item: its share of the free-energy sum
+  int ck;                 // checkpoint segment length (0: every site kept)
+  int ckSeg;              // segments per scratch buffer (ceil(maxLen / ck))
 };
 
 // Local adjoints of gamma and gamma~ at one site and energy through the
@@ -329,51 +331,150 @@ __device__ void adjointItem(const AdjointArgs &a, const SweepItemDev &it, int it
     }
   };
 
-  // ---------------- pass 1: gamma~ from the top, kept at every site ----------
+  // With checkpoints (a.ck = K > 0) each scratch buffer holds, per segment
+  // of K sites, the state before the segment's first site (EL values and
+  // the entry count, per lane), then room for one segment's values: pass 1
+  // keeps gamma~ only at the segments' starts and pass 2 integrates each
+  // segment again from there when it reaches it; pass 2 leaves gamma's
+  // state at the bottom of every segment and pass 3 integrates each segment
+  // again.  The same arithmetic as keeping every site.
+  const int K = a.ck;
+  auto ckSlot = [&](int sg, int j) { return (size_t(sg) * (EL + 1) + j) * 64 + lane; };
+  auto segSlot = [&](int i, int j) {
+    return (size_t(a.ckSeg) * (EL + 1) + size_t(i) * EL + j) * 64 + lane;
+  };
+  const int baseInT0 = v.idxInGt[col], baseInG0 = v.idxInG[col];
+  // one site of gamma~ from the top (the sweep's pass 1): t counts from it.yhi
+  auto fwdT = [&](int t, Cx<double>(&coh)[EL], Cx<double> &dPrev, double &esPrev, int &nT) {
+    const size_t o = size_t(it.yhi - t) * N1;
+    const Cx<double> D = {-F[o], F[v.fieldPlane + o]};
+    const double es = F[2 * v.fieldPlane + o];
+    const int8_t lc = lab[o];
+    if (active) {
+      if (lc == 1 || lc == 2) {
+        const StepSite<double> ps = stepSite(
+            Cx<double>{(dPrev.re + D.re) * 0.5, (dPrev.im + D.im) * 0.5}, (esPrev + es) * 0.5, h);
+        const double half = t > 0 ? 1.0 : 0.5;
+#pragma unroll
+        for (int j = 0; j < EL; ++j) coh[j] = stepEnergy(ps, ser[j] * half, coh[j]);
+      } else if (lc == 3) {
+        const int ci = baseInT0 + nT;
+        const double dr = v.distInGt[ci];
+        const StepSite<double> ps = stepSite(D, es, dr * h);
+#pragma unroll
+        for (int j = 0; j < EL; ++j) {
+          const size_t si = size_t(ci) * Ne + g * EL + j;
+          const Cx<double> b0 = eOk[j] ? Cx<double>{v.bInGt[si], v.bInGt[size_t(v.nInGt) * Ne + si]}
+                                       : Cx<double>{0.0, 0.0};
+          coh[j] = stepEnergy(ps, ser[j] * dr, b0);
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < EL; ++j) coh[j] = {0.0, 0.0};
+      }
+    }
+    if (lc == 3) ++nT;
+    dPrev = D;
+    esPrev = es;
+  };
+  // one site of gamma from the bottom (the sweep's pass 2): u counts from it.ylo
+  auto fwdG = [&](int u, Cx<double>(&coh)[EL], Cx<double> &dPrev, double &esPrev, int &nG) {
+    const size_t o = size_t(it.ylo + u) * N1;
+    const Cx<double> D = {F[o], F[v.fieldPlane + o]};
+    const double es = F[2 * v.fieldPlane + o];
+    const int8_t lc = lab[o];
+    if (active) {
+      if (lc == 1 || lc == 3) {
+        const StepSite<double> ps = stepSite(
+            Cx<double>{(dPrev.re + D.re) * 0.5, (dPrev.im + D.im) * 0.5}, (esPrev + es) * 0.5, h);
+        const double half = u > 0 ? 1.0 : 0.5;
+#pragma unroll
+        for (int j = 0; j < EL; ++j) coh[j] = stepEnergy(ps, ser[j] * half, coh[j]);
+      } else if (lc == 2) {
+        const int ci = baseInG0 + nG;
+        const double dr = v.distInG[ci];
+        const StepSite<double> ps = stepSite(D, es, dr * h);
+#pragma unroll
+        for (int j = 0; j < EL; ++j) {
+          const size_t si = size_t(ci) * Ne + g * EL + j;
+          const Cx<double> b0 = eOk[j] ? Cx<double>{v.bInG[si], v.bInG[size_t(v.nInG) * Ne + si]}
+                                       : Cx<double>{0.0, 0.0};
+          coh[j] = stepEnergy(ps, ser[j] * dr, b0);
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < EL; ++j) coh[j] = {0.0, 0.0};
+      }
+    }
+    if (lc == 2) ++nG;
+    dPrev = D;
+    esPrev = es;
+  };
+  // gamma~ of segment sg again, from its checkpoint, into the segment room
+  auto recomputeT = [&](int sg) {
+    const int t0 = sg * K, t1 = min(t0 + K, len);
+    Cx<double> cc[EL];
+#pragma unroll
+    for (int j = 0; j < EL; ++j) cc[j] = active ? scrT[ckSlot(sg, j)] : Cx<double>{0.0, 0.0};
+    int nT = active ? int(scrT[ckSlot(sg, EL)].re) : 0;
+    Cx<double> dp = {0.0, 0.0};
+    double ep = 0.0;
+    if (t0 > 0) {
+      const size_t o = size_t(it.yhi - (t0 - 1)) * N1;
+      dp = {-F[o], F[v.fieldPlane + o]};
+      ep = F[2 * v.fieldPlane + o];
+    }
+    for (int t = t0; t < t1; ++t) {
+      fwdT(t, cc, dp, ep, nT);
+      if (active) {
+#pragma unroll
+        for (int j = 0; j < EL; ++j) scrT[segSlot(t - t0, j)] = cc[j];
+      }
+    }
+  };
+  // gamma of segment sg again, from the state pass 2 left at its bottom
+  auto recomputeG = [&](int sg) {
+    const int t0 = sg * K, t1 = min(t0 + K, len) - 1;
+    Cx<double> cc[EL];
+#pragma unroll
+    for (int j = 0; j < EL; ++j) cc[j] = active ? scrG[ckSlot(sg, j)] : Cx<double>{0.0, 0.0};
+    int nG = active ? int(scrG[ckSlot(sg, EL)].re) : 0;
+    const int u0 = len - 1 - t1;
+    Cx<double> dp = {0.0, 0.0};
+    double ep = 0.0;
+    if (u0 > 0) {
+      const size_t o = size_t(it.ylo + u0 - 1) * N1;
+      dp = {F[o], F[v.fieldPlane + o]};
+      ep = F[2 * v.fieldPlane + o];
+    }
+    for (int u = u0; u <= len - 1 - t0; ++u) {
+      fwdG(u, cc, dp, ep, nG);
+      if (active) {
+#pragma unroll
+        for (int j = 0; j < EL; ++j) scrG[segSlot((len - 1 - u) - t0, j)] = cc[j];
+      }
+    }
+  };
+
+  // ------- pass 1: gamma~ from the top, kept at every site or checkpointed -------
   int numEntryT = 0;
   {
-    const int baseIn = v.idxInGt[col];
     Cx<double> coh[EL];
 #pragma unroll
     for (int j = 0; j < EL; ++j) coh[j] = {0.0, 0.0};
     Cx<double> dPrev = {0.0, 0.0};
     double esPrev = 0.0;
-    bool havePrev = false;
     for (int t = 0; t < len; ++t) {
-      const int y = it.yhi - t;
-      const size_t o = size_t(y) * N1;
-      const Cx<double> D = {-F[o], F[v.fieldPlane + o]};
-      const double es = F[2 * v.fieldPlane + o];
-      const int8_t lc = lab[o];
-      if (active) {
-        if (lc == 1 || lc == 2) {
-          const StepSite<double> ps = stepSite(
-              Cx<double>{(dPrev.re + D.re) * 0.5, (dPrev.im + D.im) * 0.5}, (esPrev + es) * 0.5, h);
-          const double half = havePrev ? 1.0 : 0.5;
+      if (K > 0 && t % K == 0 && active) {
 #pragma unroll
-          for (int j = 0; j < EL; ++j) coh[j] = stepEnergy(ps, ser[j] * half, coh[j]);
-        } else if (lc == 3) {
-          const int ci = baseIn + numEntryT;
-          const double dr = v.distInGt[ci];
-          const StepSite<double> ps = stepSite(D, es, dr * h);
-#pragma unroll
-          for (int j = 0; j < EL; ++j) {
-            const size_t si = size_t(ci) * Ne + g * EL + j;
-            const Cx<double> b0 = eOk[j] ? Cx<double>{v.bInGt[si], v.bInGt[size_t(v.nInGt) * Ne + si]}
-                                         : Cx<double>{0.0, 0.0};
-            coh[j] = stepEnergy(ps, ser[j] * dr, b0);
-          }
-        } else {
-#pragma unroll
-          for (int j = 0; j < EL; ++j) coh[j] = {0.0, 0.0};
-        }
+        for (int j = 0; j < EL; ++j) scrT[ckSlot(t / K, j)] = coh[j];
+        scrT[ckSlot(t / K, EL)] = Cx<double>{double(numEntryT), 0.0};
+      }
+      fwdT(t, coh, dPrev, esPrev, numEntryT);
+      if (K == 0 && active) {
 #pragma unroll
         for (int j = 0; j < EL; ++j) scrT[slot(t, j)] = coh[j];
       }
-      if (lc == 3) ++numEntryT;
-      dPrev = D;
-      esPrev = es;
-      havePrev = true;
     }
   }
 
@@ -400,9 +501,21 @@ __device__ void adjointItem(const AdjointArgs &a, const SweepItemDev &it, int it
       const Cx<double> D = {dre, dim}, Dt = {-dre, dim};
       const int8_t lc = lab[o];
       const double rho = RES ? 0.0 : rhoK[o];
+      if (K > 0 && (t == len - 1 || t % K == K - 1)) {
+        // the bottom of a segment: gamma's state before it, for pass 3, and
+        // gamma~ of the segment again
+        if (active) {
+#pragma unroll
+          for (int j = 0; j < EL; ++j) scrG[ckSlot(t / K, j)] = coh[j];
+          scrG[ckSlot(t / K, EL)] = Cx<double>{double(numEntryG), 0.0};
+        }
+        recomputeT(t / K);
+      }
       Cx<double> gt[EL];
 #pragma unroll
-      for (int j = 0; j < EL; ++j) gt[j] = active ? scrT[slot(t, j)] : Cx<double>{0.0, 0.0};
+      for (int j = 0; j < EL; ++j)
+        gt[j] = active ? (K > 0 ? scrT[segSlot(t - (t / K) * K, j)] : scrT[slot(t, j)])
+                       : Cx<double>{0.0, 0.0};
       SiteBar dbCur = {0.0, 0.0, 0.0};
       if (active) {
         // gamma into this site (the sweep's pass 2)
@@ -427,8 +540,10 @@ __device__ void adjointItem(const AdjointArgs &a, const SweepItemDev &it, int it
 #pragma unroll
           for (int j = 0; j < EL; ++j) coh[j] = {0.0, 0.0};
         }
+        if (K == 0) {
 #pragma unroll
-        for (int j = 0; j < EL; ++j) scrG[slot(t, j)] = coh[j];
+          for (int j = 0; j < EL; ++j) scrG[slot(t, j)] = coh[j];
+        }
         if constexpr (!RES) {
           // the free-energy density and its own dependence on D
           double su = 0.0, sw = 0.0;
@@ -467,7 +582,7 @@ __device__ void adjointItem(const AdjointArgs &a, const SweepItemDev &it, int it
             Cx<double> lg = {0.0, 0.0}, lt = {0.0, 0.0};
             if (eOk[j]) greensAdjoint(coh[j], gt[j], sd, sj, res[j], lg, lt);
             mu[j] = eOk[j] ? carry[j] + lt : Cx<double>{0.0, 0.0};
-            scrT[slot(t, j)] = lg;
+            if (K == 0) scrT[slot(t, j)] = lg;
           }
         } else {
 #pragma unroll
@@ -537,9 +652,17 @@ __device__ void adjointItem(const AdjointArgs &a, const SweepItemDev &it, int it
       const Cx<double> D = {dre, dim};
       const int8_t lc = lab[o];
       const double rho = RES ? 0.0 : rhoK[o];
+      if (K > 0 && t % K == 0) {
+        // the top of a segment: gamma of the segment again (and, for the
+        // residual's local adjoints, gamma~)
+        recomputeG(t / K);
+        if constexpr (RES) recomputeT(t / K);
+      }
       Cx<double> gc[EL];
 #pragma unroll
-      for (int j = 0; j < EL; ++j) gc[j] = active ? scrG[slot(t, j)] : Cx<double>{0.0, 0.0};
+      for (int j = 0; j < EL; ++j)
+        gc[j] = active ? (K > 0 ? scrG[segSlot(t - (t / K) * K, j)] : scrG[slot(t, j)])
+                       : Cx<double>{0.0, 0.0};
       SiteBar dbCur = {0.0, 0.0, 0.0};
       if (active) {
         Cx<double> carry[EL];
@@ -559,10 +682,21 @@ __device__ void adjointItem(const AdjointArgs &a, const SweepItemDev &it, int it
           addBar(dbA, dd, 0.5 * eb);
         }
         if constexpr (RES) {
-          // the local adjoint of gamma that pass 2 left in the gamma~ scratch
+          // the local adjoint of gamma: left by pass 2 in the gamma~ scratch,
+          // or formed here from the segment's gamma~ again
+          const Cx<double> sd = {seedK[o], seedK[a.framePlane + o]};
+          const double sj = seedK[2 * a.framePlane + o];
 #pragma unroll
-          for (int j = 0; j < EL; ++j)
-            lam[j] = eOk[j] ? carry[j] + scrT[slot(t, j)] : Cx<double>{0.0, 0.0};
+          for (int j = 0; j < EL; ++j) {
+            Cx<double> lg = {0.0, 0.0};
+            if (K == 0) {
+              lg = scrT[slot(t, j)];
+            } else if (eOk[j]) {
+              Cx<double> lt;
+              greensAdjoint(gc[j], scrT[segSlot(t - (t / K) * K, j)], sd, sj, res[j], lg, lt);
+            }
+            lam[j] = eOk[j] ? carry[j] + lg : Cx<double>{0.0, 0.0};
+          }
         } else {
           // dF/dg = rho r (D.im - i D.re)
 #pragma unroll
@@ -1037,8 +1171,7 @@ void Grain::prepareAdjoint() {
   const Setup &s = *s_;
   const size_t n0 = size_t(s.N0) * s.N0, n1 = size_t(s.N1) * s.N1;
   const long long framePlane = static_cast<long long>(s.Na) * static_cast<long long>(n1);
-  if (adjItems_) return;
-  {
+  if (!adjItems_) {
     const GrainView gv = sweepView(kAdjEL);
     std::vector<SweepItem> items = sweepItems(s, gv.cols, &adjMaxLen_);
     std::vector<SweepItemDev> di(items.size());
@@ -1051,8 +1184,6 @@ void Grain::prepareAdjoint() {
     SC_HIP(hipGetDevice(&dev));
     SC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     adjSlots_ = std::max(1, std::min(adjNumItems_, kAdjSlotsPerCU * std::max(cus, 1)));
-    const size_t stride = size_t(adjMaxLen_) * kAdjEL * 64;
-    dAdjScratch_ = devAlloc<double>(size_t(adjSlots_) * 4 * stride);
     dAdjPartial_ = devAlloc<double>(size_t(std::max(adjNumItems_, 1)));
     dAdjRho_ = devAlloc<double>(size_t(framePlane));
     dAdjDbar_ = devAlloc<double>(3 * size_t(framePlane));
@@ -1060,6 +1191,32 @@ void Grain::prepareAdjoint() {
     dAdjGrad_ = devAlloc<double>(size_t(2 * s.Nc) * n0);
     dAdjGradA_ = devAlloc<double>(2 * n0);
   }
+  // the slots' scratch for the setting of the checkpoints: every site
+  // (two buffers of maxLen sites), or segments of K sites (two buffers of
+  // ceil(maxLen / K) checkpoints and one segment each)
+  int K = 0;
+  if (adjCk_ > 0) K = std::min(adjCk_, std::max(adjMaxLen_, 1));
+  if (adjCk_ < 0) K = std::max(1, int(std::ceil(std::sqrt(double(std::max(adjMaxLen_, 1))))));
+  if (!dAdjScratch_ || K != adjCkUsed_) {
+    devFree(dAdjScratch_);
+    adjCkUsed_ = K;
+    adjCkSeg_ = K > 0 ? (adjMaxLen_ + K - 1) / K : 0;
+    adjSlotStride_ = K > 0 ? (static_cast<long long>(adjCkSeg_) * (kAdjEL + 1) +
+                              static_cast<long long>(K) * kAdjEL) * 64
+                           : static_cast<long long>(adjMaxLen_) * kAdjEL * 64;
+    // complex elements, two buffers per slot
+    dAdjScratch_ = devAlloc<double>(size_t(adjSlots_) * 4 * size_t(adjSlotStride_));
+  }
+}
+
+void Grain::setAdjointCheckpoints(int k) {
+  if (k < -1) throw std::runtime_error("checkpoint segment length must be >= -1");
+  adjCk_ = k;
+}
+
+size_t Grain::adjointScratchBytes() {
+  prepareAdjoint();
+  return size_t(adjSlots_) * 4 * size_t(adjSlotStride_) * sizeof(double);
 }
 
 double Grain::freeEnergyGradient(double *grad, double *gradA) {
@@ -1100,7 +1257,9 @@ double Grain::freeEnergyGradient(double *grad, double *gradA) {
   a.numItems = adjNumItems_;
   a.counter = dCounter_;
   a.scratch = reinterpret_cast<Cx<double> *>(dAdjScratch_);
-  a.slotStride = static_cast<long long>(adjMaxLen_) * kAdjEL * 64;
+  a.slotStride = adjSlotStride_;
+  a.ck = adjCkUsed_;
+  a.ckSeg = adjCkSeg_;
   a.rho = dAdjRho_;
   a.seed = nullptr;
   a.dbar = dAdjDbar_;
@@ -1254,7 +1413,9 @@ double Grain::residualGradient(double *grad, double *gradA) {
   a.numItems = adjNumItems_;
   a.counter = dCounter_;
   a.scratch = reinterpret_cast<Cx<double> *>(dAdjScratch_);
-  a.slotStride = static_cast<long long>(adjMaxLen_) * kAdjEL * 64;
+  a.slotStride = adjSlotStride_;
+  a.ck = adjCkUsed_;
+  a.ckSeg = adjCkSeg_;
   a.rho = dAdjRho_;
   a.seed = dResSeed_;
   a.dbar = dAdjDbar_;

@@ -472,6 +472,16 @@ int sc_free_energy_gradient(sc_context *ctx, double *grad, double *freeEnergy) {
   return sc_free_energy_gradient_ex(ctx, grad, nullptr, freeEnergy);
 }
 
+int sc_set_adjoint_checkpoints(sc_context *ctx, int k) {
+  SC_CHECK_CTX(ctx);
+  SC_CHECK_SIM(ctx);
+  if (k < -1) return fail(SC_ERR_ARGUMENT, "k must be -1 (automatic), 0 (every site) or positive");
+  return guarded([&] {
+    ctx->grain->setAdjointCheckpoints(k);
+    return SC_OK;
+  });
+}
+
 int sc_jacobian_vector(sc_context *ctx, const double *v, double *jv) {
   SC_CHECK_CTX(ctx);
   SC_CHECK_SIM(ctx);

@@ -113,7 +113,7 @@ Grain::~Grain() {
   devFree(dAdjGradA_);
   for (double *p : {dResD_, dResA_, dResPois_, dResSbar_, dResJbar_, dResZero_, dResMean_,
                     dResSeed_, dResSave_, dTanExt_, dTanFields_, dTanG_, dTanJ_, dTanE_,
-                    dTanOutG_, dTanOutGt_, dTanV_, dTanJV_})
+                    dTanOutG_, dTanOutGt_, dTanV_, dTanJV_, dTanScratch_})
     devFree(p);
   dropEndGraph();
   fft_.reset();
@@ -1764,6 +1764,7 @@ void Grain::jacobianVector(const double *dv, double *djv, bool inValues, bool sc
     dTanE_ = devAlloc<double>(n0);
     dTanOutG_ = devAlloc<double>(std::max<size_t>(1, size_t(2 * s.nOutG * s.Ne)));
     dTanOutGt_ = devAlloc<double>(std::max<size_t>(1, size_t(2 * s.nOutGt * s.Ne)));
+    dTanScratch_ = devAlloc<double>(size_t(adjSlots_) * 4 * size_t(adjMaxLen_) * kAdjointEL * 64);
   }
   // the state's fields, then the tangents of the extended order parameter
   // and of the site fields (the induced vector potential's share of the
@@ -1794,7 +1795,7 @@ void Grain::jacobianVector(const double *dv, double *djv, bool inValues, bool sc
   // the sweep's tangent
   SC_HIP(hipMemsetAsync(dTanG_, 0, 4 * size_t(framePlane) * sizeof(double), stream_));
   tangentSweep(sweepView(kAdjointEL), dTanFields_, adjItems_, adjNumItems_, dCounter_,
-               dAdjScratch_, static_cast<long long>(adjMaxLen_) * kAdjointEL * 64,
+               dTanScratch_, static_cast<long long>(adjMaxLen_) * kAdjointEL * 64,
                inValues ? dv + nx : nullptr, inValues ? dv + nx + nG : nullptr, dTanOutG_,
                dTanOutGt_, dTanG_, adjSlots_, stream_);
   // the back-rotation, angle integration and gap equation of the tangents

@@ -199,6 +199,12 @@ public:
   // The same on host vectors [Delta planes (Nc x 2 x N0^2), A planes
   // (2 x N0^2, in the setters' units)], in-values held fixed, unscreened.
   void jacobianVectorHost(const double *v, double *jv);
+  // The reverse sweeps' storage of the coherence functions: 0 keeps every
+  // site, k > 0 keeps checkpoints every k sites and integrates each segment
+  // again when the reverse passes reach it, -1 (the default) picks
+  // k = ceil(sqrt(longest trajectory)).  Results are the same either way.
+  void setAdjointCheckpoints(int k);
+  size_t adjointScratchBytes();
   void resetAccelerator();
 
   // Keldysh spectroscopy (src/postprocess.cu), for a grain built by
@@ -392,6 +398,9 @@ private:
          *dResJbar_ = nullptr, *dResZero_ = nullptr, *dResMean_ = nullptr, *dResSeed_ = nullptr,
          *dResSave_ = nullptr;
   void prepareAdjoint();
+  int adjCk_ = -1, adjCkUsed_ = 0, adjCkSeg_ = 0;
+  long long adjSlotStride_ = 0;
+  double *dTanScratch_ = nullptr; // the tangent sweep's own (every site kept)
   // the tangent-linear products' buffers
   double *dTanExt_ = nullptr, *dTanFields_ = nullptr, *dTanG_ = nullptr, *dTanJ_ = nullptr,
          *dTanE_ = nullptr, *dTanOutG_ = nullptr, *dTanOutGt_ = nullptr, *dTanV_ = nullptr,

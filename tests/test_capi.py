@@ -182,6 +182,7 @@ def test_spectroscopy_config_errors(product):
     assert L.sc_free_energy_gradient_ex(None, None, None, None) == 1
     assert L.sc_residual_gradient(None, None, None, None) == 1
     assert L.sc_jacobian_vector(None, None, None) == 1
+    assert L.sc_set_adjoint_checkpoints(None, 0) == 1
 
 
 def test_noalloc_poisson_refuses_oversized_grid(product):

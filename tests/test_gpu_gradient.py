@@ -428,3 +428,36 @@ def test_jacobian_vector_matches_oracle_differences(product, oracle, name):
     assert err < 1e-7 and err_a < 2e-7
     s.close()
 
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_checkpointed_reverse_sweeps_give_the_same_gradients(product, name):
+    # sc_set_adjoint_checkpoints: the coherence functions kept only every k
+    # sites and each segment integrated again must give the gradients of
+    # keeping every site (the same arithmetic; the transposes' atomic sums
+    # add in another order)
+    cfg = CASES[name]()
+    d, a = state(product, cfg)
+    s = product.Simulation(cfg)
+    for c in range(d.shape[0]):
+        s.set_order_parameter(c, d[c])
+    if lam(cfg) > 0:
+        s.set_vector_potential(a[0], a[1])
+    s.set_adjoint_checkpoints(0)
+    f0, g0, ga0 = s.free_energy_gradient(vector_potential=True)
+    r0, h0, ha0 = s.residual_gradient()
+    for k in (1, 3, 7, -1):
+        s.set_adjoint_checkpoints(k)
+        f, g, ga = s.free_energy_gradient(vector_potential=True)
+        r, hh, ha = s.residual_gradient()
+        errs = (abs(f - f0) / abs(f0), np.max(np.abs(g - g0)) / np.max(np.abs(g0)),
+                abs(r - r0) / r0, np.max(np.abs(hh - h0)) / np.max(np.abs(h0)))
+        print("%s, checkpoints every %d: differences %s" % (name, k, ["%.1e" % e for e in errs]))
+        assert errs[0] <= 1e-14 and errs[2] <= 1e-14
+        assert errs[1] <= 1e-13 and errs[3] <= 1e-13
+        if lam(cfg) > 0:
+            assert np.max(np.abs(ga - ga0)) <= 1e-13 * np.max(np.abs(ga0))
+            assert np.max(np.abs(ha - ha0)) <= 1e-13 * np.max(np.abs(ha0))
+    with pytest.raises(product.SupercongaError):
+        s.set_adjoint_checkpoints(-2)
+    s.close()
