@@ -259,6 +259,14 @@ int sc_jacobian_vector(sc_context *ctx, const double *v, double *jv);
  * counterpart in the reference. */
 int sc_set_adjoint_checkpoints(sc_context *ctx, int k);
 
+/* Newton-Krylov's linear solves on this grain (sc_newton_krylov(_ex), and
+ * sc_batch_newton_krylov for the batch's grains): GMRES cycles per Newton
+ * step (restarts >= 1, default 4) and the cap of the Eisenstat-Walker
+ * forcing term (0 < eta_max < 1, default 0.3), the linear residual a Newton
+ * step asks for relative to the nonlinear one.  No counterpart in the
+ * reference. */
+int sc_set_newton_options(sc_context *ctx, int restarts, double eta_max);
+
 /* Individual stages. */
 /* RiccatiSolver::computeCoherenceFunctions for every angle (RiccatiSolverConfined.h:513-671)
  * fused with ComputeProperty::computeGreensAngle; burnin = 1 only updates the

@@ -161,6 +161,7 @@ def lib():
         L.sc_residual_gradient.argtypes = [vp, _DP, _DP, _DP]
         L.sc_jacobian_vector.argtypes = [vp, _DP, _DP]
         L.sc_set_adjoint_checkpoints.argtypes = [vp, ctypes.c_int]
+        L.sc_set_newton_options.argtypes = [vp, ctypes.c_int, ctypes.c_double]
         L.sc_ldos_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(vp)]
         L.sc_ldos_compute.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.sc_ldos_run.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -377,6 +378,11 @@ class Simulation:
         _check(lib().sc_residual_gradient(self._h, _dp(g), _dp(ga), _dp(r)))
         g = g.reshape(self.Nc, 2, n, n)
         return float(r[0]), g[:, 0] + 1j * g[:, 1], ga.reshape(2, n, n)
+
+    def set_newton_options(self, restarts=4, eta_max=0.3):
+        """Newton-Krylov's GMRES cycles per Newton step and the cap of its
+        Eisenstat-Walker forcing term."""
+        _check(lib().sc_set_newton_options(self._h, int(restarts), ctypes.c_double(eta_max)))
 
     def set_adjoint_checkpoints(self, k):
         """The reverse sweeps' storage: k > 0 a checkpoint every k sites

@@ -1209,6 +1209,13 @@ void Grain::prepareAdjoint() {
   }
 }
 
+void Grain::setNewtonOptions(int restarts, double etaMax) {
+  if (restarts < 1) throw std::runtime_error("restarts must be >= 1");
+  if (!(etaMax > 0.0 && etaMax < 1.0)) throw std::runtime_error("eta_max must be in (0, 1)");
+  nkRestarts_ = restarts;
+  nkEtaMax_ = etaMax;
+}
+
 void Grain::setAdjointCheckpoints(int k) {
   if (k < -1) throw std::runtime_error("checkpoint segment length must be >= -1");
   adjCk_ = k;

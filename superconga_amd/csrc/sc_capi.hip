@@ -472,6 +472,15 @@ int sc_free_energy_gradient(sc_context *ctx, double *grad, double *freeEnergy) {
   return sc_free_energy_gradient_ex(ctx, grad, nullptr, freeEnergy);
 }
 
+int sc_set_newton_options(sc_context *ctx, int restarts, double etaMax) {
+  SC_CHECK_CTX(ctx);
+  SC_CHECK_SIM(ctx);
+  return guarded([&] {
+    ctx->grain->setNewtonOptions(restarts, etaMax);
+    return SC_OK;
+  });
+}
+
 int sc_set_adjoint_checkpoints(sc_context *ctx, int k) {
   SC_CHECK_CTX(ctx);
   SC_CHECK_SIM(ctx);

@@ -204,6 +204,11 @@ public:
   // again when the reverse passes reach it, -1 (the default) picks
   // k = ceil(sqrt(longest trajectory)).  Results are the same either way.
   void setAdjointCheckpoints(int k);
+  // Newton-Krylov's GMRES cycles per Newton step (each of at most
+  // krylovDim directions; default 4) and the cap of its Eisenstat-Walker
+  // forcing term (the linear residual a Newton step asks for, relative to
+  // the nonlinear one; default 0.3).
+  void setNewtonOptions(int restarts, double etaMax);
   size_t adjointScratchBytes();
   void resetAccelerator();
 
@@ -399,6 +404,8 @@ private:
          *dResSave_ = nullptr;
   void prepareAdjoint();
   int adjCk_ = -1, adjCkUsed_ = 0, adjCkSeg_ = 0;
+  int nkRestarts_ = 4;
+  double nkEtaMax_ = 0.3;
   long long adjSlotStride_ = 0;
   double *dTanScratch_ = nullptr; // the tangent sweep's own (every site kept)
   // the tangent-linear products' buffers

@@ -40,8 +40,6 @@ namespace scamd {
 namespace {
 
 constexpr int kMaxTerms = 64;
-// GMRES cycles per Newton step (each of at most krylovDim directions)
-constexpr int kRestarts = 4;
 struct Terms {
   const double *v[kMaxTerms];
   double c[kMaxTerms];
@@ -161,6 +159,7 @@ NewtonKrylov::NewtonKrylov(Grain &g, int maxNewton, int krylovDim, double tol,
     throw std::runtime_error("Jacobian-vector products run in fp64 or mixed precision, or "
                              "exactly by the tangent-linear sweep");
   if (g.s_->keldysh) throw std::runtime_error("a spectroscopy context only runs the LDOS iteration");
+  eta_ = std::min(eta_, g.nkEtaMax_);
   if (krylovDim < 1 || krylovDim > kMaxTerms - 1)
     throw std::runtime_error("Krylov dimension must be in [1, " + std::to_string(kMaxTerms - 1) +
                              "]");
@@ -363,7 +362,7 @@ void NewtonKrylov::endCycle() {
   std::vector<const double *> vk(V_.begin(), V_.begin() + k);
   combine(dstep_, dstep_, 1.0, vk, y);
   rn_ = std::fabs(gv_[size_t(k)]);
-  if (rn_ <= eta_ * fn_ || breakdown_ || cycle_ + 1 == kRestarts) {
+  if (rn_ <= eta_ * fn_ || breakdown_ || cycle_ + 1 == g_.nkRestarts_) {
     startLineSearch();
     return;
   }
@@ -402,7 +401,7 @@ void NewtonKrylov::lineDone() {
     rep_.history.push_back(fn_);
     ++rep_.newtonIterations;
     // Eisenstat-Walker forcing term (choice 2)
-    eta_ = std::clamp(0.9 * (fn_ / fnPrev) * (fn_ / fnPrev), 1e-4, 0.3);
+    eta_ = std::clamp(0.9 * (fn_ / fnPrev) * (fn_ / fnPrev), 1e-4, g_.nkEtaMax_);
     if (fn_ < tol_ || rep_.newtonIterations >= maxNewton_) phase_ = DONE;
     else startNewton();
     return;

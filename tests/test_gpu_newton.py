@@ -93,6 +93,29 @@ def test_newton_krylov_rejects_bad_arguments(product):
         args.update(kw)
         with pytest.raises(product.SupercongaError):
             s.newton_krylov(**args)
+    for restarts, eta in ((0, 0.3), (4, 0.0), (4, 1.0)):
+        with pytest.raises(product.SupercongaError):
+            s.set_newton_options(restarts, eta)
+
+
+def test_newton_options_keep_the_fixed_point(product):
+    # one GMRES cycle per Newton step with a tight forcing cap, and many
+    # cycles with a loose one: the same fixed point
+    cfg = cases.accelerator_case("picard")
+    ref = converged_plain(product, cfg)
+    for restarts, eta in ((1, 0.01), (12, 0.5)):
+        s = product.Simulation(cfg)
+        s.set_newton_options(restarts, eta)
+        rep = s.newton_krylov(max_newton=40, krylov_dim=20, tol=1e-10)
+        print("restarts %d, eta_max %g:" % (restarts, eta),
+              {k: v for k, v in rep.items() if k != "history"})
+        assert rep["converged"], rep
+        s.compute("fp64")
+        assert s.results()["residual_order_parameter"] < 1e-8
+        a, b = s.order_parameter(1), ref.order_parameter(1)
+        phase = np.angle(np.sum(np.conj(b) * a))
+        assert rel(a * np.exp(-1j * phase), b) < 1e-6
+        s.close()
 
 
 def test_newton_krylov_with_fp32_products_finds_the_fp64_fixed_point(product):
