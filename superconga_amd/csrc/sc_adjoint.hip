@@ -46,7 +46,7 @@ namespace scamd {
 namespace {
 
 constexpr int kAdjEL = kAdjointEL; // Ozaki energies per lane of the reverse sweep
-constexpr int kAdjSlotsPerCU = 4; // persistent wavefronts per compute unit
+constexpr int kAdjSlotsPerCU = 8; // persistent wavefronts per compute unit (2 per SIMD)
 
 __device__ __forceinline__ Cx<double> conjc(Cx<double> a) { return {a.re, -a.im}; }
 
