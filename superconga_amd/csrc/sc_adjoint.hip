@@ -145,58 +145,108 @@ __global__ void extendAdjointKernel(double *bar, const int *target, const int *o
   }
 }
 
-// 3a. Transpose of the site fields: of the order parameter
-// D_k(s) = sum_c b_c(k) bilinear(extDelta_c)(rotated s), dF/dD_k(s)
-// scattered to the four taps of every component; with aBar, of the Doppler
-// shift es_k(s) = q v_k . bilinear(A)(rotated s) (siteFields), dF/des_k(s)
-// scattered to the four taps of A_x and A_y.  One thread per frame site.
+// 3a. Transpose of the site fields, as a gather.  The site fields take the
+// order parameter D_k(s) = sum_c b_c(k) bilinear(extDelta_c)(rotated s) and,
+// with aBar, the Doppler shift es_k(s) = q v_k . bilinear(A)(rotated s)
+// (siteFields).  One thread per reference site p collects, frame by frame,
+// dF/dD_k(s) w(s, p) and dF/des_k(s) w(s, p) from the frame sites s whose
+// bilinear taps include p: those lie within sqrt(2) of p's inverse-rotated
+// position, inside a 4 x 4 window, and rotatedCell (the forward's own
+// arithmetic) decides which of them tap p and with what weight.  At the
+// grid's lower edges (p.x or p.y <= 1) rotatedCell's truncation toward zero
+// also takes rotated positions in (-1, 0) to cell 0, up to 2 sites from p
+// in that coordinate, so the window grows to 7 x 7 there.  Every output is
+// written by one thread: no atomics, the same sum on every run.
+__device__ __forceinline__ void gatherFrame(const double *const *planes, int nPlanes, int px,
+                                            int py, int N0, int N1, double c, double s,
+                                            double (&acc)[3]) {
+  const double sh = 0.5 * double(N0 - 1), h1 = 0.5 * double(N1 - 1);
+  const double Pcx = double(px) - sh, Pcy = double(py) - sh;
+  const double xf = Pcx * c + Pcy * s + h1, yf = -Pcx * s + Pcy * c + h1;
+  const bool edge = px <= 1 || py <= 1;
+  const int lo = edge ? 3 : 1, hi = edge ? 3 : 2;
+  const int x0 = max(int(floor(xf)) - lo, 0), x1 = min(int(floor(xf)) + hi, N1 - 1);
+  const int y0 = max(int(floor(yf)) - lo, 0), y1 = min(int(floor(yf)) + hi, N1 - 1);
+  for (int cy = y0; cy <= y1; ++cy)
+    for (int cx = x0; cx <= x1; ++cx) {
+      const size_t fi = size_t(cy) * N1 + cx;
+      double val[3] = {0.0, 0.0, 0.0};
+      bool any = false;
+      for (int q = 0; q < nPlanes; ++q) {
+        val[q] = planes[q][fi];
+        any = any || val[q] != 0.0;
+      }
+      if (!any) continue;
+      const RotSample r = rotatedCell(cx, cy, N1, N0, c, s);
+      if (!r.valid) continue;
+      const int Yi = r.idx / N0, Xi = r.idx - Yi * N0;
+      const int dx = px - Xi, dy = py - Yi;
+      if (dx < 0 || dx > 1 || dy < 0 || dy > 1) continue;
+      const double w = (dy ? r.fy : 1.0 - r.fy) * (dx ? r.fx : 1.0 - r.fx);
+      for (int q = 0; q < nPlanes; ++q) acc[q] += val[q] * w;
+    }
+}
+
+// The frames are split into kFieldChunks ranges (blockIdx.y), each thread
+// summing its range into its own slot of part ([chunk][plane][site], planes
+// Re/Im of every component, then A_x and A_y): many more threads to hide
+// the loads' latency, and fieldsAdjointSumKernel adds the ranges in order.
+constexpr int kFieldChunks = 8;
 __global__ void fieldsAdjointKernel(GrainView v, const double *dbar, long long framePlane,
-                                    double *extBar, double *aBar) {
-  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (i >= framePlane) return;
-  const double br = dbar[i], bi = dbar[framePlane + i];
-  const double be = aBar ? dbar[2 * framePlane + i] : 0.0;
-  if (br == 0.0 && bi == 0.0 && be == 0.0) return;
-  const int N0 = v.N0, N1 = v.N1;
-  const long long n1 = (long long)N1 * N1;
-  const int k = int(i / n1);
-  const int site = int(i - (long long)k * n1);
-  const int x = site % N1, y = site / N1;
-  const size_t n0 = size_t(N0) * N0;
-  if (be != 0.0) {
-    const RotSample q = rotatedCell(x, y, N1, N0, double(v.aCos[k]), double(v.aSin[k]));
-    if (q.valid) {
-      const double w00 = (1.0 - q.fy) * (1.0 - q.fx), w10 = (1.0 - q.fy) * q.fx;
-      const double w01 = q.fy * (1.0 - q.fx), w11 = q.fy * q.fx;
-      for (int f = 0; f < 2; ++f) {
-        const double c = be * double(v.charge) * (f == 0 ? v.vFx[k] : v.vFy[k]);
-        double *pa = aBar + size_t(f) * n0 + q.idx;
-        atomicAdd(pa, c * w00);
-        atomicAdd(pa + 1, c * w10);
-        atomicAdd(pa + N0, c * w01);
-        atomicAdd(pa + N0 + 1, c * w11);
+                                    bool withA, double *part) {
+  const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const int N0 = v.N0, N1 = v.N1, Na = v.Na, Nc = v.Nc;
+  const size_t n0 = size_t(N0) * N0, n1 = size_t(N1) * N1;
+  if (p >= (long long)n0) return;
+  const int px = int(p % N0), py = int(p / N0);
+  const int chunk = blockIdx.y;
+  const int k0 = chunk * Na / kFieldChunks, k1 = (chunk + 1) * Na / kFieldChunks;
+  double dr[4] = {0, 0, 0, 0}, di[4] = {0, 0, 0, 0};
+  double ax = 0.0, ay = 0.0;
+  for (int k = k0; k < k1; ++k) {
+    const double *pl[2] = {dbar + size_t(k) * n1, dbar + framePlane + size_t(k) * n1};
+    double acc[3] = {0.0, 0.0, 0.0};
+    gatherFrame(pl, 2, px, py, N0, N1, double(v.opCos[k]), double(v.opSin[k]), acc);
+    if (acc[0] != 0.0 || acc[1] != 0.0) {
+      for (int c = 0; c < Nc && c < 4; ++c) {
+        const double b = v.basis[c * Na + k];
+        dr[c] += b * acc[0];
+        di[c] += b * acc[1];
       }
     }
+    if (withA) {
+      const double *pe[1] = {dbar + 2 * framePlane + size_t(k) * n1};
+      double ae[3] = {0.0, 0.0, 0.0};
+      gatherFrame(pe, 1, px, py, N0, N1, double(v.aCos[k]), double(v.aSin[k]), ae);
+      ax += double(v.charge) * v.vFx[k] * ae[0];
+      ay += double(v.charge) * v.vFy[k] * ae[0];
+    }
   }
-  if (br == 0.0 && bi == 0.0) return;
-  const RotSample r = rotatedCell(x, y, N1, N0, double(v.opCos[k]), double(v.opSin[k]));
-  if (!r.valid) return;
-  const double w00 = (1.0 - r.fy) * (1.0 - r.fx), w10 = (1.0 - r.fy) * r.fx;
-  const double w01 = r.fy * (1.0 - r.fx), w11 = r.fy * r.fx;
-  for (int c = 0; c < v.Nc; ++c) {
-    const double b = v.basis[c * v.Na + k];
-    double *pr = extBar + size_t(c) * 2 * n0 + r.idx;
-    double *pi = pr + n0;
-    const double vr = b * br, vi = b * bi;
-    atomicAdd(pr, vr * w00);
-    atomicAdd(pr + 1, vr * w10);
-    atomicAdd(pr + N0, vr * w01);
-    atomicAdd(pr + N0 + 1, vr * w11);
-    atomicAdd(pi, vi * w00);
-    atomicAdd(pi + 1, vi * w10);
-    atomicAdd(pi + N0, vi * w01);
-    atomicAdd(pi + N0 + 1, vi * w11);
+  const int planes = 2 * Nc + 2;
+  double *out = part + size_t(chunk) * planes * n0 + p;
+  for (int c = 0; c < Nc && c < 4; ++c) {
+    out[size_t(2 * c) * n0] = dr[c];
+    out[size_t(2 * c + 1) * n0] = di[c];
   }
+  out[size_t(2 * Nc) * n0] = ax;
+  out[size_t(2 * Nc + 1) * n0] = ay;
+}
+
+// The frame ranges added in order: extBar (Re/Im of every component, the
+// extension's transpose follows) and, with aBar, dF/dA's Doppler part.
+__global__ void fieldsAdjointSumKernel(const double *part, int Nc, long long n0, double *extBar,
+                                       double *aBar) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const int planes = 2 * Nc + 2;
+  if (i >= planes * n0) return;
+  const int q = int(i / n0);
+  if (q >= 2 * Nc && !aBar) return;
+  double sum = 0.0;
+  for (int ch = 0; ch < kFieldChunks; ++ch) sum += part[size_t(ch) * planes * n0 + i];
+  if (q < 2 * Nc)
+    extBar[i] = sum;
+  else
+    aBar[i - 2 * Nc * n0] = sum;
 }
 
 // 3c. The magnetic term's gradient: with b_t the curl of the stored
@@ -1190,6 +1240,7 @@ void Grain::prepareAdjoint() {
     dAdjExt_ = devAlloc<double>(size_t(2 * s.Nc) * n0);
     dAdjGrad_ = devAlloc<double>(size_t(2 * s.Nc) * n0);
     dAdjGradA_ = devAlloc<double>(2 * n0);
+    dAdjPart_ = devAlloc<double>(size_t(kFieldChunks) * size_t(2 * s.Nc + 2) * n0);
   }
   // the slots' scratch for the setting of the checkpoints: every site
   // (two buffers of maxLen sites), or segments of K sites (two buffers of
@@ -1278,8 +1329,12 @@ double Grain::freeEnergyGradient(double *grad, double *gradA) {
     SC_HIP(hipGetLastError());
   }
   // 3. the rotation, basis and extension of the order parameter
-  fieldsAdjointKernel<<<blocksFor(framePlane), 256, 0, stream_>>>(
-      v, dAdjDbar_, framePlane, dAdjExt_, withA ? dAdjGradA_ : nullptr);
+  fieldsAdjointKernel<<<dim3(blocksFor(static_cast<long long>(n0)), kFieldChunks), 256, 0,
+                        stream_>>>(v, dAdjDbar_, framePlane, withA, dAdjPart_);
+  SC_HIP(hipGetLastError());
+  fieldsAdjointSumKernel<<<blocksFor(static_cast<long long>(2 * s.Nc + 2) * n0), 256, 0,
+                           stream_>>>(dAdjPart_, s.Nc, static_cast<long long>(n0), dAdjExt_,
+                                      withA ? dAdjGradA_ : nullptr);
   SC_HIP(hipGetLastError());
   if (withA) {
     curlAdjointKernel<<<blocksFor(n0), 256, 0, stream_>>>(dA_, s.N0, 1.0 / s.h, s.h * s.h,
@@ -1435,8 +1490,12 @@ double Grain::residualGradient(double *grad, double *gradA) {
   }
   // 5. the transposes of the site fields and of the order parameter's
   // extension, then the identity part
-  fieldsAdjointKernel<<<blocksFor(framePlane), 256, 0, stream_>>>(
-      v, dAdjDbar_, framePlane, dAdjExt_, withA ? dAdjGradA_ : nullptr);
+  fieldsAdjointKernel<<<dim3(blocksFor(static_cast<long long>(n0)), kFieldChunks), 256, 0,
+                        stream_>>>(v, dAdjDbar_, framePlane, withA, dAdjPart_);
+  SC_HIP(hipGetLastError());
+  fieldsAdjointSumKernel<<<blocksFor(static_cast<long long>(2 * s.Nc + 2) * n0), 256, 0,
+                           stream_>>>(dAdjPart_, s.Nc, static_cast<long long>(n0), dAdjExt_,
+                                      withA ? dAdjGradA_ : nullptr);
   SC_HIP(hipGetLastError());
   const int ne = static_cast<int>(s.extRef.target.size());
   if (ne > 0) {

@@ -111,6 +111,7 @@ Grain::~Grain() {
   devFree(dAdjExt_);
   devFree(dAdjGrad_);
   devFree(dAdjGradA_);
+  devFree(dAdjPart_);
   for (double *p : {dResD_, dResA_, dResPois_, dResSbar_, dResJbar_, dResZero_, dResMean_,
                     dResSeed_, dResSave_, dTanExt_, dTanFields_, dTanG_, dTanJ_, dTanE_,
                     dTanOutG_, dTanOutGt_, dTanV_, dTanJV_, dTanScratch_})

@@ -395,7 +395,7 @@ private:
   int adjNumItems_ = 0, adjMaxLen_ = 0, adjSlots_ = 0;
   double *dAdjScratch_ = nullptr, *dAdjPartial_ = nullptr, *dAdjRho_ = nullptr,
          *dAdjDbar_ = nullptr, *dAdjExt_ = nullptr, *dAdjGrad_ = nullptr,
-         *dAdjGradA_ = nullptr;
+         *dAdjGradA_ = nullptr, *dAdjPart_ = nullptr;
   // the residual gradient's buffers: rD, rA, the Poisson seed and its
   // transpose, jbar, two zero means, two means, the frames' Green-sum seeds,
   // the saved state
