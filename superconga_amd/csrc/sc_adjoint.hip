@@ -1272,11 +1272,6 @@ void Grain::setAdjointCheckpoints(int k) {
   adjCk_ = k;
 }
 
-size_t Grain::adjointScratchBytes() {
-  prepareAdjoint();
-  return size_t(adjSlots_) * 4 * size_t(adjSlotStride_) * sizeof(double);
-}
-
 double Grain::freeEnergyGradient(double *grad, double *gradA) {
   const Setup &s = *s_;
   if (s.keldysh) throw std::runtime_error("a spectroscopy context has no free-energy gradient");

@@ -209,7 +209,6 @@ public:
   // forcing term (the linear residual a Newton step asks for, relative to
   // the nonlinear one; default 0.3).
   void setNewtonOptions(int restarts, double etaMax);
-  size_t adjointScratchBytes();
   void resetAccelerator();
 
   // Keldysh spectroscopy (src/postprocess.cu), for a grain built by
