@@ -1284,7 +1284,10 @@ double Grain::freeEnergyGradient(double *grad, double *gradA) {
   prepareAdjoint();
   // A enters through the Doppler shift only with a penetration depth
   const bool withA = gradA != nullptr && s.lambda > 0.0;
-  SC_HIP(hipMemsetAsync(dAdjRho_, 0, size_t(framePlane) * sizeof(double), stream_));
+  // (the seed of the free-energy density depends on the grain's geometry,
+  // temperature and angle weights only: formed on the first call, kept)
+  if (!adjRhoReady_)
+    SC_HIP(hipMemsetAsync(dAdjRho_, 0, size_t(framePlane) * sizeof(double), stream_));
   SC_HIP(hipMemsetAsync(dAdjDbar_, 0, 3 * size_t(framePlane) * sizeof(double), stream_));
   if (withA) SC_HIP(hipMemsetAsync(dAdjGradA_, 0, 2 * n0 * sizeof(double), stream_));
   SC_HIP(hipMemsetAsync(dAdjExt_, 0, size_t(2 * s.Nc) * n0 * sizeof(double), stream_));
@@ -1292,16 +1295,19 @@ double Grain::freeEnergyGradient(double *grad, double *gradA) {
   const double area = s.h * s.h;
   // 1. the adjoint seed of the free-energy density per frame site
   const int ns = static_cast<int>(s.domainSites.size());
-  if (ns > 0) {
-    feSeedKernel<<<blocksFor(ns), 256, 0, stream_>>>(v, dDomainSites_, ns, area * s.temperature,
-                                                      dAdjRho_);
-    SC_HIP(hipGetLastError());
-  }
-  const int nr = static_cast<int>(s.extRot.target.size());
-  if (nr > 0) {
-    extendAdjointKernel<<<blocksFor(nr), 256, 0, stream_>>>(dAdjRho_, dExtRotT_, dExtRotO_,
-                                                            dExtRotS_, nr, 1, 0);
-    SC_HIP(hipGetLastError());
+  if (!adjRhoReady_) {
+    if (ns > 0) {
+      feSeedKernel<<<blocksFor(ns), 256, 0, stream_>>>(v, dDomainSites_, ns,
+                                                        area * s.temperature, dAdjRho_);
+      SC_HIP(hipGetLastError());
+    }
+    const int nr = static_cast<int>(s.extRot.target.size());
+    if (nr > 0) {
+      extendAdjointKernel<<<blocksFor(nr), 256, 0, stream_>>>(dAdjRho_, dExtRotT_, dExtRotO_,
+                                                              dExtRotS_, nr, 1, 0);
+      SC_HIP(hipGetLastError());
+    }
+    adjRhoReady_ = true;
   }
   // 2. the reverse sweep
   AdjointArgs a;

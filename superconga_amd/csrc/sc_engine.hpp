@@ -403,6 +403,7 @@ private:
          *dResSave_ = nullptr;
   void prepareAdjoint();
   int adjCk_ = -1, adjCkUsed_ = 0, adjCkSeg_ = 0;
+  bool adjRhoReady_ = false; // dAdjRho_ holds the free-energy seed
   int nkRestarts_ = 4;
   double nkEtaMax_ = 0.3;
   long long adjSlotStride_ = 0;
