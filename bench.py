@@ -153,10 +153,9 @@ def converge_newton(sa, cfg, cap, warm_residual=1e-3, warm_max=30, jvp="fp64",
                     workloads=None, name=""):
     """Wall time to the config's criterion with Newton-Krylov: FP32-stepping
     iterations until the residual is below warm_residual (at most warm_max),
-    Jacobian-free Newton-Krylov on the FP64 map (with FP32 products, when
-    those stop short of the tolerance, a few Newton steps on FP64 products
-    after them), then FP64 iterations until the reference's convergence test
-    passes."""
+    Jacobian-free Newton-Krylov on the FP64 map (FP32 products switch to
+    FP64 ones when a Newton step stalls), then FP64 iterations until the
+    reference's convergence test passes."""
     sim = start(sa, workloads, cfg, name)
     sim.autotune("fp64")
     sim.autotune("mixed")
@@ -167,11 +166,6 @@ def converge_newton(sa, cfg, cap, warm_residual=1e-3, warm_max=30, jvp="fp64",
         if sim.results()["residual"] < warm_residual:
             break
     rep = sim.newton_krylov(max_newton=20, krylov_dim=30, tol=1e-11, jvp_precision=jvp)
-    refine = None
-    if jvp == "mixed" and not rep["converged"]:
-        # the FP32 products' Newton steps refined by Newton steps on FP64
-        # products, as FP32 stepping is by FP64 iterations
-        refine = sim.newton_krylov(max_newton=6, krylov_dim=30, tol=1e-11, jvp_precision="fp64")
     ok = sim.run(0, 0, cap, "fp64")
     wall = time.perf_counter() - t
     r = sim.results()
@@ -182,9 +176,6 @@ def converge_newton(sa, cfg, cap, warm_residual=1e-3, warm_max=30, jvp="fp64",
            "map_evaluations": rep["map_evaluations"], "newton_residual": rep["residual"],
            "fp64_iterations_after": r["iterations"] - warm, "wall_s": round(wall, 4),
            "residual": r["residual"], "free_energy_per_area": r["free_energy_per_area"]}
-    if refine is not None:
-        out["fp64_refinement"] = {k: refine[k] for k in (
-            "newton_iterations", "krylov_iterations", "map_evaluations", "residual")}
     return out
 
 

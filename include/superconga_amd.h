@@ -192,7 +192,9 @@ int sc_newton_krylov(sc_context *ctx, int max_newton, int krylov_dim, double tol
  * SC_PRECISION_FP64 (as sc_newton_krylov), SC_PRECISION_MIXED (both ends
  * of each finite difference on the map with FP32 stepping and FP64 sums, at
  * a step of 3e-4 (1 + |u|) in the weighted norm; the Newton residuals stay
- * FP64, so the solution is the FP64 fixed point) or SC_JVP_TANGENT (exact
+ * FP64, so the solution is the FP64 fixed point; a Newton step that gains
+ * less than a tenth, or finds no descent, hands the rest of the solve to
+ * FP64 products) or SC_JVP_TANGENT (exact
  * FP64 products J v by the tangent-linear sweep, as sc_jacobian_vector
  * forms them, with the boundary condition's and the screened update's
  * tangents).  krylov_dim 1..63. */
