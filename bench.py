@@ -403,6 +403,11 @@ def batch_convergence(sa, workloads, args):
         left = [i for i, c in enumerate(conv0) if not c]
         reps = []
         if left:
+            # FP64 iterations follow: a grain whose FP32 products find no
+            # descent leaves the Newton solve to them rather than going on
+            # with FP64 products
+            for i in left:
+                b.grain(i).set_newton_options(fp64_fallback=False)
             reps = b.newton_krylov(max_newton=12, krylov_dim=20, tol=1e-10,
                                    jvp_precision="mixed", grains=left)
         conv = b.run(0, 0, 30, "fp64")

@@ -192,9 +192,9 @@ int sc_newton_krylov(sc_context *ctx, int max_newton, int krylov_dim, double tol
  * SC_PRECISION_FP64 (as sc_newton_krylov), SC_PRECISION_MIXED (both ends
  * of each finite difference on the map with FP32 stepping and FP64 sums, at
  * a step of 3e-4 (1 + |u|) in the weighted norm; the Newton residuals stay
- * FP64, so the solution is the FP64 fixed point; a Newton step that gains
- * less than a tenth, or finds no descent, hands the rest of the solve to
- * FP64 products) or SC_JVP_TANGENT (exact
+ * FP64, so the solution is the FP64 fixed point; a Newton step whose line
+ * search finds no descent hands the rest of the solve to FP64 products,
+ * see sc_set_newton_options) or SC_JVP_TANGENT (exact
  * FP64 products J v by the tangent-linear sweep, as sc_jacobian_vector
  * forms them, with the boundary condition's and the screened update's
  * tangents).  krylov_dim 1..63. */
@@ -261,13 +261,15 @@ int sc_jacobian_vector(sc_context *ctx, const double *v, double *jv);
  * counterpart in the reference. */
 int sc_set_adjoint_checkpoints(sc_context *ctx, int k);
 
-/* Newton-Krylov's linear solves on this grain (sc_newton_krylov(_ex), and
+/* Newton-Krylov on this grain (sc_newton_krylov(_ex), and
  * sc_batch_newton_krylov for the batch's grains): GMRES cycles per Newton
- * step (restarts >= 1, default 4) and the cap of the Eisenstat-Walker
- * forcing term (0 < eta_max < 1, default 0.3), the linear residual a Newton
- * step asks for relative to the nonlinear one.  No counterpart in the
- * reference. */
-int sc_set_newton_options(sc_context *ctx, int restarts, double eta_max);
+ * step (restarts >= 1, default 4), the cap of the Eisenstat-Walker forcing
+ * term (0 < eta_max < 1, default 0.3), the linear residual a Newton step
+ * asks for relative to the nonlinear one, and with SC_PRECISION_MIXED
+ * products whether a line search without descent hands the solve over to
+ * FP64 products (fp64_fallback != 0, the default) or ends it.  No
+ * counterpart in the reference. */
+int sc_set_newton_options(sc_context *ctx, int restarts, double eta_max, int fp64_fallback);
 
 /* Individual stages. */
 /* RiccatiSolver::computeCoherenceFunctions for every angle (RiccatiSolverConfined.h:513-671)

@@ -161,7 +161,7 @@ def lib():
         L.sc_residual_gradient.argtypes = [vp, _DP, _DP, _DP]
         L.sc_jacobian_vector.argtypes = [vp, _DP, _DP]
         L.sc_set_adjoint_checkpoints.argtypes = [vp, ctypes.c_int]
-        L.sc_set_newton_options.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+        L.sc_set_newton_options.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int]
         L.sc_ldos_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(vp)]
         L.sc_ldos_compute.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.sc_ldos_run.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -379,10 +379,12 @@ class Simulation:
         g = g.reshape(self.Nc, 2, n, n)
         return float(r[0]), g[:, 0] + 1j * g[:, 1], ga.reshape(2, n, n)
 
-    def set_newton_options(self, restarts=4, eta_max=0.3):
-        """Newton-Krylov's GMRES cycles per Newton step and the cap of its
-        Eisenstat-Walker forcing term."""
-        _check(lib().sc_set_newton_options(self._h, int(restarts), ctypes.c_double(eta_max)))
+    def set_newton_options(self, restarts=4, eta_max=0.3, fp64_fallback=True):
+        """Newton-Krylov's GMRES cycles per Newton step, the cap of its
+        Eisenstat-Walker forcing term, and whether FP32 products hand over
+        to FP64 ones when a line search finds no descent."""
+        _check(lib().sc_set_newton_options(self._h, int(restarts), ctypes.c_double(eta_max),
+                                           int(bool(fp64_fallback))))
 
     def set_adjoint_checkpoints(self, k):
         """The reverse sweeps' storage: k > 0 a checkpoint every k sites

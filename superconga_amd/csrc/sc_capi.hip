@@ -472,11 +472,11 @@ int sc_free_energy_gradient(sc_context *ctx, double *grad, double *freeEnergy) {
   return sc_free_energy_gradient_ex(ctx, grad, nullptr, freeEnergy);
 }
 
-int sc_set_newton_options(sc_context *ctx, int restarts, double etaMax) {
+int sc_set_newton_options(sc_context *ctx, int restarts, double etaMax, int fp64Fallback) {
   SC_CHECK_CTX(ctx);
   SC_CHECK_SIM(ctx);
   return guarded([&] {
-    ctx->grain->setNewtonOptions(restarts, etaMax);
+    ctx->grain->setNewtonOptions(restarts, etaMax, fp64Fallback != 0);
     return SC_OK;
   });
 }

@@ -205,10 +205,12 @@ public:
   // k = ceil(sqrt(longest trajectory)).  Results are the same either way.
   void setAdjointCheckpoints(int k);
   // Newton-Krylov's GMRES cycles per Newton step (each of at most
-  // krylovDim directions; default 4) and the cap of its Eisenstat-Walker
+  // krylovDim directions; default 4), the cap of its Eisenstat-Walker
   // forcing term (the linear residual a Newton step asks for, relative to
-  // the nonlinear one; default 0.3).
-  void setNewtonOptions(int restarts, double etaMax);
+  // the nonlinear one; default 0.3), and whether FP32 products hand over to
+  // FP64 ones when a line search finds no descent (default) or the solve
+  // ends there.
+  void setNewtonOptions(int restarts, double etaMax, bool fallback = true);
   void resetAccelerator();
 
   // Keldysh spectroscopy (src/postprocess.cu), for a grain built by
@@ -406,6 +408,7 @@ private:
   bool adjRhoReady_ = false; // dAdjRho_ holds the free-energy seed
   int nkRestarts_ = 4;
   double nkEtaMax_ = 0.3;
+  bool nkFallback_ = true;
   long long adjSlotStride_ = 0;
   double *dTanScratch_ = nullptr; // the tangent sweep's own (every site kept)
   // the tangent-linear products' buffers

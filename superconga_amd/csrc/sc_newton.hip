@@ -402,18 +402,16 @@ void NewtonKrylov::lineDone() {
     ++rep_.newtonIterations;
     // Eisenstat-Walker forcing term (choice 2)
     eta_ = std::clamp(0.9 * (fn_ / fnPrev) * (fn_ / fnPrev), 1e-4, g_.nkEtaMax_);
-    // FP32 products whose Newton step gained less than a tenth: their
-    // rounding limits the direction, so the steps continue on FP64 products
-    if (jvp_ == PREC_MIXED && fn_ > 0.9 * fnPrev) jvp_ = PREC_FP64;
     if (fn_ < tol_ || rep_.newtonIterations >= maxNewton_) phase_ = DONE;
     else startNewton();
     return;
   }
   lambda_ *= 0.5;
   if (++ls_ == 6) {
-    // no descent along the Newton direction: with FP32 products, the step
-    // again on FP64 products; else the end
-    if (jvp_ == PREC_MIXED && rep_.newtonIterations < maxNewton_) {
+    // no descent along the Newton direction: with FP32 products their
+    // rounding is the likely limit, so the step again on FP64 products;
+    // else the end
+    if (jvp_ == PREC_MIXED && g_.nkFallback_ && rep_.newtonIterations < maxNewton_) {
       jvp_ = PREC_FP64;
       startNewton();
     } else {

@@ -103,10 +103,11 @@ def test_newton_options_keep_the_fixed_point(product):
     # cycles with a loose one: the same fixed point
     cfg = cases.accelerator_case("picard")
     ref = converged_plain(product, cfg)
-    for restarts, eta in ((1, 0.01), (12, 0.5)):
+    for restarts, eta, fallback, jvp in ((1, 0.01, True, "fp64"), (12, 0.5, True, "fp64"),
+                                         (4, 0.3, False, "mixed")):
         s = product.Simulation(cfg)
-        s.set_newton_options(restarts, eta)
-        rep = s.newton_krylov(max_newton=40, krylov_dim=20, tol=1e-10)
+        s.set_newton_options(restarts, eta, fallback)
+        rep = s.newton_krylov(max_newton=40, krylov_dim=20, tol=1e-10, jvp_precision=jvp)
         print("restarts %d, eta_max %g:" % (restarts, eta),
               {k: v for k, v in rep.items() if k != "history"})
         assert rep["converged"], rep
