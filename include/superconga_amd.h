@@ -183,9 +183,12 @@ typedef struct {
  * finite-difference products, in the inner product weighted per block by
  * 1 / |u_block|^2,
  * followed by a backtracking line search; the forcing term follows
- * Eisenstat-Walker.  On return the state holds the last accepted u and the
- * accelerator restarts; a following sc_compute reports the reference's
- * residuals for it. */
+ * Eisenstat-Walker.  The solve ends at tol, at max_newton steps, when the
+ * line search finds no descent, or after two accepted steps in a row that
+ * each keep more than 0.9 of the residual (the rounding floor of the map:
+ * converged then stays 0).  On return the state holds the last accepted u
+ * and the accelerator restarts; a following sc_compute reports the
+ * reference's residuals for it. */
 int sc_newton_krylov(sc_context *ctx, int max_newton, int krylov_dim, double tol,
                      sc_newton_report *report);
 /* sc_newton_krylov with the precision of the Jacobian-vector products:

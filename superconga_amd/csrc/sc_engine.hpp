@@ -531,7 +531,7 @@ private:
   int prec_ = PREC_FP64;
   NewtonReport rep_;
   double fn_ = 0, eta_ = 0.1, e_ = 0, rn_ = 0, beta_ = 0, lambda_ = 1;
-  int cycle_ = 0, k_ = 0, ls_ = 0;
+  int cycle_ = 0, k_ = 0, ls_ = 0, stalled_ = 0;
   bool breakdown_ = false;
   std::vector<double> H_, R_, gv_, cs_, sn_;
 };
