@@ -270,9 +270,13 @@ int sc_set_adjoint_checkpoints(sc_context *ctx, int k);
  * term (0 < eta_max < 1, default 0.3), the linear residual a Newton step
  * asks for relative to the nonlinear one, and with SC_PRECISION_MIXED
  * products whether a line search without descent hands the solve over to
- * FP64 products (fp64_fallback != 0, the default) or ends it.  No
- * counterpart in the reference. */
-int sc_set_newton_options(sc_context *ctx, int restarts, double eta_max, int fp64_fallback);
+ * FP64 products (fp64_fallback != 0, the default) or ends it, and whether
+ * each Newton step loses its component along the generator of the order
+ * parameter's global phase rotation (fix_phase != 0; default 0), a
+ * direction in which J - I is singular because every rotation of a fixed
+ * point is one.  No counterpart in the reference. */
+int sc_set_newton_options(sc_context *ctx, int restarts, double eta_max, int fp64_fallback,
+                          int fix_phase);
 
 /* Individual stages. */
 /* RiccatiSolver::computeCoherenceFunctions for every angle (RiccatiSolverConfined.h:513-671)

@@ -161,7 +161,8 @@ def lib():
         L.sc_residual_gradient.argtypes = [vp, _DP, _DP, _DP]
         L.sc_jacobian_vector.argtypes = [vp, _DP, _DP]
         L.sc_set_adjoint_checkpoints.argtypes = [vp, ctypes.c_int]
-        L.sc_set_newton_options.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+        L.sc_set_newton_options.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                            ctypes.c_int]
         L.sc_ldos_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(vp)]
         L.sc_ldos_compute.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.sc_ldos_run.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -379,12 +380,14 @@ class Simulation:
         g = g.reshape(self.Nc, 2, n, n)
         return float(r[0]), g[:, 0] + 1j * g[:, 1], ga.reshape(2, n, n)
 
-    def set_newton_options(self, restarts=4, eta_max=0.3, fp64_fallback=True):
+    def set_newton_options(self, restarts=4, eta_max=0.3, fp64_fallback=True, fix_phase=False):
         """Newton-Krylov's GMRES cycles per Newton step, the cap of its
-        Eisenstat-Walker forcing term, and whether FP32 products hand over
-        to FP64 ones when a line search finds no descent."""
+        Eisenstat-Walker forcing term, whether FP32 products hand over to
+        FP64 ones when a line search finds no descent, and whether each
+        Newton step drops its component along the order parameter's global
+        phase rotation (a symmetry, so a singular direction of J - I)."""
         _check(lib().sc_set_newton_options(self._h, int(restarts), ctypes.c_double(eta_max),
-                                           int(bool(fp64_fallback))))
+                                           int(bool(fp64_fallback)), int(bool(fix_phase))))
 
     def set_adjoint_checkpoints(self, k):
         """The reverse sweeps' storage: k > 0 a checkpoint every k sites

@@ -1260,12 +1260,13 @@ void Grain::prepareAdjoint() {
   }
 }
 
-void Grain::setNewtonOptions(int restarts, double etaMax, bool fallback) {
+void Grain::setNewtonOptions(int restarts, double etaMax, bool fallback, bool fixPhase) {
   if (restarts < 1) throw std::runtime_error("restarts must be >= 1");
   if (!(etaMax > 0.0 && etaMax < 1.0)) throw std::runtime_error("eta_max must be in (0, 1)");
   nkRestarts_ = restarts;
   nkEtaMax_ = etaMax;
   nkFallback_ = fallback;
+  nkFixPhase_ = fixPhase;
 }
 
 void Grain::setAdjointCheckpoints(int k) {

@@ -472,11 +472,12 @@ int sc_free_energy_gradient(sc_context *ctx, double *grad, double *freeEnergy) {
   return sc_free_energy_gradient_ex(ctx, grad, nullptr, freeEnergy);
 }
 
-int sc_set_newton_options(sc_context *ctx, int restarts, double etaMax, int fp64Fallback) {
+int sc_set_newton_options(sc_context *ctx, int restarts, double etaMax, int fp64Fallback,
+                          int fixPhase) {
   SC_CHECK_CTX(ctx);
   SC_CHECK_SIM(ctx);
   return guarded([&] {
-    ctx->grain->setNewtonOptions(restarts, etaMax, fp64Fallback != 0);
+    ctx->grain->setNewtonOptions(restarts, etaMax, fp64Fallback != 0, fixPhase != 0);
     return SC_OK;
   });
 }

@@ -210,7 +210,7 @@ public:
   // the nonlinear one; default 0.3), and whether FP32 products hand over to
   // FP64 ones when a line search finds no descent (default) or the solve
   // ends there.
-  void setNewtonOptions(int restarts, double etaMax, bool fallback = true);
+  void setNewtonOptions(int restarts, double etaMax, bool fallback = true, bool fixPhase = false);
   void resetAccelerator();
 
   // Keldysh spectroscopy (src/postprocess.cu), for a grain built by
@@ -347,6 +347,10 @@ private:
   // Newton-Krylov state
   struct NkBlock {
     size_t offset, length;
+    // the block's charge under the global phase rotation of the order
+    // parameter (planar complex values: +1 rotate with it, -1 against it,
+    // 0 invariant)
+    int phase;
   };
   std::vector<NkBlock> nkBlocks() const;
   size_t nkLength() const;
@@ -409,6 +413,7 @@ private:
   int nkRestarts_ = 4;
   double nkEtaMax_ = 0.3;
   bool nkFallback_ = true;
+  bool nkFixPhase_ = false;
   long long adjSlotStride_ = 0;
   double *dTanScratch_ = nullptr; // the tangent sweep's own (every site kept)
   // the tangent-linear products' buffers
@@ -515,6 +520,7 @@ private:
   void endCycle();
   void startLineSearch();
   void lineDone();
+  void removePhase();
 
   Grain &g_;
   int maxNewton_, m_;
@@ -523,7 +529,7 @@ private:
   std::vector<Grain::NkBlock> blocks_;
   std::vector<double> wt_;
   long long nL_ = 0;
-  double *u_, *gu_, *f_, *trial_, *w_, *dstep_, *rlin_, *guJ_;
+  double *u_, *gu_, *f_, *trial_, *w_, *dstep_, *rlin_, *guJ_, *gen_;
   std::vector<double *> V_;
   Phase phase_ = INIT;
   const double *x_ = nullptr;

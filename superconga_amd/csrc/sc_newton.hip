@@ -63,6 +63,15 @@ __global__ void combineKernel(double *y, const double *base, double a, Terms t, 
   y[i] = acc;
 }
 
+// The generator of the global phase rotation on one planar complex block
+// of length 2 m: d/dtheta of exp(i sign theta) z = i sign z.
+__global__ void phaseGeneratorKernel(double *n, const double *u, long long m, double sign) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  n[i] = -sign * u[m + i];
+  n[m + i] = sign * u[i];
+}
+
 unsigned blocks256(long long n) { return static_cast<unsigned>((n + 255) / 256); }
 
 } // namespace
@@ -73,18 +82,20 @@ std::vector<Grain::NkBlock> Grain::nkBlocks() const {
   const bool doA = s.cfg.computeCurrent && s.lambda > 0.0;
   std::vector<NkBlock> b;
   size_t off = 0;
+  // gamma rotates with the order parameter, gamma~ against it, the vector
+  // potential not at all
   for (int c = 0; c < s.Nc; ++c) {
-    b.push_back({off, 2 * n0});
+    b.push_back({off, 2 * n0, 1});
     off += 2 * n0;
   }
   if (doA) {
-    b.push_back({off, 2 * n0});
+    b.push_back({off, 2 * n0, 0});
     off += 2 * n0;
   }
   const size_t g = size_t(2 * s.nInG * s.Ne), gt = size_t(2 * s.nInGt * s.Ne);
-  if (g) b.push_back({off, g});
+  if (g) b.push_back({off, g, 1});
   off += g;
-  if (gt) b.push_back({off, gt});
+  if (gt) b.push_back({off, gt, -1});
   return b;
 }
 
@@ -176,8 +187,9 @@ NewtonKrylov::NewtonKrylov(Grain &g, int maxNewton, int krylovDim, double tol,
   const size_t L = g.nkLength();
   nL_ = static_cast<long long>(L);
   // workspace: u, G(u), F, trial, w, the step d, the linear residual r,
-  // G(u) in the products' precision, and the Krylov basis
-  const int nv = 8 + krylovDim + 1;
+  // G(u) in the products' precision, the phase generator, and the Krylov
+  // basis
+  const int nv = 9 + krylovDim + 1;
   if (g.nkVecs_.size() != size_t(nv) || g.nkLen_ != L) {
     for (double *p : g.nkVecs_) devFree(p);
     g.nkVecs_.clear();
@@ -192,7 +204,8 @@ NewtonKrylov::NewtonKrylov(Grain &g, int maxNewton, int krylovDim, double tol,
   dstep_ = g.nkVecs_[5];
   rlin_ = g.nkVecs_[6];
   guJ_ = g.nkVecs_[7];
-  V_.assign(g.nkVecs_.begin() + 8, g.nkVecs_.end());
+  gen_ = g.nkVecs_[8];
+  V_.assign(g.nkVecs_.begin() + 9, g.nkVecs_.end());
   g.nkMaps_ = 0;
   // block weights 1 / |u_b|^2 of the starting state (blocks at zero: 1 / size)
   g.ensureBoundaryInit();
@@ -388,8 +401,28 @@ void NewtonKrylov::endCycle() {
   startCycle();
 }
 
+void NewtonKrylov::removePhase() {
+  // The global phase of the order parameter is a symmetry of the map: every
+  // rotation of a fixed point is one, so J - I is singular along the
+  // rotation's generator n at u, and GMRES may put an arbitrary (and, in an
+  // inconsistent least-squares solve, large) multiple of n into d, which
+  // the line search then meets as a second-order growth of |F|.  The
+  // component of d along n (in the weighted inner product) is removed.
+  combine(gen_, nullptr, 0.0, {}, {});
+  for (const Grain::NkBlock &b : blocks_) {
+    if (b.phase == 0) continue;
+    const long long m = static_cast<long long>(b.length / 2);
+    phaseGeneratorKernel<<<blocks256(m), 256, 0, g_.stream_>>>(gen_ + b.offset, u_ + b.offset,
+                                                               m, double(b.phase));
+    SC_NK_HIP(hipGetLastError());
+  }
+  const std::vector<double> d = wdots(gen_, {dstep_, gen_});
+  if (d[1] > 0.0) combine(dstep_, dstep_, 1.0, {gen_}, {-d[0] / d[1]});
+}
+
 void NewtonKrylov::startLineSearch() {
   // backtracking line search on |F| along d
+  if (g_.nkFixPhase_) removePhase();
   lambda_ = 1.0;
   ls_ = 0;
   combine(trial_, u_, 1.0, {dstep_}, {lambda_});

@@ -183,7 +183,7 @@ def test_spectroscopy_config_errors(product):
     assert L.sc_residual_gradient(None, None, None, None) == 1
     assert L.sc_jacobian_vector(None, None, None) == 1
     assert L.sc_set_adjoint_checkpoints(None, 0) == 1
-    assert L.sc_set_newton_options(None, 4, ctypes.c_double(0.3), 1) == 1
+    assert L.sc_set_newton_options(None, 4, ctypes.c_double(0.3), 1, 0) == 1
 
 
 def test_noalloc_poisson_refuses_oversized_grid(product):

@@ -103,10 +103,13 @@ def test_newton_options_keep_the_fixed_point(product):
     # cycles with a loose one: the same fixed point
     cfg = cases.accelerator_case("picard")
     ref = converged_plain(product, cfg)
-    for restarts, eta, fallback, jvp in ((1, 0.01, True, "fp64"), (12, 0.5, True, "fp64"),
-                                         (4, 0.3, False, "mixed")):
+    for restarts, eta, fallback, jvp, fix in ((1, 0.01, True, "fp64", False),
+                                              (12, 0.5, True, "fp64", False),
+                                              (4, 0.3, False, "mixed", False),
+                                              (2, 0.3, True, "fp64", True),
+                                              (2, 0.3, True, "mixed", True)):
         s = product.Simulation(cfg)
-        s.set_newton_options(restarts, eta, fallback)
+        s.set_newton_options(restarts, eta, fallback, fix)
         rep = s.newton_krylov(max_newton=40, krylov_dim=20, tol=1e-10, jvp_precision=jvp)
         print("restarts %d, eta_max %g:" % (restarts, eta),
               {k: v for k, v in rep.items() if k != "history"})
