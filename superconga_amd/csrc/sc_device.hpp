@@ -29,6 +29,16 @@ template <typename T> __device__ __forceinline__ Cx<T> scale(Cx<T> a, T s) {
   return {a.re * s, a.im * s};
 }
 
+// Device pointers that reach a kernel inside a struct are generic: accesses
+// through them compile to flat_* instructions, which also count against the
+// LDS/scalar wait counter, so each wait for a lane shuffle or a scalar load
+// would also wait for them.  Every buffer the kernels address lives in global
+// memory; through global() the accesses are global_* instructions.
+template <typename T> using GlobalPtr = __attribute__((address_space(1))) T *;
+template <typename T> __device__ __forceinline__ GlobalPtr<T> global(T *p) {
+  return (GlobalPtr<T>)p;
+}
+
 // Streaming access to the sweep scratch: every element is written once and
 // read once, so the accesses carry the non-temporal hint and leave the cache
 // to the grids the sweep samples.

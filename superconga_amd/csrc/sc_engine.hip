@@ -888,7 +888,7 @@ void Grain::ensureBoundaryInit() {
 
 template <typename T, int EL, int W, bool KEL>
 static void launchSweep(const SweepArgs &a, int slots, hipStream_t st) {
-  sweepKernel<T, EL, W, KEL><<<slots, 64, 0, st>>>(a);
+  sweepKernel<T, EL, W, KEL><<<slots, 64, 0, st>>>(a, a.views, a.items);
 }
 
 void launchSweepVariant(int precision, int el, int waves, const SweepArgs &a, int slots,

@@ -127,9 +127,8 @@ __device__ __forceinline__ Cx<T> stepE(const StepSite<T> &p, T ser, T sej, Cx<T>
 // column: the lanes of a column take consecutive rows of a chunk and pass
 // the values round with __shfl.
 template <typename T, int EL, bool KEL>
-__device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev &it,
-                                          Cx<T> *scr) {
-  const GrainView &v = a.views[it.grain];
+__device__ __forceinline__ void sweepItem(const SweepArgs &a, const GrainView &v,
+                                          const SweepItemDev &it, Cx<T> *scr) {
   const int lane = threadIdx.x;
   const int cols = v.cols;
   const int gp = v.groups; // lanes per column
@@ -142,23 +141,23 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev
   const bool colActive = c < it.ncols;
   const int x = it.x0 + (colActive ? c : it.ncols - 1);
   const bool active = colActive && (g < v.groups);
-  const int8_t *lab = v.labels + size_t(k) * n1 + x;
+  const GlobalPtr<const int8_t> lab = global(v.labels + size_t(k) * n1 + x);
   T ser[EL], sej[EL], res[EL];
   bool eOk[EL];
 #pragma unroll
   for (int j = 0; j < EL; ++j) {
     const int e = g * EL + j;
     eOk[j] = active && e < Ne;
-    ser[j] = eOk[j] ? -h * T(v.energy[e]) : T(0);
-    sej[j] = (KEL && eOk[j]) ? h * T(v.energyRe[e]) : T(0);
-    res[j] = eOk[j] ? T(v.residue[e]) : T(0);
+    ser[j] = eOk[j] ? -h * T(global(v.energy)[e]) : T(0);
+    sej[j] = (KEL && eOk[j]) ? h * T(global(v.energyRe)[e]) : T(0);
+    res[j] = eOk[j] ? T(global(v.residue)[e]) : T(0);
   }
   const int len = it.yhi - it.ylo + 1;
   const size_t col = size_t(k) * N1 + x;
 
   // ---------------- pass 1: gamma~ along -y ----------------
   {
-    const int baseIn = v.idxInGt[col], baseOut = v.idxOutGt[col];
+    const int baseIn = global(v.idxInGt)[col], baseOut = global(v.idxOutGt)[col];
     int cntIn = 0, cntOut = 0;
     Cx<T> coh[EL];
 #pragma unroll
@@ -176,9 +175,9 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev
       if (colActive && g < gp && tt < len) {
         const int y = it.yhi - tt;
         const size_t fi = size_t(k - v.fieldBase) * n1 + size_t(y) * N1 + x;
-        nre = v.fields[fi];
-        nim = v.fields[v.fieldPlane + fi];
-        nes = v.fields[2 * v.fieldPlane + fi];
+        nre = global(v.fields)[fi];
+        nim = global(v.fields)[v.fieldPlane + fi];
+        nes = global(v.fields)[2 * v.fieldPlane + fi];
         nlab = lab[size_t(y) * N1];
       } else {
         nre = nim = nes = 0.0;
@@ -211,27 +210,28 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev
             for (int j = 0; j < EL; ++j) coh[j] = stepE<KEL>(ps, ser[j] * half, sej[j] * half, coh[j]);
             if (lcur == 2) {
               const int ci = baseOut + cntOut;
-              const T dr = T(v.distOutGt[ci]), d = dr * h;
+              const T dr = T(global(v.distOutGt)[ci]), d = dr * h;
               const StepSite<T> pe = stepSite(D, es, d);
 #pragma unroll
               for (int j = 0; j < EL; ++j) {
                 if (!eOk[j]) continue;
                 const Cx<T> bv = stepE<KEL>(pe, ser[j] * dr, sej[j] * dr, coh[j]);
                 const size_t si = size_t(ci) * Ne + g * EL + j;
-                v.bOutGt[si] = double(bv.re);
-                v.bOutGt[size_t(v.nOutGt) * Ne + si] = double(bv.im);
+                global(v.bOutGt)[si] = double(bv.re);
+                global(v.bOutGt)[size_t(v.nOutGt) * Ne + si] = double(bv.im);
               }
               ++cntOut;
             }
           } else if (lcur == 3) {
             // entry of gamma~: start from the boundary value
             const int ci = baseIn + cntIn;
-            const T dr = T(v.distInGt[ci]), d = dr * h;
+            const T dr = T(global(v.distInGt)[ci]), d = dr * h;
             const StepSite<T> pe = stepSite(D, es, d);
 #pragma unroll
             for (int j = 0; j < EL; ++j) {
               const size_t si = size_t(ci) * Ne + g * EL + j;
-              const Cx<T> b0 = eOk[j] ? Cx<T>{T(v.bInGt[si]), T(v.bInGt[size_t(v.nInGt) * Ne + si])}
+              const GlobalPtr<const double> bin = global(v.bInGt);
+              const Cx<T> b0 = eOk[j] ? Cx<T>{T(bin[si]), T(bin[size_t(v.nInGt) * Ne + si])}
                                       : Cx<T>{T(0), T(0)};
               coh[j] = stepE<KEL>(pe, ser[j] * dr, sej[j] * dr, b0);
             }
@@ -255,7 +255,7 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev
 
   // ---------------- pass 2: gamma along +y, Green functions ----------------
   {
-    const int baseIn = v.idxInG[col], baseOut = v.idxOutG[col];
+    const int baseIn = global(v.idxInG)[col], baseOut = global(v.idxOutG)[col];
     int cntIn = 0, cntOut = 0;
     Cx<T> coh[EL];
 #pragma unroll
@@ -281,9 +281,9 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev
       if (colActive && g < gp && tt < len) {
         const int y = it.ylo + tt;
         const size_t fi = size_t(k - v.fieldBase) * n1 + size_t(y) * N1 + x;
-        nre = v.fields[fi];
-        nim = v.fields[v.fieldPlane + fi];
-        nes = v.fields[2 * v.fieldPlane + fi];
+        nre = global(v.fields)[fi];
+        nim = global(v.fields)[v.fieldPlane + fi];
+        nes = global(v.fields)[2 * v.fieldPlane + fi];
         nlab = lab[size_t(y) * N1];
       } else {
         nre = nim = nes = 0.0;
@@ -323,26 +323,27 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev
             for (int j = 0; j < EL; ++j) coh[j] = stepE<KEL>(ps, ser[j] * half, sej[j] * half, coh[j]);
             if (lcur == 3) {
               const int ci = baseOut + cntOut;
-              const T dr = T(v.distOutG[ci]), d = dr * h;
+              const T dr = T(global(v.distOutG)[ci]), d = dr * h;
               const StepSite<T> pe = stepSite(D, es, d);
 #pragma unroll
               for (int j = 0; j < EL; ++j) {
                 if (!eOk[j]) continue;
                 const Cx<T> bv = stepE<KEL>(pe, ser[j] * dr, sej[j] * dr, coh[j]);
                 const size_t si = size_t(ci) * Ne + g * EL + j;
-                v.bOutG[si] = double(bv.re);
-                v.bOutG[size_t(v.nOutG) * Ne + si] = double(bv.im);
+                global(v.bOutG)[si] = double(bv.re);
+                global(v.bOutG)[size_t(v.nOutG) * Ne + si] = double(bv.im);
               }
               ++cntOut;
             }
           } else if (lcur == 2) {
             const int ci = baseIn + cntIn;
-            const T dr = T(v.distInG[ci]), d = dr * h;
+            const T dr = T(global(v.distInG)[ci]), d = dr * h;
             const StepSite<T> pe = stepSite(D, es, d);
 #pragma unroll
             for (int j = 0; j < EL; ++j) {
               const size_t si = size_t(ci) * Ne + g * EL + j;
-              const Cx<T> b0 = eOk[j] ? Cx<T>{T(v.bInG[si]), T(v.bInG[size_t(v.nInG) * Ne + si])}
+              const GlobalPtr<const double> bin = global(v.bInG);
+              const Cx<T> b0 = eOk[j] ? Cx<T>{T(bin[si]), T(bin[size_t(v.nInG) * Ne + si])}
                                       : Cx<T>{T(0), T(0)};
               coh[j] = stepE<KEL>(pe, ser[j] * dr, sej[j] * dr, b0);
             }
@@ -364,7 +365,8 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev
           // LocalGreens.h:32-45): -Im(i X) with X = -(1 - p) / (1 + p),
           // i.e. Re((1 - p) / (1 + p)) = (1 - |p|^2) / |1 + p|^2.
           if (!a.burnin && lcur > 0 && active) {
-            double *Gs = v.G + (size_t(k - v.angleBase) * n1 + size_t(y) * N1 + x) * Ne + g * EL;
+            const GlobalPtr<double> Gs =
+                global(v.G + (size_t(k - v.angleBase) * n1 + size_t(y) * N1 + x) * Ne + g * EL);
 #pragma unroll
             for (int j = 0; j < EL; ++j) {
               if (!eOk[j]) continue;
@@ -418,10 +420,11 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev
             }
           }
           if (g == 0 && colActive) {
-            double2 *Gk = reinterpret_cast<double2 *>(
-                v.G + (size_t(k) * n1 + size_t(y) * N1 + x) * 4);
-            Gk[0] = make_double2(gdr, gdi);
-            Gk[1] = make_double2(gj, gf);
+            typedef typename VecOf<double>::type D2;
+            const GlobalPtr<D2> Gk =
+                global(reinterpret_cast<D2 *>(v.G + (size_t(k) * n1 + size_t(y) * N1 + x) * 4));
+            Gk[0] = D2{gdr, gdi};
+            Gk[1] = D2{gj, gf};
           }
         }
         dPrev = D;
@@ -436,8 +439,15 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const SweepItemDev
 // Persistent wavefronts pulling items from the work counter.  The second
 // launch bound asks the compiler for a register budget that leaves room for
 // WAVES wavefronts per SIMD.
+// The views and items come as restrict-qualified parameters as well: the
+// sweep never writes them, and with that known the compiler reads them with
+// scalar loads once per item (in SGPRs, uniform across the wavefront) and
+// reaches the arrays they point at with global (not flat) loads, instead of
+// reloading each field after every store the sweep makes.
 template <typename T, int EL, int WAVES, bool KEL>
-__global__ __launch_bounds__(64, WAVES) void sweepKernel(SweepArgs a) {
+__global__ __launch_bounds__(64, WAVES) void sweepKernel(SweepArgs a,
+                                                          const GrainView *__restrict__ views,
+                                                          const SweepItemDev *__restrict__ items) {
   Cx<T> *scr = reinterpret_cast<Cx<T> *>(a.scratch) + size_t(blockIdx.x) * a.slotStride;
   for (;;) {
     // every lane takes part in the fetch (lane 0 adds one, the others
@@ -449,7 +459,8 @@ __global__ __launch_bounds__(64, WAVES) void sweepKernel(SweepArgs a) {
     const int item =
         __builtin_amdgcn_readfirstlane(atomicAdd(a.counter, threadIdx.x == 0 ? 1 : 0));
     if (item >= a.numItems) break;
-    sweepItem<T, EL, KEL>(a, a.items[item], scr);
+    const SweepItemDev it = items[item];
+    sweepItem<T, EL, KEL>(a, views[it.grain], it, scr);
   }
 }
 
