@@ -309,10 +309,23 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const GrainView &v
         Cx<T> gtCur[EL];
 #pragma unroll
         for (int j = 0; j < EL; ++j) gtCur[j] = gtNext[j];
-        if (green && lnext > 0) {
+        // Every path through a site issues the same vector memory operations
+        // (this prefetch, then the Green record's two stores): the wait for
+        // the prefetched gamma~ is then counted past the stores that follow
+        // it and leaves them in flight.  A branch around either would make
+        // the count path-dependent, and the wait would also hold for the
+        // store's completion at every site.  Sites that need no gamma~ read
+        // a clamped, valid index of the slot's scratch.
+        if constexpr (KEL) {
+          if (green && lnext > 0) {
 #pragma unroll
-          for (int j = 0; j < EL; ++j)
-            gtNext[j] = loadStream(scr + (size_t(len - 2 - t) * EL + j) * 64 + lane);
+            for (int j = 0; j < EL; ++j)
+              gtNext[j] = loadStream(scr + (size_t(len - 2 - t) * EL + j) * 64 + lane);
+          }
+        } else {
+          const size_t tn = size_t(max(len - 2 - t, 0));
+#pragma unroll
+          for (int j = 0; j < EL; ++j) gtNext[j] = loadStream(scr + (tn * EL + j) * 64 + lane);
         }
         if (active) {
           if (lcur == 1 || lcur == 3) {
@@ -376,9 +389,10 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const GrainView &v
               Gs[j] = double((T(1) - (p.re * p.re + p.im * p.im)) * rcpT(den));
             }
           }
-        } else if (!a.burnin && lcur > 0) {
-          double gdr = 0.0, gdi = 0.0, gj = 0.0, su = 0.0, sw = 0.0;
-          if (active) {
+        } else {
+          const bool record = !a.burnin && lcur > 0;
+          double gdr = 0.0, gdi = 0.0, gj = 0.0, su = 0.0, sw = 0.0, gf = 0.0;
+          if (record && active) {
 #pragma unroll
             for (int j = 0; j < EL; ++j) {
               if (!eOk[j]) continue;
@@ -398,34 +412,39 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const GrainView &v
               sw += r * double(w.re);
             }
           }
-          double gf = dre * su + dim * sw;
-          // sum the energy groups of this column (lanes c, c + cols, ...)
-          // into lane c: a butterfly when the groups fill the wavefront,
-          // else the other groups' lanes read in turn
-          if (gp * cols == 64) {
-            for (int m = cols; m < 64; m <<= 1) {
-              gdr += __shfl_xor(gdr, m);
-              gdi += __shfl_xor(gdi, m);
-              gj += __shfl_xor(gj, m);
-              gf += __shfl_xor(gf, m);
-            }
-          } else {
-            const double g0r = gdr, g0i = gdi, g0j = gj, g0f = gf;
-            for (int m = 1; m < gp; ++m) {
-              const int src = c + m * cols;
-              gdr += __shfl(g0r, src);
-              gdi += __shfl(g0i, src);
-              gj += __shfl(g0j, src);
-              gf += __shfl(g0f, src);
+          if (record) {
+            gf = dre * su + dim * sw;
+            // sum the energy groups of this column (lanes c, c + cols, ...)
+            // into lane c: a butterfly when the groups fill the wavefront,
+            // else the other groups' lanes read in turn
+            if (gp * cols == 64) {
+              for (int m = cols; m < 64; m <<= 1) {
+                gdr += __shfl_xor(gdr, m);
+                gdi += __shfl_xor(gdi, m);
+                gj += __shfl_xor(gj, m);
+                gf += __shfl_xor(gf, m);
+              }
+            } else {
+              const double g0r = gdr, g0i = gdi, g0j = gj, g0f = gf;
+              for (int m = 1; m < gp; ++m) {
+                const int src = c + m * cols;
+                gdr += __shfl(g0r, src);
+                gdi += __shfl(g0i, src);
+                gj += __shfl(g0j, src);
+                gf += __shfl(g0f, src);
+              }
             }
           }
-          if (g == 0 && colActive) {
-            typedef typename VecOf<double>::type D2;
-            const GlobalPtr<D2> Gk =
-                global(reinterpret_cast<D2 *>(v.G + (size_t(k) * n1 + size_t(y) * N1 + x) * 4));
-            Gk[0] = D2{gdr, gdi};
-            Gk[1] = D2{gj, gf};
-          }
+          // lanes with no record to write store into the slot scratch of
+          // the first gamma~ this pass consumed (site len - 1, read before
+          // the first store and never again)
+          typedef typename VecOf<double>::type D2;
+          const bool real = record && g == 0 && colActive;
+          D2 *dst = real ? reinterpret_cast<D2 *>(v.G + (size_t(k) * n1 + size_t(y) * N1 + x) * 4)
+                         : reinterpret_cast<D2 *>(scr + size_t(len - 1) * EL * 64);
+          const GlobalPtr<D2> Gk = global(dst);
+          Gk[0] = D2{gdr, gdi};
+          Gk[1] = D2{gj, gf};
         }
         dPrev = D;
         esPrev = es;
