@@ -220,9 +220,10 @@ __device__ __forceinline__ RotSample rotatedCell(int x, int y, int M, int N,
   r.idx = Yi * N + Xi;
   return r;
 }
-__device__ __forceinline__ double bilinear(const double *__restrict__ p, int N,
+__device__ __forceinline__ double bilinear(const double *__restrict__ grid, int N,
                                            const RotSample &r) {
 #pragma clang fp contract(off)
+  const GlobalPtr<const double> p = global(grid);
   const double a = (1.0 - r.fx) * p[r.idx] + r.fx * p[r.idx + 1];
   const double b = (1.0 - r.fx) * p[r.idx + N] + r.fx * p[r.idx + N + 1];
   return (1.0 - r.fy) * a + r.fy * b;

@@ -17,10 +17,11 @@ __device__ __forceinline__ void siteFields(const GrainView &v, int k, int x,
   const size_t n0 = size_t(N0) * N0;
   dre = 0.0;
   dim = 0.0;
-  const RotSample r = rotatedCell(x, y, N1, N0, double(v.opCos[k]), double(v.opSin[k]));
+  const RotSample r =
+      rotatedCell(x, y, N1, N0, double(global(v.opCos)[k]), double(global(v.opSin)[k]));
   if (r.valid) {
     for (int c = 0; c < v.Nc; ++c) {
-      const double b = v.basis[c * v.Na + k];
+      const double b = global(v.basis)[c * v.Na + k];
       const double *p = v.extDelta + size_t(c) * 2 * n0;
       dre += b * bilinear(p, N0, r);
       dim += b * bilinear(p + n0, N0, r);
@@ -30,7 +31,8 @@ __device__ __forceinline__ void siteFields(const GrainView &v, int k, int x,
   if (v.useField) {
     double ax = 0.0, ay = 0.0;
     if (v.useInduced) {
-      const RotSample q = rotatedCell(x, y, N1, N0, double(v.aCos[k]), double(v.aSin[k]));
+      const RotSample q =
+          rotatedCell(x, y, N1, N0, double(global(v.aCos)[k]), double(global(v.aSin)[k]));
       if (q.valid) {
         ax = bilinear(v.A, N0, q) * v.lambdaInv2;
         ay = bilinear(v.A + n0, N0, q) * v.lambdaInv2;
@@ -38,7 +40,7 @@ __device__ __forceinline__ void siteFields(const GrainView &v, int k, int x,
     }
     const double off = -0.5 * double(N1 - 1);
     const double px0 = (double(x) + off) * v.h, py0 = (double(y) + off) * v.h;
-    const double c = v.saCos[k], s = v.saSin[k];
+    const double c = global(v.saCos)[k], s = global(v.saSin)[k];
     const double px = px0 * c - py0 * s, py = px0 * s + py0 * c;
     double ex, ey;
     if (v.gauge == 0) {
@@ -54,7 +56,7 @@ __device__ __forceinline__ void siteFields(const GrainView &v, int k, int x,
       ex = -py * f;
       ey = px * f;
     }
-    es = double(v.charge) * (v.vFx[k] * (ax + ex) + v.vFy[k] * (ay + ey));
+    es = double(v.charge) * (global(v.vFx)[k] * (ax + ex) + global(v.vFy)[k] * (ay + ey));
   }
 }
 
@@ -66,8 +68,10 @@ __global__ void resetCounter(int *c) { *c = 0; }
 // of a row of the tile are contiguous.  F is planar [3][frames][N1^2] with
 // plane stride `plane`; frame k is at (k - k0) * N1^2.
 __device__ __forceinline__ void fieldsTile(const GrainView &v, const int *tl, int k0,
-                                           long long plane, double *F) {
-  const int k = tl[0], x = tl[1] + int(threadIdx.x & 15), y0 = tl[2], y1 = tl[3];
+                                           long long plane, double *Fout) {
+  const GlobalPtr<const int> tg = global(tl);
+  const int k = tg[0], x = tg[1] + int(threadIdx.x & 15), y0 = tg[2], y1 = tg[3];
+  const GlobalPtr<double> F = global(Fout);
   const int N1 = v.N1;
   if (x >= N1) return;
   const size_t base = size_t(k - k0) * N1 * N1 + x;
@@ -89,8 +93,9 @@ __global__ void fieldsKernel(GrainView v, const int *tiles, int numTiles, int k0
 // The site fields of several grains in one launch (a batch): grain j of the
 // job list (view grainOf[j]) owns the blocks [tileStart[j], tileStart[j+1])
 // of the numTiles in total, one tile each, every frame of the grain.
-__global__ void fieldsBatchKernel(const GrainView *views, const int *grainOf,
-                                  const int *tileStart, int numJobs, int numTiles) {
+__global__ void fieldsBatchKernel(const GrainView *__restrict__ views,
+                                  const int *__restrict__ grainOf,
+                                  const int *__restrict__ tileStart, int numJobs, int numTiles) {
   const long long b = xcdBlock(numTiles);
   if (b < 0) return;
   int lo = 0, hi = numJobs - 1;
