@@ -321,13 +321,7 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const GrainView &v
         // the count path-dependent, and the wait would also hold for the
         // store's completion at every site.  Sites that need no gamma~ read
         // a clamped, valid index of the slot's scratch.
-        if constexpr (KEL) {
-          if (green && lnext > 0) {
-#pragma unroll
-            for (int j = 0; j < EL; ++j)
-              gtNext[j] = loadStream(scr + (size_t(len - 2 - t) * EL + j) * 64 + lane);
-          }
-        } else {
+        {
           const size_t tn = size_t(max(len - 2 - t, 0));
 #pragma unroll
           for (int j = 0; j < EL; ++j) gtNext[j] = loadStream(scr + (tn * EL + j) * 64 + lane);
@@ -382,17 +376,23 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const GrainView &v
           // Spectral weight per energy (ComputeLDOS.h:126-151 with
           // LocalGreens.h:32-45): -Im(i X) with X = -(1 - p) / (1 + p),
           // i.e. Re((1 - p) / (1 + p)) = (1 - |p|^2) / |1 + p|^2.
-          if (!a.burnin && lcur > 0 && active) {
-            const GlobalPtr<double> Gs =
-                global(v.G + (size_t(k - v.angleBase) * n1 + size_t(y) * N1 + x) * Ne + g * EL);
+          // Every lane stores its EL weights on every path, as the Green
+          // records below; weights with no site or energy to go to land in
+          // the consumed scratch.
+          const bool record = !a.burnin && lcur > 0 && active;
+          double *const row =
+              v.G + (size_t(k - v.angleBase) * n1 + size_t(y) * N1 + x) * Ne + g * EL;
+          double *const sink = reinterpret_cast<double *>(scr + size_t(len - 1) * EL * 64);
 #pragma unroll
-            for (int j = 0; j < EL; ++j) {
-              if (!eOk[j]) continue;
+          for (int j = 0; j < EL; ++j) {
+            double wgt = 0.0;
+            if (record && eOk[j]) {
               const Cx<T> p = coh[j] * gtCur[j];
               const T pr = T(1) + p.re;
               const T den = pr * pr + p.im * p.im;
-              Gs[j] = double((T(1) - (p.re * p.re + p.im * p.im)) * rcpT(den));
+              wgt = double((T(1) - (p.re * p.re + p.im * p.im)) * rcpT(den));
             }
+            *global(record && eOk[j] ? row + j : sink + j) = wgt;
           }
         } else {
           const bool record = !a.burnin && lcur > 0;
