@@ -195,7 +195,10 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const GrainView &v
       const int flab = nlab;
       const bool more = t0 + gp < len;
       if (more) loadChunk(t0 + gp);
-      const int tEnd = min(gp, len - t0);
+      // at least one site (t0 < len, gp >= 1): with no zero-trip path every
+      // way to the next chunk passes a site's memory operations, and the
+      // wait for this chunk's loads there is counted past them
+      const int tEnd = max(1, min(gp, len - t0));
       for (int i = 0; i < tEnd; ++i) {
         const int src = i * cols + c;
         const double dre = __shfl(fre, src), dim = __shfl(fim, src), e = __shfl(fes, src);
@@ -245,10 +248,18 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const GrainView &v
 #pragma unroll
             for (int j = 0; j < EL; ++j) coh[j] = {T(0), T(0)};
           }
-          if (!a.burnin && lcur > 0) {
+        }
+        // gamma~ into the slot's scratch on every path through a site (the
+        // second pass reads it only where it is inside and not burning in):
+        // the wait for the next chunk's loads is then counted past these
+        // stores.  A burn-in sweep reads none, and all its lanes store into
+        // one element.
+        {
+          const size_t at = a.burnin ? 0 : size_t(t) * EL;
+          const int ln = a.burnin ? 0 : lane;
 #pragma unroll
-            for (int j = 0; j < EL; ++j) storeStream(scr + (size_t(t) * EL + j) * 64 + lane, coh[j]);
-          }
+          for (int j = 0; j < EL; ++j)
+            storeStream(scr + (at + (a.burnin ? 0 : j)) * 64 + ln, coh[j]);
         }
         dPrev = D;
         esPrev = es;
@@ -301,7 +312,10 @@ __device__ __forceinline__ void sweepItem(const SweepArgs &a, const GrainView &v
       const int flab = nlab;
       const bool more = t0 + gp < len;
       if (more) loadChunk(t0 + gp);
-      const int tEnd = min(gp, len - t0);
+      // at least one site (t0 < len, gp >= 1): with no zero-trip path every
+      // way to the next chunk passes a site's memory operations, and the
+      // wait for this chunk's loads there is counted past them
+      const int tEnd = max(1, min(gp, len - t0));
       for (int i = 0; i < tEnd; ++i) {
         const int src = i * cols + c;
         const double dre = __shfl(fre, src), dim = __shfl(fim, src), e = __shfl(fes, src);
